@@ -1,0 +1,994 @@
+// engine.hip -- host runtime and C ABI of the GA* engine (libgastar.so).
+//
+// One engine owns a HIP stream for the search and one for path walks, a
+// device arena for heaps and closed sets, and a slot table of resident
+// instances.  Plain GA* (ga_solve), batched GA* (ga_solve_batch) and serving
+// (ga_serve_run) all run the same loop:
+//   admit -> launch a chunk of R iterations -> read back the previous chunk's
+//   per-instance state -> retire finished instances -> admit more.
+// Inside a chunk the host never synchronises; one launch is one iteration of
+// every resident instance (see gastar/core.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "gastar.h"
+#include "gastar/core.hpp"
+
+using namespace gastar;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// kernels instantiated for the built-in problems
+// ---------------------------------------------------------------------------
+constexpr int kBlock = 256;
+
+template <int W>
+__global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict__ wgmap,
+                                                      const InstDesc* __restrict__ descs, InstState* states,
+                                                      const uint32_t* __restrict__ iter_base, uint32_t r,
+                                                      StepParams prm) {
+    uint32_t inst, it, first;
+    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first)) return;
+    const InstDesc& d = descs[inst];
+    const uint32_t q = first + threadIdx.x / W;
+    const bool active = q < d.n_heaps;
+    switch (d.kind) {
+        case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, prm); break;
+        case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, prm); break;
+        case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, prm); break;
+        case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, prm); break;
+        default: break;
+    }
+}
+
+__global__ void seed_kernel(InstDesc d, InstDesc* descs, InstState* states, uint32_t slot, int W) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    descs[slot] = d;
+    switch (d.kind) {
+        case KIND_PUZZLE3: seed_instance<PuzzleOps<3>>(d, states + slot, W); break;
+        case KIND_PUZZLE4: seed_instance<PuzzleOps<4>>(d, states + slot, W); break;
+        case KIND_GRID4: seed_instance<GridOps<4>>(d, states + slot, W); break;
+        case KIND_GRID8: seed_instance<GridOps<8>>(d, states + slot, W); break;
+        default: break;
+    }
+}
+
+struct PathJob {
+    uint32_t inst;
+    uint32_t cap;
+    uint64_t offset;
+};
+
+__global__ void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n, uint64_t* out, uint32_t* lens) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PathJob j = jobs[i];
+    const InstDesc& d = descs[j.inst];
+    uint32_t len = kNone;
+    switch (d.kind) {
+        case KIND_PUZZLE3: len = walk_path<PuzzleOps<3>>(d, out + j.offset, j.cap); break;
+        case KIND_PUZZLE4: len = walk_path<PuzzleOps<4>>(d, out + j.offset, j.cap); break;
+        case KIND_GRID4: len = walk_path<GridOps<4>>(d, out + j.offset, j.cap); break;
+        case KIND_GRID8: len = walk_path<GridOps<8>>(d, out + j.offset, j.cap); break;
+        default: break;
+    }
+    lens[i] = len;
+}
+
+__global__ void abort_kernel(InstState* states, uint32_t slot) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) atomicCAS(&states[slot].status, (uint32_t)ST_RUNNING, (uint32_t)ST_ABORTED);
+}
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+struct GaError : std::runtime_error {
+    int code;
+    GaError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw GaError(GA_E_HIP, std::string(what) + ": " + hipGetErrorName(e));
+}
+
+uint64_t now_ns() {
+    return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+uint64_t pow2_floor(uint64_t v) {
+    uint64_t p = 1;
+    while (p * 2 <= v) p *= 2;
+    return p;
+}
+
+uint64_t pow2_ceil(uint64_t v) {
+    uint64_t p = 1;
+    while (p < v) p *= 2;
+    return p;
+}
+
+uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
+
+// ---------------------------------------------------------------------------
+// device arena: chunks from hipMalloc, first-fit with coalescing
+// ---------------------------------------------------------------------------
+class Arena {
+   public:
+    explicit Arena(uint64_t chunk_bytes) : chunk_bytes_(chunk_bytes) {}
+    ~Arena() {
+        for (auto& c : chunks_) (void)hipFree(c.base);
+    }
+
+    void* alloc(uint64_t bytes) {
+        bytes = round_up(bytes ? bytes : 1, kAlign);
+        for (auto& c : chunks_) {
+            void* p = take(c, bytes);
+            if (p) return p;
+        }
+        uint64_t sz = std::max<uint64_t>(chunk_bytes_, round_up(bytes, 1ull << 21));
+        char* base = nullptr;
+        if (hipMalloc(&base, sz) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        chunks_.push_back(Chunk{base, sz, {}});
+        chunks_.back().free[0] = sz;
+        return take(chunks_.back(), bytes);
+    }
+
+    void release(void* p) {
+        auto it = sizes_.find(p);
+        if (it == sizes_.end()) return;
+        uint64_t bytes = it->second;
+        sizes_.erase(it);
+        in_use_ -= bytes;
+        for (auto& c : chunks_) {
+            char* cp = (char*)p;
+            if (cp < c.base || cp >= c.base + c.size) continue;
+            uint64_t off = (uint64_t)(cp - c.base);
+            auto nx = c.free.lower_bound(off);
+            if (nx != c.free.end() && off + bytes == nx->first) {
+                bytes += nx->second;
+                nx = c.free.erase(nx);
+            }
+            if (nx != c.free.begin()) {
+                auto pv = std::prev(nx);
+                if (pv->first + pv->second == off) {
+                    pv->second += bytes;
+                    return;
+                }
+            }
+            c.free[off] = bytes;
+            return;
+        }
+    }
+
+    uint64_t in_use() const { return in_use_; }
+    uint64_t peak() const { return peak_; }
+
+   private:
+    static constexpr uint64_t kAlign = 256;
+    struct Chunk {
+        char* base;
+        uint64_t size;
+        std::map<uint64_t, uint64_t> free;  // offset -> bytes
+    };
+    void* take(Chunk& c, uint64_t bytes) {
+        for (auto it = c.free.begin(); it != c.free.end(); ++it) {
+            if (it->second < bytes) continue;
+            uint64_t off = it->first, rest = it->second - bytes;
+            c.free.erase(it);
+            if (rest) c.free[off + bytes] = rest;
+            void* p = c.base + off;
+            sizes_[p] = bytes;
+            in_use_ += bytes;
+            peak_ = std::max(peak_, in_use_);
+            return p;
+        }
+        return nullptr;
+    }
+    uint64_t chunk_bytes_;
+    std::vector<Chunk> chunks_;
+    std::map<void*, uint64_t> sizes_;
+    uint64_t in_use_ = 0, peak_ = 0;
+};
+
+// Pinned host buffer that grows on demand.
+struct Pinned {
+    void* p = nullptr;
+    size_t bytes = 0;
+    void ensure(size_t n) {
+        if (n <= bytes) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = 0;
+        size_t sz = std::max<size_t>(n, 4096);
+        hip_check(hipHostMalloc(&p, sz, hipHostMallocDefault), "hipHostMalloc");
+        bytes = sz;
+    }
+    ~Pinned() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
+int kind_of(const ga_request& r) {
+    if (r.kind == GA_PUZZLE) return r.side == 3 ? KIND_PUZZLE3 : KIND_PUZZLE4;
+    return r.connectivity == 8 ? KIND_GRID8 : KIND_GRID4;
+}
+
+int branching_of(int kind) { return kind == KIND_GRID8 ? 8 : 4; }
+
+// ---------------------------------------------------------------------------
+// one request moving through the engine
+// ---------------------------------------------------------------------------
+struct Job {
+    const ga_request* req = nullptr;
+    ga_result* out = nullptr;
+    uint64_t arrival_ns = 0;  // relative to the start of the run
+    uint32_t heap_shift = 0, table_shift = 0, retries = 0;
+    int kind = 0;
+    uint32_t w_min = 1;
+    // residency
+    int slot = -1;
+    uint64_t first_chunk = 0;  // first chunk whose read-back reflects this job
+    InstDesc desc{};
+    uint64_t bytes = 0;
+    std::vector<void*> allocs;
+    const uint8_t* cells_dev = nullptr;
+    uint64_t t_admit = 0;
+    uint32_t last_iterations = 0;
+    bool finished = false;
+};
+
+struct PathBatch {
+    std::vector<Job*> jobs;
+    std::vector<uint64_t> offsets;
+    std::vector<InstState> snaps;
+    void* dev = nullptr;   // PathJob[n] | lens[n] | out[total]
+    Pinned host;           // lens[n] | out[total]
+    hipEvent_t ev = nullptr;
+    uint64_t total = 0;
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+struct ga_engine {
+    int device = 0;
+    ga_config cfg{};
+    hipStream_t stream = nullptr, path_stream = nullptr;
+    Arena* arena = nullptr;
+    InstDesc* d_descs = nullptr;
+    InstState* d_states = nullptr;
+    WgEntry* d_map = nullptr;
+    uint32_t* d_iter = nullptr;
+    uint32_t map_cap = 0;
+    Pinned h_map[2];
+    hipEvent_t ev_map[2] = {nullptr, nullptr};
+    int map_buf = 0;
+    Pinned h_states[2];
+    hipEvent_t ev_chunk[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> prof_ev;  // 2 events per step of a chunk
+    uint64_t host_base = 0;           // iteration counter at the start of the next chunk
+    uint64_t chunk_count = 0;
+    std::vector<Job*> resident;
+    std::vector<int> free_slots;
+    uint32_t n_wg = 0;
+    bool map_dirty = true;
+    ga_engine_stats stats{};
+    std::string last_error;
+
+    void init(int dev, const ga_config* c) {
+        device = dev;
+        ga_config_default(&cfg);
+        if (c) cfg = *c;
+        if (cfg.heap_arity != 4 && cfg.heap_arity != 8 && cfg.heap_arity != 16) cfg.heap_arity = 4;
+        if (cfg.max_instances == 0) cfg.max_instances = 1;
+        if (cfg.chunk_iters == 0) cfg.chunk_iters = 1;
+        if (cfg.max_load_pct == 0 || cfg.max_load_pct > 95) cfg.max_load_pct = 50;
+        if (cfg.pops_per_iter == 0) cfg.pops_per_iter = 1;
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipStreamCreateWithFlags(&path_stream, hipStreamNonBlocking), "hipStreamCreate");
+        arena = new Arena(cfg.chunk_bytes);
+        hip_check(hipMalloc(&d_descs, sizeof(InstDesc) * cfg.max_instances), "hipMalloc descs");
+        hip_check(hipMalloc(&d_states, sizeof(InstState) * cfg.max_instances), "hipMalloc states");
+        hip_check(hipMemset(d_states, 0, sizeof(InstState) * cfg.max_instances), "hipMemset states");
+        hip_check(hipMalloc(&d_iter, sizeof(uint32_t)), "hipMalloc iter");
+        hip_check(hipMemset(d_iter, 0, sizeof(uint32_t)), "hipMemset iter");
+        for (int i = 0; i < 2; ++i) {
+            hip_check(hipEventCreateWithFlags(&ev_map[i], hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventCreateWithFlags(&ev_chunk[i], hipEventDisableTiming), "hipEventCreate");
+            h_states[i].ensure(sizeof(InstState) * cfg.max_instances);
+        }
+        resident.assign(cfg.max_instances, nullptr);
+        for (int s = (int)cfg.max_instances - 1; s >= 0; --s) free_slots.push_back(s);
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+    }
+
+    ~ga_engine() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        if (path_stream) (void)hipStreamSynchronize(path_stream);
+        for (auto e : prof_ev) (void)hipEventDestroy(e);
+        for (int i = 0; i < 2; ++i) {
+            if (ev_map[i]) (void)hipEventDestroy(ev_map[i]);
+            if (ev_chunk[i]) (void)hipEventDestroy(ev_chunk[i]);
+        }
+        if (d_map) (void)hipFree(d_map);
+        if (d_descs) (void)hipFree(d_descs);
+        if (d_states) (void)hipFree(d_states);
+        if (d_iter) (void)hipFree(d_iter);
+        delete arena;
+        if (stream) (void)hipStreamDestroy(stream);
+        if (path_stream) (void)hipStreamDestroy(path_stream);
+    }
+
+    // ---- planning -----------------------------------------------------------
+    struct Plan {
+        uint32_t n_heaps, heap_cap, heap_stride;
+        uint64_t table_slots;  // hashed sets; dense sets use width*height
+        uint64_t bytes;
+    };
+
+    Plan plan(const Job& j) const {
+        const ga_request& r = *j.req;
+        const uint32_t W = cfg.heap_arity;
+        Plan p{};
+        uint64_t n_req = r.num_queues;
+        uint64_t cells = (uint64_t)r.width * r.height;
+        if (j.kind == KIND_PUZZLE3) {
+            p.n_heaps = (uint32_t)pow2_floor(n_req ? n_req : 256);
+            p.heap_cap = r.heap_capacity ? r.heap_capacity : 1024;
+            p.table_slots = r.table_slots ? pow2_ceil(r.table_slots) : (1ull << 19);
+        } else if (j.kind == KIND_PUZZLE4) {
+            p.n_heaps = (uint32_t)pow2_floor(n_req ? n_req : 16384);
+            p.heap_cap = r.heap_capacity ? r.heap_capacity : 512;
+            p.table_slots = r.table_slots ? pow2_ceil(r.table_slots) : (1ull << 22);
+        } else {
+            uint64_t dflt = std::min<uint64_t>(16384, std::max<uint64_t>(64, pow2_floor(std::max<uint64_t>(cells / 64, 1))));
+            p.n_heaps = (uint32_t)pow2_floor(n_req ? n_req : dflt);
+            p.heap_cap = r.heap_capacity ? r.heap_capacity : 256;
+            p.table_slots = 0;
+        }
+        p.n_heaps = std::min<uint32_t>(std::max<uint32_t>(p.n_heaps, 1), 1u << 22);
+        uint64_t cap = (uint64_t)p.heap_cap << j.heap_shift;
+        p.heap_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(cap, 16), 1u << 28);
+        p.heap_stride = (uint32_t)round_up(p.heap_cap + W - 1, W);
+        if (p.table_slots) p.table_slots = std::min<uint64_t>(p.table_slots << j.table_shift, 1ull << 32);
+        const int K = branching_of(j.kind);
+        uint64_t b = (uint64_t)p.n_heaps * p.heap_stride * sizeof(HeapEntry);  // heaps
+        b += (uint64_t)p.n_heaps * sizeof(uint32_t);                          // heap sizes
+        b += 2ull * p.n_heaps * K * sizeof(HeapEntry);                         // candidates
+        b += p.table_slots ? p.table_slots * sizeof(Slot) : cells * sizeof(uint64_t);
+        p.bytes = b;
+        return p;
+    }
+
+    // ---- admission ------------------------------------------------------------
+    // Returns 1 if admitted, 0 if it must wait (no slot / budget), or throws.
+    int admit(Job& j) {
+        if (free_slots.empty()) return 0;
+        const ga_request& r = *j.req;
+        Plan p = plan(j);
+        uint64_t cell_bytes = (r.kind == GA_GRID && !r.cells_device) ? (uint64_t)r.width * r.height : 0;
+        uint64_t need = p.bytes + cell_bytes;
+        if (need > cfg.budget_bytes) throw GaError(GA_E_BUDGET, "request exceeds the admission budget");
+        if (arena->in_use() + need > cfg.budget_bytes) return 0;
+        const int W = (int)cfg.heap_arity;
+        const int K = branching_of(j.kind);
+        std::vector<void*> allocs;
+        auto take = [&](uint64_t bytes) -> void* {
+            void* ptr = arena->alloc(bytes);
+            if (!ptr) {
+                for (void* a : allocs) arena->release(a);
+                throw GaError(GA_E_NOMEM, "device arena exhausted");
+            }
+            allocs.push_back(ptr);
+            return ptr;
+        };
+        InstDesc d{};
+        d.kind = (uint32_t)j.kind;
+        d.start = r.start;
+        d.goal = r.goal;
+        d.heaps = (HeapEntry*)take((uint64_t)p.n_heaps * p.heap_stride * sizeof(HeapEntry));
+        d.heap_size = (uint32_t*)take((uint64_t)p.n_heaps * sizeof(uint32_t));
+        d.cand = (HeapEntry*)take(2ull * p.n_heaps * K * sizeof(HeapEntry));
+        d.n_heaps = p.n_heaps;
+        d.heap_cap = p.heap_cap;
+        d.heap_stride = p.heap_stride;
+        const uint32_t fill_grid = 1024;
+        if (r.kind == GA_PUZZLE) {
+            d.width = r.side;
+            d.height = r.side;
+            d.table = (Slot*)take(p.table_slots * sizeof(Slot));
+            d.table_mask = p.table_slots - 1;
+            d.load_limit = p.table_slots * cfg.max_load_pct / 100;
+            for (uint32_t c = 0; c < r.side * r.side; ++c) {
+                uint32_t t = (uint32_t)(r.goal >> (4 * c)) & 15u;
+                d.goal_rc[t] = (uint8_t)(((c / r.side) << 4) | (c % r.side));
+            }
+            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>((uint64_t*)d.table, p.table_slots * 2, ~0ull);
+        } else {
+            d.width = r.width;
+            d.height = r.height;
+            d.w_min = j.w_min;
+            uint64_t cells = (uint64_t)r.width * r.height;
+            if (r.cells_device) {
+                d.cells = r.cells_device;
+            } else {
+                uint8_t* c = (uint8_t*)take(cells);
+                hip_check(hipMemcpyAsync(c, r.cells, cells, hipMemcpyHostToDevice, stream), "upload cells");
+                d.cells = c;
+            }
+            d.dense = (uint64_t*)take(cells * sizeof(uint64_t));
+            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>(d.dense, cells, ~0ull);
+        }
+        fill_kernel<uint32_t><<<std::min<uint32_t>(fill_grid, (p.n_heaps + kBlock - 1) / kBlock), kBlock, 0,
+                                stream>>>(d.heap_size, p.n_heaps, 0u);
+        int slot = free_slots.back();
+        free_slots.pop_back();
+        d.start_iter = (uint32_t)host_base;
+        seed_kernel<<<1, 64, 0, stream>>>(d, d_descs, d_states, (uint32_t)slot, W);
+        hip_check(hipGetLastError(), "seed launch");
+        j.slot = slot;
+        j.desc = d;
+        j.allocs = std::move(allocs);
+        j.bytes = need;
+        j.first_chunk = chunk_count;
+        j.t_admit = now_ns();
+        j.last_iterations = 0;
+        resident[slot] = &j;
+        map_dirty = true;
+        stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena->in_use());
+        return 1;
+    }
+
+    void release(Job& j) {
+        for (void* a : j.allocs) arena->release(a);
+        j.allocs.clear();
+        if (j.slot >= 0) {
+            resident[j.slot] = nullptr;
+            free_slots.push_back(j.slot);
+            j.slot = -1;
+            map_dirty = true;
+        }
+    }
+
+    // ---- launch ---------------------------------------------------------------
+    void upload_map() {
+        const uint32_t W = cfg.heap_arity;
+        const uint32_t heaps_per_wg = kBlock / W;
+        std::vector<WgEntry> map;
+        for (uint32_t s = 0; s < cfg.max_instances; ++s) {
+            Job* j = resident[s];
+            if (!j || j->finished) continue;
+            uint32_t n = j->desc.n_heaps;
+            for (uint32_t h = 0; h < n; h += heaps_per_wg) map.push_back(WgEntry{s, h});
+        }
+        n_wg = (uint32_t)map.size();
+        map_dirty = false;
+        if (!n_wg) return;
+        if (n_wg > map_cap) {
+            if (d_map) {
+                hip_check(hipStreamSynchronize(stream), "sync");
+                hip_check(hipFree(d_map), "hipFree map");
+            }
+            map_cap = (uint32_t)pow2_ceil(n_wg);
+            hip_check(hipMalloc(&d_map, sizeof(WgEntry) * map_cap), "hipMalloc map");
+        }
+        int b = map_buf;
+        map_buf ^= 1;
+        hip_check(hipEventSynchronize(ev_map[b]), "map event");
+        h_map[b].ensure(sizeof(WgEntry) * n_wg);
+        memcpy(h_map[b].p, map.data(), sizeof(WgEntry) * n_wg);
+        hip_check(hipMemcpyAsync(d_map, h_map[b].p, sizeof(WgEntry) * n_wg, hipMemcpyHostToDevice, stream),
+                  "upload map");
+        hip_check(hipEventRecord(ev_map[b], stream), "record map");
+    }
+
+    void launch_chunk() {
+        const uint32_t R = cfg.chunk_iters;
+        StepParams prm{cfg.pops_per_iter, 1u << 16, cfg.profile ? 1u : 0u, 0};
+        if (cfg.profile && prof_ev.size() < 2 * R) {
+            for (size_t i = prof_ev.size(); i < 2 * R; ++i) {
+                hipEvent_t e;
+                hip_check(hipEventCreate(&e), "hipEventCreate");
+                prof_ev.push_back(e);
+            }
+        }
+        for (uint32_t r = 0; r < R; ++r) {
+            if (cfg.profile) hip_check(hipEventRecord(prof_ev[2 * r], stream), "event");
+            switch (cfg.heap_arity) {
+                case 8: step_kernel<8><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+                case 16: step_kernel<16><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+                default: step_kernel<4><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+            }
+            if (cfg.profile) hip_check(hipEventRecord(prof_ev[2 * r + 1], stream), "event");
+        }
+        advance_kernel<<<1, 64, 0, stream>>>(d_iter, R);
+        hip_check(hipGetLastError(), "step launch");
+        host_base += R;
+        stats.step_launches += R;
+        int b = (int)(chunk_count & 1);
+        hip_check(hipMemcpyAsync(h_states[b].p, d_states, sizeof(InstState) * cfg.max_instances,
+                                 hipMemcpyDeviceToHost, stream),
+                  "read back states");
+        hip_check(hipEventRecord(ev_chunk[b], stream), "record chunk");
+        ++chunk_count;
+        if (cfg.profile) {
+            hip_check(hipEventSynchronize(ev_chunk[b]), "sync chunk");
+            for (uint32_t r = 0; r < R; ++r) {
+                float ms = 0;
+                hip_check(hipEventElapsedTime(&ms, prof_ev[2 * r], prof_ev[2 * r + 1]), "elapsed");
+                stats.step_ms_total += ms;
+            }
+        }
+    }
+
+    // ---- results -------------------------------------------------------------
+    void fill_counters(Job& j, const InstState& s) {
+        ga_result* o = j.out;
+        o->expanded = s.expanded;
+        o->stored = j.kind == KIND_GRID4 || j.kind == KIND_GRID8 ? 0 : s.occupied;
+        o->iterations = s.iterations;
+        o->retries = j.retries;
+        stats.expanded += s.expanded;
+        stats.generated += s.generated;
+        stats.pushed += s.pushed;
+        stats.probes += s.probes;
+        stats.sift_down += s.sift_down;
+        stats.sift_up += s.sift_up;
+        stats.pops += s.pops;
+    }
+
+    void finish(Job& j, int status, uint64_t t0) {
+        j.out->status = status;
+        j.out->user_tag = j.req->user_tag;
+        uint64_t t = now_ns();
+        if (j.t_admit) j.out->search_ns = t - j.t_admit;
+        j.out->latency_ns = t - t0 - std::min(j.arrival_ns, t - t0);
+        j.finished = true;
+        ++stats.completed;
+    }
+};
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// the run loop shared by plain, batched and served GA*
+// ---------------------------------------------------------------------------
+int validate(const ga_request& r) {
+    if (r.kind == GA_PUZZLE) {
+        if (r.side != 3 && r.side != 4) return GA_E_INVALID;
+        uint32_t n = r.side * r.side;
+        for (uint64_t s : {r.start, r.goal}) {
+            uint32_t seen = 0;
+            for (uint32_t c = 0; c < n; ++c) {
+                uint32_t t = (uint32_t)(s >> (4 * c)) & 15u;
+                if (t >= n || ((seen >> t) & 1)) return GA_E_INVALID;
+                seen |= 1u << t;
+            }
+            if (n < 16 && (s >> (4 * n)) != 0) return GA_E_INVALID;
+        }
+        return GA_OK;
+    }
+    if (r.kind == GA_GRID) {
+        if (!r.width || !r.height || r.width > 65535 || r.height > 65535) return GA_E_INVALID;
+        if ((uint64_t)r.width * r.height >= 0xFFFFFFFFull) return GA_E_INVALID;
+        if (r.connectivity != 4 && r.connectivity != 8) return GA_E_INVALID;
+        if (!r.cells && !r.cells_device) return GA_E_INVALID;
+        uint64_t n = (uint64_t)r.width * r.height;
+        if (r.start >= n || r.goal >= n) return GA_E_INVALID;
+        if (r.cells && (!r.cells[r.start] || !r.cells[r.goal])) return GA_E_INVALID;
+        return GA_OK;
+    }
+    return GA_E_INVALID;
+}
+
+uint32_t grid_wmin(const ga_request& r) {
+    if (r.w_min) return r.w_min;
+    if (!r.cells) return 1;  // weights are >= 1, so 1 is always admissible
+    uint64_t n = (uint64_t)r.width * r.height;
+    uint32_t m = 256;
+    for (uint64_t i = 0; i < n && m > 1; ++i)
+        if (r.cells[i] && r.cells[i] < m) m = r.cells[i];
+    return m == 256 ? 1 : m;
+}
+
+void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* ss) {
+    const uint64_t t0 = now_ns();
+    std::deque<Job*> pending, retry;
+    std::vector<Job*> order;
+    for (auto& j : jobs) order.push_back(&j);
+    std::stable_sort(order.begin(), order.end(), [](Job* a, Job* b) { return a->arrival_ns < b->arrival_ns; });
+    for (Job* j : order) pending.push_back(j);
+    std::vector<PathBatch*> batches;
+    uint32_t inflight = 0, max_inflight = 0;
+    bool have_prev = false;
+
+    auto finalize_batch = [&](PathBatch* b) {
+        hip_check(hipEventSynchronize(b->ev), "path event");
+        size_t n = b->jobs.size();
+        const uint32_t* lens = (const uint32_t*)b->host.p;
+        const uint64_t* out = (const uint64_t*)((char*)b->host.p + round_up(n * sizeof(uint32_t), 16));
+        for (size_t i = 0; i < n; ++i) {
+            Job& j = *b->jobs[i];
+            uint32_t len = lens[i];
+            e->fill_counters(j, b->snaps[i]);
+            j.out->cost = b->snaps[i].best;
+            if (len == kNone || len == 0) {
+                e->finish(j, GA_E_OVERFLOW, t0);
+            } else {
+                j.out->path = (uint64_t*)malloc(sizeof(uint64_t) * len);
+                for (uint32_t k = 0; k < len; ++k) j.out->path[k] = out[b->offsets[i] + (len - 1 - k)];
+                j.out->path_len = len;
+                e->finish(j, GA_OK, t0);
+            }
+            e->release(j);
+            --inflight;
+        }
+        e->arena->release(b->dev);
+        (void)hipEventDestroy(b->ev);
+        delete b;
+    };
+
+    auto handle_snapshot = [&](uint64_t chunk) {
+        const InstState* snap = (const InstState*)e->h_states[chunk & 1].p;
+        std::vector<Job*> done;
+        for (uint32_t s = 0; s < e->cfg.max_instances; ++s) {
+            Job* j = e->resident[s];
+            if (!j || j->finished || j->first_chunk > chunk) continue;
+            const InstState& st = snap[s];
+            if (st.status == ST_RUNNING) {
+                j->last_iterations = st.iterations;
+                if (e->cfg.max_iterations && st.iterations > e->cfg.max_iterations) {
+                    abort_kernel<<<1, 64, 0, e->stream>>>(e->d_states, (uint32_t)s);
+                }
+                continue;
+            }
+            if (st.status == ST_DONE && st.best != kInf) {
+                j->finished = true;  // leaves the launch map; memory kept for the path walk
+                e->map_dirty = true;
+                done.push_back(j);
+                continue;
+            }
+            if (st.status == ST_OVERFLOW_HEAP || st.status == ST_OVERFLOW_TABLE) {
+                e->fill_counters(*j, st);
+                e->release(*j);
+                --inflight;
+                if (j->retries >= e->cfg.max_retries) {
+                    e->finish(*j, GA_E_OVERFLOW, t0);
+                } else {
+                    ++j->retries;
+                    if (st.status == ST_OVERFLOW_HEAP) j->heap_shift += 2;
+                    else j->table_shift += 2;
+                    retry.push_back(j);
+                }
+                continue;
+            }
+            // DONE without a goal = search space exhausted; ABORTED = iteration limit
+            e->fill_counters(*j, st);
+            j->out->cost = 0;
+            e->finish(*j, st.status == ST_DONE ? GA_NO_PATH : GA_E_TIMEOUT, t0);
+            e->release(*j);
+            --inflight;
+        }
+        if (done.empty()) return;
+        // walk the paths on the second stream once this chunk has finished
+        PathBatch* b = new PathBatch();
+        b->jobs = done;
+        uint64_t total = 0;
+        for (Job* j : done) {
+            b->offsets.push_back(total);
+            b->snaps.push_back(snap[j->slot]);
+            total += (uint64_t)snap[j->slot].best + 1;  // every move costs >= 1
+        }
+        size_t n = done.size();
+        size_t lens_bytes = round_up(n * sizeof(uint32_t), 16);
+        size_t jobs_bytes = round_up(n * sizeof(PathJob), 16);
+        b->dev = e->arena->alloc(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
+        if (!b->dev) throw GaError(GA_E_NOMEM, "device arena exhausted (paths)");
+        b->host.ensure(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
+        PathJob* pj = (PathJob*)((char*)b->host.p + lens_bytes + total * sizeof(uint64_t));
+        for (size_t i = 0; i < n; ++i)
+            pj[i] = PathJob{(uint32_t)done[i]->slot, snap[done[i]->slot].best + 1, b->offsets[i]};
+        char* dj = (char*)b->dev;
+        uint32_t* dl = (uint32_t*)(dj + jobs_bytes);
+        uint64_t* dout = (uint64_t*)(dj + jobs_bytes + lens_bytes);
+        hipEvent_t ready;
+        hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventRecord(ready, e->stream), "record");
+        hip_check(hipStreamWaitEvent(e->path_stream, ready, 0), "wait");
+        hip_check(hipMemcpyAsync(dj, pj, n * sizeof(PathJob), hipMemcpyHostToDevice, e->path_stream), "upload jobs");
+        path_kernel<<<(uint32_t)((n + 63) / 64), 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n,
+                                                                          dout, dl);
+        hip_check(hipGetLastError(), "path launch");
+        hip_check(hipMemcpyAsync(b->host.p, dl, lens_bytes + total * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                 e->path_stream),
+                  "read back paths");
+        hip_check(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventRecord(b->ev, e->path_stream), "record");
+        (void)hipEventDestroy(ready);
+        b->total = total;
+        batches.push_back(b);
+    };
+
+    for (;;) {
+        // -- admit: retries first, then arrivals in order
+        uint64_t now = now_ns() - t0;
+        for (std::deque<Job*>* q : {&retry, &pending}) {
+            while (!q->empty()) {
+                Job* j = q->front();
+                if (timed && j->arrival_ns > now) break;
+                int rc;
+                try {
+                    rc = e->admit(*j);
+                } catch (const GaError& err) {
+                    if (err.code == GA_E_HIP) throw;
+                    q->pop_front();
+                    e->finish(*j, err.code, t0);
+                    continue;
+                }
+                if (!rc) break;
+                q->pop_front();
+                ++inflight;
+            }
+        }
+        max_inflight = std::max(max_inflight, inflight);
+        // -- retire path walks that have completed
+        while (!batches.empty() && hipEventQuery(batches.front()->ev) == hipSuccess) {
+            finalize_batch(batches.front());
+            batches.erase(batches.begin());
+        }
+        uint32_t running = 0;
+        for (Job* j : e->resident)
+            if (j && !j->finished) ++running;
+        if (running > 0) {
+            if (e->map_dirty) e->upload_map();
+            if (e->n_wg) e->launch_chunk();
+            if (have_prev) {
+                hip_check(hipEventSynchronize(e->ev_chunk[(e->chunk_count - 2) & 1]), "chunk event");
+                handle_snapshot(e->chunk_count - 2);
+            }
+            have_prev = true;
+            continue;
+        }
+        if (have_prev) {
+            hip_check(hipEventSynchronize(e->ev_chunk[(e->chunk_count - 1) & 1]), "chunk event");
+            handle_snapshot(e->chunk_count - 1);
+            have_prev = false;
+            continue;
+        }
+        if (!batches.empty()) {
+            finalize_batch(batches.front());
+            batches.erase(batches.begin());
+            continue;
+        }
+        if (pending.empty() && retry.empty()) break;
+        if (retry.empty() && timed) {
+            uint64_t wait = pending.front()->arrival_ns;
+            now = now_ns() - t0;
+            if (wait > now) std::this_thread::sleep_for(std::chrono::nanoseconds(std::min<uint64_t>(wait - now, 200000)));
+        }
+    }
+    if (ss) {
+        std::vector<double> lat;
+        uint64_t last = 0, first = ~0ull;
+        ss->completed = ss->failed = 0;
+        for (auto& j : jobs) {
+            first = std::min(first, j.arrival_ns);
+            if (j.out->status == GA_OK || j.out->status == GA_NO_PATH) {
+                ++ss->completed;
+                lat.push_back(j.out->latency_ns / 1e6);
+                last = std::max(last, j.arrival_ns + j.out->latency_ns);
+            } else {
+                ++ss->failed;
+            }
+        }
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double p) {
+            if (lat.empty()) return 0.0;
+            size_t k = (size_t)std::min<double>(lat.size() - 1, std::ceil(p * lat.size()) - 1);
+            return lat[k];
+        };
+        ss->wall_ns = last > first ? last - first : 0;
+        ss->solves_per_sec = ss->wall_ns ? ss->completed / (ss->wall_ns / 1e9) : 0.0;
+        ss->p50_ms = pct(0.50);
+        ss->p99_ms = pct(0.99);
+        ss->max_inflight = max_inflight;
+    }
+}
+
+int run_checked(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, uint32_t n, ga_result* outs,
+                ga_serve_stats* ss) {
+    if (!e || (!reqs && n) || (!outs && n)) return GA_E_INVALID;
+    std::vector<Job> jobs(n);
+    int worst = GA_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        memset(&outs[i], 0, sizeof(ga_result));
+        outs[i].user_tag = reqs[i].user_tag;
+        jobs[i].req = &reqs[i];
+        jobs[i].out = &outs[i];
+        jobs[i].arrival_ns = arrivals ? arrivals[i] : 0;
+        int v = validate(reqs[i]);
+        if (v != GA_OK) {
+            outs[i].status = v;
+            jobs[i].finished = true;
+            worst = v;
+            continue;
+        }
+        jobs[i].kind = kind_of(reqs[i]);
+        if (reqs[i].kind == GA_GRID) jobs[i].w_min = grid_wmin(reqs[i]);
+        if (reqs[i].kind == GA_PUZZLE && !ga_puzzle_solvable(reqs[i].side, reqs[i].start, reqs[i].goal)) {
+            outs[i].status = GA_NO_PATH;
+            jobs[i].finished = true;
+        }
+    }
+    std::vector<Job> live;
+    std::vector<uint32_t> live_idx;
+    for (uint32_t i = 0; i < n; ++i)
+        if (!jobs[i].finished) {
+            live.push_back(jobs[i]);
+            live_idx.push_back(i);
+        }
+    try {
+        hip_check(hipSetDevice(e->device), "hipSetDevice");
+        run_jobs(e, live, arrivals != nullptr, ss);
+    } catch (const GaError& err) {
+        e->last_error = err.what();
+        return err.code;
+    } catch (const std::bad_alloc&) {
+        return GA_E_NOMEM;
+    }
+    for (auto& j : live)
+        if (j.out->status != GA_OK && j.out->status != GA_NO_PATH) worst = j.out->status;
+    return worst;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void ga_config_default(ga_config* cfg) {
+    memset(cfg, 0, sizeof(*cfg));
+    cfg->budget_bytes = 64ull << 30;
+    cfg->chunk_bytes = 4ull << 30;
+    cfg->max_instances = 4096;
+    cfg->chunk_iters = 8;
+    cfg->use_graph = 0;
+    cfg->max_retries = 6;
+    cfg->max_iterations = 0;
+    cfg->heap_arity = 4;
+    cfg->max_load_pct = 50;
+    cfg->pops_per_iter = 2;
+    cfg->profile = 0;
+}
+
+int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {
+    if (!out) return GA_E_INVALID;
+    *out = nullptr;
+    ga_engine* e = new (std::nothrow) ga_engine();
+    if (!e) return GA_E_NOMEM;
+    try {
+        e->init(device, cfg);
+    } catch (const GaError& err) {
+        delete e;
+        return err.code;
+    }
+    *out = e;
+    return GA_OK;
+}
+
+void ga_engine_destroy(ga_engine* e) { delete e; }
+
+int ga_engine_get_stats(ga_engine* e, ga_engine_stats* out) {
+    if (!e || !out) return GA_E_INVALID;
+    *out = e->stats;
+    out->peak_bytes = e->arena->peak();
+    return GA_OK;
+}
+
+void ga_engine_reset_stats(ga_engine* e) {
+    if (e) e->stats = ga_engine_stats{};
+}
+
+int ga_solve(ga_engine* e, const ga_request* req, ga_result* out) {
+    int rc = run_checked(e, req, nullptr, 1, out, nullptr);
+    return rc == GA_OK ? out->status : rc;
+}
+
+int ga_solve_puzzle(ga_engine* e, uint32_t side, uint64_t start, uint64_t goal, uint32_t num_queues,
+                    ga_result* out) {
+    ga_request r{};
+    r.kind = GA_PUZZLE;
+    r.side = side;
+    r.start = start;
+    r.goal = goal;
+    r.num_queues = num_queues;
+    return ga_solve(e, &r, out);
+}
+
+int ga_solve_grid(ga_engine* e, const uint8_t* cells, uint32_t width, uint32_t height, uint64_t start,
+                  uint64_t goal, uint32_t connectivity, uint32_t num_queues, ga_result* out) {
+    ga_request r{};
+    r.kind = GA_GRID;
+    r.cells = cells;
+    r.width = width;
+    r.height = height;
+    r.start = start;
+    r.goal = goal;
+    r.connectivity = connectivity;
+    r.num_queues = num_queues;
+    return ga_solve(e, &r, out);
+}
+
+int ga_solve_batch(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* outs) {
+    return run_checked(e, reqs, nullptr, n, outs, nullptr);
+}
+
+int ga_serve_run(ga_engine* e, const ga_request* reqs, const uint64_t* arrival_ns, uint32_t n, ga_result* outs,
+                 ga_serve_stats* stats) {
+    if (!arrival_ns) return GA_E_INVALID;
+    ga_serve_stats local{};
+    int rc = run_checked(e, reqs, arrival_ns, n, outs, stats ? stats : &local);
+    return rc;
+}
+
+void ga_result_free(ga_result* r) {
+    if (!r) return;
+    free(r->path);
+    r->path = nullptr;
+    r->path_len = 0;
+}
+
+const char* ga_status_str(int status) {
+    switch (status) {
+        case GA_OK: return "GA_OK";
+        case GA_NO_PATH: return "GA_NO_PATH";
+        case GA_E_INVALID: return "GA_E_INVALID";
+        case GA_E_NOMEM: return "GA_E_NOMEM";
+        case GA_E_BUDGET: return "GA_E_BUDGET";
+        case GA_E_HIP: return "GA_E_HIP";
+        case GA_E_TIMEOUT: return "GA_E_TIMEOUT";
+        case GA_E_OVERFLOW: return "GA_E_OVERFLOW";
+        default: return "GA_E_UNKNOWN";
+    }
+}
+
+int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device_out) {
+    if (!e || !host || !device_out) return GA_E_INVALID;
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) return GA_E_NOMEM;
+    if (hipMemcpy(p, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(p);
+        return GA_E_HIP;
+    }
+    *device_out = p;
+    return GA_OK;
+}
+
+int ga_device_free(ga_engine* e, void* device) {
+    if (!e) return GA_E_INVALID;
+    return hipFree(device) == hipSuccess ? GA_OK : GA_E_HIP;
+}
+
+}  // extern "C"

@@ -1,0 +1,186 @@
+/*
+ * gastar.h -- C ABI of the MI355X GA* engine (libgastar.so).
+ *
+ * The reference (michalborkowski96/cuda_a_star) exposes its search as a C++
+ * template, a_star<T, max_neighbours, gen, h, bucket_depth>(start, target,
+ * num_queues) -> {path, elapsed} (a_star.h:223-224), called by
+ * SlidingPuzzle::solve (a_star_sliding_puzzle.h:165-171) and
+ * Pathfinding::solve (a_star_pathfinding.h:223-238).  That template is kept
+ * as a header drop-in in gastar/a_star.h.  This header is the thin C ABI over
+ * the built-in problem instantiations: plain C types, no exceptions, integer
+ * status codes.  Every entry point below names the reference interface it
+ * replaces.
+ *
+ * States:
+ *   sliding puzzle: uint64, nibble i (bits 4i..4i+3) = tile on cell i in
+ *                   row-major order, 0 = blank; side 3 or 4.
+ *   grid:           cell index y*width + x into a row-major uint8 grid,
+ *                   0 = wall, 1..255 = cost of entering the cell.
+ * Paths are returned start..goal as states in these encodings.
+ */
+#ifndef GASTAR_H
+#define GASTAR_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (the reference throws std::runtime_error instead,
+ *      helpers.h:46-50; "no path" is its empty vector, a_star.h:268-274) ---- */
+#define GA_OK 0
+#define GA_NO_PATH 1
+#define GA_E_INVALID (-1)  /* malformed request */
+#define GA_E_NOMEM (-2)    /* device memory exhausted */
+#define GA_E_BUDGET (-3)   /* request can never fit the admission budget */
+#define GA_E_HIP (-4)      /* HIP runtime error */
+#define GA_E_TIMEOUT (-5)  /* iteration limit reached */
+#define GA_E_OVERFLOW (-6) /* capacity exceeded after all retries */
+
+/* ---- problem kinds ---- */
+#define GA_PUZZLE 1 /* sliding puzzle: a_star_sliding_puzzle.h */
+#define GA_GRID 2   /* weighted grid:  a_star_pathfinding.h    */
+
+typedef struct ga_engine ga_engine;
+
+/* One search request (the arguments of a_star<>() plus the problem data that
+ * the reference keeps in __constant__ globals, a_star_sliding_puzzle.h:125-142
+ * and a_star_pathfinding.h:72-81). */
+typedef struct ga_request {
+    uint32_t kind;          /* GA_PUZZLE or GA_GRID */
+    uint32_t side;          /* puzzle side, 3 or 4 */
+    uint64_t start;         /* start state */
+    uint64_t goal;          /* target state */
+    uint32_t width;         /* grid width  (x extent) */
+    uint32_t height;        /* grid height (y extent) */
+    uint32_t connectivity;  /* grid: 4 or 8 */
+    uint32_t w_min;         /* grid heuristic scale; 0 = minimum open-cell weight */
+    const uint8_t* cells;   /* grid cells in host memory, or NULL */
+    const uint8_t* cells_device; /* grid cells already resident on the device, or NULL */
+    uint32_t num_queues;    /* number of parallel heaps (a_star.h:224); 0 = planner's choice */
+    uint32_t heap_capacity; /* entries per heap; 0 = planner's choice */
+    uint64_t table_slots;   /* closed-set slots (puzzles); 0 = planner's choice */
+    uint64_t user_tag;      /* copied into the result */
+} ga_request;
+
+/* Result of one request (the reference returns {path, elapsed ns},
+ * a_star.h:224,281; the cost is not returned there and is added here). */
+typedef struct ga_result {
+    int32_t status;        /* GA_OK, GA_NO_PATH or an error */
+    uint32_t cost;         /* optimal path cost */
+    uint64_t expanded;     /* nodes expanded on the device */
+    uint64_t stored;       /* distinct states inserted in the closed set */
+    uint32_t iterations;   /* GA* iterations */
+    uint32_t retries;      /* restarts with larger capacities */
+    uint32_t path_len;     /* states in path */
+    uint32_t reserved;
+    uint64_t* path;        /* start..goal; owned by the library, see ga_result_free */
+    uint64_t search_ns;    /* admission to completion */
+    uint64_t latency_ns;   /* arrival to result on the host (serving) */
+    uint64_t user_tag;
+} ga_result;
+
+typedef struct ga_config {
+    uint64_t budget_bytes;   /* admission budget for heap + closed-set bytes */
+    uint64_t chunk_bytes;    /* device arena grows in chunks of this size */
+    uint32_t max_instances;  /* instances resident at once */
+    uint32_t chunk_iters;    /* GA* iterations launched between host checks */
+    uint32_t use_graph;      /* replay each chunk from a captured hipGraph */
+    uint32_t max_retries;    /* capacity-overflow restarts per request */
+    uint32_t max_iterations; /* per request; 0 = unlimited */
+    uint32_t heap_arity;     /* lanes per heap = heap arity: 4, 8 or 16 */
+    uint32_t max_load_pct;   /* closed-set load factor limit, percent */
+    uint32_t pops_per_iter;  /* heap pops tried per heap per iteration (stale skip) */
+    uint32_t profile;        /* time every step with HIP events */
+    uint32_t reserved[3];
+} ga_config;
+
+/* Engine statistics (kernel timing is collected when cfg.profile != 0). */
+typedef struct ga_engine_stats {
+    uint64_t step_launches;  /* step kernels launched */
+    double step_ms_total;    /* summed step-kernel time from HIP events */
+    uint64_t expanded;       /* node expansions by all requests */
+    uint64_t generated;      /* successors generated */
+    uint64_t pushed;         /* heap pushes */
+    uint64_t probes;         /* closed-set probes */
+    uint64_t sift_down;      /* heap levels visited by pops */
+    uint64_t sift_up;        /* heap levels visited by pushes */
+    uint64_t pops;           /* heap pops (including stale ones) */
+    uint64_t completed;      /* requests completed */
+    uint64_t peak_bytes;     /* peak admitted heap + closed-set bytes */
+} ga_engine_stats;
+
+typedef struct ga_serve_stats {
+    uint32_t completed;
+    uint32_t failed;
+    uint64_t wall_ns;        /* first arrival to last completion */
+    double solves_per_sec;
+    double p50_ms;
+    double p99_ms;
+    uint32_t max_inflight;   /* largest number of requests resident at once */
+    uint32_t reserved;
+} ga_serve_stats;
+
+void ga_config_default(ga_config* cfg);
+
+/* Engine lifetime.  An engine owns one HIP stream on `device`; it is not
+ * thread-safe (the reference is not re-entrant either: a_star.h:205-212,
+ * reduction.h:46-57). */
+int ga_engine_create(int device, const ga_config* cfg, ga_engine** out);
+void ga_engine_destroy(ga_engine* e);
+int ga_engine_get_stats(ga_engine* e, ga_engine_stats* out);
+void ga_engine_reset_stats(ga_engine* e);
+
+/* Plain GA*, one request at a time.  Replaces a_star<>() (a_star.h:223-282)
+ * as called by SlidingPuzzle::solve (a_star_sliding_puzzle.h:165-171) and
+ * Pathfinding::solve (a_star_pathfinding.h:223-238). */
+int ga_solve(ga_engine* e, const ga_request* req, ga_result* out);
+int ga_solve_puzzle(ga_engine* e, uint32_t side, uint64_t start, uint64_t goal, uint32_t num_queues,
+                    ga_result* out);
+int ga_solve_grid(ga_engine* e, const uint8_t* cells, uint32_t width, uint32_t height, uint64_t start,
+                  uint64_t goal, uint32_t connectivity, uint32_t num_queues, ga_result* out);
+
+/* Batched GA*: all requests share each iteration's launches, with heaps and
+ * closed sets partitioned per request.  No reference counterpart (the
+ * reference solves one instance per a_star<>() call). */
+int ga_solve_batch(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* outs);
+
+/* Serving: replay an arrival stream (arrival_ns[i] relative to the call)
+ * with continuous batching under cfg.budget_bytes.  No reference
+ * counterpart. */
+int ga_serve_run(ga_engine* e, const ga_request* reqs, const uint64_t* arrival_ns, uint32_t n,
+                 ga_result* outs, ga_serve_stats* stats);
+
+void ga_result_free(ga_result* r);
+const char* ga_status_str(int status);
+
+/* Device memory helpers for callers that keep grids resident in HBM. */
+int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device_out);
+int ga_device_free(ga_engine* e, void* device);
+
+/* ---- seeded instance generators (host only; splitmix64 streams) ---- */
+uint64_t ga_splitmix64(uint64_t* state);
+/* Goal = tiles 1..n-1 in row-major order, blank last. */
+uint64_t ga_puzzle_goal(uint32_t side);
+/* Random walk of `depth` legal blank moves from `goal`, moves drawn uniformly
+ * from the legal ones in the reference order (a_star_sliding_puzzle.h:88-105);
+ * no_backtrack excludes the move that undoes the previous one. */
+uint64_t ga_gen_puzzle(uint32_t side, uint64_t goal, uint32_t depth, int no_backtrack, uint64_t seed);
+/* width*height cells: wall with probability wall_ppm/1e6, else weight uniform
+ * in w_lo..w_hi; start and goal forced open; redrawn from the continuing
+ * stream until goal is reachable from start (4- or 8-connected).  Returns the
+ * number of draws (>= 1), or 0 on invalid arguments / no reachable draw in
+ * max_attempts. */
+uint32_t ga_gen_grid(uint32_t width, uint32_t height, uint32_t wall_ppm, uint32_t w_lo, uint32_t w_hi,
+                     uint64_t start, uint64_t goal, uint32_t connectivity, uint64_t seed,
+                     uint32_t max_attempts, uint8_t* cells_out);
+/* 1 if goal is reachable from start by sliding moves (parity test). */
+int ga_puzzle_solvable(uint32_t side, uint64_t start, uint64_t goal);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

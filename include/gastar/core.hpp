@@ -1,0 +1,700 @@
+// core.hpp -- device side of the GA* engine for gfx950 (CDNA4).
+//
+// One GA* iteration of every resident instance is ONE launch of step_kernel.
+// A workgroup serves heaps of one instance; W consecutive lanes ("a heap
+// group") own one W-ary heap.  Per iteration a heap group
+//   1. pushes the successor records addressed to it by the previous
+//      iteration (round-robin, as insert_nodes_into_queues, a_star.h:101-115),
+//   2. contributes its top f to the instance's min-f reduction
+//      (the termination test of a_star.h:260-262, evaluated one launch later),
+//   3. pops its minimum, skipping stale entries (lazy deletion replaces the
+//      eager erase of remove_updated_elements, a_star.h:192-203),
+//   4. tests the goal at pop (a_star.h:80-83) and expands the node through the
+//      problem's successor function, one successor per lane (a_star.h:84-97),
+//   5. inserts each successor into the closed set with a 64-bit CAS on the key
+//      and an atomicMin on (g << 32 | parent slot): a successor survives only if
+//      it lowered g (the INSERTED/UPDATED/REJECTED contract of
+//      fancy_hashmap.h:32-81,155-167), so duplicates inside one iteration
+//      resolve in the table and no sort is needed (bitonic_sort.h and
+//      mark_duplicates, a_star.h:137-146, have no counterpart here),
+//   6. writes K successor records to its slice of the candidate buffer.
+// Candidate buffers are double-buffered by iteration parity, min-f
+// reductions are triple-buffered, so one launch per iteration needs no grid
+// barrier and the host never reads back inside the search loop (the reference
+// blocks six times per iteration, a_star.h:247-264).
+//
+// This header is shared by the C ABI library (fixed problem instantiations)
+// and by the header drop-in gastar/a_star.h (user problem functors).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gastar {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;   // no parent / no instance
+constexpr uint32_t kInf = 0xFFFFFFFFu;    // infinite cost
+constexpr uint64_t kEmptyKey = ~0ull;     // free closed-set slot
+constexpr uint64_t kInfVal = ~0ull;       // g = inf, parent = none
+constexpr uint64_t kNoRecord = ~0ull;     // empty candidate record
+
+// Instance status.
+enum : uint32_t {
+    ST_FREE = 0,
+    ST_RUNNING = 1,
+    ST_DONE = 2,
+    ST_OVERFLOW_TABLE = 3,
+    ST_OVERFLOW_HEAP = 4,
+    ST_ABORTED = 5,
+};
+
+// Problem kinds compiled into the C ABI library.
+enum : uint32_t {
+    KIND_PUZZLE3 = 1,
+    KIND_PUZZLE4 = 2,
+    KIND_GRID4 = 3,
+    KIND_GRID8 = 4,
+    KIND_USER = 16,  // header drop-in: one user problem per kernel instantiation
+};
+
+// Heap entry and candidate record, 16 B (one dwordx4).
+// prio = f << 32 | ~g: one unsigned compare orders by f ascending and, on equal
+// f, by g descending -- QueueValue::operator< of a_star.h:58-63.
+struct alignas(16) HeapEntry {
+    uint64_t prio;
+    uint32_t slot;  // closed-set slot of the state
+    uint32_t aux;   // problem scratch (grid: y << 16 | x)
+};
+
+// Closed-set slot, 16 B: the key and its value in one line.
+// val = g << 32 | parent slot; atomicMin keeps the smallest g (and, for equal
+// g, the smallest parent slot, so the stored parent is deterministic per g).
+struct alignas(16) Slot {
+    uint64_t key;
+    uint64_t val;
+};
+
+__host__ __device__ inline uint64_t make_prio(uint32_t f, uint32_t g) {
+    return ((uint64_t)f << 32) | (uint64_t)(uint32_t)~g;
+}
+__host__ __device__ inline uint32_t prio_f(uint64_t p) { return (uint32_t)(p >> 32); }
+__host__ __device__ inline uint32_t prio_g(uint64_t p) { return ~(uint32_t)p; }
+__host__ __device__ inline uint32_t val_g(uint64_t v) { return (uint32_t)(v >> 32); }
+__host__ __device__ inline uint32_t val_parent(uint64_t v) { return (uint32_t)v; }
+
+// 64-bit finaliser of MurmurHash3 (public domain, Austin Appleby).  The
+// reference hashes with MurmurHash3_x86_32 into (h*B)>>32 buckets
+// (helpers.h:10-35, fancy_hashmap.h:151-153); any well-mixed hash gives the
+// same search result, only the slot placement differs.
+__host__ __device__ inline uint64_t mix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+// Read-only description of one resident instance (written once at admission).
+struct InstDesc {
+    uint32_t kind;
+    uint32_t width;    // puzzle side, or grid width
+    uint32_t height;   // grid height
+    uint32_t w_min;    // grid heuristic scale
+    uint64_t start;
+    uint64_t goal;
+    const uint8_t* cells;  // grid cells
+    HeapEntry* heaps;      // n_heaps * heap_stride entries
+    uint32_t* heap_size;   // n_heaps
+    HeapEntry* cand;       // 2 * n_heaps * K records
+    Slot* table;           // hashed closed set (puzzles, user problems)
+    uint64_t* dense;       // direct-indexed closed set (grids)
+    uint64_t table_mask;   // hashed: slots - 1
+    uint64_t load_limit;   // hashed: max occupied slots before overflow
+    uint32_t n_heaps;      // power of two
+    uint32_t heap_cap;     // max entries per heap
+    uint32_t heap_stride;  // physical entries per heap
+    uint32_t start_iter;   // value of the global iteration counter at the first step
+    uint8_t goal_rc[16];   // puzzle: row << 4 | col of each tile in the goal
+};
+
+// Mutable per-instance state.
+struct InstState {
+    uint32_t status;
+    uint32_t best;        // smallest g of a popped goal
+    uint32_t min_top[3];  // min f over heap tops, by iteration mod 3
+    uint32_t iterations;  // iterations started
+    uint32_t done_iter;
+    uint32_t pad;
+    unsigned long long occupied;  // closed-set slots in use
+    unsigned long long expanded;
+    unsigned long long generated;
+    unsigned long long pushed;
+    unsigned long long pops;
+    unsigned long long probes;
+    unsigned long long sift_down;
+    unsigned long long sift_up;
+};
+
+// Workgroup -> (instance, first heap) map of one launch.
+struct WgEntry {
+    uint32_t inst;
+    uint32_t first_heap;
+};
+
+struct StepParams {
+    uint32_t max_pops;    // pops tried per heap per iteration while entries are stale
+    uint32_t max_probes;  // closed-set probe limit (overflow guard)
+    uint32_t profile;     // count probes and sift levels
+    uint32_t pad;
+};
+
+// ---------------------------------------------------------------------------
+// wave helpers (wave64)
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        uint32_t w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ inline uint32_t load_relaxed(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline unsigned long long load_relaxed(const unsigned long long* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ inline void store_relaxed(uint32_t* p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Orders lane 0's heap writes before the other lanes of its group read them
+// (all lanes of a group are in one wave on one CU).
+__device__ inline void group_fence() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+__device__ inline void set_status(InstState* st, uint32_t s) {
+    atomicCAS(&st->status, (uint32_t)ST_RUNNING, s);
+}
+
+// ---------------------------------------------------------------------------
+// W-ary heap, logical index j at physical j + W - 1 so that the W children
+// W*j+1 .. W*j+W of node j start on a W-entry boundary (W*16 B aligned).
+// ---------------------------------------------------------------------------
+template <int W>
+__device__ inline HeapEntry heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& levels) {
+    HeapEntry top = H[0];
+    const uint32_t n = S - 1;
+    S = n;
+    if (n == 0) return top;
+    const HeapEntry last = H[n];
+    uint32_t j = 0;
+    for (;;) {
+        const uint32_t c = W * j + 1;
+        if (c >= n) break;
+        const uint32_t idx = c + (uint32_t)lane;
+        HeapEntry e;
+        e.prio = ~0ull;
+        e.slot = 0;
+        e.aux = 0;
+        if (idx < n) e = H[idx];
+        uint64_t mp = e.prio;
+        uint32_t mi = idx;
+#pragma unroll
+        for (int o = W / 2; o > 0; o >>= 1) {
+            const uint64_t op = __shfl_xor(mp, o, W);
+            const uint32_t oi = __shfl_xor(mi, o, W);
+            if (op < mp || (op == mp && oi < mi)) {
+                mp = op;
+                mi = oi;
+            }
+        }
+        if (mp >= last.prio) break;
+        if (idx == mi) H[j] = e;
+        j = mi;
+        ++levels;
+    }
+    if (lane == 0) H[j] = last;
+    return top;
+}
+
+template <int W>
+__device__ inline void heap_push(HeapEntry* H, uint32_t S, const HeapEntry& x, uint32_t& levels) {
+    uint32_t j = S;
+    while (j > 0) {
+        const uint32_t p = (j - 1) / W;
+        const HeapEntry pe = H[p];
+        if (x.prio < pe.prio) {
+            H[j] = pe;
+            j = p;
+            ++levels;
+        } else {
+            break;
+        }
+    }
+    H[j] = x;
+}
+
+// ---------------------------------------------------------------------------
+// closed sets
+// ---------------------------------------------------------------------------
+// Open addressing with linear probing.  A key is claimed with a 64-bit CAS;
+// keys are never removed, so a non-empty key read with a plain load is exact
+// and a stale "empty" read is corrected by the CAS.
+__device__ inline bool hash_find_or_insert(Slot* table, uint64_t mask, uint64_t key, uint32_t max_probes,
+                                           uint32_t& slot, bool& isnew, uint32_t& probes) {
+    uint64_t i = mix64(key) & mask;
+    for (uint32_t p = 0; p < max_probes; ++p) {
+        ++probes;
+        const uint64_t k = table[i].key;
+        if (k == key) {
+            slot = (uint32_t)i;
+            isnew = false;
+            return true;
+        }
+        if (k == kEmptyKey) {
+            const uint64_t prev = atomicCAS((unsigned long long*)&table[i].key, (unsigned long long)kEmptyKey,
+                                            (unsigned long long)key);
+            if (prev == kEmptyKey || prev == key) {
+                slot = (uint32_t)i;
+                isnew = prev == kEmptyKey;
+                return true;
+            }
+        }
+        i = (i + 1) & mask;
+    }
+    return false;
+}
+
+__device__ inline bool hash_find(const Slot* table, uint64_t mask, uint64_t key, uint64_t max_probes,
+                                 uint32_t& slot) {
+    uint64_t i = mix64(key) & mask;
+    for (uint64_t p = 0; p < max_probes; ++p) {
+        const uint64_t k = table[i].key;
+        if (k == key) {
+            slot = (uint32_t)i;
+            return true;
+        }
+        if (k == kEmptyKey) return false;
+        i = (i + 1) & mask;
+    }
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// built-in problems
+// ---------------------------------------------------------------------------
+// Sliding puzzle of side S, state = 16 nibbles (tile of cell i in nibble i).
+// Moves in the reference order (a_star_sliding_puzzle.h:88-105); unit cost
+// (:84); h = Manhattan distance of the tiles, blank excluded (the reference's
+// manhattan_heuristic, :152-163, counts the blank and is not admissible).
+template <int S>
+struct PuzzleOps {
+    static constexpr int K = 4;
+    static constexpr bool kDense = false;
+    static constexpr int kCells = S * S;
+    static constexpr uint64_t kLowBits = (kCells == 16) ? 0x1111111111111111ull
+                                                        : ((1ull << (4 * kCells)) - 1) & 0x1111111111111111ull;
+
+    __device__ static inline int blank_of(uint64_t s) {
+        uint64_t y = s | (s >> 1);
+        y |= y >> 2;  // bit 4c = OR of the bits of nibble c
+        const uint64_t z = ~y & kLowBits;
+        return (int)(__builtin_ctzll(z) >> 2);
+    }
+
+    __device__ static inline uint32_t md(const InstDesc& d, uint32_t tile, int cell) {
+        const int gr = d.goal_rc[tile] >> 4, gc = d.goal_rc[tile] & 15;
+        const int r = cell / S, c = cell % S;
+        return (uint32_t)(abs(r - gr) + abs(c - gc));
+    }
+
+    __device__ static inline uint32_t heuristic(const InstDesc& d, uint64_t s, uint32_t) {
+        uint32_t h = 0;
+#pragma unroll
+        for (int c = 0; c < kCells; ++c) {
+            const uint32_t t = (uint32_t)(s >> (4 * c)) & 15u;
+            if (t) h += md(d, t, c);
+        }
+        return h;
+    }
+
+    __device__ static inline uint32_t aux_of(const InstDesc&, uint64_t) { return 0; }
+
+    // h of the successor from h of the parent: only the moved tile changes.
+    __device__ static inline bool succ(const InstDesc& d, uint64_t s, uint32_t, uint32_t h, int i, uint64_t& t,
+                                       uint32_t& taux, uint32_t& cost, uint32_t& th) {
+        const int b = blank_of(s);
+        int r = b / S, c = b % S;
+        if (i == 0) ++r;
+        else if (i == 1) --r;
+        else if (i == 2) ++c;
+        else --c;
+        if (r < 0 || r >= S || c < 0 || c >= S) return false;
+        const int nb = r * S + c;
+        const uint32_t tile = (uint32_t)(s >> (4 * nb)) & 15u;
+        t = (s & ~(0xFull << (4 * nb))) | ((uint64_t)tile << (4 * b));
+        taux = 0;
+        cost = 1;
+        th = h - md(d, tile, nb) + md(d, tile, b);
+        return true;
+    }
+};
+
+// 4- or 8-connected grid, state = cell index y*width + x, aux = y << 16 | x.
+// Moves in the reference order (a_star_pathfinding.h:150-182); cost = weight
+// of the entered cell (:203-208), 0 = wall; h = Manhattan (4) or Chebyshev (8)
+// distance times the minimum cell weight, both admissible (the reference's
+// plain Manhattan with unit diagonals, :219-221, is not).
+template <int CONN>
+struct GridOps {
+    static constexpr int K = CONN;
+    static constexpr bool kDense = true;
+
+    __device__ static inline uint32_t hxy(const InstDesc& d, uint32_t x, uint32_t y) {
+        const uint32_t gx = (uint32_t)(d.goal % d.width), gy = (uint32_t)(d.goal / d.width);
+        const uint32_t dx = x > gx ? x - gx : gx - x;
+        const uint32_t dy = y > gy ? y - gy : gy - y;
+        if (CONN == 8) return (dx > dy ? dx : dy) * d.w_min;
+        return (dx + dy) * d.w_min;
+    }
+
+    __device__ static inline uint32_t aux_of(const InstDesc& d, uint64_t s) {
+        return ((uint32_t)(s / d.width) << 16) | (uint32_t)(s % d.width);
+    }
+
+    __device__ static inline uint32_t heuristic(const InstDesc& d, uint64_t s, uint32_t aux) {
+        return hxy(d, aux & 0xFFFFu, aux >> 16);
+    }
+
+    __device__ static inline bool succ(const InstDesc& d, uint64_t, uint32_t aux, uint32_t, int i, uint64_t& t,
+                                       uint32_t& taux, uint32_t& cost, uint32_t& th) {
+        const int DX[8] = {1, -1, 0, 0, 1, 1, -1, -1};
+        const int DY[8] = {0, 0, 1, -1, 1, -1, 1, -1};
+        const uint32_t nx = (aux & 0xFFFFu) + (uint32_t)DX[i];
+        const uint32_t ny = (aux >> 16) + (uint32_t)DY[i];
+        if (nx >= d.width || ny >= d.height) return false;  // also catches -1 wrap-around
+        const uint64_t c = (uint64_t)ny * d.width + nx;
+        const uint32_t w = d.cells[c];
+        if (w == 0) return false;
+        t = c;
+        taux = (ny << 16) | nx;
+        cost = w;
+        th = hxy(d, nx, ny);
+        return true;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// closed-set access by problem
+// ---------------------------------------------------------------------------
+template <class Ops>
+__device__ inline void load_slot(const InstDesc& d, uint32_t slot, uint64_t& key, uint64_t& val) {
+    if constexpr (Ops::kDense) {
+        key = slot;
+        val = d.dense[slot];
+    } else {
+        const Slot s = d.table[slot];
+        key = s.key;
+        val = s.val;
+    }
+}
+
+template <class Ops>
+__device__ inline uint64_t* val_ptr(const InstDesc& d, uint32_t slot) {
+    if constexpr (Ops::kDense) return d.dense + slot;
+    else return &d.table[slot].val;
+}
+
+template <class Ops>
+__device__ inline bool find_or_insert(const InstDesc& d, uint64_t key, uint32_t max_probes, uint32_t& slot,
+                                      bool& isnew, uint32_t& probes) {
+    if constexpr (Ops::kDense) {
+        slot = (uint32_t)key;
+        isnew = false;  // dense sets do not count occupancy
+        ++probes;
+        return true;
+    } else {
+        return hash_find_or_insert(d.table, d.table_mask, key, max_probes, slot, isnew, probes);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// one GA* iteration of one heap group
+// ---------------------------------------------------------------------------
+template <int W, class Ops>
+__device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t it, uint32_t q, bool active,
+                                          const StepParams& prm) {
+    constexpr int K = Ops::K;
+    const int lane = (int)(threadIdx.x & (W - 1));
+    const uint32_t N = d.n_heaps;
+
+    // -- termination test on the previous iteration's reduction (a_star.h:260)
+    const uint32_t best = load_relaxed(&st->best);
+    bool live = active;
+    if (it > 0) {
+        const uint32_t mt = load_relaxed(&st->min_top[(it - 1) % 3]);
+        if (mt == kInf || (best != kInf && best <= mt)) {
+            if (live && lane == 0) {
+                set_status(st, ST_DONE);
+                store_relaxed(&st->done_iter, it);
+            }
+            live = false;
+        }
+    }
+    if (live && q == 0 && lane == 0) {
+        store_relaxed(&st->min_top[(it + 1) % 3], kInf);
+        store_relaxed(&st->iterations, it + 1);
+    }
+    if constexpr (!Ops::kDense) {
+        if (live && load_relaxed(&st->occupied) > d.load_limit) {
+            if (lane == 0) set_status(st, ST_OVERFLOW_TABLE);
+            live = false;
+        }
+    }
+
+    HeapEntry* H = d.heaps + (size_t)q * d.heap_stride + (W - 1);
+    uint32_t S = 0;
+    uint32_t n_push = 0, n_up = 0, n_down = 0, n_pops = 0, n_exp = 0, n_gen = 0, n_new = 0, n_probe = 0;
+    bool overflow_heap = false, overflow_table = false;
+
+    // -- 1. push the records addressed to this heap (insert_nodes_into_queues,
+    //       a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N)
+    if (live) {
+        S = d.heap_size[q];
+        if (it > 0) {
+            const HeapEntry* cin = d.cand + (size_t)((it - 1) & 1) * N * K;
+            const uint32_t base = (q + it) & (N - 1);
+#pragma unroll
+            for (int j0 = 0; j0 < K; j0 += W) {
+                HeapEntry rec;
+                rec.prio = kNoRecord;
+                rec.slot = 0;
+                rec.aux = 0;
+                const int j = j0 + lane;
+                if (j < K) rec = cin[base + (size_t)j * N];
+#pragma unroll
+                for (int l = 0; l < W && j0 + l < K; ++l) {
+                    HeapEntry x;
+                    x.prio = __shfl(rec.prio, l, W);
+                    x.slot = __shfl(rec.slot, l, W);
+                    x.aux = __shfl(rec.aux, l, W);
+                    if (lane == 0 && x.prio != kNoRecord && prio_f(x.prio) < best) {
+                        if (S >= d.heap_cap) {
+                            overflow_heap = true;
+                        } else {
+                            heap_push<W>(H, S, x, n_up);
+                            ++S;
+                            ++n_push;
+                        }
+                    }
+                }
+            }
+            S = __shfl(S, 0, W);
+        }
+    }
+    group_fence();
+
+    // -- 2. min f over heap tops after the pushes (HeapsHost::minimum, heaps.h:291-294)
+    uint32_t top_f = kInf;
+    if (live && lane == 0 && S > 0) top_f = prio_f(H[0].prio);
+    top_f = wave_min_u32(top_f);
+    if ((threadIdx.x & 63) == 0 && top_f != kInf) atomicMin(&st->min_top[it % 3], top_f);
+
+    // -- 3. pop, skipping entries whose g is no longer the stored one
+    bool have = false;
+    HeapEntry top;
+    top.prio = 0;
+    top.slot = 0;
+    top.aux = 0;
+    uint64_t top_key = 0;
+    if (live) {
+        for (uint32_t t = 0; t < prm.max_pops && S > 0; ++t) {
+            top = heap_pop<W>(H, S, lane, n_down);
+            group_fence();
+            ++n_pops;
+            uint64_t key, val;
+            load_slot<Ops>(d, top.slot, key, val);
+            if (prio_g(top.prio) > val_g(val)) continue;  // stale
+            have = true;
+            top_key = key;
+            break;
+        }
+        if (lane == 0) d.heap_size[q] = S;
+    }
+
+    // -- 4. goal test at pop (a_star.h:80-83); a node with f >= best cannot improve it
+    const uint32_t g = prio_g(top.prio), f = prio_f(top.prio);
+    bool expand = have;
+    if (have && top_key == d.goal) {
+        if (lane == 0) atomicMin(&st->best, g);
+        expand = false;
+    }
+    if (expand && best != kInf && f >= best) expand = false;
+    if (expand && lane == 0) ++n_exp;
+
+    // -- 5/6. expand, insert into the closed set, write candidate records
+    if (live) {
+        HeapEntry* cout = d.cand + (size_t)(it & 1) * N * K + (size_t)q * K;
+        const uint32_t h = f - g;
+#pragma unroll
+        for (int i0 = 0; i0 < K; i0 += W) {
+            const int i = i0 + lane;
+            if (i < K) {
+                HeapEntry rec;
+                rec.prio = kNoRecord;
+                rec.slot = 0;
+                rec.aux = 0;
+                uint64_t tkey;
+                uint32_t taux, cost, th;
+                if (expand && Ops::succ(d, top_key, top.aux, h, i, tkey, taux, cost, th)) {
+                    ++n_gen;
+                    const uint32_t g2 = g + cost, f2 = g2 + th;
+                    if (f2 < best) {
+                        uint32_t slot;
+                        bool isnew;
+                        if (find_or_insert<Ops>(d, tkey, prm.max_probes, slot, isnew, n_probe)) {
+                            n_new += isnew ? 1u : 0u;
+                            const uint64_t mine = ((uint64_t)g2 << 32) | top.slot;
+                            const uint64_t old = atomicMin((unsigned long long*)val_ptr<Ops>(d, slot),
+                                                           (unsigned long long)mine);
+                            if (g2 < val_g(old)) {
+                                rec.prio = make_prio(f2, g2);
+                                rec.slot = slot;
+                                rec.aux = taux;
+                            }
+                        } else {
+                            overflow_table = true;
+                        }
+                    }
+                }
+                cout[i] = rec;
+            }
+        }
+    }
+    if (overflow_heap) set_status(st, ST_OVERFLOW_HEAP);
+    if (overflow_table) set_status(st, ST_OVERFLOW_TABLE);
+
+    // -- counters, one atomic per wave
+    const uint32_t w_new = wave_sum_u32(n_new);
+    const uint32_t w_exp = wave_sum_u32(n_exp);
+    const uint32_t w_gen = wave_sum_u32(n_gen);
+    const bool wave_lead = (threadIdx.x & 63) == 0;
+    if (wave_lead) {
+        if (w_new) atomicAdd(&st->occupied, (unsigned long long)w_new);
+        if (w_exp) atomicAdd(&st->expanded, (unsigned long long)w_exp);
+        if (w_gen) atomicAdd(&st->generated, (unsigned long long)w_gen);
+    }
+    if (prm.profile) {
+        const uint32_t w_push = wave_sum_u32(n_push);
+        const uint32_t w_pops = wave_sum_u32(lane == 0 ? n_pops : 0u);
+        const uint32_t w_probe = wave_sum_u32(n_probe);
+        const uint32_t w_down = wave_sum_u32(lane == 0 ? n_down : 0u);
+        const uint32_t w_up = wave_sum_u32(n_up);
+        if (wave_lead) {
+            if (w_push) atomicAdd(&st->pushed, (unsigned long long)w_push);
+            if (w_pops) atomicAdd(&st->pops, (unsigned long long)w_pops);
+            if (w_probe) atomicAdd(&st->probes, (unsigned long long)w_probe);
+            if (w_down) atomicAdd(&st->sift_down, (unsigned long long)w_down);
+            if (w_up) atomicAdd(&st->sift_up, (unsigned long long)w_up);
+        }
+    }
+}
+
+// Per-workgroup prologue shared by every step kernel: returns false when the
+// workgroup has no running instance this iteration.
+__device__ inline bool step_prologue(const WgEntry* wgmap, const InstDesc* descs, InstState* states,
+                                     const uint32_t* iter_base, uint32_t r, uint32_t& inst, uint32_t& it,
+                                     uint32_t& first_heap) {
+    const WgEntry we = wgmap[blockIdx.x];
+    if (we.inst == kNone) return false;
+    if (load_relaxed(&states[we.inst].status) != ST_RUNNING) return false;
+    inst = we.inst;
+    first_heap = we.first_heap;
+    it = *iter_base + r - descs[we.inst].start_iter;
+    return true;
+}
+
+// Advances the global iteration counter after a chunk of steps (a vector
+// store; the next launch reads it after the kernel boundary).
+template <int Unused = 0>
+__global__ void advance_kernel(uint32_t* iter_base, uint32_t by) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *iter_base += by;
+}
+
+// Grid-stride fill (a kernel rather than hipMemsetAsync so that it may sit in
+// captured graphs).
+template <typename T>
+__global__ void fill_kernel(T* p, uint64_t n, T v) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Resets the instance counters and seeds the search: the start state enters
+// the closed set with g = 0 (the reference never stores it, a_star.h:226 vs
+// :253, so it can be re-expanded) and heap 0 (insert_first, a_star.h:68-71).
+// The closed set must already be cleared and every heap size zeroed.
+template <class Ops>
+__device__ inline bool seed_instance(const InstDesc& d, InstState* st, int W) {
+    InstState z;
+    z.status = ST_RUNNING;
+    z.best = kInf;
+    z.min_top[0] = z.min_top[1] = z.min_top[2] = kInf;
+    z.iterations = 0;
+    z.done_iter = 0;
+    z.pad = 0;
+    z.occupied = Ops::kDense ? 0 : 1;
+    z.expanded = z.generated = z.pushed = z.pops = z.probes = z.sift_down = z.sift_up = 0;
+    uint32_t slot, probes = 0;
+    bool isnew;
+    if (!find_or_insert<Ops>(d, d.start, 1u << 20, slot, isnew, probes)) {
+        z.status = ST_OVERFLOW_TABLE;
+        *st = z;
+        return false;
+    }
+    *val_ptr<Ops>(d, slot) = kNone;  // g = 0, no parent
+    const uint32_t aux = Ops::aux_of(d, d.start);
+    HeapEntry e;
+    e.prio = make_prio(Ops::heuristic(d, d.start, aux), 0);
+    e.slot = slot;
+    e.aux = aux;
+    d.heaps[W - 1] = e;
+    d.heap_size[0] = 1;
+    *st = z;
+    return true;
+}
+
+// Walks the parent links from the goal; writes states goal..start and returns
+// their number, or kNone if cap is exceeded / the goal is not stored.
+template <class Ops>
+__device__ inline uint32_t walk_path(const InstDesc& d, uint64_t* out, uint32_t cap) {
+    uint32_t slot;
+    if constexpr (Ops::kDense) {
+        slot = (uint32_t)d.goal;
+    } else {
+        if (!hash_find(d.table, d.table_mask, d.goal, d.table_mask + 1, slot)) return kNone;
+    }
+    uint32_t n = 0;
+    while (slot != kNone) {
+        if (n >= cap) return kNone;
+        uint64_t key, val;
+        load_slot<Ops>(d, slot, key, val);
+        out[n++] = key;
+        slot = val_parent(val);
+    }
+    return n;
+}
+
+}  // namespace gastar

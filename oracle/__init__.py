@@ -1,0 +1,135 @@
+"""CPU oracle for the GA* search path -- TEST INFRASTRUCTURE ONLY.
+
+Only the test suite, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` import this package, and only as the checker or the timed CPU
+reference.  The product (``cuda_a_star_amd``) never imports it.
+
+``liboracle.so`` is built from ``oracle.c`` by ``oracle/Makefile`` (see
+``build()``).  The algorithms and the reference lines they restate are listed in
+``oracle.h``.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+OK, NO_PATH, E_INVALID, E_LIMIT, E_NOMEM = 0, 1, -1, -2, -3
+
+
+class OracleResult(ctypes.Structure):
+    _fields_ = [
+        ("cost", ctypes.c_int64),
+        ("expanded", ctypes.c_uint64),
+        ("generated", ctypes.c_uint64),
+        ("stored", ctypes.c_uint64),
+        ("path_len", ctypes.c_uint32),
+        ("pad", ctypes.c_uint32),
+    ]
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.oracle_astar_puzzle.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64,
+                                          ctypes.POINTER(OracleResult), u64p, ctypes.c_uint32]
+        L.oracle_astar_puzzle.restype = ctypes.c_int
+        L.oracle_puzzle_manhattan.argtypes = [ctypes.c_int, ctypes.c_uint64, ctypes.c_uint64]
+        L.oracle_puzzle_manhattan.restype = ctypes.c_uint32
+        L.oracle_astar_grid.argtypes = [u8p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                        ctypes.c_int, ctypes.c_uint32, ctypes.POINTER(OracleResult), u32p,
+                                        ctypes.c_uint32]
+        L.oracle_astar_grid.restype = ctypes.c_int
+        L.oracle_dial_grid.argtypes = [u8p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                       ctypes.c_int]
+        L.oracle_dial_grid.restype = ctypes.c_int64
+        L.oracle_bfs_puzzle8.argtypes = [ctypes.c_uint64, u8p, u32p]
+        L.oracle_bfs_puzzle8.restype = ctypes.c_uint32
+        L.oracle_perm_rank9.argtypes = [ctypes.c_uint64]
+        L.oracle_perm_rank9.restype = ctypes.c_uint32
+        L.oracle_replay_puzzle.argtypes = [ctypes.c_int, u64p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]
+        L.oracle_replay_puzzle.restype = ctypes.c_int64
+        L.oracle_replay_grid.argtypes = [u8p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, u32p,
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
+        L.oracle_replay_grid.restype = ctypes.c_int64
+        _lib = L
+    return _lib
+
+
+def _u8(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def astar_puzzle(side: int, start: int, goal: int, max_states: int = 0, want_path: bool = True):
+    """Optimal cost by sequential A*.  Returns (rc, OracleResult, path list)."""
+    res = OracleResult()
+    cap = 4096 if want_path else 0
+    buf = (ctypes.c_uint64 * max(cap, 1))()
+    rc = lib().oracle_astar_puzzle(side, start, goal, max_states, ctypes.byref(res),
+                                   buf if want_path else None, cap)
+    path = [buf[i] for i in range(res.path_len)] if (want_path and rc == OK and res.path_len <= cap) else []
+    return rc, res, path
+
+
+def manhattan(side: int, state: int, goal: int) -> int:
+    return lib().oracle_puzzle_manhattan(side, state, goal)
+
+
+def astar_grid(cells: np.ndarray, w: int, h: int, start: int, goal: int, connectivity: int = 4,
+               w_min: int = 0, want_path: bool = True):
+    cells = np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1)
+    res = OracleResult()
+    cap = (w * h) if want_path else 0
+    buf = np.zeros(max(cap, 1), dtype=np.uint32)
+    rc = lib().oracle_astar_grid(_u8(cells), w, h, start, goal, connectivity, w_min, ctypes.byref(res),
+                                 buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)) if want_path else None, cap)
+    path = buf[: res.path_len].tolist() if (want_path and rc == OK) else []
+    return rc, res, path
+
+
+def dial_grid(cells: np.ndarray, w: int, h: int, start: int, goal: int, connectivity: int = 4) -> int:
+    cells = np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1)
+    return lib().oracle_dial_grid(_u8(cells), w, h, start, goal, connectivity)
+
+
+def bfs_puzzle8(goal: int):
+    depth = np.zeros(362880, dtype=np.uint8)
+    hist = np.zeros(64, dtype=np.uint32)
+    n = lib().oracle_bfs_puzzle8(goal, _u8(depth), hist.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+    return n, depth, hist
+
+
+def perm_rank9(state: int) -> int:
+    return lib().oracle_perm_rank9(state)
+
+
+def replay_puzzle(side: int, path, start: int, goal: int) -> int:
+    arr = (ctypes.c_uint64 * max(len(path), 1))(*path)
+    return lib().oracle_replay_puzzle(side, arr, len(path), start, goal)
+
+
+def replay_grid(cells: np.ndarray, w: int, h: int, connectivity: int, path, start: int, goal: int) -> int:
+    cells = np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1)
+    arr = np.ascontiguousarray(np.asarray(path, dtype=np.uint32))
+    if arr.size == 0:
+        arr = np.zeros(1, dtype=np.uint32)
+    return lib().oracle_replay_grid(_u8(cells), w, h, connectivity,
+                                    arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(path), start, goal)

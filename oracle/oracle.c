@@ -1,0 +1,514 @@
+/*
+ * oracle.c -- sequential CPU oracle (test infrastructure only; see oracle.h).
+ *
+ * Reference citations are to michalborkowski96/cuda_a_star.
+ */
+#include "oracle.h"
+
+#include <stdlib.h>
+#include <string.h>
+
+#define NONE32 0xFFFFFFFFu
+#define INF32 0xFFFFFFFFu
+
+/* ------------------------------------------------------------------ */
+/* priority queue: binary min-heap on (f asc, g desc)                  */
+/* QueueValue::operator< in a_star.h:58-63: equal f -> larger g first. */
+/* ------------------------------------------------------------------ */
+typedef struct {
+    uint64_t prio; /* (f << 32) | (0xFFFFFFFF - g) */
+    uint32_t idx;
+    uint32_t g;
+} pq_item;
+
+typedef struct {
+    pq_item* a;
+    size_t n, cap;
+} pq;
+
+static uint64_t prio_of(uint32_t f, uint32_t g) {
+    return ((uint64_t)f << 32) | (uint64_t)(0xFFFFFFFFu - g);
+}
+
+static int pq_push(pq* q, pq_item it) {
+    if (q->n == q->cap) {
+        size_t nc = q->cap ? q->cap * 2 : 1024;
+        pq_item* na = (pq_item*)realloc(q->a, nc * sizeof(pq_item));
+        if (!na) return -1;
+        q->a = na;
+        q->cap = nc;
+    }
+    size_t i = q->n++;
+    while (i > 0) {
+        size_t p = (i - 1) >> 1;
+        if (it.prio < q->a[p].prio) {
+            q->a[i] = q->a[p];
+            i = p;
+        } else {
+            break;
+        }
+    }
+    q->a[i] = it;
+    return 0;
+}
+
+static pq_item pq_pop(pq* q) {
+    pq_item top = q->a[0];
+    pq_item last = q->a[--q->n];
+    size_t n = q->n, i = 0;
+    if (n) {
+        for (;;) {
+            size_t l = 2 * i + 1, r = l + 1, m = i;
+            uint64_t mp = last.prio;
+            if (l < n && q->a[l].prio < mp) { m = l; mp = q->a[l].prio; }
+            if (r < n && q->a[r].prio < mp) { m = r; }
+            if (m == i) break;
+            q->a[i] = q->a[m];
+            i = m;
+        }
+        q->a[i] = last;
+    }
+    return top;
+}
+
+/* ------------------------------------------------------------------ */
+/* sliding puzzle                                                      */
+/* ------------------------------------------------------------------ */
+static inline uint32_t nib(uint64_t s, int c) { return (uint32_t)((s >> (4 * c)) & 0xF); }
+
+static int blank_of(int side, uint64_t s) {
+    int n = side * side;
+    for (int c = 0; c < n; ++c)
+        if (nib(s, c) == 0) return c;
+    return -1;
+}
+
+uint32_t oracle_puzzle_manhattan(int side, uint64_t state, uint64_t goal) {
+    int gr[16], gc[16];
+    int n = side * side;
+    for (int c = 0; c < n; ++c) {
+        gr[nib(goal, c)] = c / side;
+        gc[nib(goal, c)] = c % side;
+    }
+    uint32_t hsum = 0;
+    for (int c = 0; c < n; ++c) {
+        uint32_t t = nib(state, c);
+        if (t == 0) continue; /* blank excluded: admissible (unlike a_star_sliding_puzzle.h:152-163) */
+        int dr = c / side - gr[t], dc = c % side - gc[t];
+        hsum += (uint32_t)((dr < 0 ? -dr : dr) + (dc < 0 ? -dc : dc));
+    }
+    return hsum;
+}
+
+/* Successor i of s in the reference order (a_star_sliding_puzzle.h:88-105):
+ * 0: blank row+1, 1: blank row-1, 2: blank col+1, 3: blank col-1. */
+static int puzzle_succ(int side, uint64_t s, int blank, int i, uint64_t* out) {
+    int r = blank / side, c = blank % side;
+    switch (i) {
+        case 0: r += 1; break;
+        case 1: r -= 1; break;
+        case 2: c += 1; break;
+        default: c -= 1; break;
+    }
+    if (r < 0 || r >= side || c < 0 || c >= side) return 0;
+    int nb = r * side + c;
+    uint64_t tile = (s >> (4 * nb)) & 0xF;
+    uint64_t t = s & ~(0xFull << (4 * nb));
+    t |= tile << (4 * blank);
+    *out = t;
+    return 1;
+}
+
+static int valid_puzzle(int side, uint64_t s) {
+    int n = side * side;
+    uint32_t seen = 0;
+    for (int c = 0; c < n; ++c) {
+        uint32_t t = nib(s, c);
+        if (t >= (uint32_t)n || (seen >> t) & 1) return 0;
+        seen |= 1u << t;
+    }
+    if (n < 16 && (s >> (4 * n)) != 0) return 0;
+    return 1;
+}
+
+typedef struct {
+    uint64_t* key;
+    uint32_t* g;
+    uint32_t* parent;
+    size_t n, cap;
+    uint32_t* slot; /* open addressing: node index + 1, 0 = empty */
+    size_t slot_mask;
+} pstore;
+
+static uint64_t mix64(uint64_t k) {
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return k;
+}
+
+static int pstore_grow_slots(pstore* st) {
+    size_t ns = st->slot_mask ? (st->slot_mask + 1) * 2 : (1u << 16);
+    uint32_t* sl = (uint32_t*)calloc(ns, sizeof(uint32_t));
+    if (!sl) return -1;
+    for (size_t i = 0; i < st->n; ++i) {
+        size_t p = mix64(st->key[i]) & (ns - 1);
+        while (sl[p]) p = (p + 1) & (ns - 1);
+        sl[p] = (uint32_t)(i + 1);
+    }
+    free(st->slot);
+    st->slot = sl;
+    st->slot_mask = ns - 1;
+    return 0;
+}
+
+/* Returns node index for key, inserting it with g = INF if absent; -1 on OOM. */
+static int64_t pstore_get(pstore* st, uint64_t key) {
+    if (2 * (st->n + 1) > st->slot_mask + 1)
+        if (pstore_grow_slots(st)) return -1;
+    size_t p = mix64(key) & st->slot_mask;
+    while (st->slot[p]) {
+        uint32_t i = st->slot[p] - 1;
+        if (st->key[i] == key) return i;
+        p = (p + 1) & st->slot_mask;
+    }
+    if (st->n == st->cap) {
+        size_t nc = st->cap ? st->cap * 2 : 4096;
+        uint64_t* k = (uint64_t*)realloc(st->key, nc * sizeof(uint64_t));
+        if (!k) return -1;
+        st->key = k;
+        uint32_t* g = (uint32_t*)realloc(st->g, nc * sizeof(uint32_t));
+        if (!g) return -1;
+        st->g = g;
+        uint32_t* pa = (uint32_t*)realloc(st->parent, nc * sizeof(uint32_t));
+        if (!pa) return -1;
+        st->parent = pa;
+        st->cap = nc;
+    }
+    size_t i = st->n++;
+    st->key[i] = key;
+    st->g[i] = INF32;
+    st->parent[i] = NONE32;
+    st->slot[p] = (uint32_t)(i + 1);
+    return (int64_t)i;
+}
+
+static void pstore_free(pstore* st) {
+    free(st->key);
+    free(st->g);
+    free(st->parent);
+    free(st->slot);
+}
+
+int oracle_astar_puzzle(int side, uint64_t start, uint64_t goal, uint64_t max_states,
+                        oracle_result* res, uint64_t* path, uint32_t path_cap) {
+    memset(res, 0, sizeof(*res));
+    res->cost = -1;
+    if ((side != 3 && side != 4) || !valid_puzzle(side, start) || !valid_puzzle(side, goal))
+        return ORACLE_E_INVALID;
+    pstore st;
+    memset(&st, 0, sizeof(st));
+    pq q;
+    memset(&q, 0, sizeof(q));
+    int rc = ORACLE_NO_PATH;
+    int64_t s0 = pstore_get(&st, start);
+    if (s0 < 0) { rc = ORACLE_E_NOMEM; goto out; }
+    st.g[s0] = 0; /* the start is stored with g = 0 (the reference never stores it, a_star.h:226 vs :253) */
+    uint32_t h0 = oracle_puzzle_manhattan(side, start, goal);
+    if (pq_push(&q, (pq_item){prio_of(h0, 0), (uint32_t)s0, 0})) { rc = ORACLE_E_NOMEM; goto out; }
+    int64_t goal_idx = -1;
+    while (q.n) {
+        pq_item it = pq_pop(&q);
+        if (it.g > st.g[it.idx]) continue; /* stale: a better copy was pushed later */
+        uint64_t s = st.key[it.idx];
+        if (s == goal) { goal_idx = it.idx; break; } /* goal test at pop: a_star.h:80-83 */
+        res->expanded++;
+        int b = blank_of(side, s);
+        for (int i = 0; i < 4; ++i) {
+            uint64_t t;
+            if (!puzzle_succ(side, s, b, i, &t)) continue;
+            res->generated++;
+            uint32_t g2 = it.g + 1; /* unit cost: a_star_sliding_puzzle.h:84 */
+            int64_t ti = pstore_get(&st, t);
+            if (ti < 0) { rc = ORACLE_E_NOMEM; goto out; }
+            if (max_states && st.n > max_states) { rc = ORACLE_E_LIMIT; goto out; }
+            if (g2 >= st.g[ti]) continue; /* rejected: fancy_hashmap.h:56-61 */
+            st.g[ti] = g2;
+            st.parent[ti] = it.idx;
+            uint32_t ht = oracle_puzzle_manhattan(side, t, goal);
+            if (pq_push(&q, (pq_item){prio_of(g2 + ht, g2), (uint32_t)ti, g2})) { rc = ORACLE_E_NOMEM; goto out; }
+        }
+    }
+    res->stored = st.n;
+    if (goal_idx >= 0) {
+        rc = ORACLE_OK;
+        res->cost = st.g[goal_idx];
+        uint32_t len = 0;
+        for (uint32_t i = (uint32_t)goal_idx; i != NONE32; i = st.parent[i]) ++len;
+        res->path_len = len;
+        if (path && path_cap >= len) {
+            uint32_t k = len;
+            for (uint32_t i = (uint32_t)goal_idx; i != NONE32; i = st.parent[i]) path[--k] = st.key[i];
+        }
+    }
+out:
+    if (rc != ORACLE_OK && rc != ORACLE_NO_PATH) res->stored = st.n;
+    pstore_free(&st);
+    free(q.a);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* grids                                                               */
+/* ------------------------------------------------------------------ */
+/* Move order of a_star_pathfinding.h:150-182: x+1, x-1, y+1, y-1,
+ * (+1,+1), (+1,-1), (-1,+1), (-1,-1). */
+static const int DX[8] = {1, -1, 0, 0, 1, 1, -1, -1};
+static const int DY[8] = {0, 0, 1, -1, 1, -1, 1, -1};
+
+static uint32_t grid_h(uint32_t x, uint32_t y, uint32_t gx, uint32_t gy, int conn, uint32_t w_min) {
+    uint32_t dx = x > gx ? x - gx : gx - x;
+    uint32_t dy = y > gy ? y - gy : gy - y;
+    if (conn == 8) return (dx > dy ? dx : dy) * w_min;
+    return (dx + dy) * w_min;
+}
+
+static uint32_t grid_wmin(const uint8_t* cells, size_t n) {
+    uint32_t m = 256;
+    for (size_t i = 0; i < n; ++i)
+        if (cells[i] && cells[i] < m) m = cells[i];
+    return m == 256 ? 1 : m;
+}
+
+int oracle_astar_grid(const uint8_t* cells, uint32_t w, uint32_t h, uint32_t start, uint32_t goal,
+                      int connectivity, uint32_t w_min, oracle_result* res, uint32_t* path,
+                      uint32_t path_cap) {
+    memset(res, 0, sizeof(*res));
+    res->cost = -1;
+    size_t n = (size_t)w * h;
+    if (!w || !h || start >= n || goal >= n || (connectivity != 4 && connectivity != 8) ||
+        !cells[start] || !cells[goal])
+        return ORACLE_E_INVALID;
+    if (w_min == 0) w_min = grid_wmin(cells, n);
+    uint32_t* g = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t* par = (uint32_t*)malloc(n * sizeof(uint32_t));
+    if (!g || !par) {
+        free(g);
+        free(par);
+        return ORACLE_E_NOMEM;
+    }
+    memset(g, 0xFF, n * sizeof(uint32_t));
+    pq q;
+    memset(&q, 0, sizeof(q));
+    int rc = ORACLE_NO_PATH;
+    uint32_t gx = goal % w, gy = goal / w;
+    g[start] = 0;
+    par[start] = NONE32;
+    if (pq_push(&q, (pq_item){prio_of(grid_h(start % w, start / w, gx, gy, connectivity, w_min), 0), start, 0})) {
+        rc = ORACLE_E_NOMEM;
+        goto out;
+    }
+    int found = 0;
+    uint64_t stored = 1;
+    while (q.n) {
+        pq_item it = pq_pop(&q);
+        if (it.g > g[it.idx]) continue;
+        if (it.idx == goal) { found = 1; break; }
+        res->expanded++;
+        uint32_t x = it.idx % w, y = it.idx / w;
+        for (int i = 0; i < connectivity; ++i) {
+            int64_t nx = (int64_t)x + DX[i], ny = (int64_t)y + DY[i];
+            if (nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+            uint32_t c = (uint32_t)ny * w + (uint32_t)nx;
+            if (!cells[c]) continue; /* wall */
+            res->generated++;
+            uint32_t g2 = it.g + cells[c]; /* cost = entered cell: a_star_pathfinding.h:203-208 */
+            if (g2 >= g[c]) continue;
+            if (g[c] == INF32) ++stored;
+            g[c] = g2;
+            par[c] = it.idx;
+            uint32_t f = g2 + grid_h((uint32_t)nx, (uint32_t)ny, gx, gy, connectivity, w_min);
+            if (pq_push(&q, (pq_item){prio_of(f, g2), c, g2})) { rc = ORACLE_E_NOMEM; goto out; }
+        }
+    }
+    res->stored = stored;
+    if (found) {
+        rc = ORACLE_OK;
+        res->cost = g[goal];
+        uint32_t len = 0;
+        for (uint32_t i = goal; i != NONE32; i = par[i]) ++len;
+        res->path_len = len;
+        if (path && path_cap >= len) {
+            uint32_t k = len;
+            for (uint32_t i = goal; i != NONE32; i = par[i]) path[--k] = i;
+        }
+    }
+out:
+    free(g);
+    free(par);
+    free(q.a);
+    return rc;
+}
+
+/* Dial's algorithm: Dijkstra with a circular array of buckets, valid because
+ * every move cost is in 1..255.  No heuristic, no binary heap: an independent
+ * check of oracle_astar_grid. */
+int64_t oracle_dial_grid(const uint8_t* cells, uint32_t w, uint32_t h, uint32_t start, uint32_t goal,
+                         int connectivity) {
+    size_t n = (size_t)w * h;
+    if (!w || !h || start >= n || goal >= n || !cells[start] || !cells[goal]) return -1;
+    enum { NB = 256 };
+    uint32_t* dist = (uint32_t*)malloc(n * sizeof(uint32_t));
+    uint32_t head[NB]; /* bucket d % NB: singly-linked list of pool entries */
+    if (!dist) return -1;
+    memset(dist, 0xFF, n * sizeof(uint32_t));
+    for (int i = 0; i < NB; ++i) head[i] = NONE32;
+    /* A cell may be queued several times; each queued copy is a pool entry
+     * (cell, d) and is skipped when popped if dist[cell] has since dropped. */
+    size_t pool_cap = n + 16, pool_n = 0;
+    uint32_t* pool_cell = (uint32_t*)malloc(pool_cap * sizeof(uint32_t));
+    uint32_t* pool_d = (uint32_t*)malloc(pool_cap * sizeof(uint32_t));
+    uint32_t* pool_next = (uint32_t*)malloc(pool_cap * sizeof(uint32_t));
+    int64_t ans = -1;
+    if (!pool_cell || !pool_d || !pool_next) goto out;
+    dist[start] = 0;
+    pool_cell[0] = start;
+    pool_d[0] = 0;
+    pool_next[0] = NONE32;
+    head[0] = 0;
+    pool_n = 1;
+    size_t live = 1;
+    for (uint32_t d = 0; live; ++d) {
+        uint32_t b = d % NB;
+        while (head[b] != NONE32) {
+            uint32_t e = head[b];
+            head[b] = pool_next[e];
+            --live;
+            /* every move costs 1..255 < NB, so all live entries lie within one
+             * lap of d and bucket d % NB holds only entries with pool_d == d */
+            uint32_t c = pool_cell[e];
+            if (dist[c] != d) continue;
+            if (c == goal) { ans = d; goto out; }
+            uint32_t x = c % w, y = c / w;
+            for (int i = 0; i < connectivity; ++i) {
+                int64_t nx = (int64_t)x + DX[i], ny = (int64_t)y + DY[i];
+                if (nx < 0 || ny < 0 || nx >= w || ny >= h) continue;
+                uint32_t t = (uint32_t)ny * w + (uint32_t)nx;
+                if (!cells[t]) continue;
+                uint32_t nd = d + cells[t];
+                if (nd >= dist[t]) continue;
+                dist[t] = nd;
+                if (pool_n == pool_cap) {
+                    size_t nc = pool_cap * 2;
+                    uint32_t* a = (uint32_t*)realloc(pool_cell, nc * sizeof(uint32_t));
+                    if (!a) goto out;
+                    pool_cell = a;
+                    a = (uint32_t*)realloc(pool_d, nc * sizeof(uint32_t));
+                    if (!a) goto out;
+                    pool_d = a;
+                    a = (uint32_t*)realloc(pool_next, nc * sizeof(uint32_t));
+                    if (!a) goto out;
+                    pool_next = a;
+                    pool_cap = nc;
+                }
+                size_t e2 = pool_n++;
+                pool_cell[e2] = t;
+                pool_d[e2] = nd;
+                pool_next[e2] = head[nd % NB];
+                head[nd % NB] = (uint32_t)e2;
+                ++live;
+            }
+        }
+    }
+out:
+    free(dist);
+    free(pool_cell);
+    free(pool_d);
+    free(pool_next);
+    return ans;
+}
+
+/* ------------------------------------------------------------------ */
+/* 8-puzzle BFS                                                        */
+/* ------------------------------------------------------------------ */
+uint32_t oracle_perm_rank9(uint64_t state) {
+    /* Lehmer code of the 9 nibbles, cell 0 most significant. */
+    static const uint32_t fact[9] = {40320, 5040, 720, 120, 24, 6, 2, 1, 1};
+    uint32_t used = 0, rank = 0;
+    for (int c = 0; c < 9; ++c) {
+        uint32_t t = nib(state, c);
+        uint32_t smaller = (uint32_t)__builtin_popcount(((1u << t) - 1) & ~used);
+        rank += smaller * fact[c];
+        used |= 1u << t;
+    }
+    return rank;
+}
+
+uint32_t oracle_bfs_puzzle8(uint64_t goal, uint8_t* depth_by_rank, uint32_t* hist) {
+    memset(depth_by_rank, 0xFF, 362880);
+    memset(hist, 0, 64 * sizeof(uint32_t));
+    uint64_t* queue = (uint64_t*)malloc(362880 * sizeof(uint64_t));
+    if (!queue) return 0;
+    size_t qh = 0, qt = 0;
+    queue[qt++] = goal;
+    depth_by_rank[oracle_perm_rank9(goal)] = 0;
+    while (qh < qt) {
+        uint64_t s = queue[qh++];
+        uint8_t d = depth_by_rank[oracle_perm_rank9(s)];
+        hist[d]++;
+        int b = blank_of(3, s);
+        for (int i = 0; i < 4; ++i) {
+            uint64_t t;
+            if (!puzzle_succ(3, s, b, i, &t)) continue;
+            uint32_t r = oracle_perm_rank9(t);
+            if (depth_by_rank[r] != 0xFF) continue;
+            depth_by_rank[r] = (uint8_t)(d + 1);
+            queue[qt++] = t;
+        }
+    }
+    free(queue);
+    return (uint32_t)qt;
+}
+
+/* ------------------------------------------------------------------ */
+/* replay                                                              */
+/* ------------------------------------------------------------------ */
+int64_t oracle_replay_puzzle(int side, const uint64_t* path, uint32_t n, uint64_t start, uint64_t goal) {
+    if (n == 0 || path[0] != start || path[n - 1] != goal) return -1;
+    if (!valid_puzzle(side, start)) return -1;
+    int64_t cost = 0;
+    for (uint32_t k = 1; k < n; ++k) {
+        uint64_t s = path[k - 1];
+        int b = blank_of(side, s);
+        int ok = 0;
+        for (int i = 0; i < 4 && !ok; ++i) {
+            uint64_t t;
+            if (puzzle_succ(side, s, b, i, &t) && t == path[k]) ok = 1;
+        }
+        if (!ok) return -1;
+        cost += 1;
+    }
+    return cost;
+}
+
+int64_t oracle_replay_grid(const uint8_t* cells, uint32_t w, uint32_t h, int connectivity,
+                           const uint32_t* path, uint32_t n, uint32_t start, uint32_t goal) {
+    size_t total = (size_t)w * h;
+    if (n == 0 || path[0] != start || path[n - 1] != goal) return -1;
+    if (start >= total || !cells[start]) return -1;
+    int64_t cost = 0;
+    for (uint32_t k = 1; k < n; ++k) {
+        uint32_t a = path[k - 1], b = path[k];
+        if (b >= total || !cells[b]) return -1;
+        int64_t dx = (int64_t)(b % w) - (int64_t)(a % w);
+        int64_t dy = (int64_t)(b / w) - (int64_t)(a / w);
+        int ok = 0;
+        for (int i = 0; i < connectivity; ++i)
+            if (DX[i] == dx && DY[i] == dy) ok = 1;
+        if (!ok) return -1;
+        cost += cells[b];
+    }
+    return cost;
+}

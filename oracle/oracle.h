@@ -1,0 +1,109 @@
+/*
+ * oracle.h -- sequential CPU oracle for the GA* search path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (the HIP engine, the C ABI,
+ * the Python package) links, loads or calls this code.  Only the test suite,
+ * the smoke check and the CPU-baseline leg of the benchmark load liboracle.so,
+ * and only as the checker / the timed CPU reference, never as the path that is
+ * measured or shipped.
+ *
+ * What it restates (reference = michalborkowski96/cuda_a_star):
+ *   - A* with re-opening and the priority order of QueueValue
+ *     (f ascending, ties to the larger g): a_star.h:55-66.
+ *   - goal test at pop time, the popped goal is not expanded: a_star.h:80-83.
+ *   - g' = g + cost of the move: a_star.h:90-95.
+ *   - closed-set rule "accept only if g_new < g_stored": fancy_hashmap.h:56-61
+ *     with FancyHashmapValue::operator<= comparing g (a_star.h:31-33).
+ *   - path = states from start to target through prev links: a_star.h:266-281.
+ *   - sliding-puzzle moves in the reference order (blank row+1, row-1, col+1,
+ *     col-1), unit cost: a_star_sliding_puzzle.h:83-123.
+ *   - grid moves in the reference order (x+1, x-1, y+1, y-1, then the four
+ *     diagonals), cost = weight of the entered cell: a_star_pathfinding.h:147-209.
+ * Heuristics are the admissible variants the workloads specify (Manhattan
+ * without the blank; Manhattan or Chebyshev times the minimum cell weight),
+ * not the reference's inadmissible ones (a_star_sliding_puzzle.h:152-163
+ * counts the blank; a_star_pathfinding.h:219-221 uses Manhattan with unit
+ * diagonals).
+ *
+ * How each function is pinned (checked by tests/test_oracle.py):
+ *   oracle_astar_puzzle  - against the full 8-puzzle BFS (every one of the
+ *                          181,440 reachable states, or a seeded sample), whose
+ *                          depth histogram must equal OEIS A089473; parity and
+ *                          bound invariants on 15-puzzle walks.
+ *   oracle_bfs_puzzle8   - histogram = OEIS A089473, total 181,440, max 31.
+ *   oracle_astar_grid    - against oracle_dial_grid (an independent Dijkstra
+ *                          with a bucket queue) and the closed form
+ *                          w*(|dx|+|dy|) / w*max(|dx|,|dy|) on wall-free grids.
+ *   oracle_dial_grid     - closed forms above and hand-built mazes.
+ *   oracle_replay_*      - hand-built legal and illegal paths.
+ *
+ * State encodings (shared with the engine's C ABI):
+ *   puzzle: uint64, nibble i (bits 4i..4i+3) = tile at cell i in row-major
+ *           order, 0 = blank; side 3 or 4.
+ *   grid:   uint8 cells[w*h] row-major, 0 = wall, 1..255 = cost of entering;
+ *           a state is the cell index y*w + x.
+ */
+#ifndef GASTAR_ORACLE_H
+#define GASTAR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORACLE_OK 0
+#define ORACLE_NO_PATH 1
+#define ORACLE_E_INVALID (-1)
+#define ORACLE_E_LIMIT (-2)
+#define ORACLE_E_NOMEM (-3)
+
+typedef struct {
+    int64_t cost;        /* optimal path cost, -1 if none */
+    uint64_t expanded;   /* nodes popped and expanded */
+    uint64_t generated;  /* successors generated */
+    uint64_t stored;     /* distinct states stored in the closed set */
+    uint32_t path_len;   /* number of states written to path (start..goal) */
+    uint32_t pad;
+} oracle_result;
+
+/* Sequential A* on a side x side sliding puzzle.  path may be NULL.
+ * max_states bounds the closed set (0 = no bound); ORACLE_E_LIMIT if hit. */
+int oracle_astar_puzzle(int side, uint64_t start, uint64_t goal,
+                        uint64_t max_states, oracle_result* res,
+                        uint64_t* path, uint32_t path_cap);
+
+/* Admissible heuristic used by the oracle (Manhattan, blank excluded). */
+uint32_t oracle_puzzle_manhattan(int side, uint64_t state, uint64_t goal);
+
+/* Sequential A* on a dense grid; connectivity 4 or 8.  w_min = 0 means
+ * "compute the minimum open-cell weight".  path holds cell indices. */
+int oracle_astar_grid(const uint8_t* cells, uint32_t w, uint32_t h,
+                      uint32_t start, uint32_t goal, int connectivity,
+                      uint32_t w_min, oracle_result* res,
+                      uint32_t* path, uint32_t path_cap);
+
+/* Dijkstra with a circular bucket queue (Dial), independent of the A* code. */
+int64_t oracle_dial_grid(const uint8_t* cells, uint32_t w, uint32_t h,
+                         uint32_t start, uint32_t goal, int connectivity);
+
+/* Full BFS of the 8-puzzle from goal.  depth_by_rank[362880] receives the
+ * distance of every permutation (0xFF = unreachable), indexed by
+ * oracle_perm_rank9; hist[64] receives the depth histogram.  Returns the
+ * number of reachable states. */
+uint32_t oracle_bfs_puzzle8(uint64_t goal, uint8_t* depth_by_rank, uint32_t* hist);
+uint32_t oracle_perm_rank9(uint64_t state);
+
+/* Move-by-move replay.  Returns the summed move cost, or -1 if the path does
+ * not start at start, end at goal, or contains an illegal move. */
+int64_t oracle_replay_puzzle(int side, const uint64_t* path, uint32_t n,
+                             uint64_t start, uint64_t goal);
+int64_t oracle_replay_grid(const uint8_t* cells, uint32_t w, uint32_t h,
+                           int connectivity, const uint32_t* path, uint32_t n,
+                           uint32_t start, uint32_t goal);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

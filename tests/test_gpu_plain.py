@@ -1,0 +1,122 @@
+"""Plain GA* on the GPU (one instance per call) against the CPU oracle.
+
+Costs must equal the oracle's exactly (integer, tolerance 0) and every path
+must pass a move-by-move replay.  Instances come from the seeded generators.
+"""
+import numpy as np
+import pytest
+
+import cuda_a_star_amd as ga
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def check_puzzle(engine, side, start, goal=None, **kw):
+    goal = ga.puzzle_goal(side) if goal is None else goal
+    sol = engine.solve_puzzle(side, start, goal, **kw)
+    rc, ref, _ = oracle.astar_puzzle(side, start, goal, want_path=False)
+    if rc == oracle.NO_PATH:
+        assert sol.status == ga.NO_PATH
+        return sol
+    assert rc == oracle.OK
+    assert sol.status == ga.OK, ga.status_str(sol.status)
+    assert sol.cost == ref.cost
+    assert oracle.replay_puzzle(side, sol.path, start, goal) == sol.cost
+    return sol
+
+
+def check_grid(engine, cells, w, h, start=0, goal=None, conn=4, **kw):
+    goal = w * h - 1 if goal is None else goal
+    sol = engine.solve_grid(cells, w, h, start, goal, conn, **kw)
+    ref = oracle.dial_grid(cells, w, h, start, goal, conn)
+    if ref < 0:
+        assert sol.status == ga.NO_PATH
+        return sol
+    assert sol.status == ga.OK, ga.status_str(sol.status)
+    assert sol.cost == ref
+    assert oracle.replay_grid(cells, w, h, conn, sol.path, start, goal) == sol.cost
+    return sol
+
+
+def test_puzzle8_start_is_goal(engine):
+    g = ga.puzzle_goal(3)
+    sol = engine.solve_puzzle(3, g, g, num_queues=256)
+    assert sol.status == ga.OK and sol.cost == 0 and sol.path == [g]
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_puzzle8_walk30(engine, seed):
+    # 8-puzzle, 30-move seeded walk with backtracks allowed, N = 256 heaps
+    start = ga.gen_puzzle(3, 30, seed, no_backtrack=False)
+    check_puzzle(engine, 3, start, num_queues=256)
+
+
+def test_puzzle8_unsolvable(engine):
+    g = ga.puzzle_goal(3)
+    tiles = ga.state_to_board(g, 3)
+    tiles[0], tiles[1] = tiles[1], tiles[0]  # one transposition: other parity class
+    sol = engine.solve_puzzle(3, ga.board_to_state(tiles), g, num_queues=256)
+    assert sol.status == ga.NO_PATH
+
+
+def test_puzzle8_hardest(engine):
+    # the two 31-move states of the 8-puzzle (BFS depth 31)
+    n, depth, hist = oracle.bfs_puzzle8(ga.puzzle_goal(3))
+    assert hist[31] == 2
+    found = 0
+    import itertools
+    for perm in itertools.permutations(range(9)):
+        s = ga.board_to_state(perm)
+        if depth[oracle.perm_rank9(s)] == 31:
+            sol = check_puzzle(engine, 3, s, num_queues=256)
+            assert sol.cost == 31
+            found += 1
+            if found == 2:
+                break
+    assert found == 2
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_puzzle15_walk60(engine, seed):
+    start = ga.gen_puzzle(4, 60, seed)
+    check_puzzle(engine, 4, start, num_queues=16384)
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_grid_small(engine, seed):
+    cells, _ = ga.gen_grid(64, 48, 0.2, seed)
+    check_grid(engine, cells, 64, 48, num_queues=256)
+
+
+def test_grid_8conn(engine):
+    cells, _ = ga.gen_grid(80, 60, 0.25, 11, connectivity=8)
+    check_grid(engine, cells, 80, 60, conn=8, num_queues=512)
+
+
+def test_grid_closed_form(engine):
+    w, h, wt = 100, 70, 3
+    cells = np.full(w * h, wt, dtype=np.uint8)
+    sol = check_grid(engine, cells, w, h, num_queues=1024)
+    assert sol.cost == wt * (w - 1 + h - 1)
+
+
+def test_grid_unreachable(engine):
+    cells, _ = ga.gen_grid(40, 40, 0.1, 3)
+    cells = cells.reshape(40, 40).copy()
+    cells[:, 20] = 0  # wall off the right half
+    cells = cells.reshape(-1)
+    sol = engine.solve_grid(cells, 40, 40, 0, 40 * 40 - 1, 4, num_queues=256)
+    assert sol.status == ga.NO_PATH
+
+
+def test_grid_1024(engine):
+    cells, _ = ga.gen_grid(1024, 1024, 0.2, 5)
+    check_grid(engine, cells, 1024, 1024, num_queues=4096)
+
+
+def test_capacity_growth(engine):
+    # tiny heaps and closed set force the overflow -> retry path
+    start = ga.gen_puzzle(4, 60, 1)
+    sol = check_puzzle(engine, 4, start, num_queues=1024, heap_capacity=16, table_slots=1 << 12)
+    assert sol.retries > 0

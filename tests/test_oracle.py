@@ -1,0 +1,151 @@
+"""Pins the CPU oracle (oracle/oracle.c) against facts it did not produce.
+
+- 8-puzzle: full BFS histogram = OEIS A089473 (181,440 states, max depth 31);
+  A* cost = BFS depth for every state of a seeded sample.
+- grids: A* cost = Dial's Dijkstra (independent code, no heuristic) on seeded
+  grids, 4- and 8-connected; closed forms on wall-free grids.
+- 15-puzzle: h <= cost <= walk length, cost = walk length = h (mod 2).
+- replay: hand-built legal and illegal paths.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+import cuda_a_star_amd as ga
+import oracle
+
+# OEIS A089473: 8-puzzle configurations at distance n from the goal with the blank in a corner.
+A089473 = [1, 2, 4, 8, 16, 20, 39, 62, 116, 152, 286, 396, 748, 1024, 1893, 2512, 4485, 5638, 9529,
+           10878, 16993, 17110, 23952, 20224, 24047, 15578, 14560, 6274, 3910, 760, 221, 2]
+
+GOAL8 = 0x087654321
+GOAL15 = 0x0FEDCBA987654321
+
+
+def test_goal_encodings():
+    assert ga.puzzle_goal(3) == GOAL8
+    assert ga.puzzle_goal(4) == GOAL15
+
+
+def test_bfs_histogram_is_a089473():
+    n, depth, hist = oracle.bfs_puzzle8(GOAL8)
+    assert n == 181440 == sum(A089473)
+    assert [int(x) for x in hist[:32]] == A089473
+    assert int(hist[32:].sum()) == 0
+
+
+def test_perm_rank_is_a_bijection():
+    ranks = {oracle.perm_rank9(ga.board_to_state(p)) for p in itertools.permutations(range(9))}
+    assert ranks == set(range(362880))
+
+
+def test_astar_equals_bfs_on_sample():
+    _, depth, _ = oracle.bfs_puzzle8(GOAL8)
+    rng = np.random.default_rng(1234)
+    checked = 0
+    for perm in rng.permuted(np.tile(np.arange(9), (4000, 1)), axis=1):
+        s = ga.board_to_state(perm)
+        d = depth[oracle.perm_rank9(s)]
+        rc, res, path = oracle.astar_puzzle(3, s, GOAL8)
+        if d == 0xFF:
+            assert rc == oracle.NO_PATH
+            continue
+        assert rc == oracle.OK and res.cost == d
+        assert oracle.replay_puzzle(3, path, s, GOAL8) == d
+        checked += 1
+    assert checked > 1500
+
+
+def test_astar_deepest_states():
+    _, depth, _ = oracle.bfs_puzzle8(GOAL8)
+    deep = [ga.board_to_state(p) for p in itertools.permutations(range(9))
+            if depth[oracle.perm_rank9(ga.board_to_state(p))] == 31]
+    assert len(deep) == 2
+    for s in deep:
+        rc, res, _ = oracle.astar_puzzle(3, s, GOAL8)
+        assert rc == oracle.OK and res.cost == 31
+
+
+def test_manhattan_excludes_blank():
+    # one blank move from the goal: tile 8 is one cell off, the blank is not counted
+    s = ga.gen_puzzle(3, 1, 0)
+    assert oracle.manhattan(3, s, GOAL8) == 1
+    rc, res, _ = oracle.astar_puzzle(3, s, GOAL8)
+    assert res.cost == 1
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_puzzle15_invariants(seed):
+    depth = 40 + seed
+    s = ga.gen_puzzle(4, depth, seed)
+    rc, res, path = oracle.astar_puzzle(4, s, GOAL15)
+    assert rc == oracle.OK
+    h = oracle.manhattan(4, s, GOAL15)
+    assert h <= res.cost <= depth
+    assert res.cost % 2 == depth % 2 == h % 2
+    assert oracle.replay_puzzle(4, path, s, GOAL15) == res.cost
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+@pytest.mark.parametrize("seed", range(6))
+def test_grid_astar_equals_dial(conn, seed):
+    w, h = 57 + seed * 13, 41 + seed * 7
+    cells, _ = ga.gen_grid(w, h, 0.1 + 0.03 * seed, seed, connectivity=conn)
+    rc, res, path = oracle.astar_grid(cells, w, h, 0, w * h - 1, conn)
+    d = oracle.dial_grid(cells, w, h, 0, w * h - 1, conn)
+    assert rc == oracle.OK and res.cost == d
+    assert oracle.replay_grid(cells, w, h, conn, path, 0, w * h - 1) == d
+
+
+@pytest.mark.parametrize("w,h,wt", [(1, 1, 5), (10, 1, 2), (33, 17, 7), (64, 64, 1)])
+def test_grid_closed_forms(w, h, wt):
+    cells = np.full(w * h, wt, dtype=np.uint8)
+    for conn, expect in ((4, wt * (w - 1 + h - 1)), (8, wt * max(w - 1, h - 1))):
+        rc, res, _ = oracle.astar_grid(cells, w, h, 0, w * h - 1, conn)
+        assert res.cost == expect
+        assert oracle.dial_grid(cells, w, h, 0, w * h - 1, conn) == expect
+
+
+def test_grid_maze_detour():
+    # a wall with one gap forces a detour whose cost is known by hand
+    w, h = 5, 5
+    cells = np.ones((h, w), dtype=np.uint8)
+    cells[2, :4] = 0           # row 2 blocked except x = 4
+    cells = cells.reshape(-1)
+    # (0,0) -> (4,0) 4 moves, -> (4,4) 4 moves, -> (0,4) 4 moves = 12
+    assert oracle.dial_grid(cells, w, h, 0, 4 * w + 0, 4) == 12
+    rc, res, _ = oracle.astar_grid(cells, w, h, 0, 4 * w + 0, 4)
+    assert res.cost == 12
+
+
+def test_grid_unreachable():
+    cells = np.ones(25, dtype=np.uint8)
+    cells[[1, 5, 6]] = 0
+    assert oracle.dial_grid(cells, 5, 5, 0, 24, 4) == -1
+    rc, _, _ = oracle.astar_grid(cells, 5, 5, 0, 24, 4)
+    assert rc == oracle.NO_PATH
+    assert oracle.dial_grid(cells, 5, 5, 0, 24, 8) == -1
+
+
+def test_replay_rejects_bad_paths():
+    s1 = ga.gen_puzzle(3, 1, 0)
+    assert oracle.replay_puzzle(3, [s1, GOAL8], s1, GOAL8) == 1
+    assert oracle.replay_puzzle(3, [s1, s1, GOAL8], s1, GOAL8) == -1      # not a move
+    assert oracle.replay_puzzle(3, [GOAL8], s1, GOAL8) == -1              # wrong start
+    cells = np.array([1, 2, 3, 4], dtype=np.uint8)                        # 2x2
+    assert oracle.replay_grid(cells, 2, 2, 4, [0, 1, 3], 0, 3) == 2 + 4
+    assert oracle.replay_grid(cells, 2, 2, 4, [0, 3], 0, 3) == -1          # diagonal in 4-conn
+    assert oracle.replay_grid(cells, 2, 2, 8, [0, 3], 0, 3) == 4
+    cells[1] = 0
+    assert oracle.replay_grid(cells, 2, 2, 4, [0, 1, 3], 0, 3) == -1       # through a wall
+
+
+def test_planted_mistakes_are_caught():
+    """A plausible oracle bug (blank counted in h, as the reference does) must
+    fail the checks above: counting the blank overestimates and breaks the BFS
+    equality on some state."""
+    _, depth, _ = oracle.bfs_puzzle8(GOAL8)
+    s = ga.gen_puzzle(3, 1, 0)
+    blank_counted = oracle.manhattan(3, s, GOAL8) + 1  # the blank is one cell off its goal
+    assert blank_counted > depth[oracle.perm_rank9(s)]  # inadmissible: h > h*
