@@ -1,0 +1,267 @@
+#!/usr/bin/env python3
+"""Served GA* vs one-at-a-time plain GA* on one MI355X.
+
+Workload: the serving stream of the benchmark configurations -- a seeded
+Poisson arrival stream at a fixed offered QPS; 60 % 15-puzzles (walk depth
+U{30..70}), 40 % grids (side U{512..4096}, weights 1..9, 20 % walls); every
+grid resident in HBM before the timed region; admission budget 200 GB of heap
+and closed-set bytes.
+
+A "step" is one stream segment of --requests requests served with continuous
+batching (ga_serve_run).  W warm-up segments, then exactly K timed segments.
+value = completed solves / wall seconds over the K timed segments.
+
+Also measured (outside the timed region): the same requests solved one at a
+time by plain GA* (ga_solve), and the CPU oracle on the host cores.
+The step kernel's roofline comes from HIP events around every step launch and
+the device counters of the timed segments.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import threading
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MAX_ORACLE_STATES = 12_000_000  # per CPU request; larger puzzles are skipped (and counted)
+METRIC = "optimal solves/sec + p50/p99 latency: served batch vs one-at-a-time GA*"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(xs, p):
+    xs = sorted(xs)
+    if not xs:
+        return 0.0
+    k = min(len(xs) - 1, max(0, int(-(-p * len(xs) // 1)) - 1))
+    return xs[k]
+
+
+def cpu_baseline(insts, seconds: float, threads: int, gpu_costs: dict):
+    """CPU oracle (sequential A*) on the host cores over a bounded sample of
+    the same stream: requests are taken in stream order by `threads` workers
+    until `seconds` have passed."""
+    import oracle
+
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0, "match": 0, "checked": 0, "capped": 0}
+    deadline = time.perf_counter() + seconds
+    t0 = time.perf_counter()
+
+    def worker():
+        while True:
+            with lock:
+                i = state["next"]
+                if i >= len(insts) or time.perf_counter() > deadline:
+                    return
+                state["next"] += 1
+            x = insts[i]
+            if x.kind == "puzzle":
+                rc, res, _ = oracle.astar_puzzle(4, x.start, x.goal, max_states=MAX_ORACLE_STATES, want_path=False)
+                if rc == oracle.E_LIMIT:
+                    with lock:
+                        state["capped"] += 1
+                    continue
+                cost = res.cost if rc == oracle.OK else -1
+            else:
+                rc, res, _ = oracle.astar_grid(x.cells, x.side, x.side, x.start, x.goal, 4, want_path=False)
+                cost = res.cost if rc == oracle.OK else -1
+            with lock:
+                state["done"] += 1
+                if i in gpu_costs:
+                    state["checked"] += 1
+                    state["match"] += int(gpu_costs[i] == cost)
+
+    ts = [threading.Thread(target=worker) for _ in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    el = time.perf_counter() - t0
+    return state["done"] / el, state["done"], el, state["checked"], state["match"], state["capped"]
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--requests", type=int, default=512, help="requests per step (stream segment)")
+    ap.add_argument("--qps", type=float, default=1500.0, help="offered load (Poisson arrivals)")
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--plain-sample", type=int, default=40, help="requests solved one at a time")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--budget-gb", type=float, default=200.0)
+    ap.add_argument("--no-profile", action="store_true", help="skip per-step HIP events and counters")
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(instances are independent: a multi-GPU run would shard the request stream)", file=sys.stderr)
+        return 2
+
+    import cuda_a_star_amd as ga
+    from cuda_a_star_amd import tuning
+    from cuda_a_star_amd import workloads as wl
+
+    M, K, W = args.requests, args.steps, args.warmup
+    n_total = (W + K) * M
+    t = time.perf_counter()
+    insts = wl.config5(args.seed, n_total, args.qps, threads=16)
+    log(f"[bench] generated {n_total} requests in {time.perf_counter() - t:.1f}s")
+
+    served_cfg = tuning.load()
+    e = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30,
+                  profile=0 if args.no_profile else 1, **served_cfg.engine)
+    dev = {}
+    t = time.perf_counter()
+    for i, x in enumerate(insts):
+        if x.kind == "grid":
+            dev[i] = e.upload(x.cells)
+    log(f"[bench] uploaded {len(dev)} grids in {time.perf_counter() - t:.1f}s")
+
+    def served_request(i):
+        x = insts[i]
+        nq = served_cfg.heaps(x.kind, x.side)
+        return x.request(cells_device=dev.get(i), num_queues=nq)
+
+    def segment(s):
+        idx = list(range(s * M, (s + 1) * M))
+        t0 = insts[idx[0]].arrival_ns
+        return idx, [served_request(i) for i in idx], [insts[i].arrival_ns - t0 for i in idx]
+
+    # ---- warm-up segments (untimed) ----
+    for s in range(W):
+        idx, reqs, arr = segment(s)
+        sols, ss = e.serve(reqs, arr)
+        log(f"[bench] warmup {s}: {ss.completed}/{len(reqs)} in {ss.wall_ns / 1e9:.2f}s "
+            f"({ss.solves_per_sec:.1f}/s, p50 {ss.p50_ms:.1f} ms, p99 {ss.p99_ms:.1f} ms)")
+
+    # ---- timed segments ----
+    e.reset_stats()
+    lat, ok, failed, served_costs = [], 0, 0, {}
+    walls = []
+    try:
+        import torch
+        sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
+    except Exception:  # torch is only used for the barrier
+        sync = lambda: None
+    sync()
+    t_start = time.perf_counter()
+    for s in range(W, W + K):
+        idx, reqs, arr = segment(s)
+        ts = time.perf_counter()
+        sols, ss = e.serve(reqs, arr)
+        walls.append(time.perf_counter() - ts)
+        for i, sol in zip(idx, sols):
+            if sol.status == ga.OK:
+                ok += 1
+                lat.append(sol.latency_ns / 1e6)
+                served_costs[i] = sol.cost
+            else:
+                failed += 1
+        log(f"[bench] step {s - W}: {ss.completed}/{len(reqs)} in {walls[-1]:.2f}s, "
+            f"p50 {ss.p50_ms:.1f} ms, p99 {ss.p99_ms:.1f} ms, inflight max {ss.max_inflight}")
+    sync()
+    t_total = time.perf_counter() - t_start
+    st = e.stats()
+    served_sps = ok / t_total
+
+    # ---- kernel roofline (timed segments) ----
+    roofline = None
+    if not args.no_profile and st.step_launches:
+        avg_ms = st.step_ms_total / st.step_launches
+        bytes_per_launch = st.algo_bytes / st.step_launches
+        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
+                    "kernel": "step_kernel<4>", "avg_launch_us": round(avg_ms * 1e3, 3),
+                    "algo_bytes_per_launch": round(bytes_per_launch, 1),
+                    "expansions_per_launch": round(st.expanded / st.step_launches, 1),
+                    "launches": st.step_launches}
+    log(f"[bench] served {ok} ok / {failed} failed in {t_total:.2f}s = {served_sps:.1f} solves/s; roofline {roofline}")
+
+    # ---- one at a time: plain GA*, same requests, sequential ----
+    timed_idx = list(range(W * M, (W + K) * M))
+    step = max(1, len(timed_idx) // max(1, args.plain_sample))
+    sample = timed_idx[::step][: args.plain_sample]
+    plain_cfg = tuning.load(mode="plain")
+    e1 = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30, **plain_cfg.engine)
+    mismatch = 0
+    t = time.perf_counter()
+    for i in sample:
+        x = insts[i]
+        sol = e1.solve(x.request(cells_device=dev.get(i), num_queues=plain_cfg.heaps(x.kind, x.side)))
+        if sol.status != ga.OK or served_costs.get(i, sol.cost) != sol.cost:
+            mismatch += 1
+    plain_t = time.perf_counter() - t
+    plain_sps = len(sample) / plain_t
+    e1.close()
+    log(f"[bench] one-at-a-time: {len(sample)} in {plain_t:.2f}s = {plain_sps:.2f} solves/s, "
+        f"served-vs-alone cost mismatches {mismatch}")
+
+    # ---- CPU oracle on the host cores ----
+    cpu = None
+    if not args.skip_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sps, n, el, checked, match, capped = cpu_baseline([insts[i] for i in timed_idx], args.cpu_seconds, threads,
+                                                  {k - W * M: v for k, v in served_costs.items()})
+        cpu = {"value": round(sps, 3), "unit": "solves/s", "cores": threads, "kind": "port",
+               "sample": f"first {n} requests of the timed stream in stream order, sequential A* "
+                         f"(binary heap, lazy deletion) per request, {threads} threads, {el:.1f}s; "
+                         f"{capped} puzzles above {MAX_ORACLE_STATES} stored states skipped",
+               "gpu_cost_matches": f"{match}/{checked}"}
+        log(f"[bench] cpu oracle: {n} in {el:.1f}s = {sps:.2f} solves/s ({threads} threads), "
+            f"GPU costs equal on {match}/{checked}")
+
+    for a in dev.values():
+        e.free_device(a)
+    e.close()
+
+    out = {
+        "metric": METRIC,
+        "value": round(served_sps, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": W,
+        "ms_per_step": round(t_total / K * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "serving stream: seeded Poisson arrivals; 60% 15-puzzles (walk depth 30..70), "
+                        "40% grids (side 512..4096, weights 1..9, 20% walls); 200 GB heap+closed-set budget",
+            "requests_per_step": M,
+            "offered_qps": args.qps,
+            "seed": args.seed,
+            "heaps_per_instance": served_cfg.describe(),
+        },
+        "p50_ms": round(pct(lat, 0.50), 3),
+        "p99_ms": round(pct(lat, 0.99), 3),
+        "completed": ok,
+        "failed": failed,
+        "one_at_a_time_solves_per_sec": round(plain_sps, 3),
+        "speedup_vs_one_at_a_time": round(served_sps / plain_sps, 2) if plain_sps else None,
+        "served_vs_alone_cost_mismatches": mismatch,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0 if failed == 0 and mismatch == 0 else 1
+
+
+if __name__ == "__main__":
+    sys.exit(main())

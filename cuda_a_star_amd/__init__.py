@@ -83,7 +83,8 @@ class Config(ctypes.Structure):
         ("max_load_pct", ctypes.c_uint32),
         ("pops_per_iter", ctypes.c_uint32),
         ("profile", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 3),
+        ("rounds", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -100,6 +101,10 @@ class EngineStats(ctypes.Structure):
         ("pops", ctypes.c_uint64),
         ("completed", ctypes.c_uint64),
         ("peak_bytes", ctypes.c_uint64),
+        ("groups", ctypes.c_uint64),
+        ("inserted", ctypes.c_uint64),
+        ("lookups", ctypes.c_uint64),
+        ("algo_bytes", ctypes.c_double),
     ]
 
 

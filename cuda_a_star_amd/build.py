@@ -13,31 +13,31 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgastar.so")
+CLI = os.path.join(HERE, "gastar_cli")
 
 SOURCES = [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "instances.cpp")]
+CLI_SOURCE = os.path.join(CSRC, "cli.hip")
 HEADERS = [os.path.join(INCLUDE, "gastar.h"), os.path.join(INCLUDE, "gastar", "core.hpp")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("GASTAR_ARCH", "gfx950")
 
-FLAGS = [
+COMMON = [
     f"--offload-arch={ARCH}",
     "-O3",
     "-std=c++17",
-    "-fPIC",
-    "-shared",
     "-Wall",
     "-Wno-unused-result",
     # keep the compiler from emitting packed fp32 instructions (none are needed here)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     f"-I{INCLUDE}",
 ]
+FLAGS = COMMON + ["-fPIC", "-shared"]
 
 
-def build(verbose: bool = False) -> str:
-    """Compile every source into cuda_a_star_amd/libgastar.so (always recompiles)."""
-    tmp = LIB + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *SOURCES]
+def _hipcc(args, out, verbose):
+    tmp = out + ".tmp"
+    cmd = [HIPCC, *args, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -47,7 +47,13 @@ def build(verbose: bool = False) -> str:
         raise RuntimeError(f"hipcc failed ({res.returncode}):\n{err}")
     if verbose and err.strip():
         print(err, file=sys.stderr)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    """Compile libgastar.so and the gastar_cli executable (always recompiles)."""
+    _hipcc([*FLAGS, *SOURCES], LIB, verbose)
+    _hipcc([*COMMON, CLI_SOURCE], CLI, verbose)
     return LIB
 
 

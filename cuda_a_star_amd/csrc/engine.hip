@@ -284,7 +284,9 @@ struct ga_engine {
     int map_buf = 0;
     Pinned h_states[2];
     hipEvent_t ev_chunk[2] = {nullptr, nullptr};
-    std::vector<hipEvent_t> prof_ev;  // 2 events per step of a chunk
+    std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
+    uint32_t n_read = 0;                 // states read back per chunk (highest used slot + 1)
+    uint32_t chunk_reads[2] = {0, 0};
     uint64_t host_base = 0;           // iteration counter at the start of the next chunk
     uint64_t chunk_count = 0;
     std::vector<Job*> resident;
@@ -303,6 +305,7 @@ struct ga_engine {
         if (cfg.chunk_iters == 0) cfg.chunk_iters = 1;
         if (cfg.max_load_pct == 0 || cfg.max_load_pct > 95) cfg.max_load_pct = 50;
         if (cfg.pops_per_iter == 0) cfg.pops_per_iter = 1;
+        if (cfg.rounds == 0) cfg.rounds = 1;
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamCreateWithFlags(&path_stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -325,7 +328,8 @@ struct ga_engine {
     ~ga_engine() {
         if (stream) (void)hipStreamSynchronize(stream);
         if (path_stream) (void)hipStreamSynchronize(path_stream);
-        for (auto e : prof_ev) (void)hipEventDestroy(e);
+        for (auto& v : prof_ev)
+            for (auto ev : v) (void)hipEventDestroy(ev);
         for (int i = 0; i < 2; ++i) {
             if (ev_map[i]) (void)hipEventDestroy(ev_map[i]);
             if (ev_chunk[i]) (void)hipEventDestroy(ev_chunk[i]);
@@ -393,11 +397,19 @@ struct ga_engine {
         const int W = (int)cfg.heap_arity;
         const int K = branching_of(j.kind);
         std::vector<void*> allocs;
+        bool others_resident = false;
+        for (Job* o : resident)
+            if (o) others_resident = true;
+        bool short_of_memory = false;
         auto take = [&](uint64_t bytes) -> void* {
+            if (short_of_memory) return nullptr;
             void* ptr = arena->alloc(bytes);
             if (!ptr) {
                 for (void* a : allocs) arena->release(a);
-                throw GaError(GA_E_NOMEM, "device arena exhausted");
+                allocs.clear();
+                if (!others_resident) throw GaError(GA_E_NOMEM, "device arena exhausted");
+                short_of_memory = true;  // retry once resident requests have released memory
+                return nullptr;
             }
             allocs.push_back(ptr);
             return ptr;
@@ -409,6 +421,13 @@ struct ga_engine {
         d.heaps = (HeapEntry*)take((uint64_t)p.n_heaps * p.heap_stride * sizeof(HeapEntry));
         d.heap_size = (uint32_t*)take((uint64_t)p.n_heaps * sizeof(uint32_t));
         d.cand = (HeapEntry*)take(2ull * p.n_heaps * K * sizeof(HeapEntry));
+        if (r.kind == GA_PUZZLE) d.table = (Slot*)take(p.table_slots * sizeof(Slot));
+        else {
+            uint64_t cells = (uint64_t)r.width * r.height;
+            d.dense = (uint64_t*)take(cells * sizeof(uint64_t));
+            if (!r.cells_device) d.cells = (const uint8_t*)take(cells);
+        }
+        if (short_of_memory) return 0;
         d.n_heaps = p.n_heaps;
         d.heap_cap = p.heap_cap;
         d.heap_stride = p.heap_stride;
@@ -416,7 +435,6 @@ struct ga_engine {
         if (r.kind == GA_PUZZLE) {
             d.width = r.side;
             d.height = r.side;
-            d.table = (Slot*)take(p.table_slots * sizeof(Slot));
             d.table_mask = p.table_slots - 1;
             d.load_limit = p.table_slots * cfg.max_load_pct / 100;
             for (uint32_t c = 0; c < r.side * r.side; ++c) {
@@ -432,11 +450,9 @@ struct ga_engine {
             if (r.cells_device) {
                 d.cells = r.cells_device;
             } else {
-                uint8_t* c = (uint8_t*)take(cells);
-                hip_check(hipMemcpyAsync(c, r.cells, cells, hipMemcpyHostToDevice, stream), "upload cells");
-                d.cells = c;
+                hip_check(hipMemcpyAsync((void*)d.cells, r.cells, cells, hipMemcpyHostToDevice, stream),
+                          "upload cells");
             }
-            d.dense = (uint64_t*)take(cells * sizeof(uint64_t));
             fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>(d.dense, cells, ~0ull);
         }
         fill_kernel<uint32_t><<<std::min<uint32_t>(fill_grid, (p.n_heaps + kBlock - 1) / kBlock), kBlock, 0,
@@ -504,48 +520,56 @@ struct ga_engine {
 
     void launch_chunk() {
         const uint32_t R = cfg.chunk_iters;
-        StepParams prm{cfg.pops_per_iter, 1u << 16, cfg.profile ? 1u : 0u, 0};
-        if (cfg.profile && prof_ev.size() < 2 * R) {
-            for (size_t i = prof_ev.size(); i < 2 * R; ++i) {
-                hipEvent_t e;
-                hip_check(hipEventCreate(&e), "hipEventCreate");
-                prof_ev.push_back(e);
+        StepParams prm{cfg.pops_per_iter, 1u << 16, cfg.profile ? 1u : 0u, cfg.rounds};
+        const int b = (int)(chunk_count & 1);
+        std::vector<hipEvent_t>& pe = prof_ev[b];
+        if (cfg.profile && pe.size() < 2 * R) {
+            for (size_t i = pe.size(); i < 2 * R; ++i) {
+                hipEvent_t ev;
+                hip_check(hipEventCreate(&ev), "hipEventCreate");
+                pe.push_back(ev);
             }
         }
         for (uint32_t r = 0; r < R; ++r) {
-            if (cfg.profile) hip_check(hipEventRecord(prof_ev[2 * r], stream), "event");
+            if (cfg.profile) hip_check(hipEventRecord(pe[2 * r], stream), "event");
             switch (cfg.heap_arity) {
                 case 8: step_kernel<8><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
                 case 16: step_kernel<16><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
                 default: step_kernel<4><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
             }
-            if (cfg.profile) hip_check(hipEventRecord(prof_ev[2 * r + 1], stream), "event");
+            if (cfg.profile) hip_check(hipEventRecord(pe[2 * r + 1], stream), "event");
         }
         advance_kernel<<<1, 64, 0, stream>>>(d_iter, R);
         hip_check(hipGetLastError(), "step launch");
         host_base += R;
         stats.step_launches += R;
-        int b = (int)(chunk_count & 1);
-        hip_check(hipMemcpyAsync(h_states[b].p, d_states, sizeof(InstState) * cfg.max_instances,
-                                 hipMemcpyDeviceToHost, stream),
+        n_read = 0;
+        for (uint32_t s = 0; s < cfg.max_instances; ++s)
+            if (resident[s]) n_read = s + 1;
+        hip_check(hipMemcpyAsync(h_states[b].p, d_states, sizeof(InstState) * n_read, hipMemcpyDeviceToHost, stream),
                   "read back states");
         hip_check(hipEventRecord(ev_chunk[b], stream), "record chunk");
+        chunk_reads[b] = n_read;
         ++chunk_count;
-        if (cfg.profile) {
-            hip_check(hipEventSynchronize(ev_chunk[b]), "sync chunk");
-            for (uint32_t r = 0; r < R; ++r) {
-                float ms = 0;
-                hip_check(hipEventElapsedTime(&ms, prof_ev[2 * r], prof_ev[2 * r + 1]), "elapsed");
-                stats.step_ms_total += ms;
-            }
+    }
+
+    // Called once the chunk's read-back event has completed.
+    void collect_profile(uint64_t chunk) {
+        if (!cfg.profile) return;
+        std::vector<hipEvent_t>& pe = prof_ev[chunk & 1];
+        for (size_t r = 0; r < cfg.chunk_iters && 2 * r + 1 < pe.size(); ++r) {
+            float ms = 0;
+            hip_check(hipEventElapsedTime(&ms, pe[2 * r], pe[2 * r + 1]), "elapsed");
+            stats.step_ms_total += ms;
         }
     }
 
     // ---- results -------------------------------------------------------------
     void fill_counters(Job& j, const InstState& s) {
         ga_result* o = j.out;
+        const bool dense = j.kind == KIND_GRID4 || j.kind == KIND_GRID8;
         o->expanded = s.expanded;
-        o->stored = j.kind == KIND_GRID4 || j.kind == KIND_GRID8 ? 0 : s.occupied;
+        o->stored = dense ? 0 : s.occupied;
         o->iterations = s.iterations;
         o->retries = j.retries;
         stats.expanded += s.expanded;
@@ -555,6 +579,31 @@ struct ga_engine {
         stats.sift_down += s.sift_down;
         stats.sift_up += s.sift_up;
         stats.pops += s.pops;
+        stats.groups += s.groups;
+        stats.inserted += s.inserted;
+        stats.lookups += s.lookups;
+        stats.algo_bytes += algo_bytes(j.kind, s);
+    }
+
+    // Bytes the step kernel's data-structure operations must move (the
+    // per-unit figures are listed in the design notes):
+    //   per heap-group iteration  2*K*16 (candidate slice written + read) + 8 (heap size)
+    //   per pop                   32 (root + last) + closed-set entry (16 hashed, 8 dense)
+    //   per sift-down level       W*16 (children) + 16 (write)
+    //   per push                  16, per sift-up level 32
+    //   per lookup                8 (atomicMin on g|parent); dense: +K*1 per expansion (cells)
+    //   hashed only               8 per probe (key read) + 8 per new key (CAS)
+    double algo_bytes(int kind, const InstState& s) const {
+        const bool dense = kind == KIND_GRID4 || kind == KIND_GRID8;
+        const double K = branching_of(kind), W = cfg.heap_arity;
+        double b = (double)s.groups * (2.0 * K * 16 + 8);
+        b += (double)s.pops * (32 + (dense ? 8 : 16));
+        b += (double)s.sift_down * (W * 16 + 16);
+        b += (double)s.pushed * 16 + (double)s.sift_up * 32;
+        b += (double)s.lookups * 8;
+        if (dense) b += (double)s.expanded * K;
+        else b += (double)s.probes * 8 + (double)s.occupied * 8;
+        return b;
     }
 
     void finish(Job& j, int status, uint64_t t0) {
@@ -649,9 +698,10 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
     };
 
     auto handle_snapshot = [&](uint64_t chunk) {
+        e->collect_profile(chunk);
         const InstState* snap = (const InstState*)e->h_states[chunk & 1].p;
         std::vector<Job*> done;
-        for (uint32_t s = 0; s < e->cfg.max_instances; ++s) {
+        for (uint32_t s = 0; s < e->chunk_reads[chunk & 1]; ++s) {
             Job* j = e->resident[s];
             if (!j || j->finished || j->first_chunk > chunk) continue;
             const InstState& st = snap[s];
@@ -881,6 +931,7 @@ void ga_config_default(ga_config* cfg) {
     cfg->max_load_pct = 50;
     cfg->pops_per_iter = 2;
     cfg->profile = 0;
+    cfg->rounds = 1;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {

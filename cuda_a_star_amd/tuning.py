@@ -1,0 +1,176 @@
+"""Autotuner for the engine's launch and sizing parameters.
+
+Tuned per mode ("served": many instances per launch; "plain": one instance
+per launch):
+  heaps per instance   by problem kind and grid size class
+  heap arity           lanes per heap group = heap arity (4, 8, 16)
+  chunk iterations     GA* iterations launched between host checks
+  pops per iteration   heap pops tried while entries are stale
+  hash load factor     closed-set load limit before a capacity retry
+
+`python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
+candidates on a seeded sample of the serving workload on device 0 and writes
+the winners to tuning.json next to this file; `load()` reads them (or the
+built-in defaults when no file exists).
+"""
+from __future__ import annotations
+
+import argparse
+import itertools
+import json
+import os
+import sys
+import time
+from dataclasses import asdict, dataclass, field
+from typing import Dict
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNING_FILE = os.path.join(HERE, "tuning.json")
+
+
+@dataclass
+class TuneConfig:
+    puzzle_heaps: int = 256
+    grid_heaps_small: int = 512     # side <= 768
+    grid_heaps_mid: int = 1024      # side <= 1536
+    grid_heaps_big: int = 2048      # larger
+    heap_arity: int = 4
+    chunk_iters: int = 8
+    pops_per_iter: int = 2
+    max_load_pct: int = 50
+
+    def heaps(self, kind: str, side: int) -> int:
+        if kind == "puzzle":
+            return self.puzzle_heaps
+        if side <= 768:
+            return self.grid_heaps_small
+        if side <= 1536:
+            return self.grid_heaps_mid
+        return self.grid_heaps_big
+
+    @property
+    def engine(self) -> Dict[str, int]:
+        return {"heap_arity": self.heap_arity, "chunk_iters": self.chunk_iters,
+                "pops_per_iter": self.pops_per_iter, "max_load_pct": self.max_load_pct}
+
+    def describe(self) -> Dict[str, int]:
+        return asdict(self)
+
+
+DEFAULTS = {
+    "served": TuneConfig(),
+    "plain": TuneConfig(puzzle_heaps=4096, grid_heaps_small=4096, grid_heaps_mid=4096, grid_heaps_big=4096),
+}
+
+
+def load(mode: str = "served") -> TuneConfig:
+    if os.path.exists(TUNING_FILE):
+        with open(TUNING_FILE) as f:
+            data = json.load(f)
+        if mode in data:
+            return TuneConfig(**data[mode])
+    return TuneConfig(**asdict(DEFAULTS[mode]))
+
+
+# ---------------------------------------------------------------------------
+# sweep
+# ---------------------------------------------------------------------------
+def _measure_served(insts, dev, cfg: TuneConfig) -> float:
+    import cuda_a_star_amd as ga
+
+    with ga.Engine(0, budget_bytes=200 << 30, chunk_bytes=8 << 30, **cfg.engine) as e:
+        reqs = [x.request(cells_device=dev.get(i), num_queues=cfg.heaps(x.kind, x.side))
+                for i, x in enumerate(insts)]
+        t = time.perf_counter()
+        sols = e.solve_batch(reqs)
+        dt = time.perf_counter() - t
+    if any(s.status != ga.OK for s in sols):
+        return 0.0
+    return len(sols) / dt
+
+
+def _measure_plain(insts, dev, cfg: TuneConfig) -> float:
+    import cuda_a_star_amd as ga
+
+    with ga.Engine(0, budget_bytes=200 << 30, chunk_bytes=8 << 30, **cfg.engine) as e:
+        t = time.perf_counter()
+        for i, x in enumerate(insts):
+            s = e.solve(x.request(cells_device=dev.get(i), num_queues=cfg.heaps(x.kind, x.side)))
+            if s.status != ga.OK:
+                return 0.0
+        dt = time.perf_counter() - t
+    return len(insts) / dt
+
+
+def sweep(mode: str, n: int, seed: int) -> TuneConfig:
+    import cuda_a_star_amd as ga
+    from cuda_a_star_amd import workloads as wl
+
+    insts = wl.config5(seed, n, 1e9, threads=16)
+    up = ga.Engine(0)
+    dev = {i: up.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
+    measure = _measure_served if mode == "served" else _measure_plain
+    best = load(mode)
+    best_v = measure(insts, dev, best)
+    print(f"[tune] {mode} start {best_v:.2f}/s {asdict(best)}", flush=True)
+    if mode == "served":
+        axes = {
+            "puzzle_heaps": [64, 128, 256, 512, 1024],
+            "grid_heaps_small": [256, 512, 1024, 2048],
+            "grid_heaps_mid": [512, 1024, 2048, 4096],
+            "grid_heaps_big": [1024, 2048, 4096, 8192],
+            "heap_arity": [4, 8],
+            "chunk_iters": [4, 8, 16, 32],
+            "pops_per_iter": [1, 2, 4],
+            "max_load_pct": [50, 70],
+        }
+    else:
+        axes = {
+            "puzzle_heaps": [1024, 2048, 4096, 8192, 16384],
+            "grid_heaps_small": [1024, 2048, 4096, 8192],
+            "grid_heaps_mid": [1024, 2048, 4096, 8192],
+            "grid_heaps_big": [2048, 4096, 8192, 16384],
+            "heap_arity": [4, 8],
+            "chunk_iters": [4, 8, 16, 32],
+            "pops_per_iter": [1, 2, 4],
+            "max_load_pct": [50, 70],
+        }
+    # coordinate descent: one axis at a time, two passes
+    for _ in range(2):
+        for name, values in axes.items():
+            for v in values:
+                if getattr(best, name) == v:
+                    continue
+                cand = TuneConfig(**{**asdict(best), name: v})
+                val = measure(insts, dev, cand)
+                print(f"[tune] {mode} {name}={v}: {val:.2f}/s (best {best_v:.2f})", flush=True)
+                if val > best_v * 1.02:
+                    best, best_v = cand, val
+    for a in dev.values():
+        up.free_device(a)
+    up.close()
+    print(f"[tune] {mode} best {best_v:.2f}/s {asdict(best)}", flush=True)
+    return best
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--mode", choices=["served", "plain", "both"], default="both")
+    ap.add_argument("--n", type=int, default=128, help="requests in the tuning sample")
+    ap.add_argument("--seed", type=int, default=77)
+    args = ap.parse_args(argv)
+    data = {}
+    if os.path.exists(TUNING_FILE):
+        with open(TUNING_FILE) as f:
+            data = json.load(f)
+    modes = ["served", "plain"] if args.mode == "both" else [args.mode]
+    for m in modes:
+        n = args.n if m == "served" else max(8, args.n // 8)
+        data[m] = asdict(sweep(m, n, args.seed))
+        with open(TUNING_FILE, "w") as f:
+            json.dump(data, f, indent=2, sort_keys=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

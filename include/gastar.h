@@ -94,7 +94,8 @@ typedef struct ga_config {
     uint32_t max_load_pct;   /* closed-set load factor limit, percent */
     uint32_t pops_per_iter;  /* heap pops tried per heap per iteration (stale skip) */
     uint32_t profile;        /* time every step with HIP events */
-    uint32_t reserved[3];
+    uint32_t rounds;         /* expansion rounds per heap per launch (local pushes between rounds) */
+    uint32_t reserved[2];
 } ga_config;
 
 /* Engine statistics (kernel timing is collected when cfg.profile != 0). */
@@ -110,6 +111,10 @@ typedef struct ga_engine_stats {
     uint64_t pops;           /* heap pops (including stale ones) */
     uint64_t completed;      /* requests completed */
     uint64_t peak_bytes;     /* peak admitted heap + closed-set bytes */
+    uint64_t groups;         /* heap-group iterations (profile) */
+    uint64_t inserted;       /* successors that lowered a stored g (profile) */
+    uint64_t lookups;        /* closed-set find-or-insert calls (profile) */
+    double algo_bytes;       /* algorithmic bytes of the step kernels (profile) */
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {

@@ -117,6 +117,8 @@ struct InstDesc {
     uint32_t heap_stride;  // physical entries per heap
     uint32_t start_iter;   // value of the global iteration counter at the first step
     uint8_t goal_rc[16];   // puzzle: row << 4 | col of each tile in the goal
+    const void* user;      // user problems: device copy of {start, goal}
+    void* keys;            // user problems with states wider than 8 bytes: key array
 };
 
 // Mutable per-instance state.
@@ -135,6 +137,9 @@ struct InstState {
     unsigned long long probes;
     unsigned long long sift_down;
     unsigned long long sift_up;
+    unsigned long long groups;    // heap-group iterations that ran (live)
+    unsigned long long inserted;  // successors whose atomicMin lowered g
+    unsigned long long lookups;   // closed-set find-or-insert calls (one atomicMin each)
 };
 
 // Workgroup -> (instance, first heap) map of one launch.
@@ -144,10 +149,10 @@ struct WgEntry {
 };
 
 struct StepParams {
-    uint32_t max_pops;    // pops tried per heap per iteration while entries are stale
+    uint32_t max_pops;    // pops tried per expansion round while entries are stale
     uint32_t max_probes;  // closed-set probe limit (overflow guard)
     uint32_t profile;     // count probes and sift levels
-    uint32_t pad;
+    uint32_t rounds;      // expansion rounds per launch (>= 1)
 };
 
 // ---------------------------------------------------------------------------
@@ -294,6 +299,64 @@ __device__ inline bool hash_find(const Slot* table, uint64_t mask, uint64_t key,
 }
 
 // ---------------------------------------------------------------------------
+// closed-set layouts (a problem policy derives from one of them)
+// ---------------------------------------------------------------------------
+// Every policy provides: Key, kDense, load (key and g|parent of a slot),
+// val_ptr, find_or_insert, find, is_goal, start_key, goal_key.
+
+// Hashed set of 64-bit keys: Slot{key, val}, open addressing.
+struct HashedU64Set {
+    using Key = uint64_t;
+    static constexpr bool kDense = false;
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
+        const Slot s = d.table[slot];
+        key = s.key;
+        val = s.val;
+        // A key claimed by a CAS earlier in this launch may still read as empty
+        // through a stale L1 line; keys are write-once, so re-read it at agent
+        // scope.  (A stale val only makes the staleness test conservative.)
+        if (key == kEmptyKey)
+            key = __hip_atomic_load(&d.table[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __device__ static inline uint64_t* val_ptr(const InstDesc& d, uint32_t slot) { return &d.table[slot].val; }
+    __device__ static inline bool find_or_insert(const InstDesc& d, Key key, uint32_t max_probes, uint32_t& slot,
+                                                 bool& isnew, uint32_t& probes) {
+        return hash_find_or_insert(d.table, d.table_mask, key, max_probes, slot, isnew, probes);
+    }
+    __device__ static inline bool find(const InstDesc& d, Key key, uint32_t& slot) {
+        return hash_find(d.table, d.table_mask, key, d.table_mask + 1, slot);
+    }
+    __device__ static inline bool is_goal(const InstDesc& d, Key key) { return key == d.goal; }
+    __device__ static inline Key start_key(const InstDesc& d) { return d.start; }
+    __device__ static inline Key goal_key(const InstDesc& d) { return d.goal; }
+};
+
+// Direct-indexed set over grid cells: slot = cell index, val in d.dense.
+struct DenseCellSet {
+    using Key = uint64_t;
+    static constexpr bool kDense = true;
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
+        key = slot;
+        val = d.dense[slot];
+    }
+    __device__ static inline uint64_t* val_ptr(const InstDesc& d, uint32_t slot) { return d.dense + slot; }
+    __device__ static inline bool find_or_insert(const InstDesc&, Key key, uint32_t, uint32_t& slot, bool& isnew,
+                                                 uint32_t& probes) {
+        slot = (uint32_t)key;
+        isnew = false;  // dense sets do not count occupancy
+        ++probes;
+        return true;
+    }
+    __device__ static inline bool find(const InstDesc&, Key key, uint32_t& slot) {
+        slot = (uint32_t)key;
+        return true;
+    }
+    __device__ static inline bool is_goal(const InstDesc& d, Key key) { return key == d.goal; }
+    __device__ static inline Key start_key(const InstDesc& d) { return d.start; }
+    __device__ static inline Key goal_key(const InstDesc& d) { return d.goal; }
+};
+
+// ---------------------------------------------------------------------------
 // built-in problems
 // ---------------------------------------------------------------------------
 // Sliding puzzle of side S, state = 16 nibbles (tile of cell i in nibble i).
@@ -301,9 +364,8 @@ __device__ inline bool hash_find(const Slot* table, uint64_t mask, uint64_t key,
 // (:84); h = Manhattan distance of the tiles, blank excluded (the reference's
 // manhattan_heuristic, :152-163, counts the blank and is not admissible).
 template <int S>
-struct PuzzleOps {
+struct PuzzleOps : HashedU64Set {
     static constexpr int K = 4;
-    static constexpr bool kDense = false;
     static constexpr int kCells = S * S;
     static constexpr uint64_t kLowBits = (kCells == 16) ? 0x1111111111111111ull
                                                         : ((1ull << (4 * kCells)) - 1) & 0x1111111111111111ull;
@@ -359,9 +421,8 @@ struct PuzzleOps {
 // distance times the minimum cell weight, both admissible (the reference's
 // plain Manhattan with unit diagonals, :219-221, is not).
 template <int CONN>
-struct GridOps {
+struct GridOps : DenseCellSet {
     static constexpr int K = CONN;
-    static constexpr bool kDense = true;
 
     __device__ static inline uint32_t hxy(const InstDesc& d, uint32_t x, uint32_t y) {
         const uint32_t gx = (uint32_t)(d.goal % d.width), gy = (uint32_t)(d.goal / d.width);
@@ -398,54 +459,57 @@ struct GridOps {
 };
 
 // ---------------------------------------------------------------------------
-// closed-set access by problem
-// ---------------------------------------------------------------------------
-template <class Ops>
-__device__ inline void load_slot(const InstDesc& d, uint32_t slot, uint64_t& key, uint64_t& val) {
-    if constexpr (Ops::kDense) {
-        key = slot;
-        val = d.dense[slot];
-    } else {
-        const Slot s = d.table[slot];
-        key = s.key;
-        val = s.val;
-    }
-}
-
-template <class Ops>
-__device__ inline uint64_t* val_ptr(const InstDesc& d, uint32_t slot) {
-    if constexpr (Ops::kDense) return d.dense + slot;
-    else return &d.table[slot].val;
-}
-
-template <class Ops>
-__device__ inline bool find_or_insert(const InstDesc& d, uint64_t key, uint32_t max_probes, uint32_t& slot,
-                                      bool& isnew, uint32_t& probes) {
-    if constexpr (Ops::kDense) {
-        slot = (uint32_t)key;
-        isnew = false;  // dense sets do not count occupancy
-        ++probes;
-        return true;
-    } else {
-        return hash_find_or_insert(d.table, d.table_mask, key, max_probes, slot, isnew, probes);
-    }
-}
-
-// ---------------------------------------------------------------------------
 // one GA* iteration of one heap group
 // ---------------------------------------------------------------------------
+// Pushes the K records held by the group's lanes (record j in lane j % W,
+// register j / W) into the group's own heap; lane 0 performs the pushes.
+template <int W, int KW>
+__device__ inline void push_records(HeapEntry* H, uint32_t& S, const HeapEntry (&rec)[KW], int K, int lane,
+                                    uint32_t best, uint32_t cap, bool& overflow, uint32_t& n_push,
+                                    uint32_t& n_up) {
+#pragma unroll
+    for (int r = 0; r < KW; ++r) {
+#pragma unroll
+        for (int l = 0; l < W; ++l) {
+            if (r * W + l >= K) break;
+            HeapEntry x;
+            x.prio = __shfl(rec[r].prio, l, W);
+            x.slot = __shfl(rec[r].slot, l, W);
+            x.aux = __shfl(rec[r].aux, l, W);
+            if (lane == 0 && x.prio != kNoRecord && prio_f(x.prio) < best) {
+                if (S >= cap) {
+                    overflow = true;
+                } else {
+                    heap_push<W>(H, S, x, n_up);
+                    ++S;
+                    ++n_push;
+                }
+            }
+        }
+    }
+    S = __shfl(S, 0, W);
+}
+
+// One launch = one GA* iteration for this heap group, made of `rounds`
+// expansion rounds.  Successors of rounds 0..rounds-2 go into the group's own
+// heap and can be expanded in the same launch; successors of the last round go
+// to the candidate buffer and are spread over all heaps at the next launch.
+// The termination test stays exact: each heap's top is taken after the
+// distributed pushes and before its first pop, and no other group touches
+// this heap, so the tops of all heaps form one snapshot of the open set.
 template <int W, class Ops>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t it, uint32_t q, bool active,
                                           const StepParams& prm) {
     constexpr int K = Ops::K;
+    constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
     const uint32_t N = d.n_heaps;
 
     // -- termination test on the previous iteration's reduction (a_star.h:260)
-    const uint32_t best = load_relaxed(&st->best);
+    uint32_t best = __builtin_amdgcn_readfirstlane(load_relaxed(&st->best));
     bool live = active;
     if (it > 0) {
-        const uint32_t mt = load_relaxed(&st->min_top[(it - 1) % 3]);
+        const uint32_t mt = __builtin_amdgcn_readfirstlane(load_relaxed(&st->min_top[(it - 1) % 3]));
         if (mt == kInf || (best != kInf && best <= mt)) {
             if (live && lane == 0) {
                 set_status(st, ST_DONE);
@@ -467,8 +531,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
 
     HeapEntry* H = d.heaps + (size_t)q * d.heap_stride + (W - 1);
     uint32_t S = 0;
-    uint32_t n_push = 0, n_up = 0, n_down = 0, n_pops = 0, n_exp = 0, n_gen = 0, n_new = 0, n_probe = 0;
+    uint32_t n_push = 0, n_up = 0, n_down = 0, n_pops = 0, n_exp = 0, n_gen = 0, n_new = 0, n_probe = 0, n_ins = 0,
+             n_look = 0;
     bool overflow_heap = false, overflow_table = false;
+    HeapEntry rec[KW];
+#pragma unroll
+    for (int r = 0; r < KW; ++r) {
+        rec[r].prio = kNoRecord;
+        rec[r].slot = 0;
+        rec[r].aux = 0;
+    }
 
     // -- 1. push the records addressed to this heap (insert_nodes_into_queues,
     //       a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N)
@@ -478,31 +550,13 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             const HeapEntry* cin = d.cand + (size_t)((it - 1) & 1) * N * K;
             const uint32_t base = (q + it) & (N - 1);
 #pragma unroll
-            for (int j0 = 0; j0 < K; j0 += W) {
-                HeapEntry rec;
-                rec.prio = kNoRecord;
-                rec.slot = 0;
-                rec.aux = 0;
-                const int j = j0 + lane;
-                if (j < K) rec = cin[base + (size_t)j * N];
-#pragma unroll
-                for (int l = 0; l < W && j0 + l < K; ++l) {
-                    HeapEntry x;
-                    x.prio = __shfl(rec.prio, l, W);
-                    x.slot = __shfl(rec.slot, l, W);
-                    x.aux = __shfl(rec.aux, l, W);
-                    if (lane == 0 && x.prio != kNoRecord && prio_f(x.prio) < best) {
-                        if (S >= d.heap_cap) {
-                            overflow_heap = true;
-                        } else {
-                            heap_push<W>(H, S, x, n_up);
-                            ++S;
-                            ++n_push;
-                        }
-                    }
-                }
+            for (int r = 0; r < KW; ++r) {
+                const int j = r * W + lane;
+                if (j < K) rec[r] = cin[base + (size_t)j * N];
             }
-            S = __shfl(S, 0, W);
+            push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
+#pragma unroll
+            for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
         }
     }
     group_fence();
@@ -513,75 +567,83 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     top_f = wave_min_u32(top_f);
     if ((threadIdx.x & 63) == 0 && top_f != kInf) atomicMin(&st->min_top[it % 3], top_f);
 
-    // -- 3. pop, skipping entries whose g is no longer the stored one
-    bool have = false;
-    HeapEntry top;
-    top.prio = 0;
-    top.slot = 0;
-    top.aux = 0;
-    uint64_t top_key = 0;
+    // -- 3..6. expansion rounds
     if (live) {
-        for (uint32_t t = 0; t < prm.max_pops && S > 0; ++t) {
-            top = heap_pop<W>(H, S, lane, n_down);
-            group_fence();
-            ++n_pops;
-            uint64_t key, val;
-            load_slot<Ops>(d, top.slot, key, val);
-            if (prio_g(top.prio) > val_g(val)) continue;  // stale
-            have = true;
-            top_key = key;
-            break;
-        }
-        if (lane == 0) d.heap_size[q] = S;
-    }
-
-    // -- 4. goal test at pop (a_star.h:80-83); a node with f >= best cannot improve it
-    const uint32_t g = prio_g(top.prio), f = prio_f(top.prio);
-    bool expand = have;
-    if (have && top_key == d.goal) {
-        if (lane == 0) atomicMin(&st->best, g);
-        expand = false;
-    }
-    if (expand && best != kInf && f >= best) expand = false;
-    if (expand && lane == 0) ++n_exp;
-
-    // -- 5/6. expand, insert into the closed set, write candidate records
-    if (live) {
-        HeapEntry* cout = d.cand + (size_t)(it & 1) * N * K + (size_t)q * K;
-        const uint32_t h = f - g;
+        for (uint32_t round = 0; round < prm.rounds; ++round) {
+            // pop, skipping entries whose g is no longer the stored one
+            bool have = false;
+            HeapEntry top;
+            top.prio = 0;
+            top.slot = 0;
+            top.aux = 0;
+            typename Ops::Key top_key{};
+            for (uint32_t t = 0; t < prm.max_pops && S > 0; ++t) {
+                top = heap_pop<W>(H, S, lane, n_down);
+                group_fence();
+                ++n_pops;
+                typename Ops::Key key;
+                uint64_t val;
+                Ops::load(d, top.slot, key, val);
+                if (prio_g(top.prio) > val_g(val)) continue;  // stale
+                have = true;
+                top_key = key;
+                break;
+            }
+            if (!have) break;
+            // goal test at pop (a_star.h:80-83); a node with f >= best cannot improve it
+            const uint32_t g = prio_g(top.prio), f = prio_f(top.prio);
+            if (Ops::is_goal(d, top_key)) {
+                if (lane == 0) atomicMin(&st->best, g);
+                best = g < best ? g : best;
+                continue;
+            }
+            if (best != kInf && f >= best) continue;
+            if (lane == 0) ++n_exp;
+            // expand, insert into the closed set
+            const uint32_t h = f - g;
 #pragma unroll
-        for (int i0 = 0; i0 < K; i0 += W) {
-            const int i = i0 + lane;
-            if (i < K) {
-                HeapEntry rec;
-                rec.prio = kNoRecord;
-                rec.slot = 0;
-                rec.aux = 0;
-                uint64_t tkey;
+            for (int r = 0; r < KW; ++r) {
+                const int i = r * W + lane;
+                rec[r].prio = kNoRecord;
+                typename Ops::Key tkey;
                 uint32_t taux, cost, th;
-                if (expand && Ops::succ(d, top_key, top.aux, h, i, tkey, taux, cost, th)) {
+                if (i < K && Ops::succ(d, top_key, top.aux, h, i, tkey, taux, cost, th)) {
                     ++n_gen;
                     const uint32_t g2 = g + cost, f2 = g2 + th;
                     if (f2 < best) {
                         uint32_t slot;
                         bool isnew;
-                        if (find_or_insert<Ops>(d, tkey, prm.max_probes, slot, isnew, n_probe)) {
+                        if (Ops::find_or_insert(d, tkey, prm.max_probes, slot, isnew, n_probe)) {
+                            ++n_look;
                             n_new += isnew ? 1u : 0u;
                             const uint64_t mine = ((uint64_t)g2 << 32) | top.slot;
-                            const uint64_t old = atomicMin((unsigned long long*)val_ptr<Ops>(d, slot),
+                            const uint64_t old = atomicMin((unsigned long long*)Ops::val_ptr(d, slot),
                                                            (unsigned long long)mine);
                             if (g2 < val_g(old)) {
-                                rec.prio = make_prio(f2, g2);
-                                rec.slot = slot;
-                                rec.aux = taux;
+                                ++n_ins;
+                                rec[r].prio = make_prio(f2, g2);
+                                rec[r].slot = slot;
+                                rec[r].aux = taux;
                             }
                         } else {
                             overflow_table = true;
                         }
                     }
                 }
-                cout[i] = rec;
             }
+            if (round + 1 == prm.rounds) break;  // last round: records go to the candidate buffer
+            push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
+#pragma unroll
+            for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
+            group_fence();
+        }
+        if (lane == 0) d.heap_size[q] = S;
+        // candidate slice of this heap: K records, written every iteration
+        HeapEntry* cout = d.cand + (size_t)(it & 1) * N * K + (size_t)q * K;
+#pragma unroll
+        for (int r = 0; r < KW; ++r) {
+            const int i = r * W + lane;
+            if (i < K) cout[i] = rec[r];
         }
     }
     if (overflow_heap) set_status(st, ST_OVERFLOW_HEAP);
@@ -603,7 +665,13 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         const uint32_t w_probe = wave_sum_u32(n_probe);
         const uint32_t w_down = wave_sum_u32(lane == 0 ? n_down : 0u);
         const uint32_t w_up = wave_sum_u32(n_up);
+        const uint32_t w_groups = wave_sum_u32((live && lane == 0) ? 1u : 0u);
+        const uint32_t w_ins = wave_sum_u32(n_ins);
+        const uint32_t w_look = wave_sum_u32(n_look);
         if (wave_lead) {
+            if (w_look) atomicAdd(&st->lookups, (unsigned long long)w_look);
+            if (w_groups) atomicAdd(&st->groups, (unsigned long long)w_groups);
+            if (w_ins) atomicAdd(&st->inserted, (unsigned long long)w_ins);
             if (w_push) atomicAdd(&st->pushed, (unsigned long long)w_push);
             if (w_pops) atomicAdd(&st->pops, (unsigned long long)w_pops);
             if (w_probe) atomicAdd(&st->probes, (unsigned long long)w_probe);
@@ -657,17 +725,19 @@ __device__ inline bool seed_instance(const InstDesc& d, InstState* st, int W) {
     z.pad = 0;
     z.occupied = Ops::kDense ? 0 : 1;
     z.expanded = z.generated = z.pushed = z.pops = z.probes = z.sift_down = z.sift_up = 0;
+    z.groups = z.inserted = z.lookups = 0;
     uint32_t slot, probes = 0;
     bool isnew;
-    if (!find_or_insert<Ops>(d, d.start, 1u << 20, slot, isnew, probes)) {
+    const typename Ops::Key start = Ops::start_key(d);
+    if (!Ops::find_or_insert(d, start, 1u << 20, slot, isnew, probes)) {
         z.status = ST_OVERFLOW_TABLE;
         *st = z;
         return false;
     }
-    *val_ptr<Ops>(d, slot) = kNone;  // g = 0, no parent
-    const uint32_t aux = Ops::aux_of(d, d.start);
+    *Ops::val_ptr(d, slot) = kNone;  // g = 0, no parent
+    const uint32_t aux = Ops::aux_of(d, start);
     HeapEntry e;
-    e.prio = make_prio(Ops::heuristic(d, d.start, aux), 0);
+    e.prio = make_prio(Ops::heuristic(d, start, aux), 0);
     e.slot = slot;
     e.aux = aux;
     d.heaps[W - 1] = e;
@@ -679,18 +749,15 @@ __device__ inline bool seed_instance(const InstDesc& d, InstState* st, int W) {
 // Walks the parent links from the goal; writes states goal..start and returns
 // their number, or kNone if cap is exceeded / the goal is not stored.
 template <class Ops>
-__device__ inline uint32_t walk_path(const InstDesc& d, uint64_t* out, uint32_t cap) {
+__device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, uint32_t cap) {
     uint32_t slot;
-    if constexpr (Ops::kDense) {
-        slot = (uint32_t)d.goal;
-    } else {
-        if (!hash_find(d.table, d.table_mask, d.goal, d.table_mask + 1, slot)) return kNone;
-    }
+    if (!Ops::find(d, Ops::goal_key(d), slot)) return kNone;
     uint32_t n = 0;
     while (slot != kNone) {
         if (n >= cap) return kNone;
-        uint64_t key, val;
-        load_slot<Ops>(d, slot, key, val);
+        typename Ops::Key key;
+        uint64_t val;
+        Ops::load(d, slot, key, val);
         out[n++] = key;
         slot = val_parent(val);
     }

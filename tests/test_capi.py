@@ -1,0 +1,122 @@
+"""The C ABI library loads on a CPU-only host, exports every symbol that
+include/gastar.h declares, and its struct layouts match the ctypes mirror.
+Only host-side entry points are called here (generators, status strings)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import cuda_a_star_amd as ga
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "gastar.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(ga_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declarations_match_python_mirror():
+    assert declared_functions() == sorted(ga.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ga.lib()
+    missing = [f for f in declared_functions() if not hasattr(L, f)]
+    assert not missing, missing
+
+
+def test_status_strings():
+    for code, name in [(0, "GA_OK"), (1, "GA_NO_PATH"), (-1, "GA_E_INVALID"), (-2, "GA_E_NOMEM"),
+                       (-3, "GA_E_BUDGET"), (-4, "GA_E_HIP"), (-5, "GA_E_TIMEOUT"), (-6, "GA_E_OVERFLOW")]:
+        assert ga.status_str(code) == name
+
+
+LAYOUT_PROGRAM = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "gastar.h"
+#define F(T, m) printf(#T "." #m " %zu\n", offsetof(T, m));
+#define S(T) printf(#T " %zu\n", sizeof(T));
+int main(void) {
+  S(ga_request) F(ga_request, start) F(ga_request, cells) F(ga_request, cells_device)
+  F(ga_request, num_queues) F(ga_request, table_slots) F(ga_request, user_tag)
+  S(ga_result) F(ga_result, expanded) F(ga_result, path) F(ga_result, latency_ns) F(ga_result, user_tag)
+  S(ga_config) F(ga_config, max_instances) F(ga_config, profile) F(ga_config, rounds)
+  S(ga_engine_stats) F(ga_engine_stats, step_ms_total) F(ga_engine_stats, peak_bytes)
+  S(ga_serve_stats) F(ga_serve_stats, solves_per_sec) F(ga_serve_stats, max_inflight)
+  return 0;
+}
+"""
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "layout.c"
+    exe = tmp_path / "layout"
+    src.write_text(LAYOUT_PROGRAM)
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    mirror = {"ga_request": ga.Request, "ga_result": ga.Result, "ga_config": ga.Config,
+              "ga_engine_stats": ga.EngineStats, "ga_serve_stats": ga.ServeStats}
+    for line in filter(None, out):
+        name, val = line.split()
+        if "." in name:
+            t, m = name.split(".")
+            assert getattr(mirror[t], m).offset == int(val), line
+        else:
+            assert ctypes.sizeof(mirror[name]) == int(val), line
+
+
+def test_splitmix64_known_values():
+    # splitmix64 from state 0: the published first outputs of the generator
+    v1, s = ga.splitmix64(0)
+    v2, s = ga.splitmix64(s)
+    assert v1 == 0xE220A8397B1DCDAF
+    assert v2 == 0x6E789E6AA1B965F4
+
+
+def test_puzzle_generator_is_deterministic_and_solvable():
+    for side in (3, 4):
+        goal = ga.puzzle_goal(side)
+        for seed in range(20):
+            a = ga.gen_puzzle(side, 40, seed)
+            assert a == ga.gen_puzzle(side, 40, seed)
+            assert ga.puzzle_solvable(side, a, goal)
+        assert ga.gen_puzzle(side, 0, 7) == goal
+
+
+def test_puzzle_solvable_parity():
+    g = ga.puzzle_goal(4)
+    t = ga.state_to_board(g, 4)
+    t[0], t[1] = t[1], t[0]
+    assert not ga.puzzle_solvable(4, ga.board_to_state(t), g)
+    t[2], t[3] = t[3], t[2]
+    assert ga.puzzle_solvable(4, ga.board_to_state(t), g)
+
+
+def test_grid_generator_properties():
+    cells, draws = ga.gen_grid(200, 150, 0.2, 42)
+    again, _ = ga.gen_grid(200, 150, 0.2, 42)
+    assert np.array_equal(cells, again) and draws >= 1
+    assert cells[0] != 0 and cells[-1] != 0
+    open_w = cells[cells != 0]
+    assert open_w.min() >= 1 and open_w.max() <= 9
+    assert abs((cells == 0).mean() - 0.2) < 0.02
+    with pytest.raises(ValueError):
+        ga.gen_grid(0, 10, 0.2, 1)
+
+
+def test_engine_refuses_without_gpu_or_builds():
+    """On a host without a GPU, engine creation reports GA_E_HIP instead of crashing."""
+    import torch
+
+    if torch.cuda.is_available():
+        pytest.skip("GPU present")
+    with pytest.raises(ga.GAStarError) as ei:
+        ga.Engine(0)
+    assert ei.value.code == ga.E_HIP

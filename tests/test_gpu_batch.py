@@ -1,0 +1,144 @@
+"""Batched GA* (many instances per launch) and serving (continuous batching)
+on the GPU.  Every cost must equal the CPU oracle's (tolerance 0), every path
+must replay, and every result must be the same alone and in the mix."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import cuda_a_star_amd as ga
+import oracle
+from cuda_a_star_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def oracle_cost(x):
+    if x.kind == "puzzle":
+        rc, res, _ = oracle.astar_puzzle(x.side, x.start, x.goal, want_path=False)
+        return res.cost if rc == oracle.OK else None
+    return oracle.dial_grid(x.cells, x.side, x.side, x.start, x.goal, 4)
+
+
+def replay_ok(x, sol):
+    if x.kind == "puzzle":
+        return oracle.replay_puzzle(x.side, sol.path, x.start, x.goal) == sol.cost
+    return oracle.replay_grid(x.cells, x.side, x.side, 4, sol.path, x.start, x.goal) == sol.cost
+
+
+def small_mix(seed=3, n_puz=24, n_grid=12):
+    st = wl.Stream(seed)
+    out = []
+    for i in range(n_puz):
+        out.append(wl.puzzle15(wl.instance_seed(seed, i), depth=st.uniform_int(20, 60), num_queues=0,
+                               table_slots=0))
+        out[-1].num_queues = (64, 256, 1024)[i % 3]
+    for i in range(n_grid):
+        side = (64, 128, 200, 256)[i % 4]
+        g = wl.grid(side, 0.1 + 0.15 * st.uniform(), wl.instance_seed(seed + 1, i))
+        g.num_queues = (64, 512, 2048)[i % 3]
+        out.append(g)
+    return out
+
+
+def test_batch_small_mix_matches_oracle_and_alone(engine):
+    insts = small_mix()
+    sols = engine.solve_batch([x.request() for x in insts])
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK, ga.status_str(sol.status)
+        assert sol.cost == oracle_cost(x)
+        assert replay_ok(x, sol)
+    for x, sol in list(zip(insts, sols))[::5]:  # alone == in the mix
+        alone = engine.solve(x.request())
+        assert alone.status == ga.OK and alone.cost == sol.cost
+
+
+def test_batch_config4_full_matches_golden(engine):
+    # 1024 mixed instances in one batch; costs against the committed oracle fixture
+    gold = json.load(open(os.path.join(GOLDEN, "config4.json")))["instances"]
+    insts = wl.config4(4, 1024, threads=16)
+    cfg = __import__("cuda_a_star_amd.tuning", fromlist=["load"]).load()
+    reqs = [x.request(num_queues=cfg.heaps(x.kind, x.side)) for x in insts]
+    sols = engine.solve_batch(reqs)
+    bad = [(i, s.status, s.cost, g["cost"]) for i, (s, g) in enumerate(zip(sols, gold))
+           if s.status != ga.OK or s.cost != g["cost"]]
+    assert not bad, bad[:10]
+    for i in range(0, 1024, 97):  # replay a spread of paths
+        assert replay_ok(insts[i], sols[i])
+
+
+def test_batch_forced_retries(engine):
+    insts = small_mix(seed=9, n_puz=8, n_grid=4)
+    reqs = []
+    for x in insts:
+        r = x.request(heap_capacity=16)
+        if x.kind == "puzzle":
+            r.table_slots = 1 << 12
+        reqs.append(r)
+    sols = engine.solve_batch(reqs)
+    assert any(s.retries > 0 for s in sols)
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK and sol.cost == oracle_cost(x)
+
+
+def test_batch_handles_invalid_and_unsolvable(engine):
+    g = ga.puzzle_goal(4)
+    t = ga.state_to_board(g, 4)
+    t[0], t[1] = t[1], t[0]
+    bad_parity = ga.puzzle_request(4, ga.board_to_state(t), g, 256)
+    invalid = ga.puzzle_request(4, 0x1111111111111111, g, 256)
+    good = wl.puzzle15(1, depth=30, num_queues=256, table_slots=0)
+    sols = engine.solve_batch([bad_parity, invalid, good.request()])
+    assert sols[0].status == ga.NO_PATH
+    assert sols[1].status == ga.E_INVALID
+    assert sols[2].status == ga.OK and sols[2].cost == oracle_cost(good)
+
+
+def serving_stream(n=64, qps=400.0, seed=21):
+    st = wl.Stream(seed)
+    out, t = [], 0.0
+    for i in range(n):
+        if st.uniform() < 0.6:
+            x = wl.puzzle15(wl.instance_seed(seed, i), depth=st.uniform_int(30, 50), num_queues=256, table_slots=0)
+        else:
+            x = wl.grid(st.uniform_int(128, 512), 0.2, wl.instance_seed(seed, i), num_queues=512)
+        t += -np.log(1.0 - st.uniform()) / qps
+        x.arrival_ns = int(t * 1e9)
+        out.append(x)
+    return out
+
+
+def test_serve_stream_matches_alone(engine):
+    insts = serving_stream()
+    sols, ss = engine.serve([x.request() for x in insts], [x.arrival_ns for x in insts])
+    assert ss.completed == len(insts) and ss.failed == 0
+    assert ss.max_inflight > 1          # requests overlapped on the device
+    assert 0 < ss.p50_ms <= ss.p99_ms
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK
+        assert sol.cost == oracle_cost(x)
+        assert replay_ok(x, sol)
+        assert sol.latency_ns > 0
+    for x, sol in list(zip(insts, sols))[::8]:
+        alone = engine.solve(x.request())
+        assert alone.cost == sol.cost
+
+
+def test_serve_tight_budget_admits_in_turn():
+    insts = serving_stream(n=24, qps=5000.0, seed=5)
+    with ga.Engine(0, budget_bytes=512 << 20, chunk_bytes=256 << 20, max_iterations=2_000_000) as e:
+        sols, ss = e.serve([x.request() for x in insts], [x.arrival_ns for x in insts])
+        assert ss.completed == len(insts)
+        st = e.stats()
+        assert st.peak_bytes <= 768 << 20
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK and sol.cost == oracle_cost(x)
+
+
+def test_serve_request_over_budget_is_rejected():
+    x = wl.grid(512, 0.2, 1, num_queues=256)
+    with ga.Engine(0, budget_bytes=1 << 20, chunk_bytes=1 << 20) as e:
+        sols, ss = e.serve([x.request()], [0])
+    assert sols[0].status == ga.E_BUDGET

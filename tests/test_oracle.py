@@ -44,7 +44,7 @@ def test_astar_equals_bfs_on_sample():
     _, depth, _ = oracle.bfs_puzzle8(GOAL8)
     rng = np.random.default_rng(1234)
     checked = 0
-    for perm in rng.permuted(np.tile(np.arange(9), (4000, 1)), axis=1):
+    for perm in rng.permuted(np.tile(np.arange(9), (1500, 1)), axis=1):
         s = ga.board_to_state(perm)
         d = depth[oracle.perm_rank9(s)]
         rc, res, path = oracle.astar_puzzle(3, s, GOAL8)
@@ -54,7 +54,7 @@ def test_astar_equals_bfs_on_sample():
         assert rc == oracle.OK and res.cost == d
         assert oracle.replay_puzzle(3, path, s, GOAL8) == d
         checked += 1
-    assert checked > 1500
+    assert checked > 600
 
 
 def test_astar_deepest_states():
