@@ -327,6 +327,7 @@ struct ga_engine {
 
     ~ga_engine() {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
         if (path_stream) (void)hipStreamSynchronize(path_stream);
         for (auto& v : prof_ev)
             for (auto ev : v) (void)hipEventDestroy(ev);
@@ -507,15 +508,43 @@ struct ga_engine {
             }
             map_cap = (uint32_t)pow2_ceil(n_wg);
             hip_check(hipMalloc(&d_map, sizeof(WgEntry) * map_cap), "hipMalloc map");
+            drop_graph();  // the captured launches refer to the old map and grid size
         }
+        // graph mode launches map_cap workgroups: pad with entries that exit at once
+        const uint32_t n_up = cfg.use_graph ? map_cap : n_wg;
+        map.resize(n_up, WgEntry{kNone, 0});
         int b = map_buf;
         map_buf ^= 1;
         hip_check(hipEventSynchronize(ev_map[b]), "map event");
-        h_map[b].ensure(sizeof(WgEntry) * n_wg);
-        memcpy(h_map[b].p, map.data(), sizeof(WgEntry) * n_wg);
-        hip_check(hipMemcpyAsync(d_map, h_map[b].p, sizeof(WgEntry) * n_wg, hipMemcpyHostToDevice, stream),
+        h_map[b].ensure(sizeof(WgEntry) * n_up);
+        memcpy(h_map[b].p, map.data(), sizeof(WgEntry) * n_up);
+        hip_check(hipMemcpyAsync(d_map, h_map[b].p, sizeof(WgEntry) * n_up, hipMemcpyHostToDevice, stream),
                   "upload map");
         hip_check(hipEventRecord(ev_map[b], stream), "record map");
+    }
+
+    // ---- graph mode: one captured graph per launch grid --------------------------
+    hipGraphExec_t graph_exec = nullptr;
+    void drop_graph() {
+        if (graph_exec) {
+            hip_check(hipStreamSynchronize(stream), "sync");
+            (void)hipGraphExecDestroy(graph_exec);
+            graph_exec = nullptr;
+        }
+    }
+
+    void enqueue_steps(uint32_t grid, const StepParams& prm, std::vector<hipEvent_t>* pe) {
+        const uint32_t R = cfg.chunk_iters;
+        for (uint32_t r = 0; r < R; ++r) {
+            if (pe) hip_check(hipEventRecord((*pe)[2 * r], stream), "event");
+            switch (cfg.heap_arity) {
+                case 8: step_kernel<8><<<grid, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+                case 16: step_kernel<16><<<grid, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+                default: step_kernel<4><<<grid, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+            }
+            if (pe) hip_check(hipEventRecord((*pe)[2 * r + 1], stream), "event");
+        }
+        advance_kernel<<<1, 64, 0, stream>>>(d_iter, R);
     }
 
     void launch_chunk() {
@@ -530,16 +559,21 @@ struct ga_engine {
                 pe.push_back(ev);
             }
         }
-        for (uint32_t r = 0; r < R; ++r) {
-            if (cfg.profile) hip_check(hipEventRecord(pe[2 * r], stream), "event");
-            switch (cfg.heap_arity) {
-                case 8: step_kernel<8><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
-                case 16: step_kernel<16><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
-                default: step_kernel<4><<<n_wg, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+        if (cfg.use_graph) {
+            if (!graph_exec) {
+                hipGraph_t g;
+                hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                enqueue_steps(map_cap, prm, nullptr);
+                hip_check(hipStreamEndCapture(stream, &g), "end capture");
+                hip_check(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0), "instantiate");
+                (void)hipGraphDestroy(g);
             }
-            if (cfg.profile) hip_check(hipEventRecord(pe[2 * r + 1], stream), "event");
+            if (cfg.profile) hip_check(hipEventRecord(pe[0], stream), "event");
+            hip_check(hipGraphLaunch(graph_exec, stream), "graph launch");
+            if (cfg.profile) hip_check(hipEventRecord(pe[1], stream), "event");
+        } else {
+            enqueue_steps(n_wg, prm, cfg.profile ? &pe : nullptr);
         }
-        advance_kernel<<<1, 64, 0, stream>>>(d_iter, R);
         hip_check(hipGetLastError(), "step launch");
         host_base += R;
         stats.step_launches += R;
@@ -557,7 +591,9 @@ struct ga_engine {
     void collect_profile(uint64_t chunk) {
         if (!cfg.profile) return;
         std::vector<hipEvent_t>& pe = prof_ev[chunk & 1];
-        for (size_t r = 0; r < cfg.chunk_iters && 2 * r + 1 < pe.size(); ++r) {
+        // graph mode times the whole chunk (R steps + the counter advance)
+        const size_t pairs = cfg.use_graph ? 1 : cfg.chunk_iters;
+        for (size_t r = 0; r < pairs && 2 * r + 1 < pe.size(); ++r) {
             float ms = 0;
             hip_check(hipEventElapsedTime(&ms, pe[2 * r], pe[2 * r + 1]), "elapsed");
             stats.step_ms_total += ms;

@@ -7,6 +7,8 @@ per launch):
   chunk iterations     GA* iterations launched between host checks
   pops per iteration   heap pops tried while entries are stale
   hash load factor     closed-set load limit before a capacity retry
+  expansion rounds     heap pops per launch with local pushes in between
+  graph replay         chunks replayed from a captured hipGraph or launched one by one
 
 `python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
 candidates on a seeded sample of the serving workload on device 0 and writes
@@ -38,6 +40,8 @@ class TuneConfig:
     chunk_iters: int = 8
     pops_per_iter: int = 2
     max_load_pct: int = 50
+    rounds: int = 1
+    use_graph: int = 0
 
     def heaps(self, kind: str, side: int) -> int:
         if kind == "puzzle":
@@ -51,7 +55,8 @@ class TuneConfig:
     @property
     def engine(self) -> Dict[str, int]:
         return {"heap_arity": self.heap_arity, "chunk_iters": self.chunk_iters,
-                "pops_per_iter": self.pops_per_iter, "max_load_pct": self.max_load_pct}
+                "pops_per_iter": self.pops_per_iter, "max_load_pct": self.max_load_pct,
+                "rounds": self.rounds, "use_graph": self.use_graph}
 
     def describe(self) -> Dict[str, int]:
         return asdict(self)
@@ -123,6 +128,8 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "chunk_iters": [4, 8, 16, 32],
             "pops_per_iter": [1, 2, 4],
             "max_load_pct": [50, 70],
+            "rounds": [1, 2, 4, 8],
+            "use_graph": [0, 1],
         }
     else:
         axes = {
@@ -134,6 +141,8 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "chunk_iters": [4, 8, 16, 32],
             "pops_per_iter": [1, 2, 4],
             "max_load_pct": [50, 70],
+            "rounds": [1, 2, 4, 8],
+            "use_graph": [0, 1],
         }
     # coordinate descent: one axis at a time, two passes
     for _ in range(2):
@@ -158,6 +167,7 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["served", "plain", "both"], default="both")
     ap.add_argument("--n", type=int, default=128, help="requests in the tuning sample")
     ap.add_argument("--seed", type=int, default=77)
+    ap.add_argument("--out", default=TUNING_FILE, help="where to write the winners")
     args = ap.parse_args(argv)
     data = {}
     if os.path.exists(TUNING_FILE):
@@ -167,7 +177,7 @@ def main(argv=None) -> int:
     for m in modes:
         n = args.n if m == "served" else max(8, args.n // 8)
         data[m] = asdict(sweep(m, n, args.seed))
-        with open(TUNING_FILE, "w") as f:
+        with open(args.out, "w") as f:
             json.dump(data, f, indent=2, sort_keys=True)
     return 0
 

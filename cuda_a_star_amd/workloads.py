@@ -41,14 +41,15 @@ class Instance:
     connectivity: int = 4
     arrival_ns: int = 0
 
-    def request(self, cells_device: Optional[int] = None, **kw) -> Request:
+    def request(self, cells_device: Optional[int] = None, num_queues: Optional[int] = None, **kw) -> Request:
+        nq = self.num_queues if num_queues is None else num_queues
         if self.kind == "puzzle":
-            r = puzzle_request(self.side, self.start, self.goal, self.num_queues, **kw)
-            if self.table_slots:
+            r = puzzle_request(self.side, self.start, self.goal, nq, **kw)
+            if self.table_slots and "table_slots" not in kw:
                 r.table_slots = self.table_slots
             return r
         return grid_request(self.cells, self.side, self.side, self.start, self.goal, self.connectivity,
-                            self.num_queues, cells_device=cells_device, **kw)
+                            nq, cells_device=cells_device, **kw)
 
 
 class Stream:

@@ -120,3 +120,20 @@ def test_engine_refuses_without_gpu_or_builds():
     with pytest.raises(ga.GAStarError) as ei:
         ga.Engine(0)
     assert ei.value.code == ga.E_HIP
+
+
+def test_workload_requests_build_on_cpu():
+    from cuda_a_star_amd import tuning
+    from cuda_a_star_amd import workloads as wl
+
+    cfg = tuning.load()
+    insts = wl.config5(5, 24, 1500.0, threads=4)
+    for x in insts:
+        r = x.request(num_queues=cfg.heaps(x.kind, x.side), cells_device=None)
+        assert r.num_queues == cfg.heaps(x.kind, x.side)
+        assert r.kind in (ga.PUZZLE, ga.GRID)
+        r2 = x.request(heap_capacity=16)
+        assert r2.heap_capacity == 16
+    assert [x.arrival_ns for x in insts] == sorted(x.arrival_ns for x in insts)
+    p = wl.puzzle15(3, num_queues=512, table_slots=1 << 20)
+    assert p.request().table_slots == 1 << 20 and p.request(table_slots=1 << 12).table_slots == 1 << 12

@@ -142,3 +142,19 @@ def test_serve_request_over_budget_is_rejected():
     with ga.Engine(0, budget_bytes=1 << 20, chunk_bytes=1 << 20) as e:
         sols, ss = e.serve([x.request()], [0])
     assert sols[0].status == ga.E_BUDGET
+
+
+@pytest.mark.parametrize("arity,rounds,graph,pops", [(4, 1, 1, 1), (8, 1, 0, 2), (16, 2, 0, 1), (4, 4, 0, 2),
+                                                     (8, 8, 1, 4), (4, 2, 1, 2)])
+def test_engine_variants_match_oracle(arity, rounds, graph, pops):
+    """Every tunable combination the autotuner may pick keeps costs exact."""
+    insts = small_mix(seed=31 + arity + rounds, n_puz=12, n_grid=8)
+    with ga.Engine(0, heap_arity=arity, rounds=rounds, use_graph=graph, pops_per_iter=pops,
+                   max_iterations=2_000_000) as e:
+        sols = e.solve_batch([x.request() for x in insts])
+        alone = [e.solve(insts[i].request()) for i in (0, len(insts) - 1)]
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK, ga.status_str(sol.status)
+        assert sol.cost == oracle_cost(x)
+        assert replay_ok(x, sol)
+    assert alone[0].cost == sols[0].cost and alone[1].cost == sols[-1].cost
