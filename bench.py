@@ -7,9 +7,11 @@ U{30..70}), 40 % grids (side U{512..4096}, weights 1..9, 20 % walls); every
 grid resident in HBM before the timed region; admission budget 200 GB of heap
 and closed-set bytes.
 
-A "step" is one stream segment of --requests requests served with continuous
-batching (ga_serve_run).  W warm-up segments, then exactly K timed segments.
-value = completed solves / wall seconds over the K timed segments.
+A "step" is one stream segment of --requests requests.  The W warm-up
+segments are served first (untimed); then the K timed segments are served as
+one continuous arrival stream with continuous batching (ga_serve_run), so no
+segment drains before the next one arrives.
+value = completed solves / wall seconds of the timed stream.
 
 Also measured (outside the timed region): the same requests solved one at a
 time by plain GA* (ga_solve), and the CPU oracle on the host cores.
@@ -148,45 +150,41 @@ def main() -> int:
         nq = served_cfg.heaps(x.kind, x.side)
         return x.request(cells_device=dev.get(i), num_queues=nq)
 
-    def segment(s):
-        idx = list(range(s * M, (s + 1) * M))
+    def segment(first, count):
+        idx = list(range(first, first + count))
         t0 = insts[idx[0]].arrival_ns
         return idx, [served_request(i) for i in idx], [insts[i].arrival_ns - t0 for i in idx]
 
-    # ---- warm-up segments (untimed) ----
-    for s in range(W):
-        idx, reqs, arr = segment(s)
+    # ---- warm-up (untimed): the first W*M requests of the stream ----
+    if W:
+        idx, reqs, arr = segment(0, W * M)
         sols, ss = e.serve(reqs, arr)
-        log(f"[bench] warmup {s}: {ss.completed}/{len(reqs)} in {ss.wall_ns / 1e9:.2f}s "
+        log(f"[bench] warmup: {ss.completed}/{len(reqs)} in {ss.wall_ns / 1e9:.2f}s "
             f"({ss.solves_per_sec:.1f}/s, p50 {ss.p50_ms:.1f} ms, p99 {ss.p99_ms:.1f} ms)")
 
-    # ---- timed segments ----
+    # ---- timed: K steps of M requests, served as one continuous arrival stream ----
     e.reset_stats()
     lat, ok, failed, served_costs = [], 0, 0, {}
-    walls = []
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
     except Exception:  # torch is only used for the barrier
         sync = lambda: None
+    idx, reqs, arr = segment(W * M, K * M)
     sync()
     t_start = time.perf_counter()
-    for s in range(W, W + K):
-        idx, reqs, arr = segment(s)
-        ts = time.perf_counter()
-        sols, ss = e.serve(reqs, arr)
-        walls.append(time.perf_counter() - ts)
-        for i, sol in zip(idx, sols):
-            if sol.status == ga.OK:
-                ok += 1
-                lat.append(sol.latency_ns / 1e6)
-                served_costs[i] = sol.cost
-            else:
-                failed += 1
-        log(f"[bench] step {s - W}: {ss.completed}/{len(reqs)} in {walls[-1]:.2f}s, "
-            f"p50 {ss.p50_ms:.1f} ms, p99 {ss.p99_ms:.1f} ms, inflight max {ss.max_inflight}")
+    sols, ss = e.serve(reqs, arr)
     sync()
     t_total = time.perf_counter() - t_start
+    for i, sol in zip(idx, sols):
+        if sol.status == ga.OK:
+            ok += 1
+            lat.append(sol.latency_ns / 1e6)
+            served_costs[i] = sol.cost
+        else:
+            failed += 1
+    log(f"[bench] timed: {ss.completed}/{len(reqs)} in {t_total:.2f}s, p50 {ss.p50_ms:.1f} ms, "
+        f"p99 {ss.p99_ms:.1f} ms, inflight max {ss.max_inflight}")
     st = e.stats()
     served_sps = ok / t_total
 

@@ -120,7 +120,7 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
     print(f"[tune] {mode} start {best_v:.2f}/s {asdict(best)}", flush=True)
     if mode == "served":
         axes = {
-            "puzzle_heaps": [64, 128, 256, 512, 1024],
+            "puzzle_heaps": [256, 512, 1024, 2048, 4096],
             "grid_heaps_small": [256, 512, 1024, 2048],
             "grid_heaps_mid": [512, 1024, 2048, 4096],
             "grid_heaps_big": [1024, 2048, 4096, 8192],

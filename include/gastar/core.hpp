@@ -198,23 +198,32 @@ __device__ inline void set_status(InstState* st, uint32_t s) {
 // W-ary heap, logical index j at physical j + W - 1 so that the W children
 // W*j+1 .. W*j+W of node j start on a W-entry boundary (W*16 B aligned).
 // ---------------------------------------------------------------------------
-template <int W>
-__device__ inline HeapEntry heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& levels) {
-    HeapEntry top = H[0];
+// Removes the minimum.  The root, the last entry and the root's W children
+// are read in one round trip; the popped state's closed-set entry (key and
+// g|parent, via `load`) is then read while the sift-down runs.
+template <int W, class Load, class Key>
+__device__ inline HeapEntry heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& levels, Load load, Key& key,
+                                     uint64_t& val) {
+    const HeapEntry top = H[0];
     const uint32_t n = S - 1;
     S = n;
+    HeapEntry last;
+    last.prio = ~0ull;
+    last.slot = 0;
+    last.aux = 0;
+    if (n > 0) last = H[n];
+    uint32_t idx = 1 + (uint32_t)lane;
+    HeapEntry e;
+    e.prio = ~0ull;
+    e.slot = 0;
+    e.aux = 0;
+    if (idx < n) e = H[idx];
+    load(top.slot, key, val);
     if (n == 0) return top;
-    const HeapEntry last = H[n];
     uint32_t j = 0;
     for (;;) {
-        const uint32_t c = W * j + 1;
-        if (c >= n) break;
-        const uint32_t idx = c + (uint32_t)lane;
-        HeapEntry e;
-        e.prio = ~0ull;
-        e.slot = 0;
-        e.aux = 0;
-        if (idx < n) e = H[idx];
+        // e holds child W*j + 1 + lane of node j (or +inf past the end)
+        if (W * j + 1 >= n) break;
         uint64_t mp = e.prio;
         uint32_t mi = idx;
 #pragma unroll
@@ -230,6 +239,9 @@ __device__ inline HeapEntry heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32
         if (idx == mi) H[j] = e;
         j = mi;
         ++levels;
+        idx = W * j + 1 + (uint32_t)lane;
+        e.prio = ~0ull;
+        if (idx < n) e = H[idx];
     }
     if (lane == 0) H[j] = last;
     return top;
@@ -260,23 +272,18 @@ __device__ inline void heap_push(HeapEntry* H, uint32_t S, const HeapEntry& x, u
 // and a stale "empty" read is corrected by the CAS.
 __device__ inline bool hash_find_or_insert(Slot* table, uint64_t mask, uint64_t key, uint32_t max_probes,
                                            uint32_t& slot, bool& isnew, uint32_t& probes) {
+    // Every probe is one CAS(empty -> key): it claims a free slot, or returns the
+    // key already there (found, or occupied by another key: probe on).  One
+    // round trip per probe instead of a load followed by a CAS.
     uint64_t i = mix64(key) & mask;
     for (uint32_t p = 0; p < max_probes; ++p) {
         ++probes;
-        const uint64_t k = table[i].key;
-        if (k == key) {
+        const uint64_t prev = atomicCAS((unsigned long long*)&table[i].key, (unsigned long long)kEmptyKey,
+                                        (unsigned long long)key);
+        if (prev == kEmptyKey || prev == key) {
             slot = (uint32_t)i;
-            isnew = false;
+            isnew = prev == kEmptyKey;
             return true;
-        }
-        if (k == kEmptyKey) {
-            const uint64_t prev = atomicCAS((unsigned long long*)&table[i].key, (unsigned long long)kEmptyKey,
-                                            (unsigned long long)key);
-            if (prev == kEmptyKey || prev == key) {
-                slot = (uint32_t)i;
-                isnew = prev == kEmptyKey;
-                return true;
-            }
         }
         i = (i + 1) & mask;
     }
@@ -467,27 +474,81 @@ template <int W, int KW>
 __device__ inline void push_records(HeapEntry* H, uint32_t& S, const HeapEntry (&rec)[KW], int K, int lane,
                                     uint32_t best, uint32_t cap, bool& overflow, uint32_t& n_push,
                                     uint32_t& n_up) {
+    // Record j = r*W + l sits in lane l, register r.  Valid records take the
+    // positions S, S+1, ... in j order.  When every new position's parent is an
+    // old entry (true unless the heap is nearly empty), the lanes check their
+    // parents in parallel and place the records that already satisfy the heap
+    // order in one round trip; lane 0 then sifts up only the others.  Sifting
+    // one record up only lowers values on its own path to the root, so a record
+    // that satisfied its parent before still does.
+    const uint32_t gbase = __lane_id() & ~(uint32_t)(W - 1);
+    const uint64_t gmask = (W == 64) ? ~0ull : ((1ull << W) - 1);
+    bool v[KW];
+    uint32_t rank[KW];
+    uint32_t c = 0;
 #pragma unroll
     for (int r = 0; r < KW; ++r) {
+        v[r] = (r * W + lane < K) && rec[r].prio != kNoRecord && prio_f(rec[r].prio) < best;
+        const uint64_t bits = (__ballot(v[r]) >> gbase) & gmask;
+        rank[r] = c + (uint32_t)__popcll(bits & ((1ull << lane) - 1));
+        c += (uint32_t)__popcll(bits);
+    }
+    if (c == 0) return;
+    if (S + c > cap) {
+        overflow = true;
+        return;
+    }
+    if (S == 0 || (int)c - 2 >= (W - 1) * (int)S) {
+        // parents may be among the new records: plain sequential pushes
 #pragma unroll
-        for (int l = 0; l < W; ++l) {
-            if (r * W + l >= K) break;
-            HeapEntry x;
-            x.prio = __shfl(rec[r].prio, l, W);
-            x.slot = __shfl(rec[r].slot, l, W);
-            x.aux = __shfl(rec[r].aux, l, W);
-            if (lane == 0 && x.prio != kNoRecord && prio_f(x.prio) < best) {
-                if (S >= cap) {
-                    overflow = true;
-                } else {
+        for (int r = 0; r < KW; ++r) {
+#pragma unroll
+            for (int l = 0; l < W; ++l) {
+                if (r * W + l >= K) break;
+                const int vl = __shfl(v[r] ? 1 : 0, l, W);
+                if (!vl) continue;
+                HeapEntry x;
+                x.prio = __shfl(rec[r].prio, l, W);
+                x.slot = __shfl(rec[r].slot, l, W);
+                x.aux = __shfl(rec[r].aux, l, W);
+                if (lane == 0) {
                     heap_push<W>(H, S, x, n_up);
                     ++S;
                     ++n_push;
                 }
             }
         }
+        S = __shfl(S, 0, W);
+        return;
     }
-    S = __shfl(S, 0, W);
+    bool bad[KW];
+#pragma unroll
+    for (int r = 0; r < KW; ++r) {
+        bad[r] = false;
+        if (v[r]) {
+            const uint32_t pos = S + rank[r];
+            const uint64_t pp = H[(pos - 1) / W].prio;
+            if (rec[r].prio >= pp) H[pos] = rec[r];
+            else bad[r] = true;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < KW; ++r) {
+#pragma unroll
+        for (int l = 0; l < W; ++l) {
+            if (r * W + l >= K) break;
+            const int bl = __shfl(bad[r] ? 1 : 0, l, W);
+            if (!bl) continue;
+            HeapEntry x;
+            x.prio = __shfl(rec[r].prio, l, W);
+            x.slot = __shfl(rec[r].slot, l, W);
+            x.aux = __shfl(rec[r].aux, l, W);
+            const uint32_t pos = S + (uint32_t)__shfl((int)rank[r], l, W);
+            if (lane == 0) heap_push<W>(H, pos, x, n_up);  // sift up from its leaf position
+        }
+    }
+    if (lane == 0) n_push += c;
+    S += c;
 }
 
 // One launch = one GA* iteration for this heap group, made of `rounds`
@@ -505,35 +566,14 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     const int lane = (int)(threadIdx.x & (W - 1));
     const uint32_t N = d.n_heaps;
 
-    // -- termination test on the previous iteration's reduction (a_star.h:260)
+    // Independent loads first: the termination inputs and this heap's size and
+    // incoming records (unused if the group turns out not to be live).
     uint32_t best = __builtin_amdgcn_readfirstlane(load_relaxed(&st->best));
-    bool live = active;
-    if (it > 0) {
-        const uint32_t mt = __builtin_amdgcn_readfirstlane(load_relaxed(&st->min_top[(it - 1) % 3]));
-        if (mt == kInf || (best != kInf && best <= mt)) {
-            if (live && lane == 0) {
-                set_status(st, ST_DONE);
-                store_relaxed(&st->done_iter, it);
-            }
-            live = false;
-        }
-    }
-    if (live && q == 0 && lane == 0) {
-        store_relaxed(&st->min_top[(it + 1) % 3], kInf);
-        store_relaxed(&st->iterations, it + 1);
-    }
-    if constexpr (!Ops::kDense) {
-        if (live && load_relaxed(&st->occupied) > d.load_limit) {
-            if (lane == 0) set_status(st, ST_OVERFLOW_TABLE);
-            live = false;
-        }
-    }
-
+    const uint32_t mt = it > 0 ? __builtin_amdgcn_readfirstlane(load_relaxed(&st->min_top[(it - 1) % 3])) : 0u;
+    uint64_t occupied = 0;
+    if constexpr (!Ops::kDense) occupied = load_relaxed(&st->occupied);
     HeapEntry* H = d.heaps + (size_t)q * d.heap_stride + (W - 1);
     uint32_t S = 0;
-    uint32_t n_push = 0, n_up = 0, n_down = 0, n_pops = 0, n_exp = 0, n_gen = 0, n_new = 0, n_probe = 0, n_ins = 0,
-             n_look = 0;
-    bool overflow_heap = false, overflow_table = false;
     HeapEntry rec[KW];
 #pragma unroll
     for (int r = 0; r < KW; ++r) {
@@ -541,12 +581,10 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         rec[r].slot = 0;
         rec[r].aux = 0;
     }
-
-    // -- 1. push the records addressed to this heap (insert_nodes_into_queues,
-    //       a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N)
-    if (live) {
+    if (active) {
         S = d.heap_size[q];
         if (it > 0) {
+            // insert_nodes_into_queues, a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N
             const HeapEntry* cin = d.cand + (size_t)((it - 1) & 1) * N * K;
             const uint32_t base = (q + it) & (N - 1);
 #pragma unroll
@@ -554,16 +592,59 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 const int j = r * W + lane;
                 if (j < K) rec[r] = cin[base + (size_t)j * N];
             }
-            push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
-#pragma unroll
-            for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
         }
+    }
+
+    // -- termination test on the previous iteration's reduction (a_star.h:260)
+    bool live = active;
+    if (it > 0 && (mt == kInf || (best != kInf && best <= mt))) {
+        if (live && lane == 0) {
+            set_status(st, ST_DONE);
+            store_relaxed(&st->done_iter, it);
+        }
+        live = false;
+    }
+    if (live && q == 0 && lane == 0) {
+        store_relaxed(&st->min_top[(it + 1) % 3], kInf);
+        store_relaxed(&st->iterations, it + 1);
+    }
+    if constexpr (!Ops::kDense) {
+        if (live && occupied > d.load_limit) {
+            if (lane == 0) set_status(st, ST_OVERFLOW_TABLE);
+            live = false;
+        }
+    }
+
+    uint32_t n_push = 0, n_up = 0, n_down = 0, n_pops = 0, n_exp = 0, n_gen = 0, n_new = 0, n_probe = 0, n_ins = 0,
+             n_look = 0;
+    bool overflow_heap = false, overflow_table = false;
+
+    // -- 1. push the records addressed to this heap
+    if (live && it > 0) {
+        push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
+#pragma unroll
+        for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
     }
     group_fence();
 
-    // -- 2. min f over heap tops after the pushes (HeapsHost::minimum, heaps.h:291-294)
-    uint32_t top_f = kInf;
-    if (live && lane == 0 && S > 0) top_f = prio_f(H[0].prio);
+    // -- 2. first pop of this launch.  Its root is the heap's top after the
+    //       pushes and feeds the min-f reduction (HeapsHost::minimum,
+    //       heaps.h:291-294); a stale root only lowers the minimum (safe).
+    auto load = [&](uint32_t slot, typename Ops::Key& k, uint64_t& v) { Ops::load(d, slot, k, v); };
+    HeapEntry top;
+    top.prio = 0;
+    top.slot = 0;
+    top.aux = 0;
+    typename Ops::Key top_key{};
+    uint64_t top_val = 0;
+    bool popped = false;
+    if (live && S > 0) {
+        top = heap_pop<W>(H, S, lane, n_down, load, top_key, top_val);
+        group_fence();
+        ++n_pops;
+        popped = true;
+    }
+    uint32_t top_f = (popped && lane == 0) ? prio_f(top.prio) : kInf;
     top_f = wave_min_u32(top_f);
     if ((threadIdx.x & 63) == 0 && top_f != kInf) atomicMin(&st->min_top[it % 3], top_f);
 
@@ -572,21 +653,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         for (uint32_t round = 0; round < prm.rounds; ++round) {
             // pop, skipping entries whose g is no longer the stored one
             bool have = false;
-            HeapEntry top;
-            top.prio = 0;
-            top.slot = 0;
-            top.aux = 0;
-            typename Ops::Key top_key{};
-            for (uint32_t t = 0; t < prm.max_pops && S > 0; ++t) {
-                top = heap_pop<W>(H, S, lane, n_down);
-                group_fence();
-                ++n_pops;
-                typename Ops::Key key;
-                uint64_t val;
-                Ops::load(d, top.slot, key, val);
-                if (prio_g(top.prio) > val_g(val)) continue;  // stale
+            for (uint32_t t = 0; t < prm.max_pops; ++t) {
+                if (!popped) {
+                    if (S == 0) break;
+                    top = heap_pop<W>(H, S, lane, n_down, load, top_key, top_val);
+                    group_fence();
+                    ++n_pops;
+                }
+                popped = false;
+                if (prio_g(top.prio) > val_g(top_val)) continue;  // stale
                 have = true;
-                top_key = key;
                 break;
             }
             if (!have) break;
