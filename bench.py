@@ -105,6 +105,7 @@ def main() -> int:
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-profile", action="store_true", help="skip per-step HIP events and counters")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -194,13 +195,42 @@ def main() -> int:
         avg_ms = st.step_ms_total / st.step_launches
         bytes_per_launch = st.algo_bytes / st.step_launches
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        traffic, traffic_src = None, None
+        tpath = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
+        if os.path.exists(tpath):  # HBM bytes per launch from the committed rocprofv3 --pmc runs
+            with open(tpath) as f:
+                tj = json.load(f)
+            traffic, traffic_src = round(tj["traffic_bytes_per_launch"], 1), f"profiles/{os.path.basename(tpath)} ({tj['workload']})"
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                    "kernel": "step_kernel<4>", "avg_launch_us": round(avg_ms * 1e3, 3),
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
+                    "kernel": f"step_kernel<{served_cfg.heap_arity}>", "avg_launch_us": round(avg_ms * 1e3, 3),
                     "algo_bytes_per_launch": round(bytes_per_launch, 1),
                     "expansions_per_launch": round(st.expanded / st.step_launches, 1),
-                    "launches": st.step_launches}
+                    "launches": st.step_launches,
+                    # global atomics issued by the step kernel: one CAS per closed-set probe (hashed sets)
+                    # plus one atomicMin per lookup; counters and reductions excluded
+                    "atomics_per_launch": round((st.probes + st.lookups) / st.step_launches, 1),
+                    "atomic_rate_G_per_s": round((st.probes + st.lookups) / st.step_launches / (avg_ms * 1e-3) / 1e9, 3)}
     log(f"[bench] served {ok} ok / {failed} failed in {t_total:.2f}s = {served_sps:.1f} solves/s; roofline {roofline}")
+
+    # ---- batched: the 1024 mixed instances of the batch configuration in one ga_solve_batch ----
+    batch = None
+    if not args.skip_batch:
+        b_insts = wl.config4(4, 1024, threads=16)
+        b_dev = {i: e.upload(x.cells) for i, x in enumerate(b_insts) if x.kind == "grid"}
+        b_reqs = [x.request(cells_device=b_dev.get(i), num_queues=served_cfg.heaps(x.kind, x.side))
+                  for i, x in enumerate(b_insts)]
+        e.solve_batch(b_reqs[:64])  # warm-up
+        t = time.perf_counter()
+        b_sols = e.solve_batch(b_reqs)
+        b_t = time.perf_counter() - t
+        b_ok = sum(1 for s_ in b_sols if s_.status == ga.OK)
+        batch = {"solves_per_sec": round(b_ok / b_t, 3), "instances": len(b_reqs), "ok": b_ok, "seconds": round(b_t, 3),
+                 "workload": "1024 mixed: even = 15-puzzle walk depth 20..60, odd = grid side {256,512,1024,2048}, "
+                             "walls 10..25%"}
+        for a in b_dev.values():
+            e.free_device(a)
+        log(f"[bench] batched: {b_ok}/{len(b_reqs)} in {b_t:.2f}s = {b_ok / b_t:.1f} solves/s")
 
     # ---- one at a time: plain GA*, same requests, sequential ----
     timed_idx = list(range(W * M, (W + K) * M))
@@ -265,6 +295,7 @@ def main() -> int:
         "completed": ok,
         "failed": failed,
         "one_at_a_time_solves_per_sec": round(plain_sps, 3),
+        "batched": batch,
         "speedup_vs_one_at_a_time": round(served_sps / plain_sps, 2) if plain_sps else None,
         "served_vs_alone_cost_mismatches": mismatch,
         "roofline": roofline,

@@ -84,7 +84,8 @@ class Config(ctypes.Structure):
         ("pops_per_iter", ctypes.c_uint32),
         ("profile", ctypes.c_uint32),
         ("rounds", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32 * 2),
+        ("adaptive_start", ctypes.c_uint32),
+        ("grow_thresh", ctypes.c_uint32),
     ]
 
 

@@ -39,16 +39,16 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
                                                       const InstDesc* __restrict__ descs, InstState* states,
                                                       const uint32_t* __restrict__ iter_base, uint32_t r,
                                                       StepParams prm) {
-    uint32_t inst, it, first;
-    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first)) return;
+    uint32_t inst, it, first, cover;
+    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover)) return;
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / W;
     const bool active = q < d.n_heaps;
     switch (d.kind) {
-        case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, prm); break;
-        case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, prm); break;
-        case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, prm); break;
-        case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, prm); break;
+        case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm); break;
+        case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm); break;
+        case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm); break;
+        case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm); break;
         default: break;
     }
 }
@@ -251,6 +251,7 @@ struct Job {
     const uint8_t* cells_dev = nullptr;
     uint64_t t_admit = 0;
     uint32_t last_iterations = 0;
+    uint32_t cover = 0;  // heaps covered by the launch map (power of two)
     bool finished = false;
 };
 
@@ -306,6 +307,7 @@ struct ga_engine {
         if (cfg.max_load_pct == 0 || cfg.max_load_pct > 95) cfg.max_load_pct = 50;
         if (cfg.pops_per_iter == 0) cfg.pops_per_iter = 1;
         if (cfg.rounds == 0) cfg.rounds = 1;
+        if (cfg.grow_thresh == 0) cfg.grow_thresh = 32;
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamCreateWithFlags(&path_stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -432,6 +434,8 @@ struct ga_engine {
         d.n_heaps = p.n_heaps;
         d.heap_cap = p.heap_cap;
         d.heap_stride = p.heap_stride;
+        d.a0 = cfg.adaptive_start ? (uint32_t)std::min<uint64_t>(p.n_heaps, pow2_floor(cfg.adaptive_start))
+                                  : p.n_heaps;
         const uint32_t fill_grid = 1024;
         if (r.kind == GA_PUZZLE) {
             d.width = r.side;
@@ -470,6 +474,12 @@ struct ga_engine {
         j.first_chunk = chunk_count;
         j.t_admit = now_ns();
         j.last_iterations = 0;
+        {
+            // the map covers twice the initial width (at least one workgroup's heaps)
+            const uint64_t per_wg = kBlock / W;
+            j.cover = (uint32_t)std::min<uint64_t>(p.n_heaps, pow2_ceil(std::max<uint64_t>(2ull * d.a0, per_wg)));
+            if (d.a0 == p.n_heaps) j.cover = p.n_heaps;
+        }
         resident[slot] = &j;
         map_dirty = true;
         stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena->in_use());
@@ -495,8 +505,8 @@ struct ga_engine {
         for (uint32_t s = 0; s < cfg.max_instances; ++s) {
             Job* j = resident[s];
             if (!j || j->finished) continue;
-            uint32_t n = j->desc.n_heaps;
-            for (uint32_t h = 0; h < n; h += heaps_per_wg) map.push_back(WgEntry{s, h});
+            const uint32_t n = std::min(j->cover, j->desc.n_heaps);
+            for (uint32_t h = 0; h < n; h += heaps_per_wg) map.push_back(WgEntry{s, h, n, 0});
         }
         n_wg = (uint32_t)map.size();
         map_dirty = false;
@@ -512,7 +522,7 @@ struct ga_engine {
         }
         // graph mode launches map_cap workgroups: pad with entries that exit at once
         const uint32_t n_up = cfg.use_graph ? map_cap : n_wg;
-        map.resize(n_up, WgEntry{kNone, 0});
+        map.resize(n_up, WgEntry{kNone, 0, 0, 0});
         int b = map_buf;
         map_buf ^= 1;
         hip_check(hipEventSynchronize(ev_map[b]), "map event");
@@ -549,7 +559,7 @@ struct ga_engine {
 
     void launch_chunk() {
         const uint32_t R = cfg.chunk_iters;
-        StepParams prm{cfg.pops_per_iter, 1u << 16, cfg.profile ? 1u : 0u, cfg.rounds};
+        StepParams prm{cfg.pops_per_iter, 1u << 16, cfg.profile ? 1u : 0u, cfg.rounds, cfg.grow_thresh, {0, 0, 0}};
         const int b = (int)(chunk_count & 1);
         std::vector<hipEvent_t>& pe = prof_ev[b];
         if (cfg.profile && pe.size() < 2 * R) {
@@ -743,6 +753,12 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
             const InstState& st = snap[s];
             if (st.status == ST_RUNNING) {
                 j->last_iterations = st.iterations;
+                // the device grows the frontier width up to the map's cover: widen the
+                // cover before the width reaches it
+                if (j->cover < j->desc.n_heaps && 2ull * st.active_now > j->cover) {
+                    j->cover = (uint32_t)std::min<uint64_t>(j->desc.n_heaps, 4ull * j->cover);
+                    e->map_dirty = true;
+                }
                 if (e->cfg.max_iterations && st.iterations > e->cfg.max_iterations) {
                     abort_kernel<<<1, 64, 0, e->stream>>>(e->d_states, (uint32_t)s);
                 }
@@ -968,6 +984,8 @@ void ga_config_default(ga_config* cfg) {
     cfg->pops_per_iter = 2;
     cfg->profile = 0;
     cfg->rounds = 1;
+    cfg->adaptive_start = 0;
+    cfg->grow_thresh = 32;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {

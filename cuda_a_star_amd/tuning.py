@@ -8,6 +8,8 @@ per launch):
   pops per iteration   heap pops tried while entries are stale
   hash load factor     closed-set load limit before a capacity retry
   expansion rounds     heap pops per launch with local pushes in between
+  adaptive width       heaps in use at admission, doubled on the device while the
+                       open list exceeds grow_thresh entries per heap
   graph replay         chunks replayed from a captured hipGraph or launched one by one
 
 `python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
@@ -42,6 +44,8 @@ class TuneConfig:
     max_load_pct: int = 50
     rounds: int = 1
     use_graph: int = 0
+    adaptive_start: int = 0     # 0 = every request uses all its heaps from the start
+    grow_thresh: int = 32
 
     def heaps(self, kind: str, side: int) -> int:
         if kind == "puzzle":
@@ -56,7 +60,8 @@ class TuneConfig:
     def engine(self) -> Dict[str, int]:
         return {"heap_arity": self.heap_arity, "chunk_iters": self.chunk_iters,
                 "pops_per_iter": self.pops_per_iter, "max_load_pct": self.max_load_pct,
-                "rounds": self.rounds, "use_graph": self.use_graph}
+                "rounds": self.rounds, "use_graph": self.use_graph,
+                "adaptive_start": self.adaptive_start, "grow_thresh": self.grow_thresh}
 
     def describe(self) -> Dict[str, int]:
         return asdict(self)
@@ -120,7 +125,7 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
     print(f"[tune] {mode} start {best_v:.2f}/s {asdict(best)}", flush=True)
     if mode == "served":
         axes = {
-            "puzzle_heaps": [256, 512, 1024, 2048, 4096],
+            "puzzle_heaps": [256, 512, 1024, 2048, 4096, 8192],
             "grid_heaps_small": [256, 512, 1024, 2048],
             "grid_heaps_mid": [512, 1024, 2048, 4096],
             "grid_heaps_big": [1024, 2048, 4096, 8192],
@@ -128,8 +133,10 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "chunk_iters": [4, 8, 16, 32],
             "pops_per_iter": [1, 2, 4],
             "max_load_pct": [50, 70],
-            "rounds": [1, 2, 4, 8],
+            "rounds": [1, 2, 4],
             "use_graph": [0, 1],
+            "adaptive_start": [0, 32, 128, 512],
+            "grow_thresh": [8, 32, 128],
         }
     else:
         axes = {
@@ -141,8 +148,10 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "chunk_iters": [4, 8, 16, 32],
             "pops_per_iter": [1, 2, 4],
             "max_load_pct": [50, 70],
-            "rounds": [1, 2, 4, 8],
+            "rounds": [1, 2, 4],
             "use_graph": [0, 1],
+            "adaptive_start": [0, 64, 256, 1024],
+            "grow_thresh": [8, 32, 128],
         }
     # coordinate descent: one axis at a time, two passes
     for _ in range(2):

@@ -95,7 +95,8 @@ typedef struct ga_config {
     uint32_t pops_per_iter;  /* heap pops tried per heap per iteration (stale skip) */
     uint32_t profile;        /* time every step with HIP events */
     uint32_t rounds;         /* expansion rounds per heap per launch (local pushes between rounds) */
-    uint32_t reserved[2];
+    uint32_t adaptive_start; /* initial heaps in use per request (doubles on demand); 0 = all num_queues */
+    uint32_t grow_thresh;    /* double the heaps in use when open entries exceed this many per heap */
 } ga_config;
 
 /* Engine statistics (kernel timing is collected when cfg.profile != 0). */

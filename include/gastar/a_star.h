@@ -215,11 +215,11 @@ __global__ __launch_bounds__(kBlock) void user_step_kernel(const WgEntry* __rest
                                                            const InstDesc* __restrict__ descs, InstState* states,
                                                            const uint32_t* __restrict__ iter_base, uint32_t r,
                                                            StepParams prm) {
-    uint32_t inst, it, first;
-    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first)) return;
+    uint32_t inst, it, first, cover;
+    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover)) return;
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / kW;
-    run_group<kW, Ops>(d, states + inst, it, q, q < d.n_heaps, prm);
+    run_group<kW, Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm);
 }
 
 template <class Ops>
@@ -266,6 +266,7 @@ std::pair<std::vector<T>, std::chrono::nanoseconds> solve(const T& start, const 
         InstDesc d{};
         d.kind = KIND_USER;
         d.n_heaps = n;
+        d.a0 = n < 256 ? n : 256;  // the frontier width starts at 256 heaps and doubles on demand
         d.heap_cap = heap_cap;
         d.heap_stride = (heap_cap + kW - 1 + kW - 1) / kW * kW;
         d.heaps = (HeapEntry*)b.get(sizeof(HeapEntry) * (size_t)n * d.heap_stride);
@@ -289,7 +290,7 @@ std::pair<std::vector<T>, std::chrono::nanoseconds> solve(const T& start, const 
         const uint32_t wgs = (n + kBlock / kW - 1) / (kBlock / kW);
         WgEntry* map = (WgEntry*)b.get(sizeof(WgEntry) * wgs);
         std::vector<WgEntry> hmap(wgs);
-        for (uint32_t w = 0; w < wgs; ++w) hmap[w] = WgEntry{0, w * (kBlock / kW)};
+        for (uint32_t w = 0; w < wgs; ++w) hmap[w] = WgEntry{0, w * (kBlock / kW), n, 0};
         check(hipMemcpy(map, hmap.data(), sizeof(WgEntry) * wgs, hipMemcpyHostToDevice), "hipMemcpy");
         check(hipMemset(iter, 0, sizeof(uint32_t)), "hipMemset");
         check(hipMemset(d.heap_size, 0, sizeof(uint32_t) * n), "hipMemset");
@@ -297,7 +298,7 @@ std::pair<std::vector<T>, std::chrono::nanoseconds> solve(const T& start, const 
         user_seed_kernel<Ops><<<1, 64>>>(d, descs, st);
         check(hipGetLastError(), "seed");
         check(hipDeviceSynchronize(), "seed");
-        StepParams prm{2, 1u << 16, 0, 1};
+        StepParams prm{2, 1u << 16, 0, 1, 32, {0, 0, 0}};
         InstState hs{};
         auto t0 = std::chrono::steady_clock::now();
         for (;;) {

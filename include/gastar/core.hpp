@@ -116,6 +116,8 @@ struct InstDesc {
     uint32_t heap_cap;     // max entries per heap
     uint32_t heap_stride;  // physical entries per heap
     uint32_t start_iter;   // value of the global iteration counter at the first step
+    uint32_t a0;           // initial frontier width (heaps in use), a power of two <= n_heaps
+    uint32_t pad0;
     uint8_t goal_rc[16];   // puzzle: row << 4 | col of each tile in the goal
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
@@ -129,6 +131,10 @@ struct InstState {
     uint32_t iterations;  // iterations started
     uint32_t done_iter;
     uint32_t pad;
+    uint32_t active[3];   // heaps in use (frontier width), by iteration mod 3
+    uint32_t open_sum[3]; // heap entries after the pushes, by iteration mod 3
+    uint32_t active_now;  // latest frontier width (for the host)
+    uint32_t pad2;
     unsigned long long occupied;  // closed-set slots in use
     unsigned long long expanded;
     unsigned long long generated;
@@ -142,17 +148,23 @@ struct InstState {
     unsigned long long lookups;   // closed-set find-or-insert calls (one atomicMin each)
 };
 
-// Workgroup -> (instance, first heap) map of one launch.
+// Workgroup -> (instance, first heap) map of one launch.  `cover` is the
+// number of heaps of the instance that the map covers (a power of two): the
+// frontier width may grow on the device up to it.
 struct WgEntry {
     uint32_t inst;
     uint32_t first_heap;
+    uint32_t cover;
+    uint32_t pad;
 };
 
 struct StepParams {
-    uint32_t max_pops;    // pops tried per expansion round while entries are stale
-    uint32_t max_probes;  // closed-set probe limit (overflow guard)
-    uint32_t profile;     // count probes and sift levels
-    uint32_t rounds;      // expansion rounds per launch (>= 1)
+    uint32_t max_pops;     // pops tried per expansion round while entries are stale
+    uint32_t max_probes;   // closed-set probe limit (overflow guard)
+    uint32_t profile;      // count probes and sift levels
+    uint32_t rounds;       // expansion rounds per launch (>= 1)
+    uint32_t grow_thresh;  // double the frontier width when open entries exceed this many per heap
+    uint32_t pad[3];
 };
 
 // ---------------------------------------------------------------------------
@@ -560,11 +572,18 @@ __device__ inline void push_records(HeapEntry* H, uint32_t& S, const HeapEntry (
 // this heap, so the tops of all heaps form one snapshot of the open set.
 template <int W, class Ops>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t it, uint32_t q, bool active,
-                                          const StepParams& prm) {
+                                          uint32_t cover, const StepParams& prm) {
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
-    const uint32_t N = d.n_heaps;
+    const uint32_t N = d.n_heaps;  // heaps allocated; candidate buffers are sized for all of them
+
+    // Frontier width: heaps q < A take part in this iteration; the records read
+    // now were written by the A_w heaps of the previous iteration.  A only
+    // grows, so a heap that joins starts empty and loses nothing.
+    const uint32_t A = __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[it % 3]));
+    const uint32_t Aw = it > 0 ? __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[(it - 1) % 3])) : A;
+    active = active && q < A;
 
     // Independent loads first: the termination inputs and this heap's size and
     // incoming records (unused if the group turns out not to be live).
@@ -583,14 +602,14 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     }
     if (active) {
         S = d.heap_size[q];
-        if (it > 0) {
+        if (it > 0 && q < Aw) {
             // insert_nodes_into_queues, a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N
             const HeapEntry* cin = d.cand + (size_t)((it - 1) & 1) * N * K;
-            const uint32_t base = (q + it) & (N - 1);
+            const uint32_t base = (q + it) & (Aw - 1);
 #pragma unroll
             for (int r = 0; r < KW; ++r) {
                 const int j = r * W + lane;
-                if (j < K) rec[r] = cin[base + (size_t)j * N];
+                if (j < K) rec[r] = cin[base + (size_t)j * Aw];
             }
         }
     }
@@ -607,6 +626,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     if (live && q == 0 && lane == 0) {
         store_relaxed(&st->min_top[(it + 1) % 3], kInf);
         store_relaxed(&st->iterations, it + 1);
+        // next iteration's width: double while the open list holds more than
+        // grow_thresh entries per heap in use (counted one launch ago)
+        uint32_t an = A;
+        if (it > 0 && A < cover) {
+            const uint32_t open = load_relaxed(&st->open_sum[(it - 1) % 3]);
+            if (open > prm.grow_thresh * A) an = 2 * A < cover ? 2 * A : cover;
+        }
+        store_relaxed(&st->active[(it + 1) % 3], an);
+        store_relaxed(&st->active_now, an);
+        store_relaxed(&st->open_sum[(it + 1) % 3], 0u);
     }
     if constexpr (!Ops::kDense) {
         if (live && occupied > d.load_limit) {
@@ -620,7 +649,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     bool overflow_heap = false, overflow_table = false;
 
     // -- 1. push the records addressed to this heap
-    if (live && it > 0) {
+    if (live && it > 0 && q < Aw) {
         push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
 #pragma unroll
         for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
@@ -646,7 +675,11 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     }
     uint32_t top_f = (popped && lane == 0) ? prio_f(top.prio) : kInf;
     top_f = wave_min_u32(top_f);
-    if ((threadIdx.x & 63) == 0 && top_f != kInf) atomicMin(&st->min_top[it % 3], top_f);
+    const uint32_t open = wave_sum_u32((popped && lane == 0) ? S + 1 : 0u);  // entries after the pushes
+    if ((threadIdx.x & 63) == 0) {
+        if (top_f != kInf) atomicMin(&st->min_top[it % 3], top_f);
+        if (open) atomicAdd(&st->open_sum[it % 3], open);
+    }
 
     // -- 3..6. expansion rounds
     if (live) {
@@ -761,12 +794,13 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
 // workgroup has no running instance this iteration.
 __device__ inline bool step_prologue(const WgEntry* wgmap, const InstDesc* descs, InstState* states,
                                      const uint32_t* iter_base, uint32_t r, uint32_t& inst, uint32_t& it,
-                                     uint32_t& first_heap) {
+                                     uint32_t& first_heap, uint32_t& cover) {
     const WgEntry we = wgmap[blockIdx.x];
     if (we.inst == kNone) return false;
     if (load_relaxed(&states[we.inst].status) != ST_RUNNING) return false;
     inst = we.inst;
     first_heap = we.first_heap;
+    cover = we.cover;
     it = *iter_base + r - descs[we.inst].start_iter;
     return true;
 }
@@ -799,6 +833,11 @@ __device__ inline bool seed_instance(const InstDesc& d, InstState* st, int W) {
     z.iterations = 0;
     z.done_iter = 0;
     z.pad = 0;
+    const uint32_t a0 = (d.a0 && d.a0 <= d.n_heaps) ? d.a0 : d.n_heaps;
+    z.active[0] = z.active[1] = z.active[2] = a0;
+    z.open_sum[0] = z.open_sum[1] = z.open_sum[2] = 0;
+    z.active_now = a0;
+    z.pad2 = 0;
     z.occupied = Ops::kDense ? 0 : 1;
     z.expanded = z.generated = z.pushed = z.pops = z.probes = z.sift_down = z.sift_up = 0;
     z.groups = z.inserted = z.lookups = 0;

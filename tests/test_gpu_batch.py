@@ -144,13 +144,16 @@ def test_serve_request_over_budget_is_rejected():
     assert sols[0].status == ga.E_BUDGET
 
 
-@pytest.mark.parametrize("arity,rounds,graph,pops", [(4, 1, 1, 1), (8, 1, 0, 2), (16, 2, 0, 1), (4, 4, 0, 2),
-                                                     (8, 8, 1, 4), (4, 2, 1, 2)])
-def test_engine_variants_match_oracle(arity, rounds, graph, pops):
-    """Every tunable combination the autotuner may pick keeps costs exact."""
-    insts = small_mix(seed=31 + arity + rounds, n_puz=12, n_grid=8)
+@pytest.mark.parametrize("arity,rounds,graph,pops,adaptive,grow", [
+    (4, 1, 1, 1, 0, 32), (8, 1, 0, 2, 0, 32), (16, 2, 0, 1, 0, 32), (4, 4, 0, 2, 0, 32),
+    (8, 8, 1, 4, 0, 32), (4, 2, 1, 2, 0, 32),
+    (4, 1, 0, 2, 16, 2), (8, 2, 1, 2, 64, 4), (16, 1, 0, 1, 1, 1), (8, 2, 0, 2, 256, 32)])
+def test_engine_variants_match_oracle(arity, rounds, graph, pops, adaptive, grow):
+    """Every tunable combination the autotuner may pick keeps costs exact,
+    including a frontier width that starts small and doubles on the device."""
+    insts = small_mix(seed=31 + arity + rounds + adaptive, n_puz=12, n_grid=8)
     with ga.Engine(0, heap_arity=arity, rounds=rounds, use_graph=graph, pops_per_iter=pops,
-                   max_iterations=2_000_000) as e:
+                   adaptive_start=adaptive, grow_thresh=grow, max_iterations=2_000_000) as e:
         sols = e.solve_batch([x.request() for x in insts])
         alone = [e.solve(insts[i].request()) for i in (0, len(insts) - 1)]
     for x, sol in zip(insts, sols):
