@@ -106,6 +106,7 @@ def main() -> int:
     ap.add_argument("--no-profile", action="store_true", help="skip per-step HIP events and counters")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
+    ap.add_argument("--override", default="", help="served tuning overrides, e.g. lanes=3,rounds=2")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (for scaling HBM byte counters)")
     args = ap.parse_args()
@@ -137,6 +138,9 @@ def main() -> int:
     log(f"[bench] generated {n_total} requests in {time.perf_counter() - t:.1f}s")
 
     served_cfg = tuning.load()
+    for kv in filter(None, args.override.split(",")):
+        k, v = kv.split("=")
+        setattr(served_cfg, k, int(v))
     e = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30,
                   profile=0 if args.no_profile else 1, **served_cfg.engine)
     dev = {}
@@ -186,6 +190,16 @@ def main() -> int:
             failed += 1
     log(f"[bench] timed: {ss.completed}/{len(reqs)} in {t_total:.2f}s, p50 {ss.p50_ms:.1f} ms, "
         f"p99 {ss.p99_ms:.1f} ms, inflight max {ss.max_inflight}")
+    by_kind = {}
+    for kind in ("puzzle", "grid"):
+        kl = [sol.latency_ns / 1e6 for i, sol in zip(idx, sols) if insts[i].kind == kind and sol.status == ga.OK]
+        if kl:
+            by_kind[kind] = {"n": len(kl), "p50_ms": round(pct(kl, 0.5), 2), "p99_ms": round(pct(kl, 0.99), 2)}
+    slow = sorted(zip(idx, sols), key=lambda p: -p[1].latency_ns)[:5]
+    slowest = [{"kind": insts[i].kind, "side": insts[i].side, "depth": insts[i].depth,
+                "latency_ms": round(sol.latency_ns / 1e6, 1), "iterations": sol.iterations,
+                "expanded": sol.expanded, "retries": sol.retries} for i, sol in slow]
+    log(f"[bench] latency by kind {by_kind}; slowest {slowest}")
     st = e.stats()
     served_sps = ok / t_total
 
@@ -292,6 +306,8 @@ def main() -> int:
         },
         "p50_ms": round(pct(lat, 0.50), 3),
         "p99_ms": round(pct(lat, 0.99), 3),
+        "latency_by_kind": by_kind,
+        "slowest_requests": slowest,
         "completed": ok,
         "failed": failed,
         "one_at_a_time_solves_per_sec": round(plain_sps, 3),

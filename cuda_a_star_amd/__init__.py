@@ -86,6 +86,9 @@ class Config(ctypes.Structure):
         ("rounds", ctypes.c_uint32),
         ("adaptive_start", ctypes.c_uint32),
         ("grow_thresh", ctypes.c_uint32),
+        ("rounds_grid", ctypes.c_uint32),
+        ("lanes", ctypes.c_uint32),
+        ("deep_cells", ctypes.c_uint64),
     ]
 
 

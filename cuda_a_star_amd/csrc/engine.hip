@@ -252,6 +252,7 @@ struct Job {
     uint64_t t_admit = 0;
     uint32_t last_iterations = 0;
     uint32_t cover = 0;  // heaps covered by the launch map (power of two)
+    uint32_t lane = 0;   // launch lane
     bool finished = false;
 };
 
@@ -270,30 +271,42 @@ struct PathBatch {
 // ---------------------------------------------------------------------------
 // engine
 // ---------------------------------------------------------------------------
-struct ga_engine {
-    int device = 0;
-    ga_config cfg{};
-    hipStream_t stream = nullptr, path_stream = nullptr;
-    Arena* arena = nullptr;
-    InstDesc* d_descs = nullptr;
-    InstState* d_states = nullptr;
+// A launch lane: one stream with its own iteration clock, launch map and up
+// to two chunks in flight.  An instance advances one iteration per launch of
+// its lane.  Lanes run concurrently, so an instance that needs many
+// iterations (a large grid) is not paced by the launch that carries all the
+// others.  Slots are partitioned by lane (slot % lanes) so that a slot is only
+// ever reused in stream order on the lane that last ran it.
+struct Lane {
+    hipStream_t stream = nullptr;
     WgEntry* d_map = nullptr;
-    uint32_t* d_iter = nullptr;
-    uint32_t map_cap = 0;
+    uint32_t map_cap = 0, n_wg = 0;
+    bool map_dirty = true;
     Pinned h_map[2];
     hipEvent_t ev_map[2] = {nullptr, nullptr};
     int map_buf = 0;
+    uint32_t* d_iter = nullptr;
+    uint64_t host_base = 0;   // iteration counter at the start of the next chunk
+    uint64_t launched = 0;    // chunks launched
+    uint64_t processed = 0;   // chunks whose read-back has been handled
     Pinned h_states[2];
     hipEvent_t ev_chunk[2] = {nullptr, nullptr};
-    std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
-    uint32_t n_read = 0;                 // states read back per chunk (highest used slot + 1)
     uint32_t chunk_reads[2] = {0, 0};
-    uint64_t host_base = 0;           // iteration counter at the start of the next chunk
-    uint64_t chunk_count = 0;
-    std::vector<Job*> resident;
+    std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
+    hipGraphExec_t graph_exec = nullptr;
+    uint32_t running = 0;     // resident jobs not yet finished
     std::vector<int> free_slots;
-    uint32_t n_wg = 0;
-    bool map_dirty = true;
+};
+
+struct ga_engine {
+    int device = 0;
+    ga_config cfg{};
+    hipStream_t path_stream = nullptr;
+    Arena* arena = nullptr;
+    InstDesc* d_descs = nullptr;
+    InstState* d_states = nullptr;
+    std::vector<Lane> lanes;
+    std::vector<Job*> resident;
     ga_engine_stats stats{};
     std::string last_error;
 
@@ -308,42 +321,68 @@ struct ga_engine {
         if (cfg.pops_per_iter == 0) cfg.pops_per_iter = 1;
         if (cfg.rounds == 0) cfg.rounds = 1;
         if (cfg.grow_thresh == 0) cfg.grow_thresh = 32;
+        if (cfg.lanes == 0) cfg.lanes = 1;
+        if (cfg.lanes > 4) cfg.lanes = 4;
+        if (cfg.max_instances < cfg.lanes) cfg.max_instances = cfg.lanes;
         hip_check(hipSetDevice(device), "hipSetDevice");
-        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
         hip_check(hipStreamCreateWithFlags(&path_stream, hipStreamNonBlocking), "hipStreamCreate");
         arena = new Arena(cfg.chunk_bytes);
         hip_check(hipMalloc(&d_descs, sizeof(InstDesc) * cfg.max_instances), "hipMalloc descs");
         hip_check(hipMalloc(&d_states, sizeof(InstState) * cfg.max_instances), "hipMalloc states");
         hip_check(hipMemset(d_states, 0, sizeof(InstState) * cfg.max_instances), "hipMemset states");
-        hip_check(hipMalloc(&d_iter, sizeof(uint32_t)), "hipMalloc iter");
-        hip_check(hipMemset(d_iter, 0, sizeof(uint32_t)), "hipMemset iter");
-        for (int i = 0; i < 2; ++i) {
-            hip_check(hipEventCreateWithFlags(&ev_map[i], hipEventDisableTiming), "hipEventCreate");
-            hip_check(hipEventCreateWithFlags(&ev_chunk[i], hipEventDisableTiming), "hipEventCreate");
-            h_states[i].ensure(sizeof(InstState) * cfg.max_instances);
+        lanes.resize(cfg.lanes);
+        for (uint32_t l = 0; l < cfg.lanes; ++l) {
+            Lane& L = lanes[l];
+            hip_check(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking), "hipStreamCreate");
+            hip_check(hipMalloc(&L.d_iter, sizeof(uint32_t)), "hipMalloc iter");
+            hip_check(hipMemset(L.d_iter, 0, sizeof(uint32_t)), "hipMemset iter");
+            for (int i = 0; i < 2; ++i) {
+                hip_check(hipEventCreateWithFlags(&L.ev_map[i], hipEventDisableTiming), "hipEventCreate");
+                hip_check(hipEventCreateWithFlags(&L.ev_chunk[i], hipEventDisableTiming), "hipEventCreate");
+                L.h_states[i].ensure(sizeof(InstState) * cfg.max_instances);
+            }
+            for (int s = (int)cfg.max_instances - 1; s >= 0; --s)
+                if ((uint32_t)s % cfg.lanes == l) L.free_slots.push_back(s);
         }
         resident.assign(cfg.max_instances, nullptr);
-        for (int s = (int)cfg.max_instances - 1; s >= 0; --s) free_slots.push_back(s);
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
     }
 
     ~ga_engine() {
-        if (stream) (void)hipStreamSynchronize(stream);
-        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
-        if (path_stream) (void)hipStreamSynchronize(path_stream);
-        for (auto& v : prof_ev)
-            for (auto ev : v) (void)hipEventDestroy(ev);
-        for (int i = 0; i < 2; ++i) {
-            if (ev_map[i]) (void)hipEventDestroy(ev_map[i]);
-            if (ev_chunk[i]) (void)hipEventDestroy(ev_chunk[i]);
+        for (Lane& L : lanes) {
+            if (L.stream) (void)hipStreamSynchronize(L.stream);
+            if (L.graph_exec) (void)hipGraphExecDestroy(L.graph_exec);
+            for (auto& v : L.prof_ev)
+                for (auto ev : v) (void)hipEventDestroy(ev);
+            for (int i = 0; i < 2; ++i) {
+                if (L.ev_map[i]) (void)hipEventDestroy(L.ev_map[i]);
+                if (L.ev_chunk[i]) (void)hipEventDestroy(L.ev_chunk[i]);
+            }
+            if (L.d_map) (void)hipFree(L.d_map);
+            if (L.d_iter) (void)hipFree(L.d_iter);
+            if (L.stream) (void)hipStreamDestroy(L.stream);
         }
-        if (d_map) (void)hipFree(d_map);
+        if (path_stream) (void)hipStreamSynchronize(path_stream);
         if (d_descs) (void)hipFree(d_descs);
         if (d_states) (void)hipFree(d_states);
-        if (d_iter) (void)hipFree(d_iter);
         delete arena;
-        if (stream) (void)hipStreamDestroy(stream);
         if (path_stream) (void)hipStreamDestroy(path_stream);
+    }
+
+    // Lane of a request: puzzles and small grids share lane 0; the largest
+    // grids (>= deep_cells cells) take the last lane and, with three or more
+    // lanes, grids of >= deep_cells/4 cells the one before it.
+    uint32_t lane_of(const Job& j) const {
+        const uint32_t n = cfg.lanes;
+        if (n == 1 || j.req->kind != GA_GRID || !cfg.deep_cells) return 0;
+        const uint64_t cells = (uint64_t)j.req->width * j.req->height;
+        uint64_t t = cfg.deep_cells;
+        uint32_t lane = n - 1;
+        while (lane > 0 && cells < t) {
+            --lane;
+            t /= 4;
+        }
+        return lane;
     }
 
     // ---- planning -----------------------------------------------------------
@@ -390,7 +429,9 @@ struct ga_engine {
     // ---- admission ------------------------------------------------------------
     // Returns 1 if admitted, 0 if it must wait (no slot / budget), or throws.
     int admit(Job& j) {
-        if (free_slots.empty()) return 0;
+        const uint32_t lane = lane_of(j);
+        Lane& L = lanes[lane];
+        if (L.free_slots.empty()) return 0;
         const ga_request& r = *j.req;
         Plan p = plan(j);
         uint64_t cell_bytes = (r.kind == GA_GRID && !r.cells_device) ? (uint64_t)r.width * r.height : 0;
@@ -436,7 +477,9 @@ struct ga_engine {
         d.heap_stride = p.heap_stride;
         d.a0 = cfg.adaptive_start ? (uint32_t)std::min<uint64_t>(p.n_heaps, pow2_floor(cfg.adaptive_start))
                                   : p.n_heaps;
+        d.rounds = (r.kind == GA_GRID && cfg.rounds_grid) ? cfg.rounds_grid : cfg.rounds;
         const uint32_t fill_grid = 1024;
+        hipStream_t stream = L.stream;
         if (r.kind == GA_PUZZLE) {
             d.width = r.side;
             d.height = r.side;
@@ -462,106 +505,114 @@ struct ga_engine {
         }
         fill_kernel<uint32_t><<<std::min<uint32_t>(fill_grid, (p.n_heaps + kBlock - 1) / kBlock), kBlock, 0,
                                 stream>>>(d.heap_size, p.n_heaps, 0u);
-        int slot = free_slots.back();
-        free_slots.pop_back();
-        d.start_iter = (uint32_t)host_base;
+        int slot = L.free_slots.back();
+        L.free_slots.pop_back();
+        d.start_iter = (uint32_t)L.host_base;
         seed_kernel<<<1, 64, 0, stream>>>(d, d_descs, d_states, (uint32_t)slot, W);
         hip_check(hipGetLastError(), "seed launch");
         j.slot = slot;
+        j.lane = lane;
         j.desc = d;
         j.allocs = std::move(allocs);
         j.bytes = need;
-        j.first_chunk = chunk_count;
+        j.first_chunk = L.launched;
         j.t_admit = now_ns();
         j.last_iterations = 0;
-        {
-            // the map covers twice the initial width (at least one workgroup's heaps)
-            const uint64_t per_wg = kBlock / W;
-            j.cover = (uint32_t)std::min<uint64_t>(p.n_heaps, pow2_ceil(std::max<uint64_t>(2ull * d.a0, per_wg)));
-            if (d.a0 == p.n_heaps) j.cover = p.n_heaps;
-        }
+        // The map covers every allocated heap, so the width can double on the
+        // device every iteration; workgroups of heaps not yet in use exit at once.
+        j.cover = p.n_heaps;
         resident[slot] = &j;
-        map_dirty = true;
+        ++L.running;
+        L.map_dirty = true;
         stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena->in_use());
         return 1;
+    }
+
+    // Marks a running job as no longer launched (its lane's map drops it).
+    void retire(Job& j) {
+        if (j.finished) return;
+        j.finished = true;
+        Lane& L = lanes[j.lane];
+        --L.running;
+        L.map_dirty = true;
     }
 
     void release(Job& j) {
         for (void* a : j.allocs) arena->release(a);
         j.allocs.clear();
         if (j.slot >= 0) {
+            Lane& L = lanes[j.lane];
             resident[j.slot] = nullptr;
-            free_slots.push_back(j.slot);
+            L.free_slots.push_back(j.slot);
             j.slot = -1;
-            map_dirty = true;
+            L.map_dirty = true;
         }
     }
 
     // ---- launch ---------------------------------------------------------------
-    void upload_map() {
+    void upload_map(Lane& L, uint32_t lane) {
         const uint32_t W = cfg.heap_arity;
         const uint32_t heaps_per_wg = kBlock / W;
         std::vector<WgEntry> map;
-        for (uint32_t s = 0; s < cfg.max_instances; ++s) {
+        for (uint32_t s = lane; s < cfg.max_instances; s += cfg.lanes) {
             Job* j = resident[s];
             if (!j || j->finished) continue;
             const uint32_t n = std::min(j->cover, j->desc.n_heaps);
             for (uint32_t h = 0; h < n; h += heaps_per_wg) map.push_back(WgEntry{s, h, n, 0});
         }
-        n_wg = (uint32_t)map.size();
-        map_dirty = false;
-        if (!n_wg) return;
-        if (n_wg > map_cap) {
-            if (d_map) {
-                hip_check(hipStreamSynchronize(stream), "sync");
-                hip_check(hipFree(d_map), "hipFree map");
+        L.n_wg = (uint32_t)map.size();
+        L.map_dirty = false;
+        if (!L.n_wg) return;
+        if (L.n_wg > L.map_cap) {
+            if (L.d_map) {
+                hip_check(hipStreamSynchronize(L.stream), "sync");
+                hip_check(hipFree(L.d_map), "hipFree map");
             }
-            map_cap = (uint32_t)pow2_ceil(n_wg);
-            hip_check(hipMalloc(&d_map, sizeof(WgEntry) * map_cap), "hipMalloc map");
-            drop_graph();  // the captured launches refer to the old map and grid size
+            L.map_cap = (uint32_t)pow2_ceil(L.n_wg);
+            hip_check(hipMalloc(&L.d_map, sizeof(WgEntry) * L.map_cap), "hipMalloc map");
+            drop_graph(L);  // the captured launches refer to the old map and grid size
         }
         // graph mode launches map_cap workgroups: pad with entries that exit at once
-        const uint32_t n_up = cfg.use_graph ? map_cap : n_wg;
+        const uint32_t n_up = cfg.use_graph ? L.map_cap : L.n_wg;
         map.resize(n_up, WgEntry{kNone, 0, 0, 0});
-        int b = map_buf;
-        map_buf ^= 1;
-        hip_check(hipEventSynchronize(ev_map[b]), "map event");
-        h_map[b].ensure(sizeof(WgEntry) * n_up);
-        memcpy(h_map[b].p, map.data(), sizeof(WgEntry) * n_up);
-        hip_check(hipMemcpyAsync(d_map, h_map[b].p, sizeof(WgEntry) * n_up, hipMemcpyHostToDevice, stream),
+        int b = L.map_buf;
+        L.map_buf ^= 1;
+        hip_check(hipEventSynchronize(L.ev_map[b]), "map event");
+        L.h_map[b].ensure(sizeof(WgEntry) * n_up);
+        memcpy(L.h_map[b].p, map.data(), sizeof(WgEntry) * n_up);
+        hip_check(hipMemcpyAsync(L.d_map, L.h_map[b].p, sizeof(WgEntry) * n_up, hipMemcpyHostToDevice, L.stream),
                   "upload map");
-        hip_check(hipEventRecord(ev_map[b], stream), "record map");
+        hip_check(hipEventRecord(L.ev_map[b], L.stream), "record map");
     }
 
-    // ---- graph mode: one captured graph per launch grid --------------------------
-    hipGraphExec_t graph_exec = nullptr;
-    void drop_graph() {
-        if (graph_exec) {
-            hip_check(hipStreamSynchronize(stream), "sync");
-            (void)hipGraphExecDestroy(graph_exec);
-            graph_exec = nullptr;
+    void drop_graph(Lane& L) {
+        if (L.graph_exec) {
+            hip_check(hipStreamSynchronize(L.stream), "sync");
+            (void)hipGraphExecDestroy(L.graph_exec);
+            L.graph_exec = nullptr;
         }
     }
 
-    void enqueue_steps(uint32_t grid, const StepParams& prm, std::vector<hipEvent_t>* pe) {
+    void enqueue_steps(Lane& L, uint32_t grid, const StepParams& prm, std::vector<hipEvent_t>* pe) {
         const uint32_t R = cfg.chunk_iters;
+        hipStream_t stream = L.stream;
         for (uint32_t r = 0; r < R; ++r) {
             if (pe) hip_check(hipEventRecord((*pe)[2 * r], stream), "event");
             switch (cfg.heap_arity) {
-                case 8: step_kernel<8><<<grid, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
-                case 16: step_kernel<16><<<grid, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
-                default: step_kernel<4><<<grid, kBlock, 0, stream>>>(d_map, d_descs, d_states, d_iter, r, prm); break;
+                case 8: step_kernel<8><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                case 16: step_kernel<16><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                default: step_kernel<4><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
             }
             if (pe) hip_check(hipEventRecord((*pe)[2 * r + 1], stream), "event");
         }
-        advance_kernel<<<1, 64, 0, stream>>>(d_iter, R);
+        advance_kernel<<<1, 64, 0, stream>>>(L.d_iter, R);
     }
 
-    void launch_chunk() {
+    void launch_chunk(Lane& L) {
         const uint32_t R = cfg.chunk_iters;
         StepParams prm{cfg.pops_per_iter, 1u << 16, cfg.profile ? 1u : 0u, cfg.rounds, cfg.grow_thresh, {0, 0, 0}};
-        const int b = (int)(chunk_count & 1);
-        std::vector<hipEvent_t>& pe = prof_ev[b];
+        const int b = (int)(L.launched & 1);
+        std::vector<hipEvent_t>& pe = L.prof_ev[b];
         if (cfg.profile && pe.size() < 2 * R) {
             for (size_t i = pe.size(); i < 2 * R; ++i) {
                 hipEvent_t ev;
@@ -570,37 +621,38 @@ struct ga_engine {
             }
         }
         if (cfg.use_graph) {
-            if (!graph_exec) {
+            if (!L.graph_exec) {
                 hipGraph_t g;
-                hip_check(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "begin capture");
-                enqueue_steps(map_cap, prm, nullptr);
-                hip_check(hipStreamEndCapture(stream, &g), "end capture");
-                hip_check(hipGraphInstantiate(&graph_exec, g, nullptr, nullptr, 0), "instantiate");
+                hip_check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "begin capture");
+                enqueue_steps(L, L.map_cap, prm, nullptr);
+                hip_check(hipStreamEndCapture(L.stream, &g), "end capture");
+                hip_check(hipGraphInstantiate(&L.graph_exec, g, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(g);
             }
-            if (cfg.profile) hip_check(hipEventRecord(pe[0], stream), "event");
-            hip_check(hipGraphLaunch(graph_exec, stream), "graph launch");
-            if (cfg.profile) hip_check(hipEventRecord(pe[1], stream), "event");
+            if (cfg.profile) hip_check(hipEventRecord(pe[0], L.stream), "event");
+            hip_check(hipGraphLaunch(L.graph_exec, L.stream), "graph launch");
+            if (cfg.profile) hip_check(hipEventRecord(pe[1], L.stream), "event");
         } else {
-            enqueue_steps(n_wg, prm, cfg.profile ? &pe : nullptr);
+            enqueue_steps(L, L.n_wg, prm, cfg.profile ? &pe : nullptr);
         }
         hip_check(hipGetLastError(), "step launch");
-        host_base += R;
+        L.host_base += R;
         stats.step_launches += R;
-        n_read = 0;
+        uint32_t n_read = 0;
         for (uint32_t s = 0; s < cfg.max_instances; ++s)
             if (resident[s]) n_read = s + 1;
-        hip_check(hipMemcpyAsync(h_states[b].p, d_states, sizeof(InstState) * n_read, hipMemcpyDeviceToHost, stream),
+        hip_check(hipMemcpyAsync(L.h_states[b].p, d_states, sizeof(InstState) * n_read, hipMemcpyDeviceToHost,
+                                 L.stream),
                   "read back states");
-        hip_check(hipEventRecord(ev_chunk[b], stream), "record chunk");
-        chunk_reads[b] = n_read;
-        ++chunk_count;
+        hip_check(hipEventRecord(L.ev_chunk[b], L.stream), "record chunk");
+        L.chunk_reads[b] = n_read;
+        ++L.launched;
     }
 
     // Called once the chunk's read-back event has completed.
-    void collect_profile(uint64_t chunk) {
+    void collect_profile(Lane& L, uint64_t chunk) {
         if (!cfg.profile) return;
-        std::vector<hipEvent_t>& pe = prof_ev[chunk & 1];
+        std::vector<hipEvent_t>& pe = L.prof_ev[chunk & 1];
         // graph mode times the whole chunk (R steps + the counter advance)
         const size_t pairs = cfg.use_graph ? 1 : cfg.chunk_iters;
         for (size_t r = 0; r < pairs && 2 * r + 1 < pe.size(); ++r) {
@@ -638,7 +690,7 @@ struct ga_engine {
     //   per sift-down level       W*16 (children) + 16 (write)
     //   per push                  16, per sift-up level 32
     //   per lookup                8 (atomicMin on g|parent); dense: +K*1 per expansion (cells)
-    //   hashed only               8 per probe (key read) + 8 per new key (CAS)
+    //   hashed only               8 per probe (CAS on the key) + 8 per new key
     double algo_bytes(int kind, const InstState& s) const {
         const bool dense = kind == KIND_GRID4 || kind == KIND_GRID8;
         const double K = branching_of(kind), W = cfg.heap_arity;
@@ -715,7 +767,6 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
     for (Job* j : order) pending.push_back(j);
     std::vector<PathBatch*> batches;
     uint32_t inflight = 0, max_inflight = 0;
-    bool have_prev = false;
 
     auto finalize_batch = [&](PathBatch* b) {
         hip_check(hipEventSynchronize(b->ev), "path event");
@@ -743,33 +794,28 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         delete b;
     };
 
-    auto handle_snapshot = [&](uint64_t chunk) {
-        e->collect_profile(chunk);
-        const InstState* snap = (const InstState*)e->h_states[chunk & 1].p;
+    auto handle_snapshot = [&](uint32_t lane, uint64_t chunk) {
+        Lane& L = e->lanes[lane];
+        e->collect_profile(L, chunk);
+        const InstState* snap = (const InstState*)L.h_states[chunk & 1].p;
         std::vector<Job*> done;
-        for (uint32_t s = 0; s < e->chunk_reads[chunk & 1]; ++s) {
+        for (uint32_t s = lane; s < L.chunk_reads[chunk & 1]; s += e->cfg.lanes) {
             Job* j = e->resident[s];
             if (!j || j->finished || j->first_chunk > chunk) continue;
             const InstState& st = snap[s];
             if (st.status == ST_RUNNING) {
                 j->last_iterations = st.iterations;
-                // the device grows the frontier width up to the map's cover: widen the
-                // cover before the width reaches it
-                if (j->cover < j->desc.n_heaps && 2ull * st.active_now > j->cover) {
-                    j->cover = (uint32_t)std::min<uint64_t>(j->desc.n_heaps, 4ull * j->cover);
-                    e->map_dirty = true;
-                }
                 if (e->cfg.max_iterations && st.iterations > e->cfg.max_iterations) {
-                    abort_kernel<<<1, 64, 0, e->stream>>>(e->d_states, (uint32_t)s);
+                    abort_kernel<<<1, 64, 0, L.stream>>>(e->d_states, (uint32_t)s);
                 }
                 continue;
             }
             if (st.status == ST_DONE && st.best != kInf) {
-                j->finished = true;  // leaves the launch map; memory kept for the path walk
-                e->map_dirty = true;
+                e->retire(*j);  // leaves the launch map; memory kept for the path walk
                 done.push_back(j);
                 continue;
             }
+            e->retire(*j);
             if (st.status == ST_OVERFLOW_HEAP || st.status == ST_OVERFLOW_TABLE) {
                 e->fill_counters(*j, st);
                 e->release(*j);
@@ -780,6 +826,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
                     ++j->retries;
                     if (st.status == ST_OVERFLOW_HEAP) j->heap_shift += 2;
                     else j->table_shift += 2;
+                    j->finished = false;
                     retry.push_back(j);
                 }
                 continue;
@@ -792,7 +839,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
             --inflight;
         }
         if (done.empty()) return;
-        // walk the paths on the second stream once this chunk has finished
+        // walk the paths on the path stream once this lane's work so far has finished
         PathBatch* b = new PathBatch();
         b->jobs = done;
         uint64_t total = 0;
@@ -815,7 +862,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         uint64_t* dout = (uint64_t*)(dj + jobs_bytes + lens_bytes);
         hipEvent_t ready;
         hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
-        hip_check(hipEventRecord(ready, e->stream), "record");
+        hip_check(hipEventRecord(ready, L.stream), "record");
         hip_check(hipStreamWaitEvent(e->path_stream, ready, 0), "wait");
         hip_check(hipMemcpyAsync(dj, pj, n * sizeof(PathJob), hipMemcpyHostToDevice, e->path_stream), "upload jobs");
         path_kernel<<<(uint32_t)((n + 63) / 64), 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n,
@@ -854,27 +901,43 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         }
         max_inflight = std::max(max_inflight, inflight);
         // -- retire path walks that have completed
+        bool progressed = false;
         while (!batches.empty() && hipEventQuery(batches.front()->ev) == hipSuccess) {
             finalize_batch(batches.front());
             batches.erase(batches.begin());
+            progressed = true;
         }
-        uint32_t running = 0;
-        for (Job* j : e->resident)
-            if (j && !j->finished) ++running;
-        if (running > 0) {
-            if (e->map_dirty) e->upload_map();
-            if (e->n_wg) e->launch_chunk();
-            if (have_prev) {
-                hip_check(hipEventSynchronize(e->ev_chunk[(e->chunk_count - 2) & 1]), "chunk event");
-                handle_snapshot(e->chunk_count - 2);
+        // -- every lane keeps up to two chunks in flight and handles read-backs in order
+        bool busy = false;
+        for (uint32_t l = 0; l < e->cfg.lanes; ++l) {
+            Lane& L = e->lanes[l];
+            if (L.launched > L.processed) {
+                const uint64_t c = L.processed;
+                const bool must_wait = L.launched - L.processed >= 2 || L.running == 0;
+                hipError_t q = hipEventQuery(L.ev_chunk[c & 1]);
+                if (q == hipErrorNotReady && must_wait && e->cfg.lanes == 1) {
+                    hip_check(hipEventSynchronize(L.ev_chunk[c & 1]), "chunk event");
+                    q = hipSuccess;
+                }
+                if (q == hipSuccess) {
+                    handle_snapshot(l, c);
+                    ++L.processed;
+                    progressed = true;
+                } else if (q != hipErrorNotReady) {
+                    hip_check(q, "chunk event");
+                }
             }
-            have_prev = true;
-            continue;
+            if (L.running > 0 && L.launched - L.processed < 2) {
+                if (L.map_dirty) e->upload_map(L, l);
+                if (L.n_wg) {
+                    e->launch_chunk(L);
+                    progressed = true;
+                }
+            }
+            if (L.running > 0 || L.launched > L.processed) busy = true;
         }
-        if (have_prev) {
-            hip_check(hipEventSynchronize(e->ev_chunk[(e->chunk_count - 1) & 1]), "chunk event");
-            handle_snapshot(e->chunk_count - 1);
-            have_prev = false;
+        if (busy) {
+            if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
             continue;
         }
         if (!batches.empty()) {
@@ -986,6 +1049,9 @@ void ga_config_default(ga_config* cfg) {
     cfg->rounds = 1;
     cfg->adaptive_start = 0;
     cfg->grow_thresh = 32;
+    cfg->rounds_grid = 0;
+    cfg->lanes = 1;
+    cfg->deep_cells = 4ull << 20;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {

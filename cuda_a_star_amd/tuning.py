@@ -10,6 +10,8 @@ per launch):
   expansion rounds     heap pops per launch with local pushes in between
   adaptive width       heaps in use at admission, doubled on the device while the
                        open list exceeds grow_thresh entries per heap
+  launch lanes         streams with their own iteration clock; the largest grids
+                       (>= deep_cells cells) get their own lane
   graph replay         chunks replayed from a captured hipGraph or launched one by one
 
 `python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
@@ -46,6 +48,9 @@ class TuneConfig:
     use_graph: int = 0
     adaptive_start: int = 0     # 0 = every request uses all its heaps from the start
     grow_thresh: int = 32
+    rounds_grid: int = 0        # expansion rounds for grids; 0 = rounds
+    lanes: int = 1              # launch lanes (streams with their own iteration clock)
+    deep_cells: int = 4 << 20   # grids with at least this many cells take the last lane
 
     def heaps(self, kind: str, side: int) -> int:
         if kind == "puzzle":
@@ -61,7 +66,8 @@ class TuneConfig:
         return {"heap_arity": self.heap_arity, "chunk_iters": self.chunk_iters,
                 "pops_per_iter": self.pops_per_iter, "max_load_pct": self.max_load_pct,
                 "rounds": self.rounds, "use_graph": self.use_graph,
-                "adaptive_start": self.adaptive_start, "grow_thresh": self.grow_thresh}
+                "adaptive_start": self.adaptive_start, "grow_thresh": self.grow_thresh,
+                "rounds_grid": self.rounds_grid, "lanes": self.lanes, "deep_cells": self.deep_cells}
 
     def describe(self) -> Dict[str, int]:
         return asdict(self)
@@ -137,6 +143,9 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "use_graph": [0, 1],
             "adaptive_start": [0, 32, 128, 512],
             "grow_thresh": [8, 32, 128],
+            "rounds_grid": [0, 1, 4, 8],
+            "lanes": [1, 2, 3, 4],
+            "deep_cells": [1 << 20, 2 << 20, 4 << 20, 8 << 20],
         }
     else:
         axes = {
@@ -152,6 +161,7 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "use_graph": [0, 1],
             "adaptive_start": [0, 64, 256, 1024],
             "grow_thresh": [8, 32, 128],
+            "rounds_grid": [0, 1, 4, 8],
         }
     # coordinate descent: one axis at a time, two passes
     for _ in range(2):

@@ -97,6 +97,9 @@ typedef struct ga_config {
     uint32_t rounds;         /* expansion rounds per heap per launch (local pushes between rounds) */
     uint32_t adaptive_start; /* initial heaps in use per request (doubles on demand); 0 = all num_queues */
     uint32_t grow_thresh;    /* double the heaps in use when open entries exceed this many per heap */
+    uint32_t rounds_grid;    /* expansion rounds for grid requests; 0 = rounds */
+    uint32_t lanes;          /* launch lanes (streams with their own iteration clock), 1..4 */
+    uint64_t deep_cells;     /* grids with at least this many cells run on the last lane */
 } ga_config;
 
 /* Engine statistics (kernel timing is collected when cfg.profile != 0). */

@@ -117,7 +117,7 @@ struct InstDesc {
     uint32_t heap_stride;  // physical entries per heap
     uint32_t start_iter;   // value of the global iteration counter at the first step
     uint32_t a0;           // initial frontier width (heaps in use), a power of two <= n_heaps
-    uint32_t pad0;
+    uint32_t rounds;       // expansion rounds per launch for this instance (0 = the launch's default)
     uint8_t goal_rc[16];   // puzzle: row << 4 | col of each tile in the goal
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
@@ -683,7 +683,8 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
 
     // -- 3..6. expansion rounds
     if (live) {
-        for (uint32_t round = 0; round < prm.rounds; ++round) {
+        const uint32_t rounds = d.rounds ? d.rounds : prm.rounds;
+        for (uint32_t round = 0; round < rounds; ++round) {
             // pop, skipping entries whose g is no longer the stored one
             bool have = false;
             for (uint32_t t = 0; t < prm.max_pops; ++t) {
@@ -740,7 +741,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                     }
                 }
             }
-            if (round + 1 == prm.rounds) break;  // last round: records go to the candidate buffer
+            if (round + 1 == rounds) break;  // last round: records go to the candidate buffer
             push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
 #pragma unroll
             for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;

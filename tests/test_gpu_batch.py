@@ -126,6 +126,19 @@ def test_serve_stream_matches_alone(engine):
         assert alone.cost == sol.cost
 
 
+@pytest.mark.parametrize("lanes", [2, 3])
+def test_serve_stream_on_several_lanes(lanes):
+    """Large grids on their own launch lanes: results equal the oracle, and
+    requests still overlap."""
+    insts = serving_stream(n=48, qps=800.0, seed=40 + lanes)
+    with ga.Engine(0, lanes=lanes, deep_cells=1 << 16, rounds=2, max_iterations=2_000_000) as e:
+        sols, ss = e.serve([x.request() for x in insts], [x.arrival_ns for x in insts])
+    assert ss.completed == len(insts) and ss.max_inflight > 1
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK and sol.cost == oracle_cost(x)
+        assert replay_ok(x, sol)
+
+
 def test_serve_tight_budget_admits_in_turn():
     insts = serving_stream(n=24, qps=5000.0, seed=5)
     with ga.Engine(0, budget_bytes=512 << 20, chunk_bytes=256 << 20, max_iterations=2_000_000) as e:
@@ -144,16 +157,32 @@ def test_serve_request_over_budget_is_rejected():
     assert sols[0].status == ga.E_BUDGET
 
 
-@pytest.mark.parametrize("arity,rounds,graph,pops,adaptive,grow", [
-    (4, 1, 1, 1, 0, 32), (8, 1, 0, 2, 0, 32), (16, 2, 0, 1, 0, 32), (4, 4, 0, 2, 0, 32),
-    (8, 8, 1, 4, 0, 32), (4, 2, 1, 2, 0, 32),
-    (4, 1, 0, 2, 16, 2), (8, 2, 1, 2, 64, 4), (16, 1, 0, 1, 1, 1), (8, 2, 0, 2, 256, 32)])
-def test_engine_variants_match_oracle(arity, rounds, graph, pops, adaptive, grow):
+VARIANTS = [
+    dict(heap_arity=4, rounds=1, use_graph=1, pops_per_iter=1),
+    dict(heap_arity=8, rounds=1, use_graph=0, pops_per_iter=2),
+    dict(heap_arity=16, rounds=2, use_graph=0, pops_per_iter=1),
+    dict(heap_arity=4, rounds=4, use_graph=0, pops_per_iter=2),
+    dict(heap_arity=8, rounds=8, use_graph=1, pops_per_iter=4),
+    dict(heap_arity=4, rounds=2, use_graph=1, pops_per_iter=2),
+    dict(heap_arity=4, rounds=1, pops_per_iter=2, adaptive_start=16, grow_thresh=2),
+    dict(heap_arity=8, rounds=2, use_graph=1, adaptive_start=64, grow_thresh=4),
+    dict(heap_arity=16, rounds=1, adaptive_start=1, grow_thresh=1),
+    dict(heap_arity=8, rounds=2, adaptive_start=256, grow_thresh=32),
+    dict(heap_arity=4, rounds=2, rounds_grid=16),
+    dict(heap_arity=8, rounds=1, rounds_grid=8, adaptive_start=32, grow_thresh=8),
+    dict(heap_arity=4, rounds=2, lanes=2, deep_cells=1 << 14),
+    dict(heap_arity=8, rounds=2, lanes=3, deep_cells=1 << 16, use_graph=1),
+    dict(heap_arity=4, rounds=1, lanes=4, deep_cells=1 << 15, chunk_iters=3),
+]
+
+
+@pytest.mark.parametrize("variant", VARIANTS, ids=lambda v: "-".join(f"{k}{x}" for k, x in v.items()))
+def test_engine_variants_match_oracle(variant):
     """Every tunable combination the autotuner may pick keeps costs exact,
     including a frontier width that starts small and doubles on the device."""
-    insts = small_mix(seed=31 + arity + rounds + adaptive, n_puz=12, n_grid=8)
-    with ga.Engine(0, heap_arity=arity, rounds=rounds, use_graph=graph, pops_per_iter=pops,
-                   adaptive_start=adaptive, grow_thresh=grow, max_iterations=2_000_000) as e:
+    insts = small_mix(seed=31 + variant["heap_arity"] + variant["rounds"] + variant.get("adaptive_start", 0),
+                      n_puz=12, n_grid=8)
+    with ga.Engine(0, max_iterations=2_000_000, **variant) as e:
         sols = e.solve_batch([x.request() for x in insts])
         alone = [e.solve(insts[i].request()) for i in (0, len(insts) - 1)]
     for x, sol in zip(insts, sols):
