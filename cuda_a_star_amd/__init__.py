@@ -89,6 +89,8 @@ class Config(ctypes.Structure):
         ("rounds_grid", ctypes.c_uint32),
         ("lanes", ctypes.c_uint32),
         ("deep_cells", ctypes.c_uint64),
+        ("slack_puzzle", ctypes.c_uint32),
+        ("slack_grid", ctypes.c_uint32),
     ]
 
 

@@ -478,6 +478,10 @@ struct ga_engine {
         d.a0 = cfg.adaptive_start ? (uint32_t)std::min<uint64_t>(p.n_heaps, pow2_floor(cfg.adaptive_start))
                                   : p.n_heaps;
         d.rounds = (r.kind == GA_GRID && cfg.rounds_grid) ? cfg.rounds_grid : cfg.rounds;
+        {
+            const uint32_t slack = r.kind == GA_GRID ? cfg.slack_grid : cfg.slack_puzzle;
+            d.f_slack = slack ? slack - 1 : kInf;  // 0 = no gating; k = slack k-1
+        }
         const uint32_t fill_grid = 1024;
         hipStream_t stream = L.stream;
         if (r.kind == GA_PUZZLE) {
@@ -1052,6 +1056,8 @@ void ga_config_default(ga_config* cfg) {
     cfg->rounds_grid = 0;
     cfg->lanes = 1;
     cfg->deep_cells = 4ull << 20;
+    cfg->slack_puzzle = 0;
+    cfg->slack_grid = 0;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {

@@ -12,6 +12,8 @@ per launch):
                        open list exceeds grow_thresh entries per heap
   launch lanes         streams with their own iteration clock; the largest grids
                        (>= deep_cells cells) get their own lane
+  f slack              a heap expands its top only within this much of the last
+                       iteration's minimum f (less speculative expansion)
   graph replay         chunks replayed from a captured hipGraph or launched one by one
 
 `python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
@@ -51,6 +53,8 @@ class TuneConfig:
     rounds_grid: int = 0        # expansion rounds for grids; 0 = rounds
     lanes: int = 1              # launch lanes (streams with their own iteration clock)
     deep_cells: int = 4 << 20   # grids with at least this many cells take the last lane
+    slack_puzzle: int = 0       # 0 = off; k: expand a heap top only if f <= previous min f + k - 1
+    slack_grid: int = 0
 
     def heaps(self, kind: str, side: int) -> int:
         if kind == "puzzle":
@@ -67,7 +71,8 @@ class TuneConfig:
                 "pops_per_iter": self.pops_per_iter, "max_load_pct": self.max_load_pct,
                 "rounds": self.rounds, "use_graph": self.use_graph,
                 "adaptive_start": self.adaptive_start, "grow_thresh": self.grow_thresh,
-                "rounds_grid": self.rounds_grid, "lanes": self.lanes, "deep_cells": self.deep_cells}
+                "rounds_grid": self.rounds_grid, "lanes": self.lanes, "deep_cells": self.deep_cells,
+                "slack_puzzle": self.slack_puzzle, "slack_grid": self.slack_grid}
 
     def describe(self) -> Dict[str, int]:
         return asdict(self)
@@ -146,6 +151,8 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "rounds_grid": [0, 1, 4, 8],
             "lanes": [1, 2, 3, 4],
             "deep_cells": [1 << 20, 2 << 20, 4 << 20, 8 << 20],
+            "slack_puzzle": [0, 1, 3, 5, 9],
+            "slack_grid": [0, 1, 2, 4, 8, 16],
         }
     else:
         axes = {
@@ -162,6 +169,8 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "adaptive_start": [0, 64, 256, 1024],
             "grow_thresh": [8, 32, 128],
             "rounds_grid": [0, 1, 4, 8],
+            "slack_puzzle": [0, 1, 3, 5, 9],
+            "slack_grid": [0, 1, 2, 4, 8, 16],
         }
     # coordinate descent: one axis at a time, two passes
     for _ in range(2):

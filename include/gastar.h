@@ -100,6 +100,8 @@ typedef struct ga_config {
     uint32_t rounds_grid;    /* expansion rounds for grid requests; 0 = rounds */
     uint32_t lanes;          /* launch lanes (streams with their own iteration clock), 1..4 */
     uint64_t deep_cells;     /* grids with at least this many cells run on the last lane */
+    uint32_t slack_puzzle;   /* 0 = off; k: expand a heap top only if f <= previous min f + k - 1 */
+    uint32_t slack_grid;     /* the same for grids */
 } ga_config;
 
 /* Engine statistics (kernel timing is collected when cfg.profile != 0). */

@@ -267,6 +267,7 @@ std::pair<std::vector<T>, std::chrono::nanoseconds> solve(const T& start, const 
         d.kind = KIND_USER;
         d.n_heaps = n;
         d.a0 = n < 256 ? n : 256;  // the frontier width starts at 256 heaps and doubles on demand
+        d.f_slack = kInf;          // no gating of heap pops
         d.heap_cap = heap_cap;
         d.heap_stride = (heap_cap + kW - 1 + kW - 1) / kW * kW;
         d.heaps = (HeapEntry*)b.get(sizeof(HeapEntry) * (size_t)n * d.heap_stride);

@@ -118,6 +118,8 @@ struct InstDesc {
     uint32_t start_iter;   // value of the global iteration counter at the first step
     uint32_t a0;           // initial frontier width (heaps in use), a power of two <= n_heaps
     uint32_t rounds;       // expansion rounds per launch for this instance (0 = the launch's default)
+    uint32_t f_slack;      // expand a heap's top only if f <= previous min f + f_slack (kInf = always)
+    uint32_t pad1;
     uint8_t goal_rc[16];   // puzzle: row << 4 | col of each tile in the goal
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
@@ -212,13 +214,14 @@ __device__ inline void set_status(InstState* st, uint32_t s) {
 // ---------------------------------------------------------------------------
 // Removes the minimum.  The root, the last entry and the root's W children
 // are read in one round trip; the popped state's closed-set entry (key and
-// g|parent, via `load`) is then read while the sift-down runs.
+// g|parent, via `load`) is then read while the sift-down runs.  If the root's
+// f exceeds gate_f the heap is left as it is and false is returned (`top`
+// still holds the root).
 template <int W, class Load, class Key>
-__device__ inline HeapEntry heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& levels, Load load, Key& key,
-                                     uint64_t& val) {
-    const HeapEntry top = H[0];
+__device__ inline bool heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& levels, Load load, Key& key,
+                                uint64_t& val, uint32_t gate_f, HeapEntry& top) {
+    top = H[0];
     const uint32_t n = S - 1;
-    S = n;
     HeapEntry last;
     last.prio = ~0ull;
     last.slot = 0;
@@ -230,8 +233,10 @@ __device__ inline HeapEntry heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32
     e.slot = 0;
     e.aux = 0;
     if (idx < n) e = H[idx];
+    if (prio_f(top.prio) > gate_f) return false;
+    S = n;
     load(top.slot, key, val);
-    if (n == 0) return top;
+    if (n == 0) return true;
     uint32_t j = 0;
     for (;;) {
         // e holds child W*j + 1 + lane of node j (or +inf past the end)
@@ -256,7 +261,7 @@ __device__ inline HeapEntry heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32
         if (idx < n) e = H[idx];
     }
     if (lane == 0) H[j] = last;
-    return top;
+    return true;
 }
 
 template <int W>
@@ -659,23 +664,30 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     // -- 2. first pop of this launch.  Its root is the heap's top after the
     //       pushes and feeds the min-f reduction (HeapsHost::minimum,
     //       heaps.h:291-294); a stale root only lowers the minimum (safe).
+    //       A root whose f is more than f_slack above the previous iteration's
+    //       minimum is left in place: the heap sits this launch out.
     auto load = [&](uint32_t slot, typename Ops::Key& k, uint64_t& v) { Ops::load(d, slot, k, v); };
+    const uint32_t gate = (it > 0 && mt != kInf && d.f_slack != kInf)
+                              ? (uint32_t)min((uint64_t)mt + d.f_slack, (uint64_t)kInf - 1)
+                              : kInf;
     HeapEntry top;
     top.prio = 0;
     top.slot = 0;
     top.aux = 0;
     typename Ops::Key top_key{};
     uint64_t top_val = 0;
-    bool popped = false;
+    bool popped = false, gated = false, have_root = false;
     if (live && S > 0) {
-        top = heap_pop<W>(H, S, lane, n_down, load, top_key, top_val);
+        popped = heap_pop<W>(H, S, lane, n_down, load, top_key, top_val, gate, top);
+        gated = !popped;
+        have_root = true;
         group_fence();
-        ++n_pops;
-        popped = true;
+        if (popped) ++n_pops;
     }
-    uint32_t top_f = (popped && lane == 0) ? prio_f(top.prio) : kInf;
+    uint32_t top_f = (have_root && lane == 0) ? prio_f(top.prio) : kInf;
     top_f = wave_min_u32(top_f);
-    const uint32_t open = wave_sum_u32((popped && lane == 0) ? S + 1 : 0u);  // entries after the pushes
+    // entries after the pushes
+    const uint32_t open = wave_sum_u32((have_root && lane == 0) ? (popped ? S + 1 : S) : 0u);
     if ((threadIdx.x & 63) == 0) {
         if (top_f != kInf) atomicMin(&st->min_top[it % 3], top_f);
         if (open) atomicAdd(&st->open_sum[it % 3], open);
@@ -684,13 +696,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     // -- 3..6. expansion rounds
     if (live) {
         const uint32_t rounds = d.rounds ? d.rounds : prm.rounds;
-        for (uint32_t round = 0; round < rounds; ++round) {
+        for (uint32_t round = 0; round < rounds && !gated; ++round) {
             // pop, skipping entries whose g is no longer the stored one
             bool have = false;
             for (uint32_t t = 0; t < prm.max_pops; ++t) {
                 if (!popped) {
                     if (S == 0) break;
-                    top = heap_pop<W>(H, S, lane, n_down, load, top_key, top_val);
+                    if (!heap_pop<W>(H, S, lane, n_down, load, top_key, top_val, gate, top)) {
+                        gated = true;
+                        break;
+                    }
                     group_fence();
                     ++n_pops;
                 }

@@ -173,6 +173,9 @@ VARIANTS = [
     dict(heap_arity=4, rounds=2, lanes=2, deep_cells=1 << 14),
     dict(heap_arity=8, rounds=2, lanes=3, deep_cells=1 << 16, use_graph=1),
     dict(heap_arity=4, rounds=1, lanes=4, deep_cells=1 << 15, chunk_iters=3),
+    dict(heap_arity=4, rounds=2, slack_puzzle=1, slack_grid=1),
+    dict(heap_arity=8, rounds=1, slack_puzzle=3, slack_grid=5, lanes=2, deep_cells=1 << 14),
+    dict(heap_arity=4, rounds=2, slack_grid=2, adaptive_start=16, grow_thresh=4),
 ]
 
 
