@@ -104,6 +104,8 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-profile", action="store_true", help="skip per-step HIP events and counters")
+    ap.add_argument("--time-every", type=int, default=8,
+                    help="time the step launches of every n-th chunk with HIP events (counters cover all)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
     ap.add_argument("--override", default="", help="served tuning overrides, e.g. lanes=3,rounds=2")
@@ -142,7 +144,7 @@ def main() -> int:
         k, v = kv.split("=")
         setattr(served_cfg, k, int(v))
     e = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30,
-                  profile=0 if args.no_profile else 1, **served_cfg.engine)
+                  profile=0 if args.no_profile else max(1, args.time_every), **served_cfg.engine)
     dev = {}
     t = time.perf_counter()
     for i, x in enumerate(insts):
@@ -205,8 +207,8 @@ def main() -> int:
 
     # ---- kernel roofline (timed segments) ----
     roofline = None
-    if not args.no_profile and st.step_launches:
-        avg_ms = st.step_ms_total / st.step_launches
+    if not args.no_profile and st.step_launches and st.timed_launches:
+        avg_ms = st.step_ms_total / st.timed_launches  # launches of the sampled chunks
         bytes_per_launch = st.algo_bytes / st.step_launches
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         traffic, traffic_src = None, None
@@ -221,6 +223,7 @@ def main() -> int:
                     "algo_bytes_per_launch": round(bytes_per_launch, 1),
                     "expansions_per_launch": round(st.expanded / st.step_launches, 1),
                     "launches": st.step_launches,
+                    "timed_launches": st.timed_launches,
                     # global atomics issued by the step kernel: one CAS per closed-set probe (hashed sets)
                     # plus one atomicMin per lookup; counters and reductions excluded
                     "atomics_per_launch": round((st.probes + st.lookups) / st.step_launches, 1),

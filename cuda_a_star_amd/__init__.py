@@ -111,6 +111,7 @@ class EngineStats(ctypes.Structure):
         ("inserted", ctypes.c_uint64),
         ("lookups", ctypes.c_uint64),
         ("algo_bytes", ctypes.c_double),
+        ("timed_launches", ctypes.c_uint64),
     ]
 
 

@@ -39,18 +39,25 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
                                                       const InstDesc* __restrict__ descs, InstState* states,
                                                       const uint32_t* __restrict__ iter_base, uint32_t r,
                                                       StepParams prm) {
+    __shared__ uint32_t tally[kTallies];
     uint32_t inst, it, first, cover;
-    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover)) return;
+    bool running;
+    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover, running)) return;
+    tally_begin(tally);
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / W;
     const bool active = q < d.n_heaps;
-    switch (d.kind) {
-        case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm); break;
-        case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm); break;
-        case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm); break;
-        case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm); break;
-        default: break;
+    const Tally tl{tally, prm.profile != 0};
+    if (running) {
+        switch (d.kind) {
+            case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm, tl); break;
+            case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm, tl); break;
+            case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm, tl); break;
+            case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm, tl); break;
+            default: break;
+        }
     }
+    tally_end(tally, states + inst, it);
 }
 
 __global__ void seed_kernel(InstDesc d, InstDesc* descs, InstState* states, uint32_t slot, int W) {
@@ -293,6 +300,7 @@ struct Lane {
     hipEvent_t ev_chunk[2] = {nullptr, nullptr};
     uint32_t chunk_reads[2] = {0, 0};
     std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
+    bool timed[2] = {false, false};      // profile: whether that chunk's steps were timed
     hipGraphExec_t graph_exec = nullptr;
     uint32_t running = 0;     // resident jobs not yet finished
     std::vector<int> free_slots;
@@ -624,6 +632,10 @@ struct ga_engine {
                 pe.push_back(ev);
             }
         }
+        // profile n > 1 times every n-th chunk of the lane (HIP events cost a few
+        // microseconds of command-processor time each)
+        const bool timed = cfg.profile && (cfg.profile == 1 || L.launched % cfg.profile == 0);
+        L.timed[b] = timed;
         if (cfg.use_graph) {
             if (!L.graph_exec) {
                 hipGraph_t g;
@@ -633,11 +645,11 @@ struct ga_engine {
                 hip_check(hipGraphInstantiate(&L.graph_exec, g, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(g);
             }
-            if (cfg.profile) hip_check(hipEventRecord(pe[0], L.stream), "event");
+            if (timed) hip_check(hipEventRecord(pe[0], L.stream), "event");
             hip_check(hipGraphLaunch(L.graph_exec, L.stream), "graph launch");
-            if (cfg.profile) hip_check(hipEventRecord(pe[1], L.stream), "event");
+            if (timed) hip_check(hipEventRecord(pe[1], L.stream), "event");
         } else {
-            enqueue_steps(L, L.n_wg, prm, cfg.profile ? &pe : nullptr);
+            enqueue_steps(L, L.n_wg, prm, timed ? &pe : nullptr);
         }
         hip_check(hipGetLastError(), "step launch");
         L.host_base += R;
@@ -655,8 +667,9 @@ struct ga_engine {
 
     // Called once the chunk's read-back event has completed.
     void collect_profile(Lane& L, uint64_t chunk) {
-        if (!cfg.profile) return;
+        if (!cfg.profile || !L.timed[chunk & 1]) return;
         std::vector<hipEvent_t>& pe = L.prof_ev[chunk & 1];
+        stats.timed_launches += cfg.chunk_iters;
         // graph mode times the whole chunk (R steps + the counter advance)
         const size_t pairs = cfg.use_graph ? 1 : cfg.chunk_iters;
         for (size_t r = 0; r < pairs && 2 * r + 1 < pe.size(); ++r) {
@@ -828,8 +841,21 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
                     e->finish(*j, GA_E_OVERFLOW, t0);
                 } else {
                     ++j->retries;
-                    if (st.status == ST_OVERFLOW_HEAP) j->heap_shift += 2;
-                    else j->table_shift += 2;
+                    // 4x the structure that overflowed; 2x the other one if it was
+                    // more than a quarter full (open entries per heap, stored states),
+                    // so that a search which outgrows both does not use up its retries
+                    // one structure at a time
+                    const ga_engine::Plan p = e->plan(*j);
+                    const uint32_t open = std::max({st.open_sum[0], st.open_sum[1], st.open_sum[2]});
+                    const bool heaps_full = (uint64_t)open * 4 > (uint64_t)p.heap_cap * p.n_heaps;
+                    const bool table_full = p.table_slots && st.occupied * 4 > p.table_slots;
+                    if (st.status == ST_OVERFLOW_HEAP) {
+                        j->heap_shift += 2;
+                        if (table_full) j->table_shift += 1;
+                    } else {
+                        j->table_shift += 2;
+                        if (heaps_full) j->heap_shift += 1;
+                    }
                     j->finished = false;
                     retry.push_back(j);
                 }

@@ -19,7 +19,9 @@ per launch):
 `python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
 candidates on a seeded sample of the serving workload on device 0 and writes
 the winners to tuning.json next to this file; `load()` reads them (or the
-built-in defaults when no file exists).
+built-in defaults when no file exists).  Served candidates are scored on a
+Poisson arrival stream (ga_serve_run, solves per second of the stream's wall
+time); plain candidates on the same requests solved one at a time.
 """
 from __future__ import annotations
 
@@ -102,12 +104,11 @@ def _measure_served(insts, dev, cfg: TuneConfig) -> float:
     with ga.Engine(0, budget_bytes=200 << 30, chunk_bytes=8 << 30, **cfg.engine) as e:
         reqs = [x.request(cells_device=dev.get(i), num_queues=cfg.heaps(x.kind, x.side))
                 for i, x in enumerate(insts)]
-        t = time.perf_counter()
-        sols = e.solve_batch(reqs)
-        dt = time.perf_counter() - t
-    if any(s.status != ga.OK for s in sols):
+        t0 = insts[0].arrival_ns
+        sols, ss = e.serve(reqs, [x.arrival_ns - t0 for x in insts])
+    if any(s.status != ga.OK for s in sols) or not ss.wall_ns:
         return 0.0
-    return len(sols) / dt
+    return len(sols) / (ss.wall_ns / 1e9)
 
 
 def _measure_plain(insts, dev, cfg: TuneConfig) -> float:
@@ -123,11 +124,11 @@ def _measure_plain(insts, dev, cfg: TuneConfig) -> float:
     return len(insts) / dt
 
 
-def sweep(mode: str, n: int, seed: int) -> TuneConfig:
+def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
     import cuda_a_star_amd as ga
     from cuda_a_star_amd import workloads as wl
 
-    insts = wl.config5(seed, n, 1e9, threads=16)
+    insts = wl.config5(seed, n, qps, threads=16)
     up = ga.Engine(0)
     dev = {i: up.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
     measure = _measure_served if mode == "served" else _measure_plain
@@ -137,9 +138,9 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
     if mode == "served":
         axes = {
             "puzzle_heaps": [256, 512, 1024, 2048, 4096, 8192],
-            "grid_heaps_small": [256, 512, 1024, 2048],
-            "grid_heaps_mid": [512, 1024, 2048, 4096],
-            "grid_heaps_big": [1024, 2048, 4096, 8192],
+            "grid_heaps_small": [128, 256, 512, 1024, 2048],
+            "grid_heaps_mid": [256, 512, 1024, 2048, 4096],
+            "grid_heaps_big": [512, 1024, 2048, 4096, 8192],
             "heap_arity": [4, 8],
             "chunk_iters": [4, 8, 16, 32],
             "pops_per_iter": [1, 2, 4],
@@ -152,7 +153,7 @@ def sweep(mode: str, n: int, seed: int) -> TuneConfig:
             "lanes": [1, 2, 3, 4],
             "deep_cells": [1 << 20, 2 << 20, 4 << 20, 8 << 20],
             "slack_puzzle": [0, 1, 3, 5, 9],
-            "slack_grid": [0, 1, 2, 4, 8, 16],
+            "slack_grid": [0, 8, 16, 24, 32, 48],
         }
     else:
         axes = {
@@ -195,6 +196,7 @@ def main(argv=None) -> int:
     ap.add_argument("--mode", choices=["served", "plain", "both"], default="both")
     ap.add_argument("--n", type=int, default=128, help="requests in the tuning sample")
     ap.add_argument("--seed", type=int, default=77)
+    ap.add_argument("--qps", type=float, default=1500.0, help="offered load of the served sample")
     ap.add_argument("--out", default=TUNING_FILE, help="where to write the winners")
     args = ap.parse_args(argv)
     data = {}
@@ -204,7 +206,7 @@ def main(argv=None) -> int:
     modes = ["served", "plain"] if args.mode == "both" else [args.mode]
     for m in modes:
         n = args.n if m == "served" else max(8, args.n // 8)
-        data[m] = asdict(sweep(m, n, args.seed))
+        data[m] = asdict(sweep(m, n, args.seed, args.qps))
         with open(args.out, "w") as f:
             json.dump(data, f, indent=2, sort_keys=True)
     return 0

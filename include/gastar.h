@@ -93,7 +93,8 @@ typedef struct ga_config {
     uint32_t heap_arity;     /* lanes per heap = heap arity: 4, 8 or 16 */
     uint32_t max_load_pct;   /* closed-set load factor limit, percent */
     uint32_t pops_per_iter;  /* heap pops tried per heap per iteration (stale skip) */
-    uint32_t profile;        /* time every step with HIP events */
+    uint32_t profile;        /* 0 off; 1 count work and time every step with HIP events;
+                                n > 1 count every step, time the steps of every n-th chunk */
     uint32_t rounds;         /* expansion rounds per heap per launch (local pushes between rounds) */
     uint32_t adaptive_start; /* initial heaps in use per request (doubles on demand); 0 = all num_queues */
     uint32_t grow_thresh;    /* double the heaps in use when open entries exceed this many per heap */
@@ -121,6 +122,7 @@ typedef struct ga_engine_stats {
     uint64_t inserted;       /* successors that lowered a stored g (profile) */
     uint64_t lookups;        /* closed-set find-or-insert calls (profile) */
     double algo_bytes;       /* algorithmic bytes of the step kernels (profile) */
+    uint64_t timed_launches; /* step launches that step_ms_total covers */
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {

@@ -95,11 +95,11 @@ struct WideKeySet {
     static constexpr bool kDense = false;
     static constexpr uint32_t kMaxIters = 1u << 20;
 
-    __device__ static inline Key* keys(const InstDesc& d) { return reinterpret_cast<Key*>(d.keys); }
+    __device__ static inline gptr<Key> keys(const InstDesc& d) { return as_global(reinterpret_cast<Key*>(d.keys)); }
     __device__ static inline Key load_key(const InstDesc& d, uint32_t slot) {
         // agent-scope loads: the key may have been written earlier in this launch
         Key k;
-        const uint32_t* src = keys(d)[slot].w;
+        const gptr<const uint32_t> src = keys(d)[slot].w;
         for (size_t i = 0; i < sizeof(T) / 4; ++i)
             k.w[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return k;
@@ -111,9 +111,11 @@ struct WideKeySet {
     }
     __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
         key = load_key(d, slot);
-        val = d.table[slot].val;
+        val = as_global(d.table)[slot].val;
     }
-    __device__ static inline uint64_t* val_ptr(const InstDesc& d, uint32_t slot) { return &d.table[slot].val; }
+    __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
+        return &as_global(d.table)[slot].val;
+    }
     // All side effects (claim, key write, publish) happen inside the loop body
     // of the iteration that claims the slot; the loop exit carries none, so a
     // lane of the same wave that saw the slot busy finds it published on its
@@ -127,14 +129,16 @@ struct WideKeySet {
         int outcome = 0;  // 0 searching, 1 found, 2 inserted, 3 table exhausted
         for (uint32_t iter = 0; iter < kMaxIters; ++iter) {
             ++probes;
-            unsigned long long* tagp = (unsigned long long*)&d.table[i].key;
+            const gptr<uint64_t> tagp = &as_global(d.table)[i].key;
             uint64_t tag = __hip_atomic_load(tagp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             if (tag == kEmptyKey) {
-                const uint64_t prev = atomicCAS(tagp, (unsigned long long)kEmptyKey, (unsigned long long)kBusyKey);
+                uint64_t prev = kEmptyKey;  // receives the tag found in the slot
+                __hip_atomic_compare_exchange_strong(tagp, &prev, kBusyKey, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
                 if (prev == kEmptyKey) {
                     keys(d)[i] = key;
                     __threadfence();
-                    __hip_atomic_store(tagp, (unsigned long long)fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    __hip_atomic_store(tagp, fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     outcome = 2;
                 } else {
                     tag = prev;
@@ -161,7 +165,7 @@ struct WideKeySet {
         const uint64_t fp = h >> 2;
         uint64_t i = h & d.table_mask;
         for (uint64_t p = 0; p <= d.table_mask; ++p) {
-            const uint64_t tag = d.table[i].key;
+            const uint64_t tag = as_global(d.table)[i].key;
             if (tag == kEmptyKey) return false;
             if (tag == fp && same(load_key(d, (uint32_t)i), key)) {
                 slot = (uint32_t)i;
@@ -172,8 +176,8 @@ struct WideKeySet {
         return false;
     }
     __device__ static inline bool is_goal(const InstDesc& d, const Key& key) { return same(key, goal_key(d)); }
-    __device__ static inline Key start_key(const InstDesc& d) { return reinterpret_cast<const Key*>(d.user)[0]; }
-    __device__ static inline Key goal_key(const InstDesc& d) { return reinterpret_cast<const Key*>(d.user)[1]; }
+    __device__ static inline Key start_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[0]; }
+    __device__ static inline Key goal_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[1]; }
     __host__ __device__ static inline T unpack(const Key& k) { return from_raw<T>(k); }
     __host__ __device__ static inline Key pack(const T& t) { return to_raw<T>(t); }
 };
@@ -191,7 +195,8 @@ struct UserOps : KeySetFor<T> {
         return heur(Set::unpack(key));
     }
     __device__ static inline uint32_t aux_of(const InstDesc&, const Key&) { return 0; }
-    __device__ static inline bool succ(const InstDesc&, const Key& key, uint32_t, uint32_t, int i, Key& t,
+    __device__ static inline uint32_t prefetch(const InstDesc&, uint32_t, int) { return 0; }
+    __device__ static inline bool succ(const InstDesc&, const Key& key, uint32_t, uint32_t, int i, uint32_t, Key& t,
                                        uint32_t& taux, uint32_t& cost, uint32_t& th) {
         const T src = Set::unpack(key);
         T dst = src;
@@ -215,11 +220,15 @@ __global__ __launch_bounds__(kBlock) void user_step_kernel(const WgEntry* __rest
                                                            const InstDesc* __restrict__ descs, InstState* states,
                                                            const uint32_t* __restrict__ iter_base, uint32_t r,
                                                            StepParams prm) {
+    __shared__ uint32_t tally[kTallies];
     uint32_t inst, it, first, cover;
-    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover)) return;
+    bool running;
+    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover, running)) return;
+    tally_begin(tally);
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / kW;
-    run_group<kW, Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm);
+    if (running) run_group<kW, Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, Tally{tally, prm.profile != 0});
+    tally_end(tally, states + inst, it);
 }
 
 template <class Ops>

@@ -29,6 +29,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace gastar {
@@ -38,6 +39,24 @@ constexpr uint32_t kInf = 0xFFFFFFFFu;    // infinite cost
 constexpr uint64_t kEmptyKey = ~0ull;     // free closed-set slot
 constexpr uint64_t kInfVal = ~0ull;       // g = inf, parent = none
 constexpr uint64_t kNoRecord = ~0ull;     // empty candidate record
+
+// Instance memory as the kernels see it.  Pointers read out of InstDesc are
+// generic to the compiler, which would access them with flat instructions;
+// a flat access also counts against the LDS counter, so every LDS wait (the
+// wave shuffles included) would wait for all outstanding memory accesses.
+// The device code therefore goes through pointers typed as global.  (The
+// host pass only parses device code; there the type is a plain pointer.)
+#if defined(__HIP_DEVICE_COMPILE__)
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+#else
+template <class T>
+using gptr = T*;
+#endif
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T* p) {
+    return (gptr<T>)p;
+}
 
 // Instance status.
 enum : uint32_t {
@@ -150,6 +169,24 @@ struct InstState {
     unsigned long long lookups;   // closed-set find-or-insert calls (one atomicMin each)
 };
 
+// Per-workgroup tallies of one step, in LDS.  Lanes add to them with LDS
+// atomics (at most one per wave and event where the event is per lane), and
+// each workgroup flushes them to its InstState with one global atomic per
+// tally at the end of the step -- rather than every wave of an instance
+// hitting the same few InstState words.  Indices below kCounters follow the
+// counter fields of InstState from `occupied` on.
+enum : uint32_t {
+    TL_OCCUPIED, TL_EXPANDED, TL_GENERATED, TL_PUSHED, TL_POPS, TL_PROBES, TL_SIFT_DOWN, TL_SIFT_UP,
+    TL_GROUPS, TL_INSERTED, TL_LOOKUPS,
+    kCounters,
+    TL_MIN_TOP = kCounters,  // min f over the heap tops
+    TL_OPEN,                 // heap entries after the pushes
+    TL_WAVES_DONE,           // waves of the workgroup that have finished the step
+    kTallies
+};
+static_assert(offsetof(InstState, lookups) == offsetof(InstState, occupied) + 8 * (kCounters - 1),
+              "InstState counters must be contiguous and in tally order");
+
 // Workgroup -> (instance, first heap) map of one launch.  `cover` is the
 // number of heaps of the instance that the map covers (a power of two): the
 // frontier width may grow on the device up to it.
@@ -208,6 +245,63 @@ __device__ inline void set_status(InstState* st, uint32_t s) {
     atomicCAS(&st->status, (uint32_t)ST_RUNNING, s);
 }
 
+// The workgroup's tallies (kTallies words of LDS).  `prof` (uniform) turns
+// on the counters that only profiling needs.
+struct Tally {
+    uint32_t* c;
+    bool prof;
+
+    // v of every active lane (always counted)
+    __device__ __forceinline__ void add(uint32_t i, uint32_t v) const {
+        if (v) atomicAdd(&c[i], v);
+    }
+    // profile only: v of every active lane
+    __device__ __forceinline__ void prof_add(uint32_t i, uint32_t v) const {
+        if (prof && v) atomicAdd(&c[i], v);
+    }
+    // one per active lane where `cond` holds, one LDS atomic per wave (always counted)
+    __device__ __forceinline__ void count(uint32_t i, bool cond) const {
+        const uint64_t b = __ballot(cond);
+        const uint64_t act = __ballot(1);
+        if (b && __lane_id() == (uint32_t)__builtin_ctzll(act)) atomicAdd(&c[i], (uint32_t)__popcll(b));
+    }
+    // profile only: as count()
+    __device__ __forceinline__ void prof_count(uint32_t i, bool cond) const {
+        if (prof) count(i, cond);
+    }
+};
+
+// Clears the tallies; every thread of the workgroup calls it.
+__device__ inline void tally_begin(uint32_t* c) {
+    if (threadIdx.x < kTallies) c[threadIdx.x] = threadIdx.x == TL_MIN_TOP ? kInf : 0u;
+    __syncthreads();
+}
+
+// Adds the tallies to the instance's state for iteration `it`; every thread
+// of the workgroup calls it.  The last wave to arrive flushes, so waves that
+// finish early leave at once instead of waiting at a barrier.
+__device__ inline void tally_end(uint32_t* c, InstState* st, uint32_t it) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this wave's tally atomics first
+    uint32_t arrived = 0;
+    if (__lane_id() == 0) arrived = atomicAdd(&c[TL_WAVES_DONE], 1u);
+    arrived = __builtin_amdgcn_readfirstlane(arrived);
+    if (arrived + 1 != (blockDim.x + 63) / 64) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    const uint32_t i = __lane_id();
+    if (i >= TL_WAVES_DONE) return;
+    const uint32_t v = __hip_atomic_load(&c[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (i < kCounters) {
+        if (v) {
+            auto* ctr = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(st) + offsetof(InstState, occupied));
+            atomicAdd(ctr + i, (unsigned long long)v);
+        }
+    } else if (i == TL_MIN_TOP) {
+        if (v != kInf) atomicMin(&st->min_top[it % 3], v);
+    } else if (v) {
+        atomicAdd(&st->open_sum[it % 3], v);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // W-ary heap, logical index j at physical j + W - 1 so that the W children
 // W*j+1 .. W*j+W of node j start on a W-entry boundary (W*16 B aligned).
@@ -217,9 +311,9 @@ __device__ inline void set_status(InstState* st, uint32_t s) {
 // g|parent, via `load`) is then read while the sift-down runs.  If the root's
 // f exceeds gate_f the heap is left as it is and false is returned (`top`
 // still holds the root).
-template <int W, class Load, class Key>
-__device__ inline bool heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& levels, Load load, Key& key,
-                                uint64_t& val, uint32_t gate_f, HeapEntry& top) {
+template <int W, class Load, class Pre, class Key>
+__device__ inline bool heap_pop(gptr<HeapEntry> H, uint32_t& S, int lane, uint32_t& levels, Load load, Pre pre,
+                                Key& key, uint64_t& val, uint32_t gate_f, HeapEntry& top) {
     top = H[0];
     const uint32_t n = S - 1;
     HeapEntry last;
@@ -236,6 +330,7 @@ __device__ inline bool heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& l
     if (prio_f(top.prio) > gate_f) return false;
     S = n;
     load(top.slot, key, val);
+    pre(top);  // the problem's early reads for expanding top
     if (n == 0) return true;
     uint32_t j = 0;
     for (;;) {
@@ -265,7 +360,7 @@ __device__ inline bool heap_pop(HeapEntry* H, uint32_t& S, int lane, uint32_t& l
 }
 
 template <int W>
-__device__ inline void heap_push(HeapEntry* H, uint32_t S, const HeapEntry& x, uint32_t& levels) {
+__device__ inline void heap_push(gptr<HeapEntry> H, uint32_t S, const HeapEntry& x, uint32_t& levels) {
     uint32_t j = S;
     while (j > 0) {
         const uint32_t p = (j - 1) / W;
@@ -287,7 +382,7 @@ __device__ inline void heap_push(HeapEntry* H, uint32_t S, const HeapEntry& x, u
 // Open addressing with linear probing.  A key is claimed with a 64-bit CAS;
 // keys are never removed, so a non-empty key read with a plain load is exact
 // and a stale "empty" read is corrected by the CAS.
-__device__ inline bool hash_find_or_insert(Slot* table, uint64_t mask, uint64_t key, uint32_t max_probes,
+__device__ inline bool hash_find_or_insert(gptr<Slot> table, uint64_t mask, uint64_t key, uint32_t max_probes,
                                            uint32_t& slot, bool& isnew, uint32_t& probes) {
     // Every probe is one CAS(empty -> key): it claims a free slot, or returns the
     // key already there (found, or occupied by another key: probe on).  One
@@ -295,8 +390,9 @@ __device__ inline bool hash_find_or_insert(Slot* table, uint64_t mask, uint64_t 
     uint64_t i = mix64(key) & mask;
     for (uint32_t p = 0; p < max_probes; ++p) {
         ++probes;
-        const uint64_t prev = atomicCAS((unsigned long long*)&table[i].key, (unsigned long long)kEmptyKey,
-                                        (unsigned long long)key);
+        uint64_t prev = kEmptyKey;  // receives the key found in the slot
+        __hip_atomic_compare_exchange_strong(&table[i].key, &prev, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT);
         if (prev == kEmptyKey || prev == key) {
             slot = (uint32_t)i;
             isnew = prev == kEmptyKey;
@@ -307,7 +403,7 @@ __device__ inline bool hash_find_or_insert(Slot* table, uint64_t mask, uint64_t 
     return false;
 }
 
-__device__ inline bool hash_find(const Slot* table, uint64_t mask, uint64_t key, uint64_t max_probes,
+__device__ inline bool hash_find(gptr<const Slot> table, uint64_t mask, uint64_t key, uint64_t max_probes,
                                  uint32_t& slot) {
     uint64_t i = mix64(key) & mask;
     for (uint64_t p = 0; p < max_probes; ++p) {
@@ -333,22 +429,24 @@ struct HashedU64Set {
     using Key = uint64_t;
     static constexpr bool kDense = false;
     __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
-        const Slot s = d.table[slot];
+        const Slot s = as_global(d.table)[slot];
         key = s.key;
         val = s.val;
         // A key claimed by a CAS earlier in this launch may still read as empty
         // through a stale L1 line; keys are write-once, so re-read it at agent
         // scope.  (A stale val only makes the staleness test conservative.)
         if (key == kEmptyKey)
-            key = __hip_atomic_load(&d.table[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            key = __hip_atomic_load(&as_global(d.table)[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    __device__ static inline uint64_t* val_ptr(const InstDesc& d, uint32_t slot) { return &d.table[slot].val; }
+    __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
+        return &as_global(d.table)[slot].val;
+    }
     __device__ static inline bool find_or_insert(const InstDesc& d, Key key, uint32_t max_probes, uint32_t& slot,
                                                  bool& isnew, uint32_t& probes) {
-        return hash_find_or_insert(d.table, d.table_mask, key, max_probes, slot, isnew, probes);
+        return hash_find_or_insert(as_global(d.table), d.table_mask, key, max_probes, slot, isnew, probes);
     }
     __device__ static inline bool find(const InstDesc& d, Key key, uint32_t& slot) {
-        return hash_find(d.table, d.table_mask, key, d.table_mask + 1, slot);
+        return hash_find(as_global<const Slot>(d.table), d.table_mask, key, d.table_mask + 1, slot);
     }
     __device__ static inline bool is_goal(const InstDesc& d, Key key) { return key == d.goal; }
     __device__ static inline Key start_key(const InstDesc& d) { return d.start; }
@@ -361,9 +459,11 @@ struct DenseCellSet {
     static constexpr bool kDense = true;
     __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
         key = slot;
-        val = d.dense[slot];
+        val = as_global(d.dense)[slot];
     }
-    __device__ static inline uint64_t* val_ptr(const InstDesc& d, uint32_t slot) { return d.dense + slot; }
+    __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
+        return as_global(d.dense) + slot;
+    }
     __device__ static inline bool find_or_insert(const InstDesc&, Key key, uint32_t, uint32_t& slot, bool& isnew,
                                                  uint32_t& probes) {
         slot = (uint32_t)key;
@@ -419,9 +519,12 @@ struct PuzzleOps : HashedU64Set {
 
     __device__ static inline uint32_t aux_of(const InstDesc&, uint64_t) { return 0; }
 
+    // successors are computed from the state alone: nothing to read early
+    __device__ static inline uint32_t prefetch(const InstDesc&, uint32_t, int) { return 0; }
+
     // h of the successor from h of the parent: only the moved tile changes.
-    __device__ static inline bool succ(const InstDesc& d, uint64_t s, uint32_t, uint32_t h, int i, uint64_t& t,
-                                       uint32_t& taux, uint32_t& cost, uint32_t& th) {
+    __device__ static inline bool succ(const InstDesc& d, uint64_t s, uint32_t, uint32_t h, int i, uint32_t,
+                                       uint64_t& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
         const int b = blank_of(s);
         int r = b / S, c = b % S;
         if (i == 0) ++r;
@@ -464,16 +567,28 @@ struct GridOps : DenseCellSet {
         return hxy(d, aux & 0xFFFFu, aux >> 16);
     }
 
-    __device__ static inline bool succ(const InstDesc& d, uint64_t, uint32_t aux, uint32_t, int i, uint64_t& t,
-                                       uint32_t& taux, uint32_t& cost, uint32_t& th) {
+    // Weight of the cell that move i enters (0: wall or off the grid).  Read
+    // as soon as the node is popped, while its heap is still being sifted.
+    __device__ static inline uint32_t prefetch(const InstDesc& d, uint32_t aux, int i) {
+        uint32_t nx, ny;
+        if (!neighbour(d, aux, i, nx, ny)) return 0;
+        return as_global(d.cells)[(uint64_t)ny * d.width + nx];
+    }
+
+    __device__ static inline bool neighbour(const InstDesc& d, uint32_t aux, int i, uint32_t& nx, uint32_t& ny) {
         const int DX[8] = {1, -1, 0, 0, 1, 1, -1, -1};
         const int DY[8] = {0, 0, 1, -1, 1, -1, 1, -1};
-        const uint32_t nx = (aux & 0xFFFFu) + (uint32_t)DX[i];
-        const uint32_t ny = (aux >> 16) + (uint32_t)DY[i];
-        if (nx >= d.width || ny >= d.height) return false;  // also catches -1 wrap-around
+        nx = (aux & 0xFFFFu) + (uint32_t)DX[i];
+        ny = (aux >> 16) + (uint32_t)DY[i];
+        return nx < d.width && ny < d.height;  // also catches -1 wrap-around
+    }
+
+    // `w` = prefetch(d, aux, i)
+    __device__ static inline bool succ(const InstDesc& d, uint64_t, uint32_t aux, uint32_t, int i, uint32_t w,
+                                       uint64_t& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
+        uint32_t nx, ny;
+        if (w == 0 || !neighbour(d, aux, i, nx, ny)) return false;
         const uint64_t c = (uint64_t)ny * d.width + nx;
-        const uint32_t w = d.cells[c];
-        if (w == 0) return false;
         t = c;
         taux = (ny << 16) | nx;
         cost = w;
@@ -488,9 +603,8 @@ struct GridOps : DenseCellSet {
 // Pushes the K records held by the group's lanes (record j in lane j % W,
 // register j / W) into the group's own heap; lane 0 performs the pushes.
 template <int W, int KW>
-__device__ inline void push_records(HeapEntry* H, uint32_t& S, const HeapEntry (&rec)[KW], int K, int lane,
-                                    uint32_t best, uint32_t cap, bool& overflow, uint32_t& n_push,
-                                    uint32_t& n_up) {
+__device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEntry (&rec)[KW], int K, int lane,
+                                    uint32_t best, uint32_t cap, bool& overflow, const Tally& tl) {
     // Record j = r*W + l sits in lane l, register r.  Valid records take the
     // positions S, S+1, ... in j order.  When every new position's parent is an
     // old entry (true unless the heap is nearly empty), the lanes check their
@@ -529,13 +643,15 @@ __device__ inline void push_records(HeapEntry* H, uint32_t& S, const HeapEntry (
                 x.slot = __shfl(rec[r].slot, l, W);
                 x.aux = __shfl(rec[r].aux, l, W);
                 if (lane == 0) {
-                    heap_push<W>(H, S, x, n_up);
+                    uint32_t lv = 0;
+                    heap_push<W>(H, S, x, lv);
+                    tl.prof_add(TL_SIFT_UP, lv);
                     ++S;
-                    ++n_push;
                 }
             }
         }
         S = __shfl(S, 0, W);
+        if (lane == 0) tl.prof_add(TL_PUSHED, c);
         return;
     }
     bool bad[KW];
@@ -561,10 +677,14 @@ __device__ inline void push_records(HeapEntry* H, uint32_t& S, const HeapEntry (
             x.slot = __shfl(rec[r].slot, l, W);
             x.aux = __shfl(rec[r].aux, l, W);
             const uint32_t pos = S + (uint32_t)__shfl((int)rank[r], l, W);
-            if (lane == 0) heap_push<W>(H, pos, x, n_up);  // sift up from its leaf position
+            if (lane == 0) {  // sift up from its leaf position
+                uint32_t lv = 0;
+                heap_push<W>(H, pos, x, lv);
+                tl.prof_add(TL_SIFT_UP, lv);
+            }
         }
     }
-    if (lane == 0) n_push += c;
+    if (lane == 0) tl.prof_add(TL_PUSHED, c);
     S += c;
 }
 
@@ -577,7 +697,7 @@ __device__ inline void push_records(HeapEntry* H, uint32_t& S, const HeapEntry (
 // this heap, so the tops of all heaps form one snapshot of the open set.
 template <int W, class Ops>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t it, uint32_t q, bool active,
-                                          uint32_t cover, const StepParams& prm) {
+                                          uint32_t cover, const StepParams& prm, const Tally& tl) {
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
@@ -586,8 +706,12 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     // Frontier width: heaps q < A take part in this iteration; the records read
     // now were written by the A_w heaps of the previous iteration.  A only
     // grows, so a heap that joins starts empty and loses nothing.
-    const uint32_t A = __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[it % 3]));
-    const uint32_t Aw = it > 0 ? __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[(it - 1) % 3])) : A;
+    // With a fixed width (the default) every heap is in use from the start and
+    // the incoming records need not wait for these loads.
+    const bool fixed = d.a0 == 0 || d.a0 >= N;
+    const uint32_t A = fixed ? N : __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[it % 3]));
+    const uint32_t Aw =
+        (fixed || it == 0) ? A : __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[(it - 1) % 3]));
     active = active && q < A;
 
     // Independent loads first: the termination inputs and this heap's size and
@@ -596,7 +720,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     const uint32_t mt = it > 0 ? __builtin_amdgcn_readfirstlane(load_relaxed(&st->min_top[(it - 1) % 3])) : 0u;
     uint64_t occupied = 0;
     if constexpr (!Ops::kDense) occupied = load_relaxed(&st->occupied);
-    HeapEntry* H = d.heaps + (size_t)q * d.heap_stride + (W - 1);
+    const gptr<HeapEntry> H = as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1);
     uint32_t S = 0;
     HeapEntry rec[KW];
 #pragma unroll
@@ -606,10 +730,10 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         rec[r].aux = 0;
     }
     if (active) {
-        S = d.heap_size[q];
+        S = as_global(d.heap_size)[q];
         if (it > 0 && q < Aw) {
             // insert_nodes_into_queues, a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N
-            const HeapEntry* cin = d.cand + (size_t)((it - 1) & 1) * N * K;
+            const gptr<const HeapEntry> cin = as_global<const HeapEntry>(d.cand) + (size_t)((it - 1) & 1) * N * K;
             const uint32_t base = (q + it) & (Aw - 1);
 #pragma unroll
             for (int r = 0; r < KW; ++r) {
@@ -649,13 +773,11 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         }
     }
 
-    uint32_t n_push = 0, n_up = 0, n_down = 0, n_pops = 0, n_exp = 0, n_gen = 0, n_new = 0, n_probe = 0, n_ins = 0,
-             n_look = 0;
     bool overflow_heap = false, overflow_table = false;
 
     // -- 1. push the records addressed to this heap
     if (live && it > 0 && q < Aw) {
-        push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
+        push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl);
 #pragma unroll
         for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
     }
@@ -667,6 +789,14 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     //       A root whose f is more than f_slack above the previous iteration's
     //       minimum is left in place: the heap sits this launch out.
     auto load = [&](uint32_t slot, typename Ops::Key& k, uint64_t& v) { Ops::load(d, slot, k, v); };
+    uint32_t pre[KW];  // per successor: what the problem reads early (grid: the entered cell's weight)
+    auto prefetch = [&](const HeapEntry& t) {
+#pragma unroll
+        for (int r = 0; r < KW; ++r) {
+            const int i = r * W + lane;
+            pre[r] = i < K ? Ops::prefetch(d, t.aux, i) : 0u;
+        }
+    };
     const uint32_t gate = (it > 0 && mt != kInf && d.f_slack != kInf)
                               ? (uint32_t)min((uint64_t)mt + d.f_slack, (uint64_t)kInf - 1)
                               : kInf;
@@ -678,19 +808,23 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     uint64_t top_val = 0;
     bool popped = false, gated = false, have_root = false;
     if (live && S > 0) {
-        popped = heap_pop<W>(H, S, lane, n_down, load, top_key, top_val, gate, top);
+        uint32_t lv = 0;
+        popped = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_val, gate, top);
         gated = !popped;
         have_root = true;
         group_fence();
-        if (popped) ++n_pops;
+        if (lane == 0) {
+            tl.prof_add(TL_SIFT_DOWN, lv);
+            tl.prof_add(TL_POPS, popped ? 1u : 0u);
+        }
     }
     uint32_t top_f = (have_root && lane == 0) ? prio_f(top.prio) : kInf;
     top_f = wave_min_u32(top_f);
     // entries after the pushes
     const uint32_t open = wave_sum_u32((have_root && lane == 0) ? (popped ? S + 1 : S) : 0u);
     if ((threadIdx.x & 63) == 0) {
-        if (top_f != kInf) atomicMin(&st->min_top[it % 3], top_f);
-        if (open) atomicAdd(&st->open_sum[it % 3], open);
+        if (top_f != kInf) atomicMin(&tl.c[TL_MIN_TOP], top_f);
+        tl.add(TL_OPEN, open);
     }
 
     // -- 3..6. expansion rounds
@@ -702,12 +836,15 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             for (uint32_t t = 0; t < prm.max_pops; ++t) {
                 if (!popped) {
                     if (S == 0) break;
-                    if (!heap_pop<W>(H, S, lane, n_down, load, top_key, top_val, gate, top)) {
+                    uint32_t lv = 0;
+                    const bool ok = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_val, gate, top);
+                    if (lane == 0) tl.prof_add(TL_SIFT_DOWN, lv);
+                    if (!ok) {
                         gated = true;
                         break;
                     }
                     group_fence();
-                    ++n_pops;
+                    if (lane == 0) tl.prof_add(TL_POPS, 1u);
                 }
                 popped = false;
                 if (prio_g(top.prio) > val_g(top_val)) continue;  // stale
@@ -723,7 +860,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 continue;
             }
             if (best != kInf && f >= best) continue;
-            if (lane == 0) ++n_exp;
+            tl.count(TL_EXPANDED, lane == 0);
             // expand, insert into the closed set
             const uint32_t h = f - g;
 #pragma unroll
@@ -732,20 +869,22 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 rec[r].prio = kNoRecord;
                 typename Ops::Key tkey;
                 uint32_t taux, cost, th;
-                if (i < K && Ops::succ(d, top_key, top.aux, h, i, tkey, taux, cost, th)) {
-                    ++n_gen;
+                const bool gen = i < K && Ops::succ(d, top_key, top.aux, h, i, pre[r], tkey, taux, cost, th);
+                tl.count(TL_GENERATED, gen);
+                if (gen) {
                     const uint32_t g2 = g + cost, f2 = g2 + th;
                     if (f2 < best) {
                         uint32_t slot;
                         bool isnew;
-                        if (Ops::find_or_insert(d, tkey, prm.max_probes, slot, isnew, n_probe)) {
-                            ++n_look;
-                            n_new += isnew ? 1u : 0u;
+                        uint32_t np = 0;
+                        const bool found = Ops::find_or_insert(d, tkey, prm.max_probes, slot, isnew, np);
+                        bool ins = false;
+                        if (found) {
                             const uint64_t mine = ((uint64_t)g2 << 32) | top.slot;
-                            const uint64_t old = atomicMin((unsigned long long*)Ops::val_ptr(d, slot),
-                                                           (unsigned long long)mine);
-                            if (g2 < val_g(old)) {
-                                ++n_ins;
+                            const uint64_t old = __hip_atomic_fetch_min(Ops::val_ptr(d, slot), mine, __ATOMIC_RELAXED,
+                                                                        __HIP_MEMORY_SCOPE_AGENT);
+                            ins = g2 < val_g(old);
+                            if (ins) {
                                 rec[r].prio = make_prio(f2, g2);
                                 rec[r].slot = slot;
                                 rec[r].aux = taux;
@@ -753,18 +892,23 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                         } else {
                             overflow_table = true;
                         }
+                        if constexpr (!Ops::kDense) tl.count(TL_OCCUPIED, found && isnew);
+                        tl.prof_count(TL_PROBES, true);  // first probe of every lookup
+                        if (np > 1) tl.prof_add(TL_PROBES, np - 1);
+                        tl.prof_count(TL_LOOKUPS, found);
+                        tl.prof_count(TL_INSERTED, ins);
                     }
                 }
             }
             if (round + 1 == rounds) break;  // last round: records go to the candidate buffer
-            push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, n_push, n_up);
+            push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl);
 #pragma unroll
             for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
             group_fence();
         }
-        if (lane == 0) d.heap_size[q] = S;
+        if (lane == 0) as_global(d.heap_size)[q] = S;
         // candidate slice of this heap: K records, written every iteration
-        HeapEntry* cout = d.cand + (size_t)(it & 1) * N * K + (size_t)q * K;
+        const gptr<HeapEntry> cout = as_global(d.cand) + (size_t)(it & 1) * N * K + (size_t)q * K;
 #pragma unroll
         for (int r = 0; r < KW; ++r) {
             const int i = r * W + lane;
@@ -774,46 +918,19 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     if (overflow_heap) set_status(st, ST_OVERFLOW_HEAP);
     if (overflow_table) set_status(st, ST_OVERFLOW_TABLE);
 
-    // -- counters, one atomic per wave
-    const uint32_t w_new = wave_sum_u32(n_new);
-    const uint32_t w_exp = wave_sum_u32(n_exp);
-    const uint32_t w_gen = wave_sum_u32(n_gen);
-    const bool wave_lead = (threadIdx.x & 63) == 0;
-    if (wave_lead) {
-        if (w_new) atomicAdd(&st->occupied, (unsigned long long)w_new);
-        if (w_exp) atomicAdd(&st->expanded, (unsigned long long)w_exp);
-        if (w_gen) atomicAdd(&st->generated, (unsigned long long)w_gen);
-    }
-    if (prm.profile) {
-        const uint32_t w_push = wave_sum_u32(n_push);
-        const uint32_t w_pops = wave_sum_u32(lane == 0 ? n_pops : 0u);
-        const uint32_t w_probe = wave_sum_u32(n_probe);
-        const uint32_t w_down = wave_sum_u32(lane == 0 ? n_down : 0u);
-        const uint32_t w_up = wave_sum_u32(n_up);
-        const uint32_t w_groups = wave_sum_u32((live && lane == 0) ? 1u : 0u);
-        const uint32_t w_ins = wave_sum_u32(n_ins);
-        const uint32_t w_look = wave_sum_u32(n_look);
-        if (wave_lead) {
-            if (w_look) atomicAdd(&st->lookups, (unsigned long long)w_look);
-            if (w_groups) atomicAdd(&st->groups, (unsigned long long)w_groups);
-            if (w_ins) atomicAdd(&st->inserted, (unsigned long long)w_ins);
-            if (w_push) atomicAdd(&st->pushed, (unsigned long long)w_push);
-            if (w_pops) atomicAdd(&st->pops, (unsigned long long)w_pops);
-            if (w_probe) atomicAdd(&st->probes, (unsigned long long)w_probe);
-            if (w_down) atomicAdd(&st->sift_down, (unsigned long long)w_down);
-            if (w_up) atomicAdd(&st->sift_up, (unsigned long long)w_up);
-        }
-    }
+    tl.prof_count(TL_GROUPS, live && lane == 0);
 }
 
-// Per-workgroup prologue shared by every step kernel: returns false when the
-// workgroup has no running instance this iteration.
+// Per-workgroup prologue shared by every step kernel: returns false (for the
+// whole workgroup) when the launch map has no instance for it.  `running` is
+// this wave's reading of the instance status; a workgroup whose instance has
+// stopped still takes part in the tally barriers, with nothing to count.
 __device__ inline bool step_prologue(const WgEntry* wgmap, const InstDesc* descs, InstState* states,
                                      const uint32_t* iter_base, uint32_t r, uint32_t& inst, uint32_t& it,
-                                     uint32_t& first_heap, uint32_t& cover) {
+                                     uint32_t& first_heap, uint32_t& cover, bool& running) {
     const WgEntry we = wgmap[blockIdx.x];
     if (we.inst == kNone) return false;
-    if (load_relaxed(&states[we.inst].status) != ST_RUNNING) return false;
+    running = load_relaxed(&states[we.inst].status) == ST_RUNNING;
     inst = we.inst;
     first_heap = we.first_heap;
     cover = we.cover;

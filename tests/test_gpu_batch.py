@@ -83,6 +83,25 @@ def test_batch_forced_retries(engine):
         assert sol.status == ga.OK and sol.cost == oracle_cost(x)
 
 
+def test_overflow_grows_both_structures():
+    """Searches that outgrow both their heaps and their closed set: a retry grows
+    the structure that overflowed 4x and the other one 2x when it was more than a
+    quarter full, so the default retry budget still reaches the optimum."""
+    insts = [wl.puzzle15(wl.instance_seed(61, i), depth=48, num_queues=256, table_slots=0) for i in range(4)]
+    reqs = []
+    for x in insts:
+        r = x.request(heap_capacity=16)
+        r.table_slots = 1 << 10
+        reqs.append(r)
+    with ga.Engine(0, max_retries=6, max_iterations=2_000_000) as e:
+        sols = e.solve_batch(reqs)
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK, ga.status_str(sol.status)
+        assert sol.cost == oracle_cost(x)
+        assert sol.retries >= 2
+        assert replay_ok(x, sol)
+
+
 def test_batch_handles_invalid_and_unsolvable(engine):
     g = ga.puzzle_goal(4)
     t = ga.state_to_board(g, 4)
