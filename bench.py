@@ -15,8 +15,12 @@ value = completed solves / wall seconds of the timed stream.
 
 Also measured (outside the timed region): the same requests solved one at a
 time by plain GA* (ga_solve), and the CPU oracle on the host cores.
-The step kernel's roofline comes from HIP events around every step launch and
-the device counters of the timed segments.
+The step kernel's roofline comes from HIP events on the engine's launch
+streams (the step launches of every --time-every-th chunk; with graph replay
+one event pair brackets a chunk) and the device counters of the timed
+segments.  Its HBM traffic is this run's algorithmic bytes per launch times
+the HBM-bytes-per-algorithmic-byte ratio of the committed rocprofv3 counter
+runs (profiles/step_kernel_traffic.json).
 """
 from __future__ import annotations
 
@@ -170,6 +174,7 @@ def main() -> int:
             f"({ss.solves_per_sec:.1f}/s, p50 {ss.p50_ms:.1f} ms, p99 {ss.p99_ms:.1f} ms)")
 
     # ---- timed: K steps of M requests, served as one continuous arrival stream ----
+    launches_before = e.stats().step_launches  # step dispatches of this process before the timed segment
     e.reset_stats()
     lat, ok, failed, served_costs = [], 0, 0, {}
     try:
@@ -213,10 +218,13 @@ def main() -> int:
         achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
         traffic, traffic_src = None, None
         tpath = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
-        if os.path.exists(tpath):  # HBM bytes per launch from the committed rocprofv3 --pmc runs
+        if os.path.exists(tpath):  # HBM bytes per algorithmic byte from the committed rocprofv3 --pmc runs
             with open(tpath) as f:
                 tj = json.load(f)
-            traffic, traffic_src = round(tj["traffic_bytes_per_launch"], 1), f"profiles/{os.path.basename(tpath)} ({tj['workload']})"
+            if "hbm_bytes_per_algo_byte" in tj:
+                traffic = round(tj["hbm_bytes_per_algo_byte"] * bytes_per_launch, 1)
+                traffic_src = (f"profiles/{os.path.basename(tpath)}: {tj['hbm_bytes_per_algo_byte']:.3f} HBM bytes "
+                               f"per algorithmic byte x this run's algorithmic bytes per launch")
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
                     "kernel": f"step_kernel<{served_cfg.heap_arity}>", "avg_launch_us": round(avg_ms * 1e3, 3),
@@ -224,6 +232,12 @@ def main() -> int:
                     "expansions_per_launch": round(st.expanded / st.step_launches, 1),
                     "launches": st.step_launches,
                     "timed_launches": st.timed_launches,
+                    "launches_before_timed": launches_before,
+                    "event_timing": ("per chunk of step launches (graph replay): chunk time / launches"
+                                     if served_cfg.use_graph else "per step launch"),
+                    # launch lanes run concurrently, so the per-launch figure above shares the device;
+                    # this one is all algorithmic bytes of the timed stream over its wall time
+                    "achieved_device_GBps": round(st.algo_bytes / t_total / 1e9, 2),
                     # global atomics issued by the step kernel: one CAS per closed-set probe (hashed sets)
                     # plus one atomicMin per lookup; counters and reductions excluded
                     "atomics_per_launch": round((st.probes + st.lookups) / st.step_launches, 1),
@@ -251,22 +265,23 @@ def main() -> int:
 
     # ---- one at a time: plain GA*, same requests, sequential ----
     timed_idx = list(range(W * M, (W + K) * M))
-    step = max(1, len(timed_idx) // max(1, args.plain_sample))
-    sample = timed_idx[::step][: args.plain_sample]
-    plain_cfg = tuning.load(mode="plain")
-    e1 = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30, **plain_cfg.engine)
-    mismatch = 0
-    t = time.perf_counter()
-    for i in sample:
-        x = insts[i]
-        sol = e1.solve(x.request(cells_device=dev.get(i), num_queues=plain_cfg.heaps(x.kind, x.side)))
-        if sol.status != ga.OK or served_costs.get(i, sol.cost) != sol.cost:
-            mismatch += 1
-    plain_t = time.perf_counter() - t
-    plain_sps = len(sample) / plain_t
-    e1.close()
-    log(f"[bench] one-at-a-time: {len(sample)} in {plain_t:.2f}s = {plain_sps:.2f} solves/s, "
-        f"served-vs-alone cost mismatches {mismatch}")
+    plain_sps, mismatch = None, 0
+    if args.plain_sample > 0:
+        step = max(1, len(timed_idx) // args.plain_sample)
+        sample = timed_idx[::step][: args.plain_sample]
+        plain_cfg = tuning.load(mode="plain")
+        e1 = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30, **plain_cfg.engine)
+        t = time.perf_counter()
+        for i in sample:
+            x = insts[i]
+            sol = e1.solve(x.request(cells_device=dev.get(i), num_queues=plain_cfg.heaps(x.kind, x.side)))
+            if sol.status != ga.OK or served_costs.get(i, sol.cost) != sol.cost:
+                mismatch += 1
+        plain_t = time.perf_counter() - t
+        plain_sps = len(sample) / plain_t
+        e1.close()
+        log(f"[bench] one-at-a-time: {len(sample)} in {plain_t:.2f}s = {plain_sps:.2f} solves/s, "
+            f"served-vs-alone cost mismatches {mismatch}")
 
     # ---- CPU oracle on the host cores ----
     cpu = None
@@ -313,7 +328,7 @@ def main() -> int:
         "slowest_requests": slowest,
         "completed": ok,
         "failed": failed,
-        "one_at_a_time_solves_per_sec": round(plain_sps, 3),
+        "one_at_a_time_solves_per_sec": round(plain_sps, 3) if plain_sps else None,
         "batched": batch,
         "speedup_vs_one_at_a_time": round(served_sps / plain_sps, 2) if plain_sps else None,
         "served_vs_alone_cost_mismatches": mismatch,

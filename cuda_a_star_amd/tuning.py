@@ -137,12 +137,12 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
     print(f"[tune] {mode} start {best_v:.2f}/s {asdict(best)}", flush=True)
     if mode == "served":
         axes = {
-            "puzzle_heaps": [256, 512, 1024, 2048, 4096, 8192],
+            "puzzle_heaps": [256, 512, 1024, 2048, 4096, 8192, 16384],
             "grid_heaps_small": [128, 256, 512, 1024, 2048],
             "grid_heaps_mid": [256, 512, 1024, 2048, 4096],
             "grid_heaps_big": [512, 1024, 2048, 4096, 8192],
             "heap_arity": [4, 8],
-            "chunk_iters": [4, 8, 16, 32],
+            "chunk_iters": [8, 16, 32, 64, 128],
             "pops_per_iter": [1, 2, 4],
             "max_load_pct": [50, 70],
             "rounds": [1, 2, 4],
