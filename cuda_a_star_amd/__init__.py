@@ -134,6 +134,7 @@ EXPORTS = (
     "ga_engine_reset_stats", "ga_solve", "ga_solve_puzzle", "ga_solve_grid", "ga_solve_batch",
     "ga_serve_run", "ga_result_free", "ga_status_str", "ga_device_upload", "ga_device_free",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
+    "ga_gen_board", "ga_board_solvable",
 )
 
 _lib = None
@@ -179,6 +180,8 @@ def lib() -> ctypes.CDLL:
                               _u8p]
     L.ga_gen_grid.restype = ctypes.c_uint32
     L.ga_puzzle_solvable.argtypes = [ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint64]
+    L.ga_gen_board.argtypes = [ctypes.c_uint32, _u8p, ctypes.c_uint32, ctypes.c_int, ctypes.c_uint64, _u8p]
+    L.ga_board_solvable.argtypes = [ctypes.c_uint32, _u8p, _u8p]
     _lib = L
     return L
 
@@ -243,6 +246,29 @@ def state_to_board(state: int, side: int) -> List[int]:
     return [(state >> (4 * c)) & 0xF for c in range(side * side)]
 
 
+# Boards of side 2..5 (side*side tile numbers row-major, 0 = blank): the form
+# in which the engine takes 24-puzzles (side 5).
+def board_goal(side: int) -> tuple:
+    return tuple(range(1, side * side)) + (0,)
+
+
+def _u8buf(values) -> ctypes.Array:
+    return (ctypes.c_uint8 * len(values))(*values)
+
+
+def gen_board(side: int, depth: int, seed: int, no_backtrack: bool = True, goal=None) -> tuple:
+    """The walk of gen_puzzle on a board (same draws and moves for sides 3 and 4)."""
+    g = _u8buf(board_goal(side) if goal is None else goal)
+    out = (ctypes.c_uint8 * (side * side))()
+    if not lib().ga_gen_board(side, g, depth, 1 if no_backtrack else 0, seed, out):
+        raise ValueError("invalid board generator arguments")
+    return tuple(out)
+
+
+def board_solvable(side: int, start, goal) -> bool:
+    return bool(lib().ga_board_solvable(side, _u8buf(start), _u8buf(goal)))
+
+
 # ---------------------------------------------------------------------------
 # engine
 # ---------------------------------------------------------------------------
@@ -264,8 +290,14 @@ class Solution:
         return self.status == OK
 
 
-def _take(res: Result) -> Solution:
-    path = [res.path[i] for i in range(res.path_len)] if res.path_len else []
+def _take(res: Result, board_side: int = 0) -> Solution:
+    """board_side 5: the path holds 32-byte boards (4 words per state)."""
+    if board_side and res.path_len:
+        raw = ctypes.string_at(res.path, 32 * res.path_len)
+        n = board_side * board_side
+        path = [tuple(raw[32 * k:32 * k + n]) for k in range(res.path_len)]
+    else:
+        path = [res.path[i] for i in range(res.path_len)] if res.path_len else []
     sol = Solution(res.status, res.cost, path, res.expanded, res.stored, res.iterations, res.retries,
                    res.search_ns, res.latency_ns, res.user_tag)
     lib().ga_result_free(ctypes.byref(res))
@@ -288,6 +320,25 @@ def puzzle_request(side: int, start: int, goal: Optional[int] = None, num_queues
     for k, v in kw.items():
         setattr(r, k, v)
     return r
+
+
+def puzzle5_request(start, goal=None, num_queues: int = 0, **kw) -> Request:
+    """A 24-puzzle request: start and goal boards of 25 tiles (0 = blank) go in
+    ``cells`` (50 bytes); the buffer is kept on the request object."""
+    g = board_goal(5) if goal is None else tuple(goal)
+    r = Request()
+    r.kind = PUZZLE
+    r.side = 5
+    r._boards = _u8buf(tuple(start) + g)
+    r.cells = ctypes.addressof(r._boards)
+    r.num_queues = num_queues
+    for k, v in kw.items():
+        setattr(r, k, v)
+    return r
+
+
+def _board_side(req: Request) -> int:
+    return 5 if (req.kind == PUZZLE and req.side == 5) else 0
 
 
 def grid_request(cells: np.ndarray, width: int, height: int, start: int = 0, goal: Optional[int] = None,
@@ -343,7 +394,7 @@ class Engine:
     def solve(self, req: Request) -> Solution:
         res = Result()
         lib().ga_solve(self._h, ctypes.byref(req), ctypes.byref(res))
-        return _take(res)
+        return _take(res, _board_side(req))
 
     def solve_puzzle(self, side: int, start: int, goal: Optional[int] = None, num_queues: int = 0,
                      **kw) -> Solution:
@@ -360,7 +411,7 @@ class Engine:
         arr = (Request * n)(*reqs)
         outs = (Result * n)()
         lib().ga_solve_batch(self._h, arr, n, outs)
-        return [_take(outs[i]) for i in range(n)]
+        return [_take(outs[i], _board_side(reqs[i])) for i in range(n)]
 
     def serve(self, reqs: Sequence[Request], arrival_ns: Sequence[int]):
         n = len(reqs)
@@ -369,7 +420,7 @@ class Engine:
         outs = (Result * n)()
         ss = ServeStats()
         lib().ga_serve_run(self._h, arr, arrivals, n, outs, ctypes.byref(ss))
-        return [_take(outs[i]) for i in range(n)], ss
+        return [_take(outs[i], _board_side(reqs[i])) for i in range(n)], ss
 
     def stats(self) -> EngineStats:
         s = EngineStats()

@@ -60,6 +60,27 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
     tally_end(tally, states + inst, it);
 }
 
+// Step kernel of the wide-key kinds (24-puzzle): its own instantiation so that
+// the 128-bit keys do not raise the register count of step_kernel.  Its
+// instances run on a launch lane of their own.
+template <int W>
+__global__ __launch_bounds__(kBlock) void step_kernel_wide(const WgEntry* __restrict__ wgmap,
+                                                           const InstDesc* __restrict__ descs, InstState* states,
+                                                           const uint32_t* __restrict__ iter_base, uint32_t r,
+                                                           StepParams prm) {
+    __shared__ uint32_t tally[kTallies];
+    uint32_t inst, it, first, cover;
+    bool running;
+    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover, running)) return;
+    tally_begin(tally);
+    const InstDesc& d = descs[inst];
+    const uint32_t q = first + threadIdx.x / W;
+    const Tally tl{tally, prm.profile != 0};
+    if (running && d.kind == KIND_PUZZLE5)
+        run_group<W, Puzzle5Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, tl);
+    tally_end(tally, states + inst, it);
+}
+
 __global__ void seed_kernel(InstDesc d, InstDesc* descs, InstState* states, uint32_t slot, int W) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     descs[slot] = d;
@@ -68,14 +89,15 @@ __global__ void seed_kernel(InstDesc d, InstDesc* descs, InstState* states, uint
         case KIND_PUZZLE4: seed_instance<PuzzleOps<4>>(d, states + slot, W); break;
         case KIND_GRID4: seed_instance<GridOps<4>>(d, states + slot, W); break;
         case KIND_GRID8: seed_instance<GridOps<8>>(d, states + slot, W); break;
+        case KIND_PUZZLE5: seed_instance<Puzzle5Ops>(d, states + slot, W); break;
         default: break;
     }
 }
 
 struct PathJob {
     uint32_t inst;
-    uint32_t cap;
-    uint64_t offset;
+    uint32_t cap;     // states
+    uint64_t offset;  // words of the output buffer
 };
 
 __global__ void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n, uint64_t* out, uint32_t* lens) {
@@ -89,6 +111,9 @@ __global__ void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t
         case KIND_PUZZLE4: len = walk_path<PuzzleOps<4>>(d, out + j.offset, j.cap); break;
         case KIND_GRID4: len = walk_path<GridOps<4>>(d, out + j.offset, j.cap); break;
         case KIND_GRID8: len = walk_path<GridOps<8>>(d, out + j.offset, j.cap); break;
+        case KIND_PUZZLE5:
+            len = walk_path<Puzzle5Ops>(d, reinterpret_cast<Puzzle5Ops::Key*>(out + j.offset), j.cap);
+            break;
         default: break;
     }
     lens[i] = len;
@@ -233,11 +258,23 @@ struct Pinned {
 };
 
 int kind_of(const ga_request& r) {
-    if (r.kind == GA_PUZZLE) return r.side == 3 ? KIND_PUZZLE3 : KIND_PUZZLE4;
+    if (r.kind == GA_PUZZLE) return r.side == 3 ? KIND_PUZZLE3 : r.side == 4 ? KIND_PUZZLE4 : KIND_PUZZLE5;
     return r.connectivity == 8 ? KIND_GRID8 : KIND_GRID4;
 }
 
 int branching_of(int kind) { return kind == KIND_GRID8 ? 8 : 4; }
+
+bool is_wide(int kind) { return kind == KIND_PUZZLE5; }
+
+// 64-bit words per path state in the path buffers
+uint32_t words_of(int kind) { return is_wide(kind) ? 2 : 1; }
+
+// a 24-puzzle board (25 bytes) in the device packing
+Puzzle5Ops::Key pack_board5(const uint8_t* b) {
+    Board5 x{0, 0};
+    for (int c = 0; c < 25; ++c) Puzzle5Ops::set(x, c, b[c]);
+    return to_raw(x);
+}
 
 // ---------------------------------------------------------------------------
 // one request moving through the engine
@@ -261,6 +298,7 @@ struct Job {
     uint32_t cover = 0;  // heaps covered by the launch map (power of two)
     uint32_t lane = 0;   // launch lane
     bool finished = false;
+    Puzzle5Ops::Key ends[2] = {};  // 24-puzzle: start and goal in the device packing (uploaded at admission)
 };
 
 struct PathBatch {
@@ -313,7 +351,8 @@ struct ga_engine {
     Arena* arena = nullptr;
     InstDesc* d_descs = nullptr;
     InstState* d_states = nullptr;
-    std::vector<Lane> lanes;
+    std::vector<Lane> lanes;  // cfg.lanes lanes, then the wide-key lane
+    uint32_t n_lanes = 1;     // cfg.lanes + 1; slots are partitioned by slot % n_lanes
     std::vector<Job*> resident;
     ga_engine_stats stats{};
     std::string last_error;
@@ -331,15 +370,16 @@ struct ga_engine {
         if (cfg.grow_thresh == 0) cfg.grow_thresh = 32;
         if (cfg.lanes == 0) cfg.lanes = 1;
         if (cfg.lanes > 4) cfg.lanes = 4;
-        if (cfg.max_instances < cfg.lanes) cfg.max_instances = cfg.lanes;
+        n_lanes = cfg.lanes + 1;
+        if (cfg.max_instances < n_lanes) cfg.max_instances = n_lanes;
         hip_check(hipSetDevice(device), "hipSetDevice");
         hip_check(hipStreamCreateWithFlags(&path_stream, hipStreamNonBlocking), "hipStreamCreate");
         arena = new Arena(cfg.chunk_bytes);
         hip_check(hipMalloc(&d_descs, sizeof(InstDesc) * cfg.max_instances), "hipMalloc descs");
         hip_check(hipMalloc(&d_states, sizeof(InstState) * cfg.max_instances), "hipMalloc states");
         hip_check(hipMemset(d_states, 0, sizeof(InstState) * cfg.max_instances), "hipMemset states");
-        lanes.resize(cfg.lanes);
-        for (uint32_t l = 0; l < cfg.lanes; ++l) {
+        lanes.resize(n_lanes);
+        for (uint32_t l = 0; l < n_lanes; ++l) {
             Lane& L = lanes[l];
             hip_check(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking), "hipStreamCreate");
             hip_check(hipMalloc(&L.d_iter, sizeof(uint32_t)), "hipMalloc iter");
@@ -350,7 +390,7 @@ struct ga_engine {
                 L.h_states[i].ensure(sizeof(InstState) * cfg.max_instances);
             }
             for (int s = (int)cfg.max_instances - 1; s >= 0; --s)
-                if ((uint32_t)s % cfg.lanes == l) L.free_slots.push_back(s);
+                if ((uint32_t)s % n_lanes == l) L.free_slots.push_back(s);
         }
         resident.assign(cfg.max_instances, nullptr);
         hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
@@ -377,10 +417,12 @@ struct ga_engine {
         if (path_stream) (void)hipStreamDestroy(path_stream);
     }
 
-    // Lane of a request: puzzles and small grids share lane 0; the largest
-    // grids (>= deep_cells cells) take the last lane and, with three or more
-    // lanes, grids of >= deep_cells/4 cells the one before it.
+    // Lane of a request: 24-puzzles take the wide-key lane; puzzles and small
+    // grids share lane 0; the largest grids (>= deep_cells cells) take the last
+    // configured lane and, with three or more lanes, grids of >= deep_cells/4
+    // cells the one before it.
     uint32_t lane_of(const Job& j) const {
+        if (is_wide(j.kind)) return n_lanes - 1;
         const uint32_t n = cfg.lanes;
         if (n == 1 || j.req->kind != GA_GRID || !cfg.deep_cells) return 0;
         const uint64_t cells = (uint64_t)j.req->width * j.req->height;
@@ -410,7 +452,7 @@ struct ga_engine {
             p.n_heaps = (uint32_t)pow2_floor(n_req ? n_req : 256);
             p.heap_cap = r.heap_capacity ? r.heap_capacity : 1024;
             p.table_slots = r.table_slots ? pow2_ceil(r.table_slots) : (1ull << 19);
-        } else if (j.kind == KIND_PUZZLE4) {
+        } else if (j.kind == KIND_PUZZLE4 || j.kind == KIND_PUZZLE5) {
             p.n_heaps = (uint32_t)pow2_floor(n_req ? n_req : 16384);
             p.heap_cap = r.heap_capacity ? r.heap_capacity : 512;
             p.table_slots = r.table_slots ? pow2_ceil(r.table_slots) : (1ull << 22);
@@ -430,6 +472,7 @@ struct ga_engine {
         b += (uint64_t)p.n_heaps * sizeof(uint32_t);                          // heap sizes
         b += 2ull * p.n_heaps * K * sizeof(HeapEntry);                         // candidates
         b += p.table_slots ? p.table_slots * sizeof(Slot) : cells * sizeof(uint64_t);
+        if (is_wide(j.kind)) b += p.table_slots * sizeof(Puzzle5Ops::Key) + 2 * sizeof(Puzzle5Ops::Key);  // keys, ends
         p.bytes = b;
         return p;
     }
@@ -473,8 +516,13 @@ struct ga_engine {
         d.heaps = (HeapEntry*)take((uint64_t)p.n_heaps * p.heap_stride * sizeof(HeapEntry));
         d.heap_size = (uint32_t*)take((uint64_t)p.n_heaps * sizeof(uint32_t));
         d.cand = (HeapEntry*)take(2ull * p.n_heaps * K * sizeof(HeapEntry));
-        if (r.kind == GA_PUZZLE) d.table = (Slot*)take(p.table_slots * sizeof(Slot));
-        else {
+        if (r.kind == GA_PUZZLE) {
+            d.table = (Slot*)take(p.table_slots * sizeof(Slot));
+            if (is_wide(j.kind)) {
+                d.keys = take(p.table_slots * sizeof(Puzzle5Ops::Key));
+                d.user = take(2 * sizeof(Puzzle5Ops::Key));
+            }
+        } else {
             uint64_t cells = (uint64_t)r.width * r.height;
             d.dense = (uint64_t*)take(cells * sizeof(uint64_t));
             if (!r.cells_device) d.cells = (const uint8_t*)take(cells);
@@ -498,8 +546,15 @@ struct ga_engine {
             d.table_mask = p.table_slots - 1;
             d.load_limit = p.table_slots * cfg.max_load_pct / 100;
             for (uint32_t c = 0; c < r.side * r.side; ++c) {
-                uint32_t t = (uint32_t)(r.goal >> (4 * c)) & 15u;
+                const uint32_t t = r.side == 5 ? r.cells[25 + c] : (uint32_t)(r.goal >> (4 * c)) & 15u;
                 d.goal_rc[t] = (uint8_t)(((c / r.side) << 4) | (c % r.side));
+            }
+            if (is_wide(j.kind)) {  // start and goal keys for the seed, the goal test and the path walk
+                j.ends[0] = pack_board5(r.cells);
+                j.ends[1] = pack_board5(r.cells + 25);
+                d.start = d.goal = 0;
+                hip_check(hipMemcpyAsync((void*)d.user, j.ends, sizeof(j.ends), hipMemcpyHostToDevice, stream),
+                          "upload ends");
             }
             fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>((uint64_t*)d.table, p.table_slots * 2, ~0ull);
         } else {
@@ -566,7 +621,7 @@ struct ga_engine {
         const uint32_t W = cfg.heap_arity;
         const uint32_t heaps_per_wg = kBlock / W;
         std::vector<WgEntry> map;
-        for (uint32_t s = lane; s < cfg.max_instances; s += cfg.lanes) {
+        for (uint32_t s = lane; s < cfg.max_instances; s += n_lanes) {
             Job* j = resident[s];
             if (!j || j->finished) continue;
             const uint32_t n = std::min(j->cover, j->desc.n_heaps);
@@ -608,12 +663,21 @@ struct ga_engine {
     void enqueue_steps(Lane& L, uint32_t grid, const StepParams& prm, std::vector<hipEvent_t>* pe) {
         const uint32_t R = cfg.chunk_iters;
         hipStream_t stream = L.stream;
+        const bool wide = &L == &lanes.back();
         for (uint32_t r = 0; r < R; ++r) {
             if (pe) hip_check(hipEventRecord((*pe)[2 * r], stream), "event");
-            switch (cfg.heap_arity) {
-                case 8: step_kernel<8><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
-                case 16: step_kernel<16><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
-                default: step_kernel<4><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+            if (wide) {
+                switch (cfg.heap_arity) {
+                    case 8: step_kernel_wide<8><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                    case 16: step_kernel_wide<16><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                    default: step_kernel_wide<4><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                }
+            } else {
+                switch (cfg.heap_arity) {
+                    case 8: step_kernel<8><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                    case 16: step_kernel<16><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                    default: step_kernel<4><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                }
             }
             if (pe) hip_check(hipEventRecord((*pe)[2 * r + 1], stream), "event");
         }
@@ -718,6 +782,7 @@ struct ga_engine {
         b += (double)s.lookups * 8;
         if (dense) b += (double)s.expanded * K;
         else b += (double)s.probes * 8 + (double)s.occupied * 8;
+        if (is_wide(kind)) b += (double)s.pops * 16 + (double)s.lookups * 16;  // 16-byte keys read / written
         return b;
     }
 
@@ -738,6 +803,18 @@ namespace {
 // the run loop shared by plain, batched and served GA*
 // ---------------------------------------------------------------------------
 int validate(const ga_request& r) {
+    if (r.kind == GA_PUZZLE && r.side == 5) {  // boards in cells: start, then goal
+        if (!r.cells) return GA_E_INVALID;
+        for (int k = 0; k < 2; ++k) {
+            uint32_t seen = 0;
+            for (int c = 0; c < 25; ++c) {
+                const uint8_t t = r.cells[25 * k + c];
+                if (t >= 25 || ((seen >> t) & 1u)) return GA_E_INVALID;
+                seen |= 1u << t;
+            }
+        }
+        return GA_OK;
+    }
     if (r.kind == GA_PUZZLE) {
         if (r.side != 3 && r.side != 4) return GA_E_INVALID;
         uint32_t n = r.side * r.side;
@@ -797,6 +874,17 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
             j.out->cost = b->snaps[i].best;
             if (len == kNone || len == 0) {
                 e->finish(j, GA_E_OVERFLOW, t0);
+            } else if (is_wide(j.kind)) {  // 24-puzzle: 32-byte boards (4 words) per state
+                j.out->path = (uint64_t*)calloc((size_t)len * 4, sizeof(uint64_t));
+                const uint64_t* src = out + b->offsets[i];
+                for (uint32_t k = 0; k < len; ++k) {
+                    Board5 x;
+                    std::memcpy(&x, src + 2 * (size_t)(len - 1 - k), sizeof(x));
+                    uint8_t* dst = (uint8_t*)(j.out->path + 4 * (size_t)k);
+                    for (int c = 0; c < 25; ++c) dst[c] = (uint8_t)Puzzle5Ops::get(x, c);
+                }
+                j.out->path_len = len;
+                e->finish(j, GA_OK, t0);
             } else {
                 j.out->path = (uint64_t*)malloc(sizeof(uint64_t) * len);
                 for (uint32_t k = 0; k < len; ++k) j.out->path[k] = out[b->offsets[i] + (len - 1 - k)];
@@ -816,7 +904,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         e->collect_profile(L, chunk);
         const InstState* snap = (const InstState*)L.h_states[chunk & 1].p;
         std::vector<Job*> done;
-        for (uint32_t s = lane; s < L.chunk_reads[chunk & 1]; s += e->cfg.lanes) {
+        for (uint32_t s = lane; s < L.chunk_reads[chunk & 1]; s += e->n_lanes) {
             Job* j = e->resident[s];
             if (!j || j->finished || j->first_chunk > chunk) continue;
             const InstState& st = snap[s];
@@ -876,7 +964,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         for (Job* j : done) {
             b->offsets.push_back(total);
             b->snaps.push_back(snap[j->slot]);
-            total += (uint64_t)snap[j->slot].best + 1;  // every move costs >= 1
+            total += ((uint64_t)snap[j->slot].best + 1) * words_of(j->kind);  // every move costs >= 1
         }
         size_t n = done.size();
         size_t lens_bytes = round_up(n * sizeof(uint32_t), 16);
@@ -939,13 +1027,15 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         }
         // -- every lane keeps up to two chunks in flight and handles read-backs in order
         bool busy = false;
-        for (uint32_t l = 0; l < e->cfg.lanes; ++l) {
+        uint32_t lanes_busy = 0;  // a lone busy lane may block on its chunk instead of polling
+        for (const Lane& L : e->lanes) lanes_busy += (L.running > 0 || L.launched > L.processed) ? 1u : 0u;
+        for (uint32_t l = 0; l < e->n_lanes; ++l) {
             Lane& L = e->lanes[l];
             if (L.launched > L.processed) {
                 const uint64_t c = L.processed;
                 const bool must_wait = L.launched - L.processed >= 2 || L.running == 0;
                 hipError_t q = hipEventQuery(L.ev_chunk[c & 1]);
-                if (q == hipErrorNotReady && must_wait && e->cfg.lanes == 1) {
+                if (q == hipErrorNotReady && must_wait && lanes_busy == 1) {
                     hip_check(hipEventSynchronize(L.ev_chunk[c & 1]), "chunk event");
                     q = hipSuccess;
                 }
@@ -1030,7 +1120,10 @@ int run_checked(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, 
         }
         jobs[i].kind = kind_of(reqs[i]);
         if (reqs[i].kind == GA_GRID) jobs[i].w_min = grid_wmin(reqs[i]);
-        if (reqs[i].kind == GA_PUZZLE && !ga_puzzle_solvable(reqs[i].side, reqs[i].start, reqs[i].goal)) {
+        const bool solvable = reqs[i].kind != GA_PUZZLE ||
+                              (reqs[i].side == 5 ? ga_board_solvable(5, reqs[i].cells, reqs[i].cells + 25)
+                                                 : ga_puzzle_solvable(reqs[i].side, reqs[i].start, reqs[i].goal));
+        if (!solvable) {
             outs[i].status = GA_NO_PATH;
             jobs[i].finished = true;
         }

@@ -148,6 +148,74 @@ int ga_puzzle_solvable(uint32_t side, uint64_t start, uint64_t goal) {
     return parity == (d & 1);
 }
 
+// Boards (side 2..5): the same walk as ga_gen_puzzle -- the same draws and
+// the same move choices -- on a board of side*side bytes, so that sides 3 and
+// 4 give exactly the boards of ga_gen_puzzle.
+int ga_gen_board(uint32_t side, const uint8_t* goal, uint32_t depth, int no_backtrack, uint64_t seed,
+                 uint8_t* out) {
+    if (side < 2 || side > 5 || !goal || !out) return 0;
+    const uint32_t n = side * side;
+    uint32_t seen = 0;
+    int blank = -1;
+    for (uint32_t c = 0; c < n; ++c) {
+        if (goal[c] >= n || ((seen >> goal[c]) & 1u)) return 0;
+        seen |= 1u << goal[c];
+        if (goal[c] == 0) blank = (int)c;
+        out[c] = goal[c];
+    }
+    uint64_t rng = seed;
+    int last = -1;
+    static const int kInverse[4] = {1, 0, 3, 2};
+    for (uint32_t step = 0; step < depth; ++step) {
+        const int r = blank / (int)side, c = blank % (int)side;
+        int moves[4], cells[4], nm = 0;
+        for (int i = 0; i < 4; ++i) {
+            if (no_backtrack && last >= 0 && i == kInverse[last]) continue;
+            const int nr = r + (i == 0) - (i == 1), nc = c + (i == 2) - (i == 3);  // a_star_sliding_puzzle.h:88-105
+            if (nr < 0 || nc < 0 || nr >= (int)side || nc >= (int)side) continue;
+            moves[nm] = i;
+            cells[nm] = nr * (int)side + nc;
+            ++nm;
+        }
+        const int k = (int)(ga_splitmix64(&rng) % (uint64_t)nm);
+        out[blank] = out[cells[k]];
+        out[cells[k]] = 0;
+        blank = cells[k];
+        last = moves[k];
+    }
+    return 1;
+}
+
+int ga_board_solvable(uint32_t side, const uint8_t* start, const uint8_t* goal) {
+    if (side < 2 || side > 5 || !start || !goal) return 0;
+    const uint32_t n = side * side;
+    int pos_goal[25], perm[25];
+    uint32_t seen_s = 0, seen_g = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        if (start[c] >= n || goal[c] >= n || ((seen_s >> start[c]) & 1u) || ((seen_g >> goal[c]) & 1u)) return 0;
+        seen_s |= 1u << start[c];
+        seen_g |= 1u << goal[c];
+        pos_goal[goal[c]] = (int)c;
+    }
+    for (uint32_t c = 0; c < n; ++c) perm[c] = pos_goal[start[c]];
+    // same rule as ga_puzzle_solvable: permutation parity = parity of the blank's displacement
+    int parity = 0, bs = 0, bg = 0;
+    bool done[25] = {false};
+    for (uint32_t c = 0; c < n; ++c) {
+        if (start[c] == 0) bs = (int)c;
+        if (goal[c] == 0) bg = (int)c;
+        if (done[c]) continue;
+        int len = 0;
+        for (int j = (int)c; !done[j]; j = perm[j]) {
+            done[j] = true;
+            ++len;
+        }
+        parity ^= (len - 1) & 1;
+    }
+    const int d = abs(bs / (int)side - bg / (int)side) + abs(bs % (int)side - bg % (int)side);
+    return parity == (d & 1);
+}
+
 uint32_t ga_gen_grid(uint32_t width, uint32_t height, uint32_t wall_ppm, uint32_t w_lo, uint32_t w_hi,
                      uint64_t start, uint64_t goal, uint32_t connectivity, uint64_t seed,
                      uint32_t max_attempts, uint8_t* cells_out) {

@@ -14,6 +14,12 @@
  * States:
  *   sliding puzzle: uint64, nibble i (bits 4i..4i+3) = tile on cell i in
  *                   row-major order, 0 = blank; side 3 or 4.
+ *   24-puzzle:      (side 5, the reference CLI's default, main.cu:72) a board
+ *                   of 25 bytes, tile on each cell row-major, 0 = blank.  The
+ *                   request carries start and goal boards in `cells` (50
+ *                   bytes: start, then goal; `start`/`goal` are ignored), and
+ *                   each path state is returned as 32 bytes (4 words): the 25
+ *                   board bytes, then 7 zero bytes.
  *   grid:           cell index y*width + x into a row-major uint8 grid,
  *                   0 = wall, 1..255 = cost of entering the cell.
  * Paths are returned start..goal as states in these encodings.
@@ -50,14 +56,14 @@ typedef struct ga_engine ga_engine;
  * and a_star_pathfinding.h:72-81). */
 typedef struct ga_request {
     uint32_t kind;          /* GA_PUZZLE or GA_GRID */
-    uint32_t side;          /* puzzle side, 3 or 4 */
+    uint32_t side;          /* puzzle side, 3, 4 or 5 */
     uint64_t start;         /* start state */
     uint64_t goal;          /* target state */
     uint32_t width;         /* grid width  (x extent) */
     uint32_t height;        /* grid height (y extent) */
     uint32_t connectivity;  /* grid: 4 or 8 */
     uint32_t w_min;         /* grid heuristic scale; 0 = minimum open-cell weight */
-    const uint8_t* cells;   /* grid cells in host memory, or NULL */
+    const uint8_t* cells;   /* grid cells in host memory, or NULL; side-5 puzzle: start and goal boards */
     const uint8_t* cells_device; /* grid cells already resident on the device, or NULL */
     uint32_t num_queues;    /* number of parallel heaps (a_star.h:224); 0 = planner's choice */
     uint32_t heap_capacity; /* entries per heap; 0 = planner's choice */
@@ -76,7 +82,7 @@ typedef struct ga_result {
     uint32_t retries;      /* restarts with larger capacities */
     uint32_t path_len;     /* states in path */
     uint32_t reserved;
-    uint64_t* path;        /* start..goal; owned by the library, see ga_result_free */
+    uint64_t* path;        /* start..goal (side 5: 4 words per state); owned by the library, see ga_result_free */
     uint64_t search_ns;    /* admission to completion */
     uint64_t latency_ns;   /* arrival to result on the host (serving) */
     uint64_t user_tag;
@@ -191,6 +197,13 @@ uint32_t ga_gen_grid(uint32_t width, uint32_t height, uint32_t wall_ppm, uint32_
                      uint32_t max_attempts, uint8_t* cells_out);
 /* 1 if goal is reachable from start by sliding moves (parity test). */
 int ga_puzzle_solvable(uint32_t side, uint64_t start, uint64_t goal);
+/* Boards of side 2..5 (side*side bytes, 0 = blank): the walk of ga_gen_puzzle
+ * (same draws and moves, so sides 3 and 4 give the same boards) into out;
+ * returns 1, or 0 on invalid arguments. */
+int ga_gen_board(uint32_t side, const uint8_t* goal, uint32_t depth, int no_backtrack, uint64_t seed,
+                 uint8_t* out);
+/* ga_puzzle_solvable on boards of side 2..5. */
+int ga_board_solvable(uint32_t side, const uint8_t* start, const uint8_t* goal);
 
 #ifdef __cplusplus
 }

@@ -41,35 +41,6 @@
 namespace gastar {
 namespace dropin {
 
-constexpr uint64_t kBusyKey = ~0ull - 1;  // wide-key slot being written
-
-// The bytes of a state as 32-bit words.  Keys are handled in this form so
-// that T needs no default constructor (the reference's states delete it).
-template <typename T>
-struct RawKey {
-    static_assert(sizeof(T) % 4 == 0, "state size must be a multiple of 4 bytes");
-    uint32_t w[sizeof(T) / 4];
-};
-
-template <typename T>
-__host__ __device__ inline RawKey<T> to_raw(const T& t) {
-    RawKey<T> k;
-    __builtin_memcpy(&k, &t, sizeof(T));
-    return k;
-}
-
-template <typename T>
-__host__ __device__ inline T from_raw(const RawKey<T>& k) {
-    return *reinterpret_cast<const T*>(&k);
-}
-
-template <typename T>
-__host__ __device__ inline uint64_t hash_words(const RawKey<T>& k) {
-    uint64_t h = 0x9E3779B97F4A7C15ull ^ sizeof(T);
-    for (size_t i = 0; i < sizeof(T) / 4; ++i) h = mix64(h ^ ((uint64_t)k.w[i] << 7) ^ i);
-    return h;
-}
-
 // States of at most 8 bytes: packed into a 64-bit key (zero-extended).
 template <typename T>
 struct PackedKeySet : HashedU64Set {
@@ -81,105 +52,6 @@ struct PackedKeySet : HashedU64Set {
     __host__ __device__ static inline T unpack(uint64_t k) {
         return *reinterpret_cast<const T*>(&k);
     }
-};
-
-// States wider than 8 bytes: Slot.key holds a tag (empty / busy / a 62-bit
-// fingerprint) and d.keys[slot] the state.  A lane that claims an empty slot
-// writes the state, fences, then publishes the fingerprint in the same loop
-// iteration; lanes that meet a busy slot look again on their next iteration,
-// so no lane waits inside an inner loop on another lane of its own wave.
-// The loop is bounded; running out counts as a closed-set overflow.
-template <typename T>
-struct WideKeySet {
-    using Key = RawKey<T>;
-    static constexpr bool kDense = false;
-    static constexpr uint32_t kMaxIters = 1u << 20;
-
-    __device__ static inline gptr<Key> keys(const InstDesc& d) { return as_global(reinterpret_cast<Key*>(d.keys)); }
-    __device__ static inline Key load_key(const InstDesc& d, uint32_t slot) {
-        // agent-scope loads: the key may have been written earlier in this launch
-        Key k;
-        const gptr<const uint32_t> src = keys(d)[slot].w;
-        for (size_t i = 0; i < sizeof(T) / 4; ++i)
-            k.w[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        return k;
-    }
-    __device__ static inline bool same(const Key& a, const Key& b) {
-        for (size_t i = 0; i < sizeof(T) / 4; ++i)
-            if (a.w[i] != b.w[i]) return false;
-        return true;
-    }
-    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
-        key = load_key(d, slot);
-        val = as_global(d.table)[slot].val;
-    }
-    __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
-        return &as_global(d.table)[slot].val;
-    }
-    // All side effects (claim, key write, publish) happen inside the loop body
-    // of the iteration that claims the slot; the loop exit carries none, so a
-    // lane of the same wave that saw the slot busy finds it published on its
-    // next iteration (divergent exits are only taken once the wave leaves).
-    __device__ static inline bool find_or_insert(const InstDesc& d, const Key& key, uint32_t max_probes,
-                                                 uint32_t& slot, bool& isnew, uint32_t& probes) {
-        const uint64_t h = hash_words(key);
-        const uint64_t fp = h >> 2;  // never equal to the empty or busy tag
-        uint64_t i = h & d.table_mask;
-        uint32_t probed = 0;
-        int outcome = 0;  // 0 searching, 1 found, 2 inserted, 3 table exhausted
-        for (uint32_t iter = 0; iter < kMaxIters; ++iter) {
-            ++probes;
-            const gptr<uint64_t> tagp = &as_global(d.table)[i].key;
-            uint64_t tag = __hip_atomic_load(tagp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (tag == kEmptyKey) {
-                uint64_t prev = kEmptyKey;  // receives the tag found in the slot
-                __hip_atomic_compare_exchange_strong(tagp, &prev, kBusyKey, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                if (prev == kEmptyKey) {
-                    keys(d)[i] = key;
-                    __threadfence();
-                    __hip_atomic_store(tagp, fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    outcome = 2;
-                } else {
-                    tag = prev;
-                }
-            }
-            if (outcome == 0 && tag != kBusyKey) {  // a busy slot is looked at again next iteration
-                if (tag == fp) {
-                    __threadfence();
-                    if (same(load_key(d, (uint32_t)i), key)) outcome = 1;
-                }
-                if (outcome == 0) {
-                    if (++probed > max_probes || probed > d.table_mask) outcome = 3;
-                    else i = (i + 1) & d.table_mask;
-                }
-            }
-            if (outcome != 0) break;
-        }
-        slot = (uint32_t)i;
-        isnew = outcome == 2;
-        return outcome == 1 || outcome == 2;
-    }
-    __device__ static inline bool find(const InstDesc& d, const Key& key, uint32_t& slot) {
-        const uint64_t h = hash_words(key);
-        const uint64_t fp = h >> 2;
-        uint64_t i = h & d.table_mask;
-        for (uint64_t p = 0; p <= d.table_mask; ++p) {
-            const uint64_t tag = as_global(d.table)[i].key;
-            if (tag == kEmptyKey) return false;
-            if (tag == fp && same(load_key(d, (uint32_t)i), key)) {
-                slot = (uint32_t)i;
-                return true;
-            }
-            i = (i + 1) & d.table_mask;
-        }
-        return false;
-    }
-    __device__ static inline bool is_goal(const InstDesc& d, const Key& key) { return same(key, goal_key(d)); }
-    __device__ static inline Key start_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[0]; }
-    __device__ static inline Key goal_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[1]; }
-    __host__ __device__ static inline T unpack(const Key& k) { return from_raw<T>(k); }
-    __host__ __device__ static inline Key pack(const T& t) { return to_raw<T>(t); }
 };
 
 template <typename T>

@@ -74,6 +74,7 @@ enum : uint32_t {
     KIND_PUZZLE4 = 2,
     KIND_GRID4 = 3,
     KIND_GRID8 = 4,
+    KIND_PUZZLE5 = 5,  // 24-puzzle: 128-bit keys, its own kernel instantiation
     KIND_USER = 16,  // header drop-in: one user problem per kernel instantiation
 };
 
@@ -139,7 +140,7 @@ struct InstDesc {
     uint32_t rounds;       // expansion rounds per launch for this instance (0 = the launch's default)
     uint32_t f_slack;      // expand a heap's top only if f <= previous min f + f_slack (kInf = always)
     uint32_t pad1;
-    uint8_t goal_rc[16];   // puzzle: row << 4 | col of each tile in the goal
+    uint8_t goal_rc[32];   // puzzle: row << 4 | col of each tile in the goal
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
 };
@@ -481,6 +482,137 @@ struct DenseCellSet {
 };
 
 // ---------------------------------------------------------------------------
+// keys wider than 64 bits (24-puzzle, user states of more than 8 bytes)
+// ---------------------------------------------------------------------------
+constexpr uint64_t kBusyKey = ~0ull - 1;  // wide-key slot being written
+
+// The bytes of a state as 32-bit words.  Keys are handled in this form so
+// that T needs no default constructor (the reference's states delete it).
+template <typename T>
+struct RawKey {
+    static_assert(sizeof(T) % 4 == 0, "state size must be a multiple of 4 bytes");
+    uint32_t w[sizeof(T) / 4];
+};
+
+template <typename T>
+__host__ __device__ inline RawKey<T> to_raw(const T& t) {
+    RawKey<T> k;
+    __builtin_memcpy(&k, &t, sizeof(T));
+    return k;
+}
+
+template <typename T>
+__host__ __device__ inline T from_raw(const RawKey<T>& k) {
+    return *reinterpret_cast<const T*>(&k);
+}
+
+template <typename T>
+__host__ __device__ inline uint64_t hash_words(const RawKey<T>& k) {
+    uint64_t h = 0x9E3779B97F4A7C15ull ^ sizeof(T);
+    for (size_t i = 0; i < sizeof(T) / 4; ++i) h = mix64(h ^ ((uint64_t)k.w[i] << 7) ^ i);
+    return h;
+}
+
+// States wider than 8 bytes: Slot.key holds a tag (empty / busy / a 62-bit
+// fingerprint) and d.keys[slot] the state.  A lane that claims an empty slot
+// writes the state, fences, then publishes the fingerprint in the same loop
+// iteration; lanes that meet a busy slot look again on their next iteration,
+// so no lane waits inside an inner loop on another lane of its own wave.
+// The loop is bounded; running out counts as a closed-set overflow.
+template <typename T>
+struct WideKeySet {
+    using Key = RawKey<T>;
+    static constexpr bool kDense = false;
+    static constexpr uint32_t kMaxIters = 1u << 20;
+
+    __device__ static inline gptr<Key> keys(const InstDesc& d) { return as_global(reinterpret_cast<Key*>(d.keys)); }
+    __device__ static inline Key load_key(const InstDesc& d, uint32_t slot) {
+        // agent-scope loads: the key may have been written earlier in this launch
+        Key k;
+        const gptr<const uint32_t> src = keys(d)[slot].w;
+        for (size_t i = 0; i < sizeof(T) / 4; ++i)
+            k.w[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return k;
+    }
+    __device__ static inline bool same(const Key& a, const Key& b) {
+        for (size_t i = 0; i < sizeof(T) / 4; ++i)
+            if (a.w[i] != b.w[i]) return false;
+        return true;
+    }
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
+        key = load_key(d, slot);
+        val = as_global(d.table)[slot].val;
+    }
+    __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
+        return &as_global(d.table)[slot].val;
+    }
+    // All side effects (claim, key write, publish) happen inside the loop body
+    // of the iteration that claims the slot; the loop exit carries none, so a
+    // lane of the same wave that saw the slot busy finds it published on its
+    // next iteration (divergent exits are only taken once the wave leaves).
+    __device__ static inline bool find_or_insert(const InstDesc& d, const Key& key, uint32_t max_probes,
+                                                 uint32_t& slot, bool& isnew, uint32_t& probes) {
+        const uint64_t h = hash_words(key);
+        const uint64_t fp = h >> 2;  // never equal to the empty or busy tag
+        uint64_t i = h & d.table_mask;
+        uint32_t probed = 0;
+        int outcome = 0;  // 0 searching, 1 found, 2 inserted, 3 table exhausted
+        for (uint32_t iter = 0; iter < kMaxIters; ++iter) {
+            ++probes;
+            const gptr<uint64_t> tagp = &as_global(d.table)[i].key;
+            uint64_t tag = __hip_atomic_load(tagp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (tag == kEmptyKey) {
+                uint64_t prev = kEmptyKey;  // receives the tag found in the slot
+                __hip_atomic_compare_exchange_strong(tagp, &prev, kBusyKey, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
+                if (prev == kEmptyKey) {
+                    keys(d)[i] = key;
+                    __threadfence();
+                    __hip_atomic_store(tagp, fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    outcome = 2;
+                } else {
+                    tag = prev;
+                }
+            }
+            if (outcome == 0 && tag != kBusyKey) {  // a busy slot is looked at again next iteration
+                if (tag == fp) {
+                    __threadfence();
+                    if (same(load_key(d, (uint32_t)i), key)) outcome = 1;
+                }
+                if (outcome == 0) {
+                    if (++probed > max_probes || probed > d.table_mask) outcome = 3;
+                    else i = (i + 1) & d.table_mask;
+                }
+            }
+            if (outcome != 0) break;
+        }
+        slot = (uint32_t)i;
+        isnew = outcome == 2;
+        return outcome == 1 || outcome == 2;
+    }
+    __device__ static inline bool find(const InstDesc& d, const Key& key, uint32_t& slot) {
+        const uint64_t h = hash_words(key);
+        const uint64_t fp = h >> 2;
+        uint64_t i = h & d.table_mask;
+        for (uint64_t p = 0; p <= d.table_mask; ++p) {
+            const uint64_t tag = as_global(d.table)[i].key;
+            if (tag == kEmptyKey) return false;
+            if (tag == fp && same(load_key(d, (uint32_t)i), key)) {
+                slot = (uint32_t)i;
+                return true;
+            }
+            i = (i + 1) & d.table_mask;
+        }
+        return false;
+    }
+    __device__ static inline bool is_goal(const InstDesc& d, const Key& key) { return same(key, goal_key(d)); }
+    __device__ static inline Key start_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[0]; }
+    __device__ static inline Key goal_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[1]; }
+    __host__ __device__ static inline T unpack(const Key& k) { return from_raw<T>(k); }
+    __host__ __device__ static inline Key pack(const T& t) { return to_raw<T>(t); }
+};
+
+// ---------------------------------------------------------------------------
 // built-in problems
 // ---------------------------------------------------------------------------
 // Sliding puzzle of side S, state = 16 nibbles (tile of cell i in nibble i).
@@ -593,6 +725,80 @@ struct GridOps : DenseCellSet {
         taux = (ny << 16) | nx;
         cost = w;
         th = hxy(d, nx, ny);
+        return true;
+    }
+};
+
+// 24-puzzle (5x5 board).  Board5 packs cells 0..11 into lo and 12..23 into hi,
+// 5 bits each (bits 5c..5c+4), and cell 24 into the spare bits: its low 4
+// bits in lo[60..63], its high bit in hi[60].  aux = the blank's cell.  Moves,
+// cost and heuristic as PuzzleOps (a_star_sliding_puzzle.h:83-123, Manhattan
+// without the blank).
+struct Board5 {
+    uint64_t lo, hi;
+};
+
+struct Puzzle5Ops : WideKeySet<Board5> {
+    static constexpr int K = 4;
+    using Key = RawKey<Board5>;
+
+    __host__ __device__ static inline uint32_t get(const Board5& b, int c) {
+        if (c < 12) return (uint32_t)(b.lo >> (5 * c)) & 31u;
+        if (c < 24) return (uint32_t)(b.hi >> (5 * (c - 12))) & 31u;
+        return (uint32_t)(b.lo >> 60) | ((uint32_t)(b.hi >> 60) & 1u) << 4;
+    }
+    __host__ __device__ static inline void set(Board5& b, int c, uint32_t t) {
+        if (c < 12) {
+            b.lo = (b.lo & ~(31ull << (5 * c))) | ((uint64_t)t << (5 * c));
+        } else if (c < 24) {
+            b.hi = (b.hi & ~(31ull << (5 * (c - 12)))) | ((uint64_t)t << (5 * (c - 12)));
+        } else {
+            b.lo = (b.lo & ~(15ull << 60)) | ((uint64_t)(t & 15u) << 60);
+            b.hi = (b.hi & ~(1ull << 60)) | ((uint64_t)(t >> 4) << 60);
+        }
+    }
+    __device__ static inline Board5 board(const Key& k) { return from_raw<Board5>(k); }
+
+    __device__ static inline uint32_t md(const InstDesc& d, uint32_t tile, int cell) {
+        const int gr = d.goal_rc[tile] >> 4, gc = d.goal_rc[tile] & 15;
+        const int r = cell / 5, c = cell % 5;
+        return (uint32_t)((r > gr ? r - gr : gr - r) + (c > gc ? c - gc : gc - c));
+    }
+    __device__ static inline uint32_t heuristic(const InstDesc& d, const Key& k, uint32_t) {
+        const Board5 b = board(k);
+        uint32_t h = 0;
+        for (int c = 0; c < 25; ++c) {
+            const uint32_t t = get(b, c);
+            if (t) h += md(d, t, c);
+        }
+        return h;
+    }
+    __device__ static inline uint32_t aux_of(const InstDesc&, const Key& k) {
+        const Board5 b = board(k);
+        for (int c = 0; c < 25; ++c)
+            if (get(b, c) == 0) return (uint32_t)c;
+        return 0;
+    }
+    __device__ static inline uint32_t prefetch(const InstDesc&, uint32_t, int) { return 0; }
+    // h of the successor from h of the parent: only the moved tile changes.
+    __device__ static inline bool succ(const InstDesc& d, const Key& k, uint32_t blank, uint32_t h, int i, uint32_t,
+                                       Key& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
+        const int b = (int)blank;
+        int r = b / 5, c = b % 5;
+        if (i == 0) ++r;
+        else if (i == 1) --r;
+        else if (i == 2) ++c;
+        else --c;
+        if (r < 0 || r >= 5 || c < 0 || c >= 5) return false;
+        const int nb = r * 5 + c;
+        Board5 s = board(k);
+        const uint32_t tile = get(s, nb);
+        set(s, b, tile);
+        set(s, nb, 0u);
+        t = to_raw(s);
+        taux = (uint32_t)nb;
+        cost = 1;
+        th = h - md(d, tile, nb) + md(d, tile, b);
         return true;
     }
 };

@@ -70,6 +70,15 @@ def lib() -> ctypes.CDLL:
         L.oracle_replay_grid.argtypes = [u8p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, u32p,
                                          ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32]
         L.oracle_replay_grid.restype = ctypes.c_int64
+        L.oracle_astar_board.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_uint64, ctypes.POINTER(OracleResult),
+                                         u8p, ctypes.c_uint32]
+        L.oracle_astar_board.restype = ctypes.c_int
+        L.oracle_board_manhattan.argtypes = [ctypes.c_int, u8p, u8p]
+        L.oracle_board_manhattan.restype = ctypes.c_uint32
+        L.oracle_bfs_board.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_uint32, ctypes.c_uint64]
+        L.oracle_bfs_board.restype = ctypes.c_int64
+        L.oracle_replay_board.argtypes = [ctypes.c_int, u8p, ctypes.c_uint32, u8p, u8p]
+        L.oracle_replay_board.restype = ctypes.c_int64
         _lib = L
     return _lib
 
@@ -133,3 +142,69 @@ def replay_grid(cells: np.ndarray, w: int, h: int, connectivity: int, path, star
         arr = np.zeros(1, dtype=np.uint32)
     return lib().oracle_replay_grid(_u8(cells), w, h, connectivity,
                                     arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), len(path), start, goal)
+
+
+# ---------------------------------------------------------------------------
+# boards of side 2..5 (the 24-puzzle): tuples of side*side tile numbers, 0 = blank
+# ---------------------------------------------------------------------------
+def _board(b, side: int) -> np.ndarray:
+    a = np.ascontiguousarray(np.asarray(b, dtype=np.uint8).reshape(-1))
+    assert a.size == side * side
+    return a
+
+
+def astar_board(side: int, start, goal, max_states: int = 0, want_path: bool = True):
+    """Optimal cost by sequential A* on boards.  Returns (rc, OracleResult, path of boards)."""
+    s, g = _board(start, side), _board(goal, side)
+    res = OracleResult()
+    cap = 4096 if want_path else 0
+    buf = np.zeros(max(cap, 1) * side * side, dtype=np.uint8)
+    rc = lib().oracle_astar_board(side, _u8(s), _u8(g), max_states, ctypes.byref(res),
+                                  _u8(buf) if want_path else None, cap)
+    path = []
+    if want_path and rc == OK and res.path_len <= cap:
+        path = [tuple(int(x) for x in buf[i * side * side:(i + 1) * side * side]) for i in range(res.path_len)]
+    return rc, res, path
+
+
+def board_manhattan(side: int, state, goal) -> int:
+    s, g = _board(state, side), _board(goal, side)
+    return lib().oracle_board_manhattan(side, _u8(s), _u8(g))
+
+
+def bfs_board(side: int, start, goal, max_depth: int, max_states: int = 0) -> int:
+    s, g = _board(start, side), _board(goal, side)
+    return lib().oracle_bfs_board(side, _u8(s), _u8(g), max_depth, max_states)
+
+
+def replay_board(side: int, path, start, goal) -> int:
+    if not path:
+        return -1
+    arr = np.ascontiguousarray(np.concatenate([_board(b, side) for b in path]))
+    s, g = _board(start, side), _board(goal, side)
+    return lib().oracle_replay_board(side, _u8(arr), len(path), _u8(s), _u8(g))
+
+
+def walk_board(side: int, depth: int, seed: int, no_backtrack: bool = True):
+    """A random walk of the blank from the ordered goal board (test inputs only;
+    moves in the order of a_star_sliding_puzzle.h:88-105, xorshift64* choices)."""
+    goal = list(range(1, side * side)) + [0]
+    b = goal[:]
+    x = (seed * 0x9E3779B97F4A7C15 + 1) & 0xFFFFFFFFFFFFFFFF
+    prev = -1
+    for _ in range(depth):
+        p = b.index(0)
+        r, c = divmod(p, side)
+        moves = []
+        for i, (dr, dc) in enumerate(((1, 0), (-1, 0), (0, 1), (0, -1))):
+            nr, nc = r + dr, c + dc
+            if 0 <= nr < side and 0 <= nc < side and not (no_backtrack and nr * side + nc == prev):
+                moves.append(nr * side + nc)
+        x ^= x >> 12
+        x ^= (x << 25) & 0xFFFFFFFFFFFFFFFF
+        x ^= x >> 27
+        k = ((x * 0x2545F4914F6CDD1D) & 0xFFFFFFFFFFFFFFFF) >> 33
+        nb = moves[k % len(moves)]
+        b[p], b[nb] = b[nb], 0
+        prev = p
+    return tuple(b), tuple(goal)

@@ -512,3 +512,257 @@ int64_t oracle_replay_grid(const uint8_t* cells, uint32_t w, uint32_t h, int con
     }
     return cost;
 }
+
+/* ------------------------------------------------------------------ */
+/* boards of side <= 5 (up to the 24-puzzle): 5 bits per cell, 128 bits */
+/* ------------------------------------------------------------------ */
+/* Same rules as the 64-bit puzzle code above (move order
+ * a_star_sliding_puzzle.h:88-105, unit cost, goal at pop, re-opening), on a
+ * board given as side*side bytes (tile numbers, 0 = blank, row-major). */
+typedef unsigned __int128 u128;
+
+static inline uint32_t cell5(u128 s, int c) { return (uint32_t)(s >> (5 * c)) & 31u; }
+
+static u128 board_pack(int side, const uint8_t* b) {
+    u128 s = 0;
+    for (int c = 0; c < side * side; ++c) s |= (u128)b[c] << (5 * c);
+    return s;
+}
+
+static void board_unpack(int side, u128 s, uint8_t* b) {
+    for (int c = 0; c < side * side; ++c) b[c] = (uint8_t)cell5(s, c);
+}
+
+static int board_valid(int side, const uint8_t* b) {
+    if (side < 2 || side > 5) return 0;
+    int n = side * side;
+    uint32_t seen = 0;
+    for (int c = 0; c < n; ++c) {
+        if (b[c] >= n || (seen >> b[c]) & 1u) return 0;
+        seen |= 1u << b[c];
+    }
+    return 1;
+}
+
+static int board_blank(int side, u128 s) {
+    for (int c = 0; c < side * side; ++c)
+        if (cell5(s, c) == 0) return c;
+    return -1;
+}
+
+static int board_succ(int side, u128 s, int blank, int i, u128* out) {
+    int r = blank / side, c = blank % side;
+    switch (i) {
+        case 0: r += 1; break;
+        case 1: r -= 1; break;
+        case 2: c += 1; break;
+        default: c -= 1; break;
+    }
+    if (r < 0 || r >= side || c < 0 || c >= side) return 0;
+    int nb = r * side + c;
+    u128 tile = (u128)cell5(s, nb);
+    u128 t = s & ~((u128)31 << (5 * nb));
+    t |= tile << (5 * blank);
+    *out = t;
+    return 1;
+}
+
+static void goal_rc(int side, u128 goal, int* gr, int* gc) {
+    for (int c = 0; c < side * side; ++c) {
+        gr[cell5(goal, c)] = c / side;
+        gc[cell5(goal, c)] = c % side;
+    }
+}
+
+static uint32_t manh128(int side, u128 s, const int* gr, const int* gc) {
+    uint32_t hsum = 0;
+    for (int c = 0; c < side * side; ++c) {
+        uint32_t t = cell5(s, c);
+        if (t == 0) continue; /* blank excluded */
+        int dr = c / side - gr[t], dc = c % side - gc[t];
+        hsum += (uint32_t)((dr < 0 ? -dr : dr) + (dc < 0 ? -dc : dc));
+    }
+    return hsum;
+}
+
+uint32_t oracle_board_manhattan(int side, const uint8_t* state, const uint8_t* goal) {
+    int gr[32], gc[32];
+    goal_rc(side, board_pack(side, goal), gr, gc);
+    return manh128(side, board_pack(side, state), gr, gc);
+}
+
+/* node store keyed by 128-bit states (open addressing over node indices) */
+typedef struct {
+    u128* key;
+    uint32_t* g;
+    uint32_t* parent;
+    size_t n, cap;
+    uint32_t* slot;
+    size_t slot_mask;
+} bstore;
+
+static uint64_t hash128(u128 k) { return mix64((uint64_t)k ^ mix64((uint64_t)(k >> 64) + 0x9E3779B97F4A7C15ull)); }
+
+static int bstore_grow_slots(bstore* st) {
+    size_t ns = st->slot_mask ? (st->slot_mask + 1) * 2 : (1u << 16);
+    uint32_t* sl = (uint32_t*)calloc(ns, sizeof(uint32_t));
+    if (!sl) return -1;
+    for (size_t i = 0; i < st->n; ++i) {
+        size_t p = hash128(st->key[i]) & (ns - 1);
+        while (sl[p]) p = (p + 1) & (ns - 1);
+        sl[p] = (uint32_t)(i + 1);
+    }
+    free(st->slot);
+    st->slot = sl;
+    st->slot_mask = ns - 1;
+    return 0;
+}
+
+static int64_t bstore_get(bstore* st, u128 key) {
+    if (2 * (st->n + 1) > st->slot_mask + 1)
+        if (bstore_grow_slots(st)) return -1;
+    size_t p = hash128(key) & st->slot_mask;
+    while (st->slot[p]) {
+        uint32_t i = st->slot[p] - 1;
+        if (st->key[i] == key) return i;
+        p = (p + 1) & st->slot_mask;
+    }
+    if (st->n == st->cap) {
+        size_t nc = st->cap ? st->cap * 2 : 4096;
+        u128* k = (u128*)realloc(st->key, nc * sizeof(u128));
+        if (!k) return -1;
+        st->key = k;
+        uint32_t* g = (uint32_t*)realloc(st->g, nc * sizeof(uint32_t));
+        if (!g) return -1;
+        st->g = g;
+        uint32_t* pa = (uint32_t*)realloc(st->parent, nc * sizeof(uint32_t));
+        if (!pa) return -1;
+        st->parent = pa;
+        st->cap = nc;
+    }
+    size_t i = st->n++;
+    st->key[i] = key;
+    st->g[i] = INF32;
+    st->parent[i] = NONE32;
+    st->slot[p] = (uint32_t)(i + 1);
+    return (int64_t)i;
+}
+
+static void bstore_free(bstore* st) {
+    free(st->key);
+    free(st->g);
+    free(st->parent);
+    free(st->slot);
+}
+
+int oracle_astar_board(int side, const uint8_t* start_b, const uint8_t* goal_b, uint64_t max_states,
+                       oracle_result* res, uint8_t* path, uint32_t path_cap) {
+    memset(res, 0, sizeof(*res));
+    res->cost = -1;
+    if (!board_valid(side, start_b) || !board_valid(side, goal_b)) return ORACLE_E_INVALID;
+    const u128 start = board_pack(side, start_b), goal = board_pack(side, goal_b);
+    int gr[32], gc[32];
+    goal_rc(side, goal, gr, gc);
+    bstore st;
+    memset(&st, 0, sizeof(st));
+    pq q;
+    memset(&q, 0, sizeof(q));
+    int rc = ORACLE_NO_PATH;
+    int64_t s0 = bstore_get(&st, start);
+    if (s0 < 0) { rc = ORACLE_E_NOMEM; goto out; }
+    st.g[s0] = 0;
+    if (pq_push(&q, (pq_item){prio_of(manh128(side, start, gr, gc), 0), (uint32_t)s0, 0})) { rc = ORACLE_E_NOMEM; goto out; }
+    int64_t goal_idx = -1;
+    while (q.n) {
+        pq_item it = pq_pop(&q);
+        if (it.g > st.g[it.idx]) continue; /* stale */
+        u128 s = st.key[it.idx];
+        if (s == goal) { goal_idx = it.idx; break; } /* goal test at pop: a_star.h:80-83 */
+        res->expanded++;
+        int b = board_blank(side, s);
+        for (int i = 0; i < 4; ++i) {
+            u128 t;
+            if (!board_succ(side, s, b, i, &t)) continue;
+            res->generated++;
+            uint32_t g2 = it.g + 1;
+            int64_t ti = bstore_get(&st, t);
+            if (ti < 0) { rc = ORACLE_E_NOMEM; goto out; }
+            if (max_states && st.n > max_states) { rc = ORACLE_E_LIMIT; goto out; }
+            if (g2 >= st.g[ti]) continue; /* fancy_hashmap.h:56-61 */
+            st.g[ti] = g2;
+            st.parent[ti] = it.idx;
+            if (pq_push(&q, (pq_item){prio_of(g2 + manh128(side, t, gr, gc), g2), (uint32_t)ti, g2})) { rc = ORACLE_E_NOMEM; goto out; }
+        }
+    }
+    res->stored = st.n;
+    if (goal_idx >= 0) {
+        rc = ORACLE_OK;
+        res->cost = st.g[goal_idx];
+        uint32_t len = 0;
+        for (uint32_t i = (uint32_t)goal_idx; i != NONE32; i = st.parent[i]) ++len;
+        res->path_len = len;
+        if (path && path_cap >= len) {
+            uint32_t k = len;
+            for (uint32_t i = (uint32_t)goal_idx; i != NONE32; i = st.parent[i])
+                board_unpack(side, st.key[i], path + (size_t)(--k) * side * side);
+        }
+    }
+out:
+    if (rc != ORACLE_OK && rc != ORACLE_NO_PATH) res->stored = st.n;
+    bstore_free(&st);
+    free(q.a);
+    return rc;
+}
+
+int64_t oracle_bfs_board(int side, const uint8_t* start_b, const uint8_t* goal_b, uint32_t max_depth,
+                         uint64_t max_states) {
+    if (!board_valid(side, start_b) || !board_valid(side, goal_b)) return -2;
+    const u128 start = board_pack(side, start_b), goal = board_pack(side, goal_b);
+    if (start == goal) return 0;
+    bstore st;
+    memset(&st, 0, sizeof(st));
+    int64_t result = -1;
+    int64_t s0 = bstore_get(&st, start);
+    if (s0 < 0) { result = -3; goto out; }
+    st.g[s0] = 0;
+    /* the store's insertion order is the BFS queue: node i is expanded after every node before it */
+    for (size_t head = 0; head < st.n; ++head) {
+        uint32_t d = st.g[head];
+        if (d >= max_depth) break;
+        u128 s = st.key[head];
+        int b = board_blank(side, s);
+        for (int i = 0; i < 4; ++i) {
+            u128 t;
+            if (!board_succ(side, s, b, i, &t)) continue;
+            int64_t ti = bstore_get(&st, t);
+            if (ti < 0) { result = -3; goto out; }
+            if (st.g[ti] != INF32) continue;
+            st.g[ti] = d + 1;
+            if (t == goal) { result = d + 1; goto out; }
+            if (max_states && st.n > max_states) { result = -3; goto out; }
+        }
+    }
+out:
+    bstore_free(&st);
+    return result;
+}
+
+int64_t oracle_replay_board(int side, const uint8_t* path, uint32_t n, const uint8_t* start_b, const uint8_t* goal_b) {
+    const size_t cells = (size_t)side * side;
+    if (n == 0 || !board_valid(side, start_b) || !board_valid(side, goal_b)) return -1;
+    if (memcmp(path, start_b, cells) || memcmp(path + (n - 1) * cells, goal_b, cells)) return -1;
+    int64_t cost = 0;
+    for (uint32_t k = 1; k < n; ++k) {
+        if (!board_valid(side, path + k * cells)) return -1;
+        u128 s = board_pack(side, path + (k - 1) * cells), want = board_pack(side, path + k * cells);
+        int b = board_blank(side, s);
+        int ok = 0;
+        for (int i = 0; i < 4 && !ok; ++i) {
+            u128 t;
+            if (board_succ(side, s, b, i, &t) && t == want) ok = 1;
+        }
+        if (!ok) return -1;
+        cost += 1;
+    }
+    return cost;
+}

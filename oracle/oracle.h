@@ -36,10 +36,18 @@
  *                          w*(|dx|+|dy|) / w*max(|dx|,|dy|) on wall-free grids.
  *   oracle_dial_grid     - closed forms above and hand-built mazes.
  *   oracle_replay_*      - hand-built legal and illegal paths.
+ *   oracle_astar_board   - equal to oracle_astar_puzzle on 3x3 and 4x4 boards
+ *                          (a second encoding of the same search); equal to
+ *                          oracle_bfs_board on 5x5 walks; closed form k for a
+ *                          blank slid k cells along a row or column; parity
+ *                          and bound invariants on deeper 5x5 walks.
+ *   oracle_bfs_board     - equal to the 8-puzzle BFS table on 3x3 boards.
  *
  * State encodings (shared with the engine's C ABI):
  *   puzzle: uint64, nibble i (bits 4i..4i+3) = tile at cell i in row-major
  *           order, 0 = blank; side 3 or 4.
+ *   board:  uint8 board[side*side], tile at each cell row-major, 0 = blank;
+ *           side 2..5 (the engine takes 5x5 boards this way).
  *   grid:   uint8 cells[w*h] row-major, 0 = wall, 1..255 = cost of entering;
  *           a state is the cell index y*w + x.
  */
@@ -101,6 +109,21 @@ int64_t oracle_replay_puzzle(int side, const uint64_t* path, uint32_t n,
 int64_t oracle_replay_grid(const uint8_t* cells, uint32_t w, uint32_t h,
                            int connectivity, const uint32_t* path, uint32_t n,
                            uint32_t start, uint32_t goal);
+
+/* Boards of side 2..5 (up to the 24-puzzle), given as side*side bytes (tile
+ * numbers row-major, 0 = blank), packed internally 5 bits per cell into 128
+ * bits.  Same search rules as oracle_astar_puzzle.  path receives path_len
+ * boards of side*side bytes each (start first). */
+int oracle_astar_board(int side, const uint8_t* start, const uint8_t* goal,
+                       uint64_t max_states, oracle_result* res,
+                       uint8_t* path, uint32_t path_cap);
+uint32_t oracle_board_manhattan(int side, const uint8_t* state, const uint8_t* goal);
+/* Breadth-first distance from start to goal, or -1 if it exceeds max_depth,
+ * -2 for an invalid board, -3 if max_states (0 = none) is exceeded. */
+int64_t oracle_bfs_board(int side, const uint8_t* start, const uint8_t* goal,
+                         uint32_t max_depth, uint64_t max_states);
+int64_t oracle_replay_board(int side, const uint8_t* path, uint32_t n,
+                            const uint8_t* start, const uint8_t* goal);
 
 #ifdef __cplusplus
 }

@@ -137,3 +137,43 @@ def test_workload_requests_build_on_cpu():
     assert [x.arrival_ns for x in insts] == sorted(x.arrival_ns for x in insts)
     p = wl.puzzle15(3, num_queues=512, table_slots=1 << 20)
     assert p.request().table_slots == 1 << 20 and p.request(table_slots=1 << 12).table_slots == 1 << 12
+
+
+def test_gen_board_equals_gen_puzzle_on_sides_3_and_4():
+    """ga_gen_board follows ga_gen_puzzle draw for draw."""
+    for side in (3, 4):
+        for seed in range(6):
+            for depth in (0, 9, 31, 60):
+                for nb in (True, False):
+                    b = ga.gen_board(side, depth, seed, no_backtrack=nb)
+                    assert list(b) == ga.state_to_board(ga.gen_puzzle(side, depth, seed, no_backtrack=nb), side)
+
+
+def test_board_solvable_matches_puzzle_solvable_and_parity():
+    for side in (3, 4):
+        g = ga.puzzle_goal(side)
+        for seed in range(8):
+            s = ga.gen_puzzle(side, 30, seed)
+            t = ga.state_to_board(s, side)
+            a, b = [i for i, x in enumerate(t) if x][:2]
+            t[a], t[b] = t[b], t[a]  # one transposition of tiles flips reachability
+            for board in (ga.state_to_board(s, side), t):
+                assert ga.board_solvable(side, board, ga.state_to_board(g, side)) == \
+                    ga.puzzle_solvable(side, ga.board_to_state(board), g)
+    for seed in range(8):
+        b = ga.gen_board(5, 25 + seed, seed)
+        assert ga.board_solvable(5, b, ga.board_goal(5))
+        t = list(b)
+        i, j = t.index(3), t.index(4)
+        t[i], t[j] = t[j], t[i]
+        assert not ga.board_solvable(5, t, ga.board_goal(5))
+
+
+def test_puzzle5_request_carries_both_boards():
+    b = ga.gen_board(5, 33, 4)
+    r = ga.puzzle5_request(b, num_queues=2048)
+    raw = ctypes.string_at(r.cells, 50)
+    assert r.kind == ga.PUZZLE and r.side == 5 and r.num_queues == 2048
+    assert tuple(raw[:25]) == b and tuple(raw[25:]) == ga.board_goal(5)
+    arr = (ga.Request * 1)(r)  # the batch array copies the pointer, the buffer stays on r
+    assert ctypes.string_at(arr[0].cells, 50) == raw

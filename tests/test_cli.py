@@ -55,6 +55,24 @@ def test_sliding_matches_oracle(side, depth, seed):
     assert oracle.replay_puzzle(side, path, start, goal) == ref.cost
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("depth,seed", [(20, 3), (36, 4)])
+def test_sliding_size5_matches_oracle(depth, seed):
+    """The reference's default size (main.cu:72): 5x5 Position structs through
+    the drop-in's wide-key closed set."""
+    start = ga.gen_board(5, depth, seed)
+    goal = ga.board_goal(5)
+    text = lambda b: ",".join(str(t) if t else "_" for t in b)
+    r = run(["--version", "sliding", "--size", "5", "--num-queues", "1024"], f"{text(start)} {text(goal)}\n")
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.strip().split("\n")
+    int(lines[0])  # elapsed milliseconds
+    path = [tuple(0 if t == "_" else int(t) for t in l.split(",")) for l in lines[1:]]
+    rc, ref, _ = oracle.astar_board(5, start, goal, want_path=False)
+    assert rc == oracle.OK and len(path) - 1 == ref.cost
+    assert oracle.replay_board(5, path, start, goal) == ref.cost
+
+
 def sparse_grid(w, h, seed, wall_p=0.2):
     """A grid in the reference's sparse format and as a dense array (0 = wall)."""
     cells, _ = ga.gen_grid(w, h, wall_p, seed, connectivity=8)

@@ -149,3 +149,90 @@ def test_planted_mistakes_are_caught():
     s = ga.gen_puzzle(3, 1, 0)
     blank_counted = oracle.manhattan(3, s, GOAL8) + 1  # the blank is one cell off its goal
     assert blank_counted > depth[oracle.perm_rank9(s)]  # inadmissible: h > h*
+
+
+# ---------------------------------------------------------------------------
+# boards (up to the 24-puzzle): a second encoding of the same search
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("side,depth,seed", [(3, 25, 1), (3, 40, 2), (4, 30, 3), (4, 45, 4), (4, 60, 5)])
+def test_board_astar_equals_packed_astar(side, depth, seed):
+    """oracle_astar_board (5 bits per cell, 128-bit keys) and oracle_astar_puzzle
+    (4 bits per cell, 64-bit keys) are independent codings: same cost, and the
+    board path replays under the packed replay too."""
+    s = ga.gen_puzzle(side, depth, seed)
+    g = ga.puzzle_goal(side)
+    sb, gb = ga.state_to_board(s, side), ga.state_to_board(g, side)
+    rc1, r1, _ = oracle.astar_puzzle(side, s, g, want_path=False)
+    rc2, r2, path = oracle.astar_board(side, sb, gb)
+    assert rc1 == rc2 == oracle.OK and r1.cost == r2.cost
+    assert oracle.board_manhattan(side, sb, gb) == oracle.manhattan(side, s, g)
+    assert oracle.replay_puzzle(side, [ga.board_to_state(b) for b in path], s, g) == r2.cost
+    assert oracle.replay_board(side, path, sb, gb) == r2.cost
+
+
+def test_board_bfs_equals_8puzzle_table():
+    """The board BFS reproduces the full 8-puzzle BFS (itself pinned to OEIS A089473)."""
+    _, depth, _ = oracle.bfs_puzzle8(GOAL8)
+    gb = ga.state_to_board(GOAL8, 3)
+    rng = np.random.default_rng(77)
+    seen = 0
+    for perm in rng.permuted(np.tile(np.arange(9), (60, 1)), axis=1):
+        d = depth[oracle.perm_rank9(ga.board_to_state(perm))]
+        if d == 0xFF:
+            assert oracle.bfs_board(3, perm, gb, 31) == -1
+            continue
+        assert oracle.bfs_board(3, perm, gb, 31) == d
+        seen += 1
+    assert seen >= 20
+
+
+@pytest.mark.parametrize("depth,seed", [(8, 1), (11, 2), (13, 3), (14, 4), (15, 5), (16, 6)])
+def test_board_astar_equals_bfs_5x5(depth, seed):
+    start, goal = oracle.walk_board(5, depth, seed)
+    d = oracle.bfs_board(5, start, goal, depth, max_states=20_000_000)
+    assert 0 <= d <= depth
+    rc, res, path = oracle.astar_board(5, start, goal)
+    assert rc == oracle.OK and res.cost == d
+    assert oracle.replay_board(5, path, start, goal) == d
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4])
+def test_board_closed_form_straight_slide(k):
+    """The blank slid k cells left along the bottom row (and k cells up the last
+    column): Manhattan = k and a k-move path exists, so the optimum is k."""
+    goal = list(range(1, 25)) + [0]
+    row = goal[:]
+    for _ in range(k):
+        p = row.index(0)
+        row[p], row[p - 1] = row[p - 1], 0
+    col = goal[:]
+    for _ in range(k):
+        p = col.index(0)
+        col[p], col[p - 5] = col[p - 5], 0
+    for start in (row, col):
+        assert oracle.board_manhattan(5, start, goal) == k
+        rc, res, path = oracle.astar_board(5, start, goal)
+        assert rc == oracle.OK and res.cost == k and len(path) == k + 1
+
+
+@pytest.mark.parametrize("depth,seed", [(30, 11), (40, 12), (50, 13)])
+def test_board_5x5_invariants(depth, seed):
+    start, goal = oracle.walk_board(5, depth, seed)
+    h = oracle.board_manhattan(5, start, goal)
+    rc, res, path = oracle.astar_board(5, start, goal, max_states=30_000_000)
+    assert rc == oracle.OK
+    assert h <= res.cost <= depth
+    assert res.cost % 2 == depth % 2 == h % 2
+    assert oracle.replay_board(5, path, start, goal) == res.cost
+
+
+def test_board_rejects_bad_inputs_and_paths():
+    start, goal = oracle.walk_board(5, 6, 9)
+    bad = list(goal)
+    bad[0] = bad[1]  # not a permutation
+    assert oracle.astar_board(5, bad, goal)[0] == oracle.E_INVALID
+    rc, res, path = oracle.astar_board(5, start, goal)
+    assert oracle.replay_board(5, path[:-1], start, goal) == -1  # does not reach the goal
+    jump = [path[0]] + path[2:]
+    if len(path) > 2:
+        assert oracle.replay_board(5, jump, start, goal) == -1  # two moves at once
