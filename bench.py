@@ -51,6 +51,66 @@ def pct(xs, p):
     return xs[k]
 
 
+P24_N, P24_DEPTH = 256, (30, 60)
+
+
+def puzzle24_leg(e, heaps: int, threads: int, cpu_seconds: float):
+    """256 seeded 24-puzzles (walks of depth U{30..60} from the ordered board, no
+    immediate backtrack) in one ga_solve_batch; the CPU oracle (sequential A*
+    on boards) takes the same instances in order on `threads` workers for
+    `cpu_seconds`, and every instance it finishes checks the GPU cost."""
+    import cuda_a_star_amd as ga
+    from cuda_a_star_amd import workloads as wl
+
+    st = wl.Stream(24)
+    boards = [ga.gen_board(5, st.uniform_int(*P24_DEPTH), wl.instance_seed(24, i)) for i in range(P24_N)]
+    reqs = [ga.puzzle5_request(b, num_queues=heaps) for b in boards]
+    e.solve_batch(reqs[:16])  # warm-up
+    t = time.perf_counter()
+    sols = e.solve_batch(reqs)
+    gpu_t = time.perf_counter() - t
+    ok = sum(s.status == ga.OK for s in sols)
+    out = {"solves_per_sec": round(ok / gpu_t, 3), "instances": P24_N, "ok": ok, "seconds": round(gpu_t, 3),
+           "workload": f"24-puzzle walks of depth U{{{P24_DEPTH[0]}..{P24_DEPTH[1]}}} from the ordered board, "
+                       f"{heaps} heaps each, one batch"}
+    if cpu_seconds <= 0:
+        return out
+    import oracle
+
+    lock = threading.Lock()
+    state = {"next": 0, "done": 0, "match": 0, "capped": 0}
+    deadline = time.perf_counter() + cpu_seconds
+    t0 = time.perf_counter()
+
+    def worker():
+        while True:
+            with lock:
+                i = state["next"]
+                if i >= P24_N or time.perf_counter() > deadline:
+                    return
+                state["next"] += 1
+            rc, res, _ = oracle.astar_board(5, boards[i], ga.board_goal(5), max_states=MAX_ORACLE_STATES,
+                                            want_path=False)
+            with lock:
+                if rc == oracle.E_LIMIT:
+                    state["capped"] += 1
+                    continue
+                state["done"] += 1
+                state["match"] += int(rc == oracle.OK and sols[i].status == ga.OK and sols[i].cost == res.cost)
+
+    ts = [threading.Thread(target=worker) for _ in range(threads)]
+    for th in ts:
+        th.start()
+    for th in ts:
+        th.join()
+    el = time.perf_counter() - t0
+    out["cpu_oracle"] = {"solves_per_sec": round(state["done"] / el, 3), "cores": threads, "kind": "port",
+                         "sample": f"first {state['done']} instances in order, {el:.1f}s; {state['capped']} above "
+                                   f"{MAX_ORACLE_STATES} stored states skipped"}
+    out["gpu_cost_matches"] = f"{state['match']}/{state['done']}"
+    return out
+
+
 def cpu_baseline(insts, seconds: float, threads: int, gpu_costs: dict):
     """CPU oracle (sequential A*) on the host cores over a bounded sample of
     the same stream: requests are taken in stream order by `threads` workers
@@ -112,6 +172,7 @@ def main() -> int:
                     help="time the step launches of every n-th chunk with HIP events (counters cover all)")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
+    ap.add_argument("--skip-p24", action="store_true", help="skip the 24-puzzle leg")
     ap.add_argument("--override", default="", help="served tuning overrides, e.g. lanes=3,rounds=2")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (for scaling HBM byte counters)")
@@ -263,6 +324,13 @@ def main() -> int:
             e.free_device(a)
         log(f"[bench] batched: {b_ok}/{len(b_reqs)} in {b_t:.2f}s = {b_ok / b_t:.1f} solves/s")
 
+    # ---- 24-puzzle: 5x5 boards on the wide-key kernel, one batch, vs the CPU oracle ----
+    p24 = None
+    if not args.skip_p24:
+        p24 = puzzle24_leg(e, served_cfg.heaps("puzzle", 5), args.cpu_threads or min(16, os.cpu_count() or 1),
+                           0.0 if args.skip_cpu else 8.0)
+        log(f"[bench] 24-puzzle: {p24}")
+
     # ---- one at a time: plain GA*, same requests, sequential ----
     timed_idx = list(range(W * M, (W + K) * M))
     plain_sps, mismatch = None, 0
@@ -330,6 +398,7 @@ def main() -> int:
         "failed": failed,
         "one_at_a_time_solves_per_sec": round(plain_sps, 3) if plain_sps else None,
         "batched": batch,
+        "puzzle24": p24,
         "speedup_vs_one_at_a_time": round(served_sps / plain_sps, 2) if plain_sps else None,
         "served_vs_alone_cost_mismatches": mismatch,
         "roofline": roofline,
