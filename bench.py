@@ -291,6 +291,8 @@ def main() -> int:
                     "kernel": f"step_kernel<{served_cfg.heap_arity}>", "avg_launch_us": round(avg_ms * 1e3, 3),
                     "algo_bytes_per_launch": round(bytes_per_launch, 1),
                     "expansions_per_launch": round(st.expanded / st.step_launches, 1),
+                    # heap-group iterations with work / launched (every mapped heap, every launch)
+                    "working_group_fraction": round(st.work_groups / st.heap_groups, 4) if st.heap_groups else None,
                     "launches": st.step_launches,
                     "timed_launches": st.timed_launches,
                     "launches_before_timed": launches_before,

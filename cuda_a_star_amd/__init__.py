@@ -112,6 +112,8 @@ class EngineStats(ctypes.Structure):
         ("lookups", ctypes.c_uint64),
         ("algo_bytes", ctypes.c_double),
         ("timed_launches", ctypes.c_uint64),
+        ("heap_groups", ctypes.c_uint64),
+        ("work_groups", ctypes.c_uint64),
     ]
 
 

@@ -39,25 +39,24 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
                                                       const InstDesc* __restrict__ descs, InstState* states,
                                                       const uint32_t* __restrict__ iter_base, uint32_t r,
                                                       StepParams prm) {
-    __shared__ uint32_t tally[kTallies];
+    __shared__ StepShared sh;
     uint32_t inst, it, first, cover;
-    bool running;
-    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover, running)) return;
-    tally_begin(tally);
+    if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
+    const bool running = step_begin(sh, states + inst);  // workgroup-uniform
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / W;
     const bool active = q < d.n_heaps;
-    const Tally tl{tally, prm.profile != 0};
+    const Tally tl{sh.tally, prm.profile != 0};
     if (running) {
         switch (d.kind) {
-            case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm, tl); break;
-            case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm, tl); break;
-            case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm, tl); break;
-            case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm, tl); break;
+            case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
+            case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
+            case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
+            case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
             default: break;
         }
     }
-    tally_end(tally, states + inst, it);
+    tally_end(sh.tally, states + inst, it);
 }
 
 // Step kernel of the wide-key kinds (24-puzzle): its own instantiation so that
@@ -68,17 +67,16 @@ __global__ __launch_bounds__(kBlock) void step_kernel_wide(const WgEntry* __rest
                                                            const InstDesc* __restrict__ descs, InstState* states,
                                                            const uint32_t* __restrict__ iter_base, uint32_t r,
                                                            StepParams prm) {
-    __shared__ uint32_t tally[kTallies];
+    __shared__ StepShared sh;
     uint32_t inst, it, first, cover;
-    bool running;
-    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover, running)) return;
-    tally_begin(tally);
+    if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
+    const bool running = step_begin(sh, states + inst);  // workgroup-uniform
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / W;
-    const Tally tl{tally, prm.profile != 0};
+    const Tally tl{sh.tally, prm.profile != 0};
     if (running && d.kind == KIND_PUZZLE5)
-        run_group<W, Puzzle5Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, tl);
-    tally_end(tally, states + inst, it);
+        run_group<W, Puzzle5Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, tl, sh.work);
+    tally_end(sh.tally, states + inst, it);
 }
 
 __global__ void seed_kernel(InstDesc d, InstDesc* descs, InstState* states, uint32_t slot, int W) {
@@ -326,6 +324,7 @@ struct Lane {
     hipStream_t stream = nullptr;
     WgEntry* d_map = nullptr;
     uint32_t map_cap = 0, n_wg = 0;
+    uint64_t n_heaps = 0;     // heaps the map covers
     bool map_dirty = true;
     Pinned h_map[2];
     hipEvent_t ev_map[2] = {nullptr, nullptr};
@@ -621,10 +620,12 @@ struct ga_engine {
         const uint32_t W = cfg.heap_arity;
         const uint32_t heaps_per_wg = kBlock / W;
         std::vector<WgEntry> map;
+        L.n_heaps = 0;
         for (uint32_t s = lane; s < cfg.max_instances; s += n_lanes) {
             Job* j = resident[s];
             if (!j || j->finished) continue;
             const uint32_t n = std::min(j->cover, j->desc.n_heaps);
+            L.n_heaps += n;
             for (uint32_t h = 0; h < n; h += heaps_per_wg) map.push_back(WgEntry{s, h, n, 0});
         }
         L.n_wg = (uint32_t)map.size();
@@ -718,6 +719,7 @@ struct ga_engine {
         hip_check(hipGetLastError(), "step launch");
         L.host_base += R;
         stats.step_launches += R;
+        stats.heap_groups += L.n_heaps * R;
         uint32_t n_read = 0;
         for (uint32_t s = 0; s < cfg.max_instances; ++s)
             if (resident[s]) n_read = s + 1;
@@ -761,6 +763,7 @@ struct ga_engine {
         stats.groups += s.groups;
         stats.inserted += s.inserted;
         stats.lookups += s.lookups;
+        stats.work_groups += s.work_groups;
         stats.algo_bytes += algo_bytes(j.kind, s);
     }
 

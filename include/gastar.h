@@ -124,11 +124,13 @@ typedef struct ga_engine_stats {
     uint64_t pops;           /* heap pops (including stale ones) */
     uint64_t completed;      /* requests completed */
     uint64_t peak_bytes;     /* peak admitted heap + closed-set bytes */
-    uint64_t groups;         /* heap-group iterations (profile) */
+    uint64_t groups;         /* heap-group iterations of running instances (profile) */
     uint64_t inserted;       /* successors that lowered a stored g (profile) */
     uint64_t lookups;        /* closed-set find-or-insert calls (profile) */
     double algo_bytes;       /* algorithmic bytes of the step kernels (profile) */
     uint64_t timed_launches; /* step launches that step_ms_total covers */
+    uint64_t heap_groups;    /* heap-group iterations launched (every mapped heap, every launch) */
+    uint64_t work_groups;    /* heap-group iterations with work: pushes, or a pop within the f gate (profile) */
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {

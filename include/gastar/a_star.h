@@ -92,15 +92,16 @@ __global__ __launch_bounds__(kBlock) void user_step_kernel(const WgEntry* __rest
                                                            const InstDesc* __restrict__ descs, InstState* states,
                                                            const uint32_t* __restrict__ iter_base, uint32_t r,
                                                            StepParams prm) {
-    __shared__ uint32_t tally[kTallies];
+    __shared__ StepShared sh;
     uint32_t inst, it, first, cover;
-    bool running;
-    if (!step_prologue(wgmap, descs, states, iter_base, r, inst, it, first, cover, running)) return;
-    tally_begin(tally);
+    if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
+    const bool running = step_begin(sh, states + inst);  // workgroup-uniform
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / kW;
-    if (running) run_group<kW, Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, Tally{tally, prm.profile != 0});
-    tally_end(tally, states + inst, it);
+    if (running)
+        run_group<kW, Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, Tally{sh.tally, prm.profile != 0},
+                           sh.work);
+    tally_end(sh.tally, states + inst, it);
 }
 
 template <class Ops>

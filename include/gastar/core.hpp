@@ -168,6 +168,7 @@ struct InstState {
     unsigned long long groups;    // heap-group iterations that ran (live)
     unsigned long long inserted;  // successors whose atomicMin lowered g
     unsigned long long lookups;   // closed-set find-or-insert calls (one atomicMin each)
+    unsigned long long work_groups;  // heap-group iterations with work (pushes or a pop within the gate)
 };
 
 // Per-workgroup tallies of one step, in LDS.  Lanes add to them with LDS
@@ -178,14 +179,14 @@ struct InstState {
 // counter fields of InstState from `occupied` on.
 enum : uint32_t {
     TL_OCCUPIED, TL_EXPANDED, TL_GENERATED, TL_PUSHED, TL_POPS, TL_PROBES, TL_SIFT_DOWN, TL_SIFT_UP,
-    TL_GROUPS, TL_INSERTED, TL_LOOKUPS,
+    TL_GROUPS, TL_INSERTED, TL_LOOKUPS, TL_WORK,
     kCounters,
     TL_MIN_TOP = kCounters,  // min f over the heap tops
     TL_OPEN,                 // heap entries after the pushes
     TL_WAVES_DONE,           // waves of the workgroup that have finished the step
     kTallies
 };
-static_assert(offsetof(InstState, lookups) == offsetof(InstState, occupied) + 8 * (kCounters - 1),
+static_assert(offsetof(InstState, work_groups) == offsetof(InstState, occupied) + 8 * (kCounters - 1),
               "InstState counters must be contiguous and in tally order");
 
 // Workgroup -> (instance, first heap) map of one launch.  `cover` is the
@@ -272,10 +273,32 @@ struct Tally {
     }
 };
 
-// Clears the tallies; every thread of the workgroup calls it.
-__device__ inline void tally_begin(uint32_t* c) {
-    if (threadIdx.x < kTallies) c[threadIdx.x] = threadIdx.x == TL_MIN_TOP ? kInf : 0u;
+// Heaps of one workgroup with work this launch: filled in phase 1 of
+// run_group, read in phase 2 (at most 64 heap groups per workgroup).
+struct WorkList {
+    uint32_t n;
+    uint32_t q[64];  // heap
+    uint32_t S[64];  // its size before the pushes
+};
+
+// LDS of one step-kernel workgroup.
+struct StepShared {
+    uint32_t tally[kTallies];
+    uint32_t running;  // the instance's status is ST_RUNNING, read once for the whole workgroup
+    WorkList work;
+};
+
+// Clears the tallies and the work list and reads the instance status once, so
+// that every wave of the workgroup takes the same path (run_group holds a
+// barrier).  Every thread of the workgroup calls it; returns "running".
+__device__ inline bool step_begin(StepShared& sh, const InstState* st) {
+    if (threadIdx.x < kTallies) sh.tally[threadIdx.x] = threadIdx.x == TL_MIN_TOP ? kInf : 0u;
+    if (threadIdx.x == 0) {
+        sh.work.n = 0;
+        sh.running = load_relaxed(&st->status) == ST_RUNNING ? 1u : 0u;
+    }
     __syncthreads();
+    return sh.running != 0;
 }
 
 // Adds the tallies to the instance's state for iteration `it`; every thread
@@ -894,16 +917,34 @@ __device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEn
     S += c;
 }
 
-// One launch = one GA* iteration for this heap group, made of `rounds`
-// expansion rounds.  Successors of rounds 0..rounds-2 go into the group's own
-// heap and can be expanded in the same launch; successors of the last round go
-// to the candidate buffer and are spread over all heaps at the next launch.
-// The termination test stays exact: each heap's top is taken after the
-// distributed pushes and before its first pop, and no other group touches
-// this heap, so the tops of all heaps form one snapshot of the open set.
+// One launch = one GA* iteration for the heaps of one workgroup, in two phases.
+//
+// Phase 1 (every heap group) reads, in one round trip, its heap's size and
+// root and the records addressed to it.  It contributes the heap's top after
+// the pushes -- min(root f, f of the incoming records), the pushes have not
+// happened yet -- to the min-f reduction (HeapsHost::minimum, heaps.h:291-294)
+// and its size to the open count, and it decides whether the heap has work:
+// records to push, or a root within the f gate.  Idle heaps clear their
+// candidate slice.
+// Phase 2: the heaps with work are packed into the first groups of the
+// workgroup through an LDS list, and those groups push, pop and expand them;
+// waves left without a heap are done.  A wave lives as long as its slowest
+// group, so packing spares the other waves from waiting on a few busy heaps.
+//
+// Expansion: `rounds` rounds per launch.  Successors of rounds 0..rounds-2 go
+// into the group's own heap and can be expanded in the same launch; successors
+// of the last round go to the candidate buffer and are spread over all heaps
+// at the next launch.  The termination test stays exact: each heap's top is
+// its top after the distributed pushes and before its first pop, and no other
+// group touches the heap, so the tops of all heaps form one snapshot of the
+// open set.  A root whose f is more than f_slack above the previous
+// iteration's minimum is left in place (the heap sits the launch out) and
+// still enters the reduction.
+//
+// Every thread of the workgroup calls run_group (it holds a barrier).
 template <int W, class Ops>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t it, uint32_t q, bool active,
-                                          uint32_t cover, const StepParams& prm, const Tally& tl) {
+                                          uint32_t cover, const StepParams& prm, const Tally& tl, WorkList& wl) {
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
@@ -919,15 +960,15 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     const uint32_t Aw =
         (fixed || it == 0) ? A : __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[(it - 1) % 3]));
     active = active && q < A;
+    const gptr<const HeapEntry> cin = as_global<const HeapEntry>(d.cand) + (size_t)((it - 1) & 1) * N * K;
 
-    // Independent loads first: the termination inputs and this heap's size and
-    // incoming records (unused if the group turns out not to be live).
+    // ---------------- phase 1 ----------------
     uint32_t best = __builtin_amdgcn_readfirstlane(load_relaxed(&st->best));
     const uint32_t mt = it > 0 ? __builtin_amdgcn_readfirstlane(load_relaxed(&st->min_top[(it - 1) % 3])) : 0u;
     uint64_t occupied = 0;
     if constexpr (!Ops::kDense) occupied = load_relaxed(&st->occupied);
-    const gptr<HeapEntry> H = as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1);
     uint32_t S = 0;
+    uint64_t root_prio = ~0ull;
     HeapEntry rec[KW];
 #pragma unroll
     for (int r = 0; r < KW; ++r) {
@@ -937,9 +978,10 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     }
     if (active) {
         S = as_global(d.heap_size)[q];
+        // the root slot always lies inside the heap's storage; it is used only if S > 0
+        if (lane == 0) root_prio = (as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1))[0].prio;
         if (it > 0 && q < Aw) {
             // insert_nodes_into_queues, a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N
-            const gptr<const HeapEntry> cin = as_global<const HeapEntry>(d.cand) + (size_t)((it - 1) & 1) * N * K;
             const uint32_t base = (q + it) & (Aw - 1);
 #pragma unroll
             for (int r = 0; r < KW; ++r) {
@@ -978,22 +1020,95 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             live = false;
         }
     }
+    const uint32_t gate = (it > 0 && mt != kInf && d.f_slack != kInf)
+                              ? (uint32_t)min((uint64_t)mt + d.f_slack, (uint64_t)kInf - 1)
+                              : kInf;
 
+    // the heap's top after its pushes, and its size after them
+    uint32_t rec_f = kInf, n_rec = 0;
+#pragma unroll
+    for (int r = 0; r < KW; ++r) {
+        const bool v = live && rec[r].prio != kNoRecord && prio_f(rec[r].prio) < best;  // the pushes' filter
+        if (v) {
+            rec_f = min(rec_f, prio_f(rec[r].prio));
+            ++n_rec;
+        }
+    }
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        rec_f = min(rec_f, (uint32_t)__shfl_xor((int)rec_f, o, W));
+        n_rec += (uint32_t)__shfl_xor((int)n_rec, o, W);
+    }
+    const uint32_t root_f = (live && S > 0) ? (uint32_t)__shfl((int)prio_f(root_prio), 0, W) : kInf;
+    const bool work = live && (n_rec > 0 || root_f <= gate);  // root_f = kInf when empty
+    {
+        const uint32_t top_f = wave_min_u32(lane == 0 ? min(root_f, rec_f) : kInf);
+        const uint32_t open = wave_sum_u32((live && lane == 0) ? S + n_rec : 0u);
+        if ((threadIdx.x & 63) == 0) {
+            if (top_f != kInf) atomicMin(&tl.c[TL_MIN_TOP], top_f);
+            tl.add(TL_OPEN, open);
+        }
+    }
+    tl.prof_count(TL_GROUPS, live && lane == 0);
+    if (live && !work) {  // an idle heap sends nothing: clear its candidate slice
+        const gptr<HeapEntry> cout = as_global(d.cand) + (size_t)(it & 1) * N * K + (size_t)q * K;
+        HeapEntry none;
+        none.prio = kNoRecord;
+        none.slot = 0;
+        none.aux = 0;
+#pragma unroll
+        for (int r = 0; r < KW; ++r) {
+            const int i = r * W + lane;
+            if (i < K) cout[i] = none;
+        }
+    }
+    // append the heaps with work to the list: one LDS atomic per wave
+    {
+        const bool lead = work && lane == 0;
+        const uint64_t m = __ballot(lead);
+        if (m) {
+            const int first = __builtin_ctzll(m);
+            uint32_t base = 0;
+            if ((int)__lane_id() == first) base = atomicAdd(&wl.n, (uint32_t)__popcll(m));
+            base = (uint32_t)__shfl((int)base, first, 64);
+            if (lead) {
+                const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1));
+                wl.q[k] = q;
+                wl.S[k] = S;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---------------- phase 2 ----------------
+    const uint32_t n_work = wl.n;
+    if ((threadIdx.x & ~63u) / W >= n_work) return;  // no heap for this wave
+    const uint32_t g = threadIdx.x / W;
+    const bool mine = g < n_work;
+    q = mine ? wl.q[g] : 0u;
+    S = mine ? wl.S[g] : 0u;
+    const gptr<HeapEntry> H = as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1);
+#pragma unroll
+    for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
+    if (mine && it > 0 && q < Aw) {  // the records phase 1 read for this heap (cached)
+        const uint32_t base = (q + it) & (Aw - 1);
+#pragma unroll
+        for (int r = 0; r < KW; ++r) {
+            const int j = r * W + lane;
+            if (j < K) rec[r] = cin[base + (size_t)j * Aw];
+        }
+    }
     bool overflow_heap = false, overflow_table = false;
 
     // -- 1. push the records addressed to this heap
-    if (live && it > 0 && q < Aw) {
+    if (mine && it > 0 && q < Aw) {
         push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl);
 #pragma unroll
         for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
     }
     group_fence();
 
-    // -- 2. first pop of this launch.  Its root is the heap's top after the
-    //       pushes and feeds the min-f reduction (HeapsHost::minimum,
-    //       heaps.h:291-294); a stale root only lowers the minimum (safe).
-    //       A root whose f is more than f_slack above the previous iteration's
-    //       minimum is left in place: the heap sits this launch out.
+    // -- 2. first pop of this launch, unless the root lies above the f gate
     auto load = [&](uint32_t slot, typename Ops::Key& k, uint64_t& v) { Ops::load(d, slot, k, v); };
     uint32_t pre[KW];  // per successor: what the problem reads early (grid: the entered cell's weight)
     auto prefetch = [&](const HeapEntry& t) {
@@ -1003,38 +1118,26 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             pre[r] = i < K ? Ops::prefetch(d, t.aux, i) : 0u;
         }
     };
-    const uint32_t gate = (it > 0 && mt != kInf && d.f_slack != kInf)
-                              ? (uint32_t)min((uint64_t)mt + d.f_slack, (uint64_t)kInf - 1)
-                              : kInf;
     HeapEntry top;
     top.prio = 0;
     top.slot = 0;
     top.aux = 0;
     typename Ops::Key top_key{};
     uint64_t top_val = 0;
-    bool popped = false, gated = false, have_root = false;
-    if (live && S > 0) {
+    bool popped = false, gated = false;
+    if (mine && S > 0) {
         uint32_t lv = 0;
         popped = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_val, gate, top);
         gated = !popped;
-        have_root = true;
         group_fence();
         if (lane == 0) {
             tl.prof_add(TL_SIFT_DOWN, lv);
             tl.prof_add(TL_POPS, popped ? 1u : 0u);
         }
     }
-    uint32_t top_f = (have_root && lane == 0) ? prio_f(top.prio) : kInf;
-    top_f = wave_min_u32(top_f);
-    // entries after the pushes
-    const uint32_t open = wave_sum_u32((have_root && lane == 0) ? (popped ? S + 1 : S) : 0u);
-    if ((threadIdx.x & 63) == 0) {
-        if (top_f != kInf) atomicMin(&tl.c[TL_MIN_TOP], top_f);
-        tl.add(TL_OPEN, open);
-    }
 
     // -- 3..6. expansion rounds
-    if (live) {
+    if (mine) {
         const uint32_t rounds = d.rounds ? d.rounds : prm.rounds;
         for (uint32_t round = 0; round < rounds && !gated; ++round) {
             // pop, skipping entries whose g is no longer the stored one
@@ -1059,16 +1162,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             }
             if (!have) break;
             // goal test at pop (a_star.h:80-83); a node with f >= best cannot improve it
-            const uint32_t g = prio_g(top.prio), f = prio_f(top.prio);
+            const uint32_t g0 = prio_g(top.prio), f = prio_f(top.prio);
             if (Ops::is_goal(d, top_key)) {
-                if (lane == 0) atomicMin(&st->best, g);
-                best = g < best ? g : best;
+                if (lane == 0) atomicMin(&st->best, g0);
+                best = g0 < best ? g0 : best;
                 continue;
             }
             if (best != kInf && f >= best) continue;
             tl.count(TL_EXPANDED, lane == 0);
             // expand, insert into the closed set
-            const uint32_t h = f - g;
+            const uint32_t h = f - g0;
 #pragma unroll
             for (int r = 0; r < KW; ++r) {
                 const int i = r * W + lane;
@@ -1078,7 +1181,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 const bool gen = i < K && Ops::succ(d, top_key, top.aux, h, i, pre[r], tkey, taux, cost, th);
                 tl.count(TL_GENERATED, gen);
                 if (gen) {
-                    const uint32_t g2 = g + cost, f2 = g2 + th;
+                    const uint32_t g2 = g0 + cost, f2 = g2 + th;
                     if (f2 < best) {
                         uint32_t slot;
                         bool isnew;
@@ -1086,9 +1189,9 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                         const bool found = Ops::find_or_insert(d, tkey, prm.max_probes, slot, isnew, np);
                         bool ins = false;
                         if (found) {
-                            const uint64_t mine = ((uint64_t)g2 << 32) | top.slot;
-                            const uint64_t old = __hip_atomic_fetch_min(Ops::val_ptr(d, slot), mine, __ATOMIC_RELAXED,
-                                                                        __HIP_MEMORY_SCOPE_AGENT);
+                            const uint64_t mine_val = ((uint64_t)g2 << 32) | top.slot;
+                            const uint64_t old = __hip_atomic_fetch_min(Ops::val_ptr(d, slot), mine_val,
+                                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                             ins = g2 < val_g(old);
                             if (ins) {
                                 rec[r].prio = make_prio(f2, g2);
@@ -1113,7 +1216,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             group_fence();
         }
         if (lane == 0) as_global(d.heap_size)[q] = S;
-        // candidate slice of this heap: K records, written every iteration
+        // candidate slice of this heap: K records
         const gptr<HeapEntry> cout = as_global(d.cand) + (size_t)(it & 1) * N * K + (size_t)q * K;
 #pragma unroll
         for (int r = 0; r < KW; ++r) {
@@ -1124,19 +1227,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     if (overflow_heap) set_status(st, ST_OVERFLOW_HEAP);
     if (overflow_table) set_status(st, ST_OVERFLOW_TABLE);
 
-    tl.prof_count(TL_GROUPS, live && lane == 0);
+    tl.prof_count(TL_WORK, mine && lane == 0);
 }
 
 // Per-workgroup prologue shared by every step kernel: returns false (for the
-// whole workgroup) when the launch map has no instance for it.  `running` is
-// this wave's reading of the instance status; a workgroup whose instance has
-// stopped still takes part in the tally barriers, with nothing to count.
-__device__ inline bool step_prologue(const WgEntry* wgmap, const InstDesc* descs, InstState* states,
-                                     const uint32_t* iter_base, uint32_t r, uint32_t& inst, uint32_t& it,
-                                     uint32_t& first_heap, uint32_t& cover, bool& running) {
+// whole workgroup) when the launch map has no instance for it.
+__device__ inline bool step_prologue(const WgEntry* wgmap, const InstDesc* descs, const uint32_t* iter_base,
+                                     uint32_t r, uint32_t& inst, uint32_t& it, uint32_t& first_heap,
+                                     uint32_t& cover) {
     const WgEntry we = wgmap[blockIdx.x];
     if (we.inst == kNone) return false;
-    running = load_relaxed(&states[we.inst].status) == ST_RUNNING;
     inst = we.inst;
     first_heap = we.first_heap;
     cover = we.cover;
@@ -1179,7 +1279,7 @@ __device__ inline bool seed_instance(const InstDesc& d, InstState* st, int W) {
     z.pad2 = 0;
     z.occupied = Ops::kDense ? 0 : 1;
     z.expanded = z.generated = z.pushed = z.pops = z.probes = z.sift_down = z.sift_up = 0;
-    z.groups = z.inserted = z.lookups = 0;
+    z.groups = z.inserted = z.lookups = z.work_groups = 0;
     uint32_t slot, probes = 0;
     bool isnew;
     const typename Ops::Key start = Ops::start_key(d);
