@@ -34,7 +34,15 @@ namespace {
 // ---------------------------------------------------------------------------
 constexpr int kBlock = 256;
 
-template <int W>
+constexpr uint32_t kind_bit(uint32_t k) { return 1u << k; }
+constexpr uint32_t kAllKinds =
+    kind_bit(KIND_PUZZLE3) | kind_bit(KIND_PUZZLE4) | kind_bit(KIND_GRID4) | kind_bit(KIND_GRID8);
+
+// The step kernel for the kinds in KINDS.  The register count of a kernel is
+// the maximum over the kinds it can run, so a lane that holds one kind runs an
+// instantiation for that kind alone: the 15-puzzle and 4-connected grid paths
+// fit 64 VGPRs (8 waves per SIMD) where the kernel for all kinds needs 70 (7).
+template <int W, uint32_t KINDS>
 __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict__ wgmap,
                                                       const InstDesc* __restrict__ descs, InstState* states,
                                                       const uint32_t* __restrict__ iter_base, uint32_t r,
@@ -48,13 +56,29 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
     const bool active = q < d.n_heaps;
     const Tally tl{sh.tally, prm.profile != 0};
     if (running) {
-        switch (d.kind) {
-            case KIND_PUZZLE3: run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
-            case KIND_PUZZLE4: run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
-            case KIND_GRID4: run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
-            case KIND_GRID8: run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm, tl, sh.work); break;
-            default: break;
-        }
+        bool ran = false;
+        if constexpr ((KINDS & kind_bit(KIND_PUZZLE3)) != 0)
+            if (d.kind == KIND_PUZZLE3) {
+                run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                ran = true;
+            }
+        if constexpr ((KINDS & kind_bit(KIND_PUZZLE4)) != 0)
+            if (d.kind == KIND_PUZZLE4) {
+                run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                ran = true;
+            }
+        if constexpr ((KINDS & kind_bit(KIND_GRID4)) != 0)
+            if (d.kind == KIND_GRID4) {
+                run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                ran = true;
+            }
+        if constexpr ((KINDS & kind_bit(KIND_GRID8)) != 0)
+            if (d.kind == KIND_GRID8) {
+                run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                ran = true;
+            }
+        // a kind this instantiation cannot run: stop the instance rather than let it stall
+        if (!ran && threadIdx.x == 0) set_status(states + inst, ST_ABORTED);
     }
     tally_end(sh.tally, states + inst, it);
 }
@@ -325,6 +349,7 @@ struct Lane {
     WgEntry* d_map = nullptr;
     uint32_t map_cap = 0, n_wg = 0;
     uint64_t n_heaps = 0;     // heaps the map covers
+    uint32_t variant = 0;     // kind mask of the step-kernel instantiation the lane launches
     bool map_dirty = true;
     Pinned h_map[2];
     hipEvent_t ev_map[2] = {nullptr, nullptr};
@@ -621,16 +646,27 @@ struct ga_engine {
         const uint32_t heaps_per_wg = kBlock / W;
         std::vector<WgEntry> map;
         L.n_heaps = 0;
+        uint32_t kinds = 0;
         for (uint32_t s = lane; s < cfg.max_instances; s += n_lanes) {
             Job* j = resident[s];
             if (!j || j->finished) continue;
             const uint32_t n = std::min(j->cover, j->desc.n_heaps);
             L.n_heaps += n;
+            kinds |= kind_bit((uint32_t)j->kind);
             for (uint32_t h = 0; h < n; h += heaps_per_wg) map.push_back(WgEntry{s, h, n, 0});
         }
         L.n_wg = (uint32_t)map.size();
         L.map_dirty = false;
         if (!L.n_wg) return;
+        // a single-kind instantiation when the lane holds one kind that has one
+        const uint32_t variant =
+            (kinds == kind_bit(KIND_PUZZLE3) || kinds == kind_bit(KIND_PUZZLE4) || kinds == kind_bit(KIND_GRID4))
+                ? kinds
+                : kAllKinds;
+        if (variant != L.variant) {
+            drop_graph(L);  // the captured launches are of the other instantiation
+            L.variant = variant;
+        }
         if (L.n_wg > L.map_cap) {
             if (L.d_map) {
                 hip_check(hipStreamSynchronize(L.stream), "sync");
@@ -661,6 +697,25 @@ struct ga_engine {
         }
     }
 
+    template <int W>
+    void launch_step(const Lane& L, uint32_t grid, const StepParams& prm, uint32_t r) {
+        const dim3 g(grid), b(kBlock);
+        switch (L.variant) {
+            case kind_bit(KIND_PUZZLE3):
+                step_kernel<W, kind_bit(KIND_PUZZLE3)><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
+                break;
+            case kind_bit(KIND_PUZZLE4):
+                step_kernel<W, kind_bit(KIND_PUZZLE4)><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
+                break;
+            case kind_bit(KIND_GRID4):
+                step_kernel<W, kind_bit(KIND_GRID4)><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
+                break;
+            default:
+                step_kernel<W, kAllKinds><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
+                break;
+        }
+    }
+
     void enqueue_steps(Lane& L, uint32_t grid, const StepParams& prm, std::vector<hipEvent_t>* pe) {
         const uint32_t R = cfg.chunk_iters;
         hipStream_t stream = L.stream;
@@ -675,9 +730,9 @@ struct ga_engine {
                 }
             } else {
                 switch (cfg.heap_arity) {
-                    case 8: step_kernel<8><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
-                    case 16: step_kernel<16><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
-                    default: step_kernel<4><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
+                    case 8: launch_step<8>(L, grid, prm, r); break;
+                    case 16: launch_step<16>(L, grid, prm, r); break;
+                    default: launch_step<4>(L, grid, prm, r); break;
                 }
             }
             if (pe) hip_check(hipEventRecord((*pe)[2 * r + 1], stream), "event");

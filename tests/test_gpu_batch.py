@@ -212,3 +212,32 @@ def test_engine_variants_match_oracle(variant):
         assert sol.cost == oracle_cost(x)
         assert replay_ok(x, sol)
     assert alone[0].cost == sols[0].cost and alone[1].cost == sols[-1].cost
+
+
+@pytest.mark.parametrize("use_graph", [0, 1])
+def test_lane_kernel_follows_resident_kinds(use_graph):
+    """A lane runs a single-kind step kernel while it holds one kind and the
+    all-kinds kernel otherwise; arrivals in phases (8-puzzles, then 15-puzzles
+    and grids together, then grids alone) make the lanes switch back and forth,
+    with and without graph replay."""
+    st = wl.Stream(77)
+    phases = []
+    for i in range(10):
+        p8 = wl.puzzle8(wl.instance_seed(70, i))
+        p8.num_queues = 64
+        phases.append(p8)
+    for i in range(12):
+        phases.append(wl.puzzle15(wl.instance_seed(71, i), depth=st.uniform_int(30, 45), num_queues=256,
+                                  table_slots=0))
+        phases.append(wl.grid(st.uniform_int(96, 320), 0.2, wl.instance_seed(72, i), num_queues=256))
+    for i in range(8):
+        phases.append(wl.grid(st.uniform_int(96, 320), 0.2, wl.instance_seed(73, i), num_queues=256))
+    arrivals = [k * 3_000_000 for k in range(len(phases))]
+    with ga.Engine(0, lanes=2, deep_cells=1 << 14, use_graph=use_graph, chunk_iters=4,
+                   max_iterations=2_000_000) as e:
+        sols, ss = e.serve([x.request() for x in phases], arrivals)
+    assert ss.completed == len(phases) and ss.failed == 0
+    for x, sol in zip(phases, sols):
+        assert sol.status == ga.OK, ga.status_str(sol.status)
+        assert sol.cost == oracle_cost(x)
+        assert replay_ok(x, sol)
