@@ -173,6 +173,7 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
     ap.add_argument("--skip-p24", action="store_true", help="skip the 24-puzzle leg")
+    ap.add_argument("--p24-heaps", type=int, default=0, help="heaps per 24-puzzle (0 = the served puzzle count)")
     ap.add_argument("--override", default="", help="served tuning overrides, e.g. lanes=3,rounds=2")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (for scaling HBM byte counters)")
@@ -329,7 +330,7 @@ def main() -> int:
     # ---- 24-puzzle: 5x5 boards on the wide-key kernel, one batch, vs the CPU oracle ----
     p24 = None
     if not args.skip_p24:
-        p24 = puzzle24_leg(e, served_cfg.heaps("puzzle", 5), args.cpu_threads or min(16, os.cpu_count() or 1),
+        p24 = puzzle24_leg(e, args.p24_heaps or served_cfg.heaps("puzzle", 5), args.cpu_threads or min(16, os.cpu_count() or 1),
                            0.0 if args.skip_cpu else 8.0)
         log(f"[bench] 24-puzzle: {p24}")
 
