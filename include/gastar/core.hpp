@@ -226,6 +226,79 @@ __device__ inline uint32_t wave_sum_u32(uint32_t v) {
     return v;
 }
 
+// ---------------------------------------------------------------------------
+// exchanges inside a heap group of W lanes
+// ---------------------------------------------------------------------------
+// A group of 4 lanes is a DPP quad: quad permutes run on the VALU without the
+// LDS round trip (and the LDS-counter wait) of ds_bpermute, which __shfl uses.
+// Wider groups use __shfl.  Every lane of the group must be active.
+template <int CTRL>
+__device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ uint64_t quad_perm64(uint64_t v) {
+    return ((uint64_t)quad_perm<CTRL>((uint32_t)(v >> 32)) << 32) | quad_perm<CTRL>((uint32_t)v);
+}
+
+// value of lane (lane ^ O) of the group, O a power of two below W
+template <int W, int O>
+__device__ __forceinline__ uint32_t grp_xor(uint32_t v) {
+    if constexpr (W == 4) return quad_perm<O == 1 ? 0xB1 : 0x4E>(v);  // [1,0,3,2] / [2,3,0,1]
+    else return (uint32_t)__shfl_xor((int)v, O, W);
+}
+template <int W, int O>
+__device__ __forceinline__ uint64_t grp_xor64(uint64_t v) {
+    if constexpr (W == 4) return quad_perm64<O == 1 ? 0xB1 : 0x4E>(v);
+    else return __shfl_xor(v, O, W);
+}
+
+// value of lane l of the group
+template <int W>
+__device__ __forceinline__ uint32_t grp_bcast(uint32_t v, int l) {
+    if constexpr (W == 4) {
+        switch (l) {
+            case 0: return quad_perm<0x00>(v);
+            case 1: return quad_perm<0x55>(v);
+            case 2: return quad_perm<0xAA>(v);
+            default: return quad_perm<0xFF>(v);
+        }
+    } else {
+        return (uint32_t)__shfl((int)v, l, W);
+    }
+}
+template <int W>
+__device__ __forceinline__ uint64_t grp_bcast64(uint64_t v, int l) {
+    return ((uint64_t)grp_bcast<W>((uint32_t)(v >> 32), l) << 32) | grp_bcast<W>((uint32_t)v, l);
+}
+
+// minimum of (prio, index) over the group, ties to the smaller index
+template <int W, int O = W / 2>
+__device__ __forceinline__ void grp_min_idx(uint64_t& mp, uint32_t& mi) {
+    if constexpr (O > 0) {
+        const uint64_t op = grp_xor64<W, O>(mp);
+        const uint32_t oi = grp_xor<W, O>(mi);
+        if (op < mp || (op == mp && oi < mi)) {
+            mp = op;
+            mi = oi;
+        }
+        grp_min_idx<W, O / 2>(mp, mi);
+    }
+}
+template <int W, int O = W / 2>
+__device__ __forceinline__ uint32_t grp_min_u32(uint32_t v) {
+    if constexpr (O > 0) {
+        const uint32_t w = grp_xor<W, O>(v);
+        return grp_min_u32<W, O / 2>(w < v ? w : v);
+    }
+    return v;
+}
+template <int W, int O = W / 2>
+__device__ __forceinline__ uint32_t grp_sum_u32(uint32_t v) {
+    if constexpr (O > 0) return grp_sum_u32<W, O / 2>(v + grp_xor<W, O>(v));
+    return v;
+}
+
 __device__ inline uint32_t load_relaxed(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -362,15 +435,7 @@ __device__ inline bool heap_pop(gptr<HeapEntry> H, uint32_t& S, int lane, uint32
         if (W * j + 1 >= n) break;
         uint64_t mp = e.prio;
         uint32_t mi = idx;
-#pragma unroll
-        for (int o = W / 2; o > 0; o >>= 1) {
-            const uint64_t op = __shfl_xor(mp, o, W);
-            const uint32_t oi = __shfl_xor(mi, o, W);
-            if (op < mp || (op == mp && oi < mi)) {
-                mp = op;
-                mi = oi;
-            }
-        }
+        grp_min_idx<W>(mp, mi);
         if (mp >= last.prio) break;
         if (idx == mi) H[j] = e;
         j = mi;
@@ -865,12 +930,12 @@ __device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEn
 #pragma unroll
             for (int l = 0; l < W; ++l) {
                 if (r * W + l >= K) break;
-                const int vl = __shfl(v[r] ? 1 : 0, l, W);
+                const int vl = (int)grp_bcast<W>(v[r] ? 1u : 0u, l);
                 if (!vl) continue;
                 HeapEntry x;
-                x.prio = __shfl(rec[r].prio, l, W);
-                x.slot = __shfl(rec[r].slot, l, W);
-                x.aux = __shfl(rec[r].aux, l, W);
+                x.prio = grp_bcast64<W>(rec[r].prio, l);
+                x.slot = grp_bcast<W>(rec[r].slot, l);
+                x.aux = grp_bcast<W>(rec[r].aux, l);
                 if (lane == 0) {
                     uint32_t lv = 0;
                     heap_push<W>(H, S, x, lv);
@@ -879,7 +944,7 @@ __device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEn
                 }
             }
         }
-        S = __shfl(S, 0, W);
+        S = grp_bcast<W>(S, 0);
         if (lane == 0) tl.prof_add(TL_PUSHED, c);
         return;
     }
@@ -899,13 +964,13 @@ __device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEn
 #pragma unroll
         for (int l = 0; l < W; ++l) {
             if (r * W + l >= K) break;
-            const int bl = __shfl(bad[r] ? 1 : 0, l, W);
+            const int bl = (int)grp_bcast<W>(bad[r] ? 1u : 0u, l);
             if (!bl) continue;
             HeapEntry x;
-            x.prio = __shfl(rec[r].prio, l, W);
-            x.slot = __shfl(rec[r].slot, l, W);
-            x.aux = __shfl(rec[r].aux, l, W);
-            const uint32_t pos = S + (uint32_t)__shfl((int)rank[r], l, W);
+            x.prio = grp_bcast64<W>(rec[r].prio, l);
+            x.slot = grp_bcast<W>(rec[r].slot, l);
+            x.aux = grp_bcast<W>(rec[r].aux, l);
+            const uint32_t pos = S + grp_bcast<W>(rank[r], l);
             if (lane == 0) {  // sift up from its leaf position
                 uint32_t lv = 0;
                 heap_push<W>(H, pos, x, lv);
@@ -1034,12 +1099,10 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             ++n_rec;
         }
     }
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) {
-        rec_f = min(rec_f, (uint32_t)__shfl_xor((int)rec_f, o, W));
-        n_rec += (uint32_t)__shfl_xor((int)n_rec, o, W);
-    }
-    const uint32_t root_f = (live && S > 0) ? (uint32_t)__shfl((int)prio_f(root_prio), 0, W) : kInf;
+    rec_f = grp_min_u32<W>(rec_f);
+    n_rec = grp_sum_u32<W>(n_rec);
+    const uint32_t root_f0 = grp_bcast<W>(prio_f(root_prio), 0);
+    const uint32_t root_f = (live && S > 0) ? root_f0 : kInf;
     const bool work = live && (n_rec > 0 || root_f <= gate);  // root_f = kInf when empty
     {
         const uint32_t top_f = wave_min_u32(lane == 0 ? min(root_f, rec_f) : kInf);
