@@ -34,6 +34,7 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+MAX_ITERATIONS = 1_000_000    # per request on the GPU
 MAX_ORACLE_STATES = 12_000_000  # per CPU request; larger puzzles are skipped (and counted)
 METRIC = "optimal solves/sec + p50/p99 latency: served batch vs one-at-a-time GA*"
 HBM_PEAK_GBS = 8000.0
@@ -209,8 +210,11 @@ def main() -> int:
     for kv in filter(None, args.override.split(",")):
         k, v = kv.split("=")
         setattr(served_cfg, k, int(v))
+    # The iteration limit only bounds a pathological request (it would count as failed); the longest
+    # searches of this stream take about 6,000 iterations.
     e = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30,
-                  profile=0 if args.no_profile else max(1, args.time_every), **served_cfg.engine)
+                  profile=0 if args.no_profile else max(1, args.time_every), max_iterations=MAX_ITERATIONS,
+                  **served_cfg.engine)
     dev = {}
     t = time.perf_counter()
     for i, x in enumerate(insts):
@@ -341,7 +345,8 @@ def main() -> int:
         step = max(1, len(timed_idx) // args.plain_sample)
         sample = timed_idx[::step][: args.plain_sample]
         plain_cfg = tuning.load(mode="plain")
-        e1 = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30, **plain_cfg.engine)
+        e1 = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30,
+                       max_iterations=MAX_ITERATIONS, **plain_cfg.engine)
         t = time.perf_counter()
         for i in sample:
             x = insts[i]
