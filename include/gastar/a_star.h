@@ -67,7 +67,7 @@ struct UserOps : KeySetFor<T> {
         return heur(Set::unpack(key));
     }
     __device__ static inline uint32_t aux_of(const InstDesc&, const Key&) { return 0; }
-    __device__ static inline uint32_t prefetch(const InstDesc&, uint32_t, int) { return 0; }
+    __device__ static inline uint32_t prefetch(const InstDesc&, const HeapEntry&, int) { return 0; }
     __device__ static inline bool succ(const InstDesc&, const Key& key, uint32_t, uint32_t, int i, uint32_t, Key& t,
                                        uint32_t& taux, uint32_t& cost, uint32_t& th) {
         const T src = Set::unpack(key);

@@ -101,6 +101,8 @@ __host__ __device__ inline uint64_t make_prio(uint32_t f, uint32_t g) {
 __host__ __device__ inline uint32_t prio_f(uint64_t p) { return (uint32_t)(p >> 32); }
 __host__ __device__ inline uint32_t prio_g(uint64_t p) { return ~(uint32_t)p; }
 __host__ __device__ inline uint32_t val_g(uint64_t v) { return (uint32_t)(v >> 32); }
+// g of a closed-set value in memory: its high word, read on its own
+__device__ inline uint32_t g_of(gptr<uint64_t> v) { return ((gptr<uint32_t>)v)[1]; }
 __host__ __device__ inline uint32_t val_parent(uint64_t v) { return (uint32_t)v; }
 
 // 64-bit finaliser of MurmurHash3 (public domain, Austin Appleby).  The
@@ -404,13 +406,13 @@ __device__ inline void tally_end(uint32_t* c, InstState* st, uint32_t it) {
 // W*j+1 .. W*j+W of node j start on a W-entry boundary (W*16 B aligned).
 // ---------------------------------------------------------------------------
 // Removes the minimum.  The root, the last entry and the root's W children
-// are read in one round trip; the popped state's closed-set entry (key and
-// g|parent, via `load`) is then read while the sift-down runs.  If the root's
+// are read in one round trip; the popped state's key and stored g (via
+// `load`) are then read while the sift-down runs.  If the root's
 // f exceeds gate_f the heap is left as it is and false is returned (`top`
 // still holds the root).
 template <int W, class Load, class Pre, class Key>
 __device__ inline bool heap_pop(gptr<HeapEntry> H, uint32_t& S, int lane, uint32_t& levels, Load load, Pre pre,
-                                Key& key, uint64_t& val, uint32_t gate_f, HeapEntry& top) {
+                                Key& key, uint32_t& g, uint32_t gate_f, HeapEntry& top) {
     top = H[0];
     const uint32_t n = S - 1;
     HeapEntry last;
@@ -426,16 +428,19 @@ __device__ inline bool heap_pop(gptr<HeapEntry> H, uint32_t& S, int lane, uint32
     if (idx < n) e = H[idx];
     if (prio_f(top.prio) > gate_f) return false;
     S = n;
-    load(top.slot, key, val);
-    pre(top);  // the problem's early reads for expanding top
+    load(top, key, g);
+    // The first level's choice uses loads that arrived with the root; it is
+    // made before pre() issues its memory operations, so that no wait in the
+    // sift-down below covers them (waits on loads issued later still do).
+    uint64_t mp = e.prio;
+    uint32_t mi = idx;
+    grp_min_idx<W>(mp, mi);
+    pre(top);  // the problem's early work for expanding top
     if (n == 0) return true;
     uint32_t j = 0;
     for (;;) {
-        // e holds child W*j + 1 + lane of node j (or +inf past the end)
+        // e holds child W*j + 1 + lane of node j (or +inf past the end); mp/mi its group minimum
         if (W * j + 1 >= n) break;
-        uint64_t mp = e.prio;
-        uint32_t mi = idx;
-        grp_min_idx<W>(mp, mi);
         if (mp >= last.prio) break;
         if (idx == mi) H[j] = e;
         j = mi;
@@ -443,6 +448,9 @@ __device__ inline bool heap_pop(gptr<HeapEntry> H, uint32_t& S, int lane, uint32
         idx = W * j + 1 + (uint32_t)lane;
         e.prio = ~0ull;
         if (idx < n) e = H[idx];
+        mp = e.prio;
+        mi = idx;
+        grp_min_idx<W>(mp, mi);
     }
     if (lane == 0) H[j] = last;
     return true;
@@ -517,13 +525,14 @@ __device__ inline bool hash_find(gptr<const Slot> table, uint64_t mask, uint64_t
 struct HashedU64Set {
     using Key = uint64_t;
     static constexpr bool kDense = false;
-    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
-        const Slot s = as_global(d.table)[slot];
-        key = s.key;
-        val = s.val;
+    // The key and the stored g (the high word of val; the parent half is not
+    // needed at a pop, and a load of it would only hold a register).
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g) {
+        key = as_global(d.table)[slot].key;
+        g = g_of(&as_global(d.table)[slot].val);
         // A key claimed by a CAS earlier in this launch may still read as empty
         // through a stale L1 line; keys are write-once, so re-read it at agent
-        // scope.  (A stale val only makes the staleness test conservative.)
+        // scope.  (A stale g only makes the staleness test conservative.)
         if (key == kEmptyKey)
             key = __hip_atomic_load(&as_global(d.table)[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -546,9 +555,9 @@ struct HashedU64Set {
 struct DenseCellSet {
     using Key = uint64_t;
     static constexpr bool kDense = true;
-    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g) {
         key = slot;
-        val = as_global(d.dense)[slot];
+        g = g_of(as_global(d.dense) + slot);
     }
     __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
         return as_global(d.dense) + slot;
@@ -627,9 +636,9 @@ struct WideKeySet {
             if (a.w[i] != b.w[i]) return false;
         return true;
     }
-    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint64_t& val) {
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g) {
         key = load_key(d, slot);
-        val = as_global(d.table)[slot].val;
+        g = g_of(&as_global(d.table)[slot].val);
     }
     __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
         return &as_global(d.table)[slot].val;
@@ -740,7 +749,7 @@ struct PuzzleOps : HashedU64Set {
     __device__ static inline uint32_t aux_of(const InstDesc&, uint64_t) { return 0; }
 
     // successors are computed from the state alone: nothing to read early
-    __device__ static inline uint32_t prefetch(const InstDesc&, uint32_t, int) { return 0; }
+    __device__ static inline uint32_t prefetch(const InstDesc&, const HeapEntry&, int) { return 0; }
 
     // h of the successor from h of the parent: only the moved tile changes.
     __device__ static inline bool succ(const InstDesc& d, uint64_t s, uint32_t, uint32_t h, int i, uint32_t,
@@ -787,19 +796,20 @@ struct GridOps : DenseCellSet {
         return hxy(d, aux & 0xFFFFu, aux >> 16);
     }
 
-    // Weight of the cell that move i enters (0: wall or off the grid).  Read
-    // as soon as the node is popped, while its heap is still being sifted.
-    __device__ static inline uint32_t prefetch(const InstDesc& d, uint32_t aux, int i) {
+    // Weight of the cell that move i enters (0: wall or off the grid), read as
+    // soon as the node is popped, while its heap is still being sifted.
+    __device__ static inline uint32_t prefetch(const InstDesc& d, const HeapEntry& e, int i) {
         uint32_t nx, ny;
-        if (!neighbour(d, aux, i, nx, ny)) return 0;
+        if (!neighbour(d, e.aux, i, nx, ny)) return 0;
         return as_global(d.cells)[(uint64_t)ny * d.width + nx];
     }
 
     __device__ static inline bool neighbour(const InstDesc& d, uint32_t aux, int i, uint32_t& nx, uint32_t& ny) {
-        const int DX[8] = {1, -1, 0, 0, 1, 1, -1, -1};
-        const int DY[8] = {0, 0, 1, -1, 1, -1, 1, -1};
-        nx = (aux & 0xFFFFu) + (uint32_t)DX[i];
-        ny = (aux >> 16) + (uint32_t)DY[i];
+        // move i: dx = {1,-1,0,0,1,1,-1,-1}[i], dy = {0,0,1,-1,1,-1,1,-1}[i],
+        // each stored +1 in nibble i of an immediate (no table in memory)
+        const uint32_t dx = (0x00221102u >> (4 * i)) & 15u, dy = (0x02020211u >> (4 * i)) & 15u;
+        nx = (aux & 0xFFFFu) + dx - 1u;
+        ny = (aux >> 16) + dy - 1u;
         return nx < d.width && ny < d.height;  // also catches -1 wrap-around
     }
 
@@ -867,7 +877,7 @@ struct Puzzle5Ops : WideKeySet<Board5> {
             if (get(b, c) == 0) return (uint32_t)c;
         return 0;
     }
-    __device__ static inline uint32_t prefetch(const InstDesc&, uint32_t, int) { return 0; }
+    __device__ static inline uint32_t prefetch(const InstDesc&, const HeapEntry&, int) { return 0; }
     // h of the successor from h of the parent: only the moved tile changes.
     __device__ static inline bool succ(const InstDesc& d, const Key& k, uint32_t blank, uint32_t h, int i, uint32_t,
                                        Key& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
@@ -1172,13 +1182,13 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     group_fence();
 
     // -- 2. first pop of this launch, unless the root lies above the f gate
-    auto load = [&](uint32_t slot, typename Ops::Key& k, uint64_t& v) { Ops::load(d, slot, k, v); };
+    auto load = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& g) { Ops::load(d, t.slot, k, g); };
     uint32_t pre[KW];  // per successor: what the problem reads early (grid: the entered cell's weight)
     auto prefetch = [&](const HeapEntry& t) {
 #pragma unroll
         for (int r = 0; r < KW; ++r) {
             const int i = r * W + lane;
-            pre[r] = i < K ? Ops::prefetch(d, t.aux, i) : 0u;
+            pre[r] = i < K ? Ops::prefetch(d, t, i) : 0u;
         }
     };
     HeapEntry top;
@@ -1186,11 +1196,11 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     top.slot = 0;
     top.aux = 0;
     typename Ops::Key top_key{};
-    uint64_t top_val = 0;
+    uint32_t top_g = 0;  // g stored in the closed set for the popped node
     bool popped = false, gated = false;
     if (mine && S > 0) {
         uint32_t lv = 0;
-        popped = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_val, gate, top);
+        popped = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_g, gate, top);
         gated = !popped;
         group_fence();
         if (lane == 0) {
@@ -1209,7 +1219,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 if (!popped) {
                     if (S == 0) break;
                     uint32_t lv = 0;
-                    const bool ok = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_val, gate, top);
+                    const bool ok = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_g, gate, top);
                     if (lane == 0) tl.prof_add(TL_SIFT_DOWN, lv);
                     if (!ok) {
                         gated = true;
@@ -1219,7 +1229,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                     if (lane == 0) tl.prof_add(TL_POPS, 1u);
                 }
                 popped = false;
-                if (prio_g(top.prio) > val_g(top_val)) continue;  // stale
+                if (prio_g(top.prio) > top_g) continue;  // stale
                 have = true;
                 break;
             }
@@ -1233,15 +1243,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             }
             if (best != kInf && f >= best) continue;
             tl.count(TL_EXPANDED, lane == 0);
-            // expand, insert into the closed set
+            // expansion: successors, closed-set insertion (a successor survives
+            // only if its atomicMin of (g << 32 | parent) lowered g)
             const uint32_t h = f - g0;
 #pragma unroll
             for (int r = 0; r < KW; ++r) {
                 const int i = r * W + lane;
                 rec[r].prio = kNoRecord;
-                typename Ops::Key tkey;
+                typename Ops::Key skey;
                 uint32_t taux, cost, th;
-                const bool gen = i < K && Ops::succ(d, top_key, top.aux, h, i, pre[r], tkey, taux, cost, th);
+                const bool gen = i < K && Ops::succ(d, top_key, top.aux, h, i, pre[r], skey, taux, cost, th);
                 tl.count(TL_GENERATED, gen);
                 if (gen) {
                     const uint32_t g2 = g0 + cost, f2 = g2 + th;
@@ -1249,7 +1260,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                         uint32_t slot;
                         bool isnew;
                         uint32_t np = 0;
-                        const bool found = Ops::find_or_insert(d, tkey, prm.max_probes, slot, isnew, np);
+                        const bool found = Ops::find_or_insert(d, skey, prm.max_probes, slot, isnew, np);
                         bool ins = false;
                         if (found) {
                             const uint64_t mine_val = ((uint64_t)g2 << 32) | top.slot;
@@ -1373,10 +1384,10 @@ __device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, 
     while (slot != kNone) {
         if (n >= cap) return kNone;
         typename Ops::Key key;
-        uint64_t val;
-        Ops::load(d, slot, key, val);
+        uint32_t g;
+        Ops::load(d, slot, key, g);
         out[n++] = key;
-        slot = val_parent(val);
+        slot = val_parent(*Ops::val_ptr(d, slot));
     }
     return n;
 }
