@@ -146,6 +146,7 @@ struct InstDesc {
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
 };
+static_assert(offsetof(InstDesc, goal_rc) % 4 == 0, "goal_rc is read as 32-bit words");
 
 // Mutable per-instance state.
 struct InstState {
@@ -456,12 +457,25 @@ __device__ inline bool heap_pop(gptr<HeapEntry> H, uint32_t& S, int lane, uint32
     return true;
 }
 
+// One 16-byte load of a heap entry.  Read field by field, the compiler may
+// load the priority alone, compare, and fetch slot and aux only on the branch
+// that moves the entry: a second round trip per level.
+__device__ __forceinline__ HeapEntry load_entry(gptr<HeapEntry> p) {
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    const u32x4 v = *reinterpret_cast<gptr<const u32x4>>(p);
+    HeapEntry e;
+    e.prio = ((uint64_t)v.y << 32) | v.x;
+    e.slot = v.z;
+    e.aux = v.w;
+    return e;
+}
+
 template <int W>
 __device__ inline void heap_push(gptr<HeapEntry> H, uint32_t S, const HeapEntry& x, uint32_t& levels) {
     uint32_t j = S;
     while (j > 0) {
         const uint32_t p = (j - 1) / W;
-        const HeapEntry pe = H[p];
+        const HeapEntry pe = load_entry(H + p);
         if (x.prio < pe.prio) {
             H[j] = pe;
             j = p;
@@ -528,13 +542,12 @@ struct HashedU64Set {
     // The key and the stored g (the high word of val; the parent half is not
     // needed at a pop, and a load of it would only hold a register).
     __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g) {
-        key = as_global(d.table)[slot].key;
+        // The key may have been claimed by a CAS earlier in this launch, so it
+        // is read at agent scope (a plain load could see a stale empty L1
+        // line, and a re-read on that case would be a branch that waits for
+        // the load).  A stale g only makes the staleness test conservative.
+        key = __hip_atomic_load(&as_global(d.table)[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         g = g_of(&as_global(d.table)[slot].val);
-        // A key claimed by a CAS earlier in this launch may still read as empty
-        // through a stale L1 line; keys are write-once, so re-read it at agent
-        // scope.  (A stale g only makes the staleness test conservative.)
-        if (key == kEmptyKey)
-            key = __hip_atomic_load(&as_global(d.table)[slot].key, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
         return &as_global(d.table)[slot].val;
@@ -550,6 +563,25 @@ struct HashedU64Set {
     __device__ static inline Key start_key(const InstDesc& d) { return d.start; }
     __device__ static inline Key goal_key(const InstDesc& d) { return d.goal; }
 };
+
+// Goal position (row << 4 | col) of puzzle tile t, t < TILES.  The table is
+// read as whole words at uniform addresses (scalar loads, once per wave) and
+// the byte picked in registers: a byte load indexed by the lane's tile would
+// put a memory round trip inside every expansion.
+template <int TILES>
+__device__ inline uint32_t goal_of(const InstDesc& d, uint32_t t) {
+    static_assert(TILES <= (int)sizeof(d.goal_rc), "goal table too small");
+    // (readfirstlane makes each word an opaque uniform value, so that the
+    // selection below is not turned back into a lane-indexed load)
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(d.goal_rc);
+    uint32_t word = __builtin_amdgcn_readfirstlane(w[0]);
+#pragma unroll
+    for (int k = 1; k < (TILES + 3) / 4; ++k) {
+        const uint32_t wk = __builtin_amdgcn_readfirstlane(w[k]);
+        word = (t >> 2) == (uint32_t)k ? wk : word;
+    }
+    return (word >> (8 * (t & 3))) & 255u;
+}
 
 // Direct-indexed set over grid cells: slot = cell index, val in d.dense.
 struct DenseCellSet {
@@ -731,7 +763,8 @@ struct PuzzleOps : HashedU64Set {
     }
 
     __device__ static inline uint32_t md(const InstDesc& d, uint32_t tile, int cell) {
-        const int gr = d.goal_rc[tile] >> 4, gc = d.goal_rc[tile] & 15;
+        const uint32_t g = goal_of<kCells>(d, tile);
+        const int gr = (int)(g >> 4), gc = (int)(g & 15u);
         const int r = cell / S, c = cell % S;
         return (uint32_t)(abs(r - gr) + abs(c - gc));
     }
@@ -858,7 +891,8 @@ struct Puzzle5Ops : WideKeySet<Board5> {
     __device__ static inline Board5 board(const Key& k) { return from_raw<Board5>(k); }
 
     __device__ static inline uint32_t md(const InstDesc& d, uint32_t tile, int cell) {
-        const int gr = d.goal_rc[tile] >> 4, gc = d.goal_rc[tile] & 15;
+        const uint32_t g = goal_of<25>(d, tile);
+        const int gr = (int)(g >> 4), gc = (int)(g & 15u);
         const int r = cell / 5, c = cell % 5;
         return (uint32_t)((r > gr ? r - gr : gr - r) + (c > gc ? c - gc : gc - c));
     }
