@@ -50,34 +50,36 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
     __shared__ StepShared sh;
     uint32_t inst, it, first, cover;
     if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
-    const bool running = step_begin(sh, states + inst);  // workgroup-uniform
+    const uint32_t status = load_relaxed(&states[inst].status);
+    step_begin(sh);
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / W;
     const bool active = q < d.n_heaps;
     const Tally tl{sh.tally, prm.profile != 0};
-    if (running) {
+    {
         bool ran = false;
         if constexpr ((KINDS & kind_bit(KIND_PUZZLE3)) != 0)
             if (d.kind == KIND_PUZZLE3) {
-                run_group<W, PuzzleOps<3>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, PuzzleOps<3>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_PUZZLE4)) != 0)
             if (d.kind == KIND_PUZZLE4) {
-                run_group<W, PuzzleOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, PuzzleOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID4)) != 0)
             if (d.kind == KIND_GRID4) {
-                run_group<W, GridOps<4>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, GridOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID8)) != 0)
             if (d.kind == KIND_GRID8) {
-                run_group<W, GridOps<8>>(d, states + inst, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, GridOps<8>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
                 ran = true;
             }
-        // a kind this instantiation cannot run: stop the instance rather than let it stall
+        // a kind this instantiation cannot run: stop the instance rather than let
+        // it stall (set_status changes only a running instance)
         if (!ran && threadIdx.x == 0) set_status(states + inst, ST_ABORTED);
     }
     tally_end(sh.tally, states + inst, it);
@@ -94,12 +96,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel_wide(const WgEntry* __rest
     __shared__ StepShared sh;
     uint32_t inst, it, first, cover;
     if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
-    const bool running = step_begin(sh, states + inst);  // workgroup-uniform
+    const uint32_t status = load_relaxed(&states[inst].status);
+    step_begin(sh);
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / W;
     const Tally tl{sh.tally, prm.profile != 0};
-    if (running && d.kind == KIND_PUZZLE5)
-        run_group<W, Puzzle5Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, tl, sh.work);
+    if (d.kind == KIND_PUZZLE5)
+        run_group<W, Puzzle5Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm, tl, sh.work);
     tally_end(sh.tally, states + inst, it);
 }
 

@@ -95,12 +95,12 @@ __global__ __launch_bounds__(kBlock) void user_step_kernel(const WgEntry* __rest
     __shared__ StepShared sh;
     uint32_t inst, it, first, cover;
     if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
-    const bool running = step_begin(sh, states + inst);  // workgroup-uniform
+    const uint32_t status = load_relaxed(&states[inst].status);
+    step_begin(sh);
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / kW;
-    if (running)
-        run_group<kW, Ops>(d, states + inst, it, q, q < d.n_heaps, cover, prm, Tally{sh.tally, prm.profile != 0},
-                           sh.work);
+    run_group<kW, Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm,
+                       Tally{sh.tally, prm.profile != 0}, sh.work);
     tally_end(sh.tally, states + inst, it);
 }
 

@@ -360,21 +360,18 @@ struct WorkList {
 // LDS of one step-kernel workgroup.
 struct StepShared {
     uint32_t tally[kTallies];
-    uint32_t running;  // the instance's status is ST_RUNNING, read once for the whole workgroup
     WorkList work;
 };
 
-// Clears the tallies and the work list and reads the instance status once, so
-// that every wave of the workgroup takes the same path (run_group holds a
-// barrier).  Every thread of the workgroup calls it; returns "running".
-__device__ inline bool step_begin(StepShared& sh, const InstState* st) {
+// Clears the tallies and the work list; every thread of the workgroup calls
+// it.  The kernels read the instance's status just before (each wave its own
+// load, issued as soon as the instance is known, so that it travels with the
+// instance descriptor's loads) and pass it to run_group, which every wave
+// calls whatever the status (it holds a barrier).
+__device__ inline void step_begin(StepShared& sh) {
     if (threadIdx.x < kTallies) sh.tally[threadIdx.x] = threadIdx.x == TL_MIN_TOP ? kInf : 0u;
-    if (threadIdx.x == 0) {
-        sh.work.n = 0;
-        sh.running = load_relaxed(&st->status) == ST_RUNNING ? 1u : 0u;
-    }
+    if (threadIdx.x == 0) sh.work.n = 0;
     __syncthreads();
-    return sh.running != 0;
 }
 
 // Adds the tallies to the instance's state for iteration `it`; every thread
@@ -1052,8 +1049,9 @@ __device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEn
 //
 // Every thread of the workgroup calls run_group (it holds a barrier).
 template <int W, class Ops>
-__device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t it, uint32_t q, bool active,
-                                          uint32_t cover, const StepParams& prm, const Tally& tl, WorkList& wl) {
+__device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t status, uint32_t it, uint32_t q,
+                                          bool active, uint32_t cover, const StepParams& prm, const Tally& tl,
+                                          WorkList& wl) {
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
@@ -1068,14 +1066,20 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     const uint32_t A = fixed ? N : __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[it % 3]));
     const uint32_t Aw =
         (fixed || it == 0) ? A : __builtin_amdgcn_readfirstlane(load_relaxed(&st->active[(it - 1) % 3]));
-    active = active && q < A;
-    const gptr<const HeapEntry> cin = as_global<const HeapEntry>(d.cand) + (size_t)((it - 1) & 1) * N * K;
 
     // ---------------- phase 1 ----------------
-    uint32_t best = __builtin_amdgcn_readfirstlane(load_relaxed(&st->best));
-    const uint32_t mt = it > 0 ? __builtin_amdgcn_readfirstlane(load_relaxed(&st->min_top[(it - 1) % 3])) : 0u;
+    // The loads of phase 1 travel together: best and the previous min f, then
+    // (for a running instance: the status was loaded before the descriptor
+    // arrived) the heap's size and root and the records addressed to it.
+    // Waves may see different statuses (another workgroup may end the
+    // instance meanwhile); each only decides whether its own heaps take part.
+    const uint32_t best_ld = load_relaxed(&st->best);
+    const uint32_t mt_ld = load_relaxed(&st->min_top[(it + 2) % 3]);  // (it - 1) mod 3; unused at it = 0
     uint64_t occupied = 0;
     if constexpr (!Ops::kDense) occupied = load_relaxed(&st->occupied);
+    const bool running = __builtin_amdgcn_readfirstlane(status) == ST_RUNNING;
+    active = active && running && q < A;
+    const gptr<const HeapEntry> cin = as_global<const HeapEntry>(d.cand) + (size_t)((it - 1) & 1) * N * K;
     uint32_t S = 0;
     uint64_t root_prio = ~0ull;
     HeapEntry rec[KW];
@@ -1099,6 +1103,9 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             }
         }
     }
+
+    uint32_t best = __builtin_amdgcn_readfirstlane(best_ld);
+    const uint32_t mt = it > 0 ? __builtin_amdgcn_readfirstlane(mt_ld) : 0u;
 
     // -- termination test on the previous iteration's reduction (a_star.h:260)
     bool live = active;
@@ -1349,6 +1356,16 @@ __device__ inline bool step_prologue(const WgEntry* wgmap, const InstDesc* descs
     first_heap = we.first_heap;
     cover = we.cover;
     it = *iter_base + r - descs[we.inst].start_iter;
+    // The descriptor's first two 64-byte lines hold every field that phase 1
+    // of the step addresses its loads with; start_iter (above) and kind are
+    // read here, with the instance's status, so that both lines are in the
+    // scalar cache before the workgroup barrier that the reads after it wait
+    // behind.  (The empty statement keeps the kind load from being dropped.)
+    static_assert(offsetof(InstDesc, start_iter) / 64 == 1 && offsetof(InstDesc, a0) / 64 == 1 &&
+                      offsetof(InstDesc, heap_size) / 64 == 0,
+                  "descriptor layout: phase-1 fields in lines 0 and 1");
+    const uint32_t kind = descs[we.inst].kind;
+    asm volatile("" ::"s"(kind));
     return true;
 }
 
