@@ -32,7 +32,7 @@ namespace {
 // ---------------------------------------------------------------------------
 // kernels instantiated for the built-in problems
 // ---------------------------------------------------------------------------
-constexpr int kBlock = 256;
+constexpr int kBlock = kStepBlock;
 
 constexpr uint32_t kind_bit(uint32_t k) { return 1u << k; }
 constexpr uint32_t kAllKinds =
@@ -60,22 +60,26 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
         bool ran = false;
         if constexpr ((KINDS & kind_bit(KIND_PUZZLE3)) != 0)
             if (d.kind == KIND_PUZZLE3) {
-                run_group<W, PuzzleOps<3>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, PuzzleOps<3>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
+                                           sh.heap_top);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_PUZZLE4)) != 0)
             if (d.kind == KIND_PUZZLE4) {
-                run_group<W, PuzzleOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, PuzzleOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
+                                           sh.heap_top);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID4)) != 0)
             if (d.kind == KIND_GRID4) {
-                run_group<W, GridOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, GridOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
+                                         sh.heap_top);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID8)) != 0)
             if (d.kind == KIND_GRID8) {
-                run_group<W, GridOps<8>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work);
+                run_group<W, GridOps<8>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
+                                         sh.heap_top);
                 ran = true;
             }
         // a kind this instantiation cannot run: stop the instance rather than let
@@ -102,7 +106,8 @@ __global__ __launch_bounds__(kBlock) void step_kernel_wide(const WgEntry* __rest
     const uint32_t q = first + threadIdx.x / W;
     const Tally tl{sh.tally, prm.profile != 0};
     if (d.kind == KIND_PUZZLE5)
-        run_group<W, Puzzle5Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm, tl, sh.work);
+        run_group<W, Puzzle5Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm, tl, sh.work,
+                                 sh.heap_top);
     tally_end(sh.tally, states + inst, it);
 }
 

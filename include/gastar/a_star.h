@@ -85,7 +85,7 @@ struct UserOps : KeySetFor<T> {
 };
 
 constexpr int kW = 4;  // lanes per heap group
-constexpr int kBlock = 256;
+constexpr int kBlock = kStepBlock;
 
 template <class Ops>
 __global__ __launch_bounds__(kBlock) void user_step_kernel(const WgEntry* __restrict__ wgmap,
@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void user_step_kernel(const WgEntry* __rest
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / kW;
     run_group<kW, Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm,
-                       Tally{sh.tally, prm.profile != 0}, sh.work);
+                       Tally{sh.tally, prm.profile != 0}, sh.work, sh.heap_top);
     tally_end(sh.tally, states + inst, it);
 }
 

@@ -49,13 +49,21 @@ constexpr uint64_t kNoRecord = ~0ull;     // empty candidate record
 #if defined(__HIP_DEVICE_COMPILE__)
 template <class T>
 using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+using lptr = __attribute__((address_space(3))) T*;  // LDS
 #else
 template <class T>
 using gptr = T*;
+template <class T>
+using lptr = T*;
 #endif
 template <class T>
 __device__ __forceinline__ gptr<T> as_global(T* p) {
     return (gptr<T>)p;
+}
+template <class T>
+__device__ __forceinline__ lptr<T> as_local(T* p) {
+    return (lptr<T>)p;
 }
 
 // Instance status.
@@ -357,8 +365,16 @@ struct WorkList {
     uint32_t S[64];  // its size before the pushes
 };
 
+// Threads per step-kernel workgroup (the work list holds one entry per heap
+// group, so W >= 4).
+constexpr int kStepBlock = 256;
+// LDS heap tops of a workgroup: (kStepBlock / W) groups of 1 + 4W entries,
+// most for W = 4.
+constexpr uint32_t kHeapTopEntries = (kStepBlock / 4) * (1 + 4 * 4);
+
 // LDS of one step-kernel workgroup.
 struct StepShared {
+    HeapEntry heap_top[kHeapTopEntries];
     uint32_t tally[kTallies];
     WorkList work;
 };
@@ -403,57 +419,6 @@ __device__ inline void tally_end(uint32_t* c, InstState* st, uint32_t it) {
 // W-ary heap, logical index j at physical j + W - 1 so that the W children
 // W*j+1 .. W*j+W of node j start on a W-entry boundary (W*16 B aligned).
 // ---------------------------------------------------------------------------
-// Removes the minimum.  The root, the last entry and the root's W children
-// are read in one round trip; the popped state's key and stored g (via
-// `load`) are then read while the sift-down runs.  If the root's
-// f exceeds gate_f the heap is left as it is and false is returned (`top`
-// still holds the root).
-template <int W, class Load, class Pre, class Key>
-__device__ inline bool heap_pop(gptr<HeapEntry> H, uint32_t& S, int lane, uint32_t& levels, Load load, Pre pre,
-                                Key& key, uint32_t& g, uint32_t gate_f, HeapEntry& top) {
-    top = H[0];
-    const uint32_t n = S - 1;
-    HeapEntry last;
-    last.prio = ~0ull;
-    last.slot = 0;
-    last.aux = 0;
-    if (n > 0) last = H[n];
-    uint32_t idx = 1 + (uint32_t)lane;
-    HeapEntry e;
-    e.prio = ~0ull;
-    e.slot = 0;
-    e.aux = 0;
-    if (idx < n) e = H[idx];
-    if (prio_f(top.prio) > gate_f) return false;
-    S = n;
-    load(top, key, g);
-    // The first level's choice uses loads that arrived with the root; it is
-    // made before pre() issues its memory operations, so that no wait in the
-    // sift-down below covers them (waits on loads issued later still do).
-    uint64_t mp = e.prio;
-    uint32_t mi = idx;
-    grp_min_idx<W>(mp, mi);
-    pre(top);  // the problem's early work for expanding top
-    if (n == 0) return true;
-    uint32_t j = 0;
-    for (;;) {
-        // e holds child W*j + 1 + lane of node j (or +inf past the end); mp/mi its group minimum
-        if (W * j + 1 >= n) break;
-        if (mp >= last.prio) break;
-        if (idx == mi) H[j] = e;
-        j = mi;
-        ++levels;
-        idx = W * j + 1 + (uint32_t)lane;
-        e.prio = ~0ull;
-        if (idx < n) e = H[idx];
-        mp = e.prio;
-        mi = idx;
-        grp_min_idx<W>(mp, mi);
-    }
-    if (lane == 0) H[j] = last;
-    return true;
-}
-
 // One 16-byte load of a heap entry.  Read field by field, the compiler may
 // load the priority alone, compare, and fetch slot and aux only on the branch
 // that moves the entry: a second round trip per level.
@@ -467,21 +432,123 @@ __device__ __forceinline__ HeapEntry load_entry(gptr<HeapEntry> p) {
     return e;
 }
 
+// A heap group's heap during one launch.  Logical entries [0, kTop) -- the
+// root, its W children and the children of nodes 1..3 -- are held in the
+// group's LDS slice, copied in from global memory before the launch works on
+// the heap and written back after; deeper entries stay in global memory.  The
+// heaps of a search are shallow (tens of entries), so most sift steps and
+// parent checks are LDS accesses instead of memory round trips.  A node's W
+// children lie on one side of the boundary (kTop = 1 + 4W), so the lanes that
+// read one level take the same path.
 template <int W>
-__device__ inline void heap_push(gptr<HeapEntry> H, uint32_t S, const HeapEntry& x, uint32_t& levels) {
+struct HeapRef {
+    static constexpr uint32_t kTop = 1 + 4 * W;
+    static constexpr int kPer = (int)((kTop + W - 1) / W);  // LDS entries per lane in fill/write_back
+    gptr<HeapEntry> g;  // logical entry 0 in global memory
+    lptr<HeapEntry> t;  // logical entry 0 in LDS
+
+    __device__ __forceinline__ HeapEntry get(uint32_t j) const {
+        if (j < kTop) return t[j];
+        return load_entry(g + j);
+    }
+    __device__ __forceinline__ uint64_t prio(uint32_t j) const { return j < kTop ? t[j].prio : g[j].prio; }
+    __device__ __forceinline__ void put(uint32_t j, const HeapEntry& e) const {
+        if (j < kTop)
+            t[j] = e;
+        else
+            g[j] = e;
+    }
+    // Copies entries [0, min(S, kTop)) into LDS (lane l: l, l+W, ...).  The
+    // loads carry no branch (positions at or past S read entry 0, which every
+    // heap's storage holds, and are dropped), so they share one round trip.
+    __device__ __forceinline__ void fill(uint32_t S, int lane) const {
+        HeapEntry v[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t j = (uint32_t)lane + (uint32_t)i * W;
+            v[i] = load_entry(g + (j < S && j < kTop ? j : 0u));
+        }
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t j = (uint32_t)lane + (uint32_t)i * W;
+            if (j < S && j < kTop) t[j] = v[i];
+        }
+    }
+    // Writes entries [0, min(S, kTop)) back to global memory.
+    __device__ __forceinline__ void write_back(uint32_t S, int lane) const {
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const uint32_t j = (uint32_t)lane + (uint32_t)i * W;
+            if (j < S && j < kTop) g[j] = t[j];
+        }
+    }
+};
+
+// Removes the minimum.  The root, the last entry and the root's W children
+// are read together; the popped state's key and stored g (via `load`) are
+// then read while the sift-down runs.  If the root's f exceeds gate_f the
+// heap is left as it is and false is returned (`top` still holds the root).
+template <int W, class Load, class Pre, class Key>
+__device__ inline bool heap_pop(const HeapRef<W>& H, uint32_t& S, int lane, uint32_t& levels, Load load, Pre pre,
+                                Key& key, uint32_t& g, uint32_t gate_f, HeapEntry& top) {
+    top = H.get(0);
+    const uint32_t n = S - 1;
+    HeapEntry last;
+    last.prio = ~0ull;
+    last.slot = 0;
+    last.aux = 0;
+    if (n > 0) last = H.get(n);
+    uint32_t idx = 1 + (uint32_t)lane;
+    HeapEntry e;
+    e.prio = ~0ull;
+    e.slot = 0;
+    e.aux = 0;
+    if (idx < n) e = H.get(idx);
+    if (prio_f(top.prio) > gate_f) return false;
+    S = n;
+    load(top, key, g);
+    // The first level's choice uses entries read with the root; it is made
+    // before pre() issues its memory operations, so that no wait in the
+    // sift-down below covers them (waits on loads issued later still do).
+    uint64_t mp = e.prio;
+    uint32_t mi = idx;
+    grp_min_idx<W>(mp, mi);
+    pre(top);  // the problem's early work for expanding top
+    if (n == 0) return true;
+    uint32_t j = 0;
+    for (;;) {
+        // e holds child W*j + 1 + lane of node j (or +inf past the end); mp/mi its group minimum
+        if (W * j + 1 >= n) break;
+        if (mp >= last.prio) break;
+        if (idx == mi) H.put(j, e);
+        j = mi;
+        ++levels;
+        idx = W * j + 1 + (uint32_t)lane;
+        e.prio = ~0ull;
+        if (idx < n) e = H.get(idx);
+        mp = e.prio;
+        mi = idx;
+        grp_min_idx<W>(mp, mi);
+    }
+    if (lane == 0) H.put(j, last);
+    return true;
+}
+
+template <int W>
+__device__ inline void heap_push(const HeapRef<W>& H, uint32_t S, const HeapEntry& x, uint32_t& levels) {
     uint32_t j = S;
     while (j > 0) {
         const uint32_t p = (j - 1) / W;
-        const HeapEntry pe = load_entry(H + p);
+        const HeapEntry pe = H.get(p);
         if (x.prio < pe.prio) {
-            H[j] = pe;
+            H.put(j, pe);
             j = p;
             ++levels;
         } else {
             break;
         }
     }
-    H[j] = x;
+    H.put(j, x);
 }
 
 // ---------------------------------------------------------------------------
@@ -938,7 +1005,7 @@ struct Puzzle5Ops : WideKeySet<Board5> {
 // Pushes the K records held by the group's lanes (record j in lane j % W,
 // register j / W) into the group's own heap; lane 0 performs the pushes.
 template <int W, int KW>
-__device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEntry (&rec)[KW], int K, int lane,
+__device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const HeapEntry (&rec)[KW], int K, int lane,
                                     uint32_t best, uint32_t cap, bool& overflow, const Tally& tl) {
     // Record j = r*W + l sits in lane l, register r.  Valid records take the
     // positions S, S+1, ... in j order.  When every new position's parent is an
@@ -995,8 +1062,8 @@ __device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEn
         bad[r] = false;
         if (v[r]) {
             const uint32_t pos = S + rank[r];
-            const uint64_t pp = H[(pos - 1) / W].prio;
-            if (rec[r].prio >= pp) H[pos] = rec[r];
+            const uint64_t pp = H.prio((pos - 1) / W);
+            if (rec[r].prio >= pp) H.put(pos, rec[r]);
             else bad[r] = true;
         }
     }
@@ -1051,7 +1118,8 @@ __device__ inline void push_records(gptr<HeapEntry> H, uint32_t& S, const HeapEn
 template <int W, class Ops>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t status, uint32_t it, uint32_t q,
                                           bool active, uint32_t cover, const StepParams& prm, const Tally& tl,
-                                          WorkList& wl) {
+                                          WorkList& wl, HeapEntry* heap_top) {
+    static_assert(W >= 4 && (kStepBlock / W) * HeapRef<W>::kTop <= kHeapTopEntries, "LDS heap tops");
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
@@ -1201,7 +1269,8 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     const bool mine = g < n_work;
     q = mine ? wl.q[g] : 0u;
     S = mine ? wl.S[g] : 0u;
-    const gptr<HeapEntry> H = as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1);
+    const HeapRef<W> H{as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1),
+                       as_local(heap_top) + (size_t)g * HeapRef<W>::kTop};
 #pragma unroll
     for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
     if (mine && it > 0 && q < Aw) {  // the records phase 1 read for this heap (cached)
@@ -1212,6 +1281,8 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             if (j < K) rec[r] = cin[base + (size_t)j * Aw];
         }
     }
+    if (mine) H.fill(S, lane);  // the heap's top levels into LDS, with the records' loads
+    group_fence();
     bool overflow_heap = false, overflow_table = false;
 
     // -- 1. push the records addressed to this heap
@@ -1223,7 +1294,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     group_fence();
 
     // -- 2. first pop of this launch, unless the root lies above the f gate
-    auto load = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& g) { Ops::load(d, t.slot, k, g); };
+    auto load = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& gs) { Ops::load(d, t.slot, k, gs); };
     uint32_t pre[KW];  // per successor: what the problem reads early (grid: the entered cell's weight)
     auto prefetch = [&](const HeapEntry& t) {
 #pragma unroll
@@ -1330,6 +1401,8 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
             group_fence();
         }
+        group_fence();
+        H.write_back(S, lane);
         if (lane == 0) as_global(d.heap_size)[q] = S;
         // candidate slice of this heap: K records
         const gptr<HeapEntry> cout = as_global(d.cand) + (size_t)(it & 1) * N * K + (size_t)q * K;
