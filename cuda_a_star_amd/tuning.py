@@ -145,11 +145,11 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
             "chunk_iters": [8, 16, 32, 64, 128],
             "pops_per_iter": [1, 2, 4],
             "max_load_pct": [50, 70],
-            "rounds": [1, 2, 4],
+            "rounds": [1, 2, 3, 4],
             "use_graph": [0, 1],
             "adaptive_start": [0, 32, 128, 512],
             "grow_thresh": [8, 32, 128],
-            "rounds_grid": [0, 1, 4, 8],
+            "rounds_grid": [0, 1, 2, 3, 4, 6, 8],
             "lanes": [1, 2, 3, 4],
             "deep_cells": [1 << 20, 2 << 20, 4 << 20, 8 << 20],
             "slack_puzzle": [0, 1, 3, 5, 9],
