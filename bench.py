@@ -15,10 +15,10 @@ value = completed solves / wall seconds of the timed stream.
 
 Also measured (outside the timed region): the same requests solved one at a
 time by plain GA* (ga_solve), and the CPU oracle on the host cores.
-The step kernel's roofline comes from HIP events on the engine's launch
-streams (the step launches of every --time-every-th chunk; with graph replay
-one event pair brackets a chunk) and the device counters of the timed
-segments.  Its HBM traffic is this run's algorithmic bytes per launch times
+The step kernel's roofline comes from the sampled chunks (every
+--time-every-th chunk of each launch stream): HIP events around their step
+launches (with graph replay one event pair brackets a chunk) and the device
+counters, which count in those launches only.  Its HBM traffic is this run's algorithmic bytes per launch times
 the HBM-bytes-per-algorithmic-byte ratio of the committed rocprofv3 counter
 runs (profiles/step_kernel_traffic.json).
 """
@@ -170,7 +170,7 @@ def main() -> int:
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-profile", action="store_true", help="skip per-step HIP events and counters")
     ap.add_argument("--time-every", type=int, default=8,
-                    help="time the step launches of every n-th chunk with HIP events (counters cover all)")
+                    help="sample every n-th chunk: HIP events and device counters in its step launches")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
     ap.add_argument("--skip-p24", action="store_true", help="skip the 24-puzzle leg")
@@ -308,8 +308,9 @@ def main() -> int:
                     "achieved_device_GBps": round(st.algo_bytes / t_total / 1e9, 2),
                     # global atomics issued by the step kernel: one CAS per closed-set probe (hashed sets)
                     # plus one atomicMin per lookup; counters and reductions excluded
-                    "atomics_per_launch": round((st.probes + st.lookups) / st.step_launches, 1),
-                    "atomic_rate_G_per_s": round((st.probes + st.lookups) / st.step_launches / (avg_ms * 1e-3) / 1e9, 3)}
+                    # (probes and lookups are counted in the sampled launches only)
+                    "atomics_per_launch": round((st.probes + st.lookups) / st.timed_launches, 1),
+                    "atomic_rate_G_per_s": round((st.probes + st.lookups) / st.timed_launches / (avg_ms * 1e-3) / 1e9, 3)}
     log(f"[bench] served {ok} ok / {failed} failed in {t_total:.2f}s = {served_sps:.1f} solves/s; roofline {roofline}")
 
     # ---- batched: the 1024 mixed instances of the batch configuration in one ga_solve_batch ----

@@ -371,7 +371,7 @@ struct Lane {
     uint32_t chunk_reads[2] = {0, 0};
     std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
     bool timed[2] = {false, false};      // profile: whether that chunk's steps were timed
-    hipGraphExec_t graph_exec = nullptr;
+    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // graph mode: the chunk without / with device counters
     uint32_t running = 0;     // resident jobs not yet finished
     std::vector<int> free_slots;
 };
@@ -387,6 +387,7 @@ struct ga_engine {
     uint32_t n_lanes = 1;     // cfg.lanes + 1; slots are partitioned by slot % n_lanes
     std::vector<Job*> resident;
     ga_engine_stats stats{};
+    double algo_sampled = 0, algo_always = 0;  // algorithmic bytes: profiled-launch terms, all-launch terms
     std::string last_error;
 
     void init(int dev, const ga_config* c) {
@@ -431,7 +432,8 @@ struct ga_engine {
     ~ga_engine() {
         for (Lane& L : lanes) {
             if (L.stream) (void)hipStreamSynchronize(L.stream);
-            if (L.graph_exec) (void)hipGraphExecDestroy(L.graph_exec);
+            for (hipGraphExec_t& ge : L.graph_exec)
+                if (ge) (void)hipGraphExecDestroy(ge);
             for (auto& v : L.prof_ev)
                 for (auto ev : v) (void)hipEventDestroy(ev);
             for (int i = 0; i < 2; ++i) {
@@ -698,10 +700,12 @@ struct ga_engine {
     }
 
     void drop_graph(Lane& L) {
-        if (L.graph_exec) {
-            hip_check(hipStreamSynchronize(L.stream), "sync");
-            (void)hipGraphExecDestroy(L.graph_exec);
-            L.graph_exec = nullptr;
+        for (hipGraphExec_t& ge : L.graph_exec) {
+            if (ge) {
+                hip_check(hipStreamSynchronize(L.stream), "sync");
+                (void)hipGraphExecDestroy(ge);
+                ge = nullptr;
+            }
         }
     }
 
@@ -750,7 +754,11 @@ struct ga_engine {
 
     void launch_chunk(Lane& L) {
         const uint32_t R = cfg.chunk_iters;
-        StepParams prm{cfg.pops_per_iter, 1u << 16, cfg.profile ? 1u : 0u, cfg.rounds, cfg.grow_thresh, {0, 0, 0}};
+        // profile n > 1 samples every n-th chunk of the lane: HIP events around
+        // it, and the device counters that only profiling needs (their ballots
+        // and LDS atomics slow every expansion) in its steps only
+        const bool timed = cfg.profile && (cfg.profile == 1 || L.launched % cfg.profile == 0);
+        StepParams prm{cfg.pops_per_iter, 1u << 16, timed ? 1u : 0u, cfg.rounds, cfg.grow_thresh, {0, 0, 0}};
         const int b = (int)(L.launched & 1);
         std::vector<hipEvent_t>& pe = L.prof_ev[b];
         if (cfg.profile && pe.size() < 2 * R) {
@@ -760,21 +768,19 @@ struct ga_engine {
                 pe.push_back(ev);
             }
         }
-        // profile n > 1 times every n-th chunk of the lane (HIP events cost a few
-        // microseconds of command-processor time each)
-        const bool timed = cfg.profile && (cfg.profile == 1 || L.launched % cfg.profile == 0);
         L.timed[b] = timed;
         if (cfg.use_graph) {
-            if (!L.graph_exec) {
+            hipGraphExec_t& ge = L.graph_exec[timed ? 1 : 0];
+            if (!ge) {
                 hipGraph_t g;
                 hip_check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "begin capture");
                 enqueue_steps(L, L.map_cap, prm, nullptr);
                 hip_check(hipStreamEndCapture(L.stream, &g), "end capture");
-                hip_check(hipGraphInstantiate(&L.graph_exec, g, nullptr, nullptr, 0), "instantiate");
+                hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(g);
             }
             if (timed) hip_check(hipEventRecord(pe[0], L.stream), "event");
-            hip_check(hipGraphLaunch(L.graph_exec, L.stream), "graph launch");
+            hip_check(hipGraphLaunch(ge, L.stream), "graph launch");
             if (timed) hip_check(hipEventRecord(pe[1], L.stream), "event");
         } else {
             enqueue_steps(L, L.n_wg, prm, timed ? &pe : nullptr);
@@ -782,7 +788,7 @@ struct ga_engine {
         hip_check(hipGetLastError(), "step launch");
         L.host_base += R;
         stats.step_launches += R;
-        stats.heap_groups += L.n_heaps * R;
+        if (timed) stats.heap_groups += L.n_heaps * R;
         uint32_t n_read = 0;
         for (uint32_t s = 0; s < cfg.max_instances; ++s)
             if (resident[s]) n_read = s + 1;
@@ -827,7 +833,8 @@ struct ga_engine {
         stats.inserted += s.inserted;
         stats.lookups += s.lookups;
         stats.work_groups += s.work_groups;
-        stats.algo_bytes += algo_bytes(j.kind, s);
+        algo_sampled += algo_bytes(j.kind, s, true);
+        algo_always += algo_bytes(j.kind, s, false);
     }
 
     // Bytes the step kernel's data-structure operations must move (the
@@ -838,18 +845,26 @@ struct ga_engine {
     //   per push                  16, per sift-up level 32
     //   per lookup                8 (atomicMin on g|parent); dense: +K*1 per expansion (cells)
     //   hashed only               8 per probe (CAS on the key) + 8 per new key
-    double algo_bytes(int kind, const InstState& s) const {
+    // `sampled`: the terms of the counters that only profiled launches count
+    // (every term but the cells and the new keys); otherwise those two.
+    double algo_bytes(int kind, const InstState& s, bool sampled) const {
         const bool dense = kind == KIND_GRID4 || kind == KIND_GRID8;
         const double K = branching_of(kind), W = cfg.heap_arity;
+        if (!sampled) return dense ? (double)s.expanded * K : (double)s.occupied * 8;
         double b = (double)s.groups * (2.0 * K * 16 + 8);
         b += (double)s.pops * (32 + (dense ? 8 : 16));
         b += (double)s.sift_down * (W * 16 + 16);
         b += (double)s.pushed * 16 + (double)s.sift_up * 32;
         b += (double)s.lookups * 8;
-        if (dense) b += (double)s.expanded * K;
-        else b += (double)s.probes * 8 + (double)s.occupied * 8;
+        if (!dense) b += (double)s.probes * 8;
         if (is_wide(kind)) b += (double)s.pops * 16 + (double)s.lookups * 16;  // 16-byte keys read / written
         return b;
+    }
+    // algorithmic bytes of all step launches: the sampled terms scaled from the
+    // profiled launches to all of them
+    double algo_total() const {
+        const double scale = stats.timed_launches ? (double)stats.step_launches / (double)stats.timed_launches : 0.0;
+        return algo_always + algo_sampled * scale;
     }
 
     void finish(Job& j, int status, uint64_t t0) {
@@ -1265,12 +1280,15 @@ void ga_engine_destroy(ga_engine* e) { delete e; }
 int ga_engine_get_stats(ga_engine* e, ga_engine_stats* out) {
     if (!e || !out) return GA_E_INVALID;
     *out = e->stats;
+    out->algo_bytes = e->algo_total();
     out->peak_bytes = e->arena->peak();
     return GA_OK;
 }
 
 void ga_engine_reset_stats(ga_engine* e) {
-    if (e) e->stats = ga_engine_stats{};
+    if (!e) return;
+    e->stats = ga_engine_stats{};
+    e->algo_sampled = e->algo_always = 0;
 }
 
 int ga_solve(ga_engine* e, const ga_request* req, ga_result* out) {

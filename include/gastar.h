@@ -100,7 +100,8 @@ typedef struct ga_config {
     uint32_t max_load_pct;   /* closed-set load factor limit, percent */
     uint32_t pops_per_iter;  /* heap pops tried per heap per iteration (stale skip) */
     uint32_t profile;        /* 0 off; 1 count work and time every step with HIP events;
-                                n > 1 count every step, time the steps of every n-th chunk */
+                                n > 1 sample every n-th chunk: time its steps, and count the
+                                work marked (profile) below in those steps only */
     uint32_t rounds;         /* expansion rounds per heap per launch (local pushes between rounds) */
     uint32_t adaptive_start; /* initial heaps in use per request (doubles on demand); 0 = all num_queues */
     uint32_t grow_thresh;    /* double the heaps in use when open entries exceed this many per heap */
@@ -111,25 +112,28 @@ typedef struct ga_config {
     uint32_t slack_grid;     /* the same for grids */
 } ga_config;
 
-/* Engine statistics (kernel timing is collected when cfg.profile != 0). */
+/* Engine statistics.  Kernel timing and the counts marked (profile) cover the
+   sampled step launches only (timed_launches of them; all launches when
+   cfg.profile == 1), and are zero when cfg.profile == 0. */
 typedef struct ga_engine_stats {
     uint64_t step_launches;  /* step kernels launched */
-    double step_ms_total;    /* summed step-kernel time from HIP events */
+    double step_ms_total;    /* summed step-kernel time from HIP events (profile) */
     uint64_t expanded;       /* node expansions by all requests */
     uint64_t generated;      /* successors generated */
-    uint64_t pushed;         /* heap pushes */
-    uint64_t probes;         /* closed-set probes */
-    uint64_t sift_down;      /* heap levels visited by pops */
-    uint64_t sift_up;        /* heap levels visited by pushes */
-    uint64_t pops;           /* heap pops (including stale ones) */
+    uint64_t pushed;         /* heap pushes (profile) */
+    uint64_t probes;         /* closed-set probes (profile) */
+    uint64_t sift_down;      /* heap levels visited by pops (profile) */
+    uint64_t sift_up;        /* heap levels visited by pushes (profile) */
+    uint64_t pops;           /* heap pops, including stale ones (profile) */
     uint64_t completed;      /* requests completed */
     uint64_t peak_bytes;     /* peak admitted heap + closed-set bytes */
     uint64_t groups;         /* heap-group iterations of running instances (profile) */
     uint64_t inserted;       /* successors that lowered a stored g (profile) */
     uint64_t lookups;        /* closed-set find-or-insert calls (profile) */
-    double algo_bytes;       /* algorithmic bytes of the step kernels (profile) */
-    uint64_t timed_launches; /* step launches that step_ms_total covers */
-    uint64_t heap_groups;    /* heap-group iterations launched (every mapped heap, every launch) */
+    double algo_bytes;       /* algorithmic bytes of all step launches, the profiled terms scaled
+                                from the sampled launches to all of them */
+    uint64_t timed_launches; /* step launches sampled: step_ms_total and the (profile) counts cover them */
+    uint64_t heap_groups;    /* heap-group iterations launched in the sampled launches (every mapped heap) */
     uint64_t work_groups;    /* heap-group iterations with work: pushes, or a pop within the f gate (profile) */
 } ga_engine_stats;
 

@@ -1,0 +1,48 @@
+"""Engine profiling on the GPU: profile = 1 counts and times every step,
+profile = n > 1 samples every n-th chunk (HIP events and the profile-only
+device counters in its steps only), profile = 0 leaves both off.  The search
+itself must not depend on it."""
+import pytest
+
+import cuda_a_star_amd as ga
+import oracle
+from cuda_a_star_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+def _solve(profile, chunk_iters=4):
+    g = wl.grid(256, 0.2, wl.instance_seed(11, 0))
+    with ga.Engine(0, profile=profile, chunk_iters=chunk_iters, use_graph=1, lanes=1) as e:
+        sol = e.solve(g.request(num_queues=256))
+        return g, sol, e.stats()
+
+
+def test_profile_off_counts_nothing():
+    g, sol, st = _solve(0)
+    assert sol.status == ga.OK
+    assert sol.cost == oracle.dial_grid(g.cells, g.side, g.side, g.start, g.goal, 4)
+    assert st.step_launches > 0 and st.expanded > 0  # always counted
+    assert st.timed_launches == 0 and st.pops == 0 and st.lookups == 0 and st.heap_groups == 0
+
+
+def test_profile_every_chunk_counts_every_launch():
+    g, sol, st = _solve(1)
+    assert sol.status == ga.OK
+    assert st.timed_launches == st.step_launches
+    assert st.pops >= st.expanded > 0
+    assert 0 < st.work_groups <= st.heap_groups
+
+
+def test_profile_sampled_chunks_scale_to_every_launch():
+    _, full, st1 = _solve(1)
+    _, samp, st4 = _solve(4)
+    assert samp.status == ga.OK and samp.cost == full.cost
+    # a chunk in four is sampled (the first chunk of the lane always is)
+    assert 0 < st4.timed_launches < st4.step_launches
+    assert st4.timed_launches % 4 == 0  # whole chunks of chunk_iters = 4 launches
+    # counters of the sampled launches only, yet covering real work
+    assert 0 < st4.pops < st1.pops
+    assert 0 < st4.work_groups <= st4.heap_groups
+    # algorithmic bytes are scaled from the sample to all launches: same order as counting everything
+    assert 0.5 * st1.algo_bytes < st4.algo_bytes < 2.0 * st1.algo_bytes
