@@ -368,9 +368,9 @@ struct WorkList {
 // Threads per step-kernel workgroup (the work list holds one entry per heap
 // group, so W >= 4).
 constexpr int kStepBlock = 256;
-// LDS heap tops of a workgroup: (kStepBlock / W) groups of 1 + 4W entries,
+// LDS heap tops of a workgroup: (kStepBlock / W) groups of 1 + 3W entries,
 // most for W = 4.
-constexpr uint32_t kHeapTopEntries = (kStepBlock / 4) * (1 + 4 * 4);
+constexpr uint32_t kHeapTopEntries = (kStepBlock / 4) * (1 + 3 * 4);
 
 // LDS of one step-kernel workgroup.
 struct StepShared {
@@ -433,16 +433,18 @@ __device__ __forceinline__ HeapEntry load_entry(gptr<HeapEntry> p) {
 }
 
 // A heap group's heap during one launch.  Logical entries [0, kTop) -- the
-// root, its W children and the children of nodes 1..3 -- are held in the
+// root, its W children and the children of nodes 1 and 2 -- are held in the
 // group's LDS slice, copied in from global memory before the launch works on
 // the heap and written back after; deeper entries stay in global memory.  The
 // heaps of a search are shallow (tens of entries), so most sift steps and
 // parent checks are LDS accesses instead of memory round trips.  A node's W
-// children lie on one side of the boundary (kTop = 1 + 4W), so the lanes that
-// read one level take the same path.
+// children lie on one side of the boundary (kTop = 1 + 3W), so the lanes that
+// read one level take the same path.  (Measured on the serving stream: 1 + 3W
+// entries beat 1 + 2W and 1 + 4W by 1-2 %, 1 + W and 1 + 5W by 3-4 %; the copy
+// in and out grows with the slice, the memory accesses it saves shrink.)
 template <int W>
 struct HeapRef {
-    static constexpr uint32_t kTop = 1 + 4 * W;
+    static constexpr uint32_t kTop = 1 + 3 * W;
     static constexpr int kPer = (int)((kTop + W - 1) / W);  // LDS entries per lane in fill/write_back
     gptr<HeapEntry> g;  // logical entry 0 in global memory
     lptr<HeapEntry> t;  // logical entry 0 in LDS
