@@ -1286,6 +1286,9 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     if (mine) H.fill(S, lane);  // the heap's top levels into LDS, with the records' loads
     group_fence();
     bool overflow_heap = false, overflow_table = false;
+    // always-on counts, per lane: one wave reduction and one LDS atomic each at
+    // the end (a ballot and an LDS atomic per event cost ~15 instructions each)
+    uint32_t n_expanded = 0, n_generated = 0, n_occupied = 0;
 
     // -- 1. push the records addressed to this heap
     if (mine && it > 0 && q < Aw) {
@@ -1356,7 +1359,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 continue;
             }
             if (best != kInf && f >= best) continue;
-            tl.count(TL_EXPANDED, lane == 0);
+            n_expanded += lane == 0 ? 1u : 0u;
             // expansion: successors, closed-set insertion (a successor survives
             // only if its atomicMin of (g << 32 | parent) lowered g)
             const uint32_t h = f - g0;
@@ -1367,7 +1370,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 typename Ops::Key skey;
                 uint32_t taux, cost, th;
                 const bool gen = i < K && Ops::succ(d, top_key, top.aux, h, i, pre[r], skey, taux, cost, th);
-                tl.count(TL_GENERATED, gen);
+                n_generated += gen ? 1u : 0u;
                 if (gen) {
                     const uint32_t g2 = g0 + cost, f2 = g2 + th;
                     if (f2 < best) {
@@ -1389,7 +1392,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                         } else {
                             overflow_table = true;
                         }
-                        if constexpr (!Ops::kDense) tl.count(TL_OCCUPIED, found && isnew);
+                        if constexpr (!Ops::kDense) n_occupied += (found && isnew) ? 1u : 0u;
                         tl.prof_count(TL_PROBES, true);  // first probe of every lookup
                         if (np > 1) tl.prof_add(TL_PROBES, np - 1);
                         tl.prof_count(TL_LOOKUPS, found);
@@ -1416,6 +1419,15 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     }
     if (overflow_heap) set_status(st, ST_OVERFLOW_HEAP);
     if (overflow_table) set_status(st, ST_OVERFLOW_TABLE);
+    {
+        const uint32_t e_sum = wave_sum_u32(n_expanded), g_sum = wave_sum_u32(n_generated);
+        const uint32_t o_sum = Ops::kDense ? 0u : wave_sum_u32(n_occupied);
+        if ((threadIdx.x & 63) == 0) {
+            tl.add(TL_EXPANDED, e_sum);
+            tl.add(TL_GENERATED, g_sum);
+            tl.add(TL_OCCUPIED, o_sum);
+        }
+    }
 
     tl.prof_count(TL_WORK, mine && lane == 0);
 }
