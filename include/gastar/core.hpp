@@ -605,9 +605,14 @@ __device__ inline bool hash_find(gptr<const Slot> table, uint64_t mask, uint64_t
 struct HashedU64Set {
     using Key = uint64_t;
     static constexpr bool kDense = false;
+    // successor i's prefetched value, or 0 (no successor) when it is the
+    // popped node's stored parent; the sets without parents keep every move
+    __device__ static inline uint32_t drop_parent(const InstDesc&, uint32_t, int, uint32_t w, uint32_t) { return w; }
     // The key and the stored g (the high word of val; the parent half is not
     // needed at a pop, and a load of it would only hold a register).
-    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g) {
+    // (par: the stored parent, which only the dense sets read: kNone here)
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g, uint32_t& par) {
+        par = kNone;
         // The key may have been claimed by a CAS earlier in this launch, so it
         // is read at agent scope (a plain load could see a stale empty L1
         // line, and a re-read on that case would be a branch that waits for
@@ -653,9 +658,16 @@ __device__ inline uint32_t goal_of(const InstDesc& d, uint32_t t) {
 struct DenseCellSet {
     using Key = uint64_t;
     static constexpr bool kDense = true;
-    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g) {
+    // successor i's prefetched value, or 0 (no successor) when it is the
+    // popped node's stored parent; the sets without parents keep every move
+    __device__ static inline uint32_t drop_parent(const InstDesc&, uint32_t, int, uint32_t w, uint32_t) { return w; }
+    // the stored g and parent cell in one 8-byte load (the parent lets the
+    // expansion skip the move back to it: see GridOps::drop_parent)
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g, uint32_t& par) {
         key = slot;
-        g = g_of(as_global(d.dense) + slot);
+        const uint64_t v = as_global(d.dense)[slot];
+        g = val_g(v);
+        par = val_parent(v);
     }
     __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
         return as_global(d.dense) + slot;
@@ -718,6 +730,9 @@ template <typename T>
 struct WideKeySet {
     using Key = RawKey<T>;
     static constexpr bool kDense = false;
+    // successor i's prefetched value, or 0 (no successor) when it is the
+    // popped node's stored parent; the sets without parents keep every move
+    __device__ static inline uint32_t drop_parent(const InstDesc&, uint32_t, int, uint32_t w, uint32_t) { return w; }
     static constexpr uint32_t kMaxIters = 1u << 20;
 
     __device__ static inline gptr<Key> keys(const InstDesc& d) { return as_global(reinterpret_cast<Key*>(d.keys)); }
@@ -734,7 +749,8 @@ struct WideKeySet {
             if (a.w[i] != b.w[i]) return false;
         return true;
     }
-    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g) {
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g, uint32_t& par) {
+        par = kNone;
         key = load_key(d, slot);
         g = g_of(&as_global(d.table)[slot].val);
     }
@@ -851,7 +867,10 @@ struct PuzzleOps : HashedU64Set {
     __device__ static inline uint32_t prefetch(const InstDesc&, const HeapEntry&, int) { return 0; }
 
     // h of the successor from h of the parent: only the moved tile changes.
-    __device__ static inline bool succ(const InstDesc& d, uint64_t s, uint32_t, uint32_t h, int i, uint32_t,
+    // aux = the cell the blank left to reach this state, + 1 (0 at the start):
+    // moving it back would recreate the parent, whose stored g is lower, so
+    // that move is not generated (its closed-set atomic could not succeed).
+    __device__ static inline bool succ(const InstDesc& d, uint64_t s, uint32_t from, uint32_t h, int i, uint32_t,
                                        uint64_t& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
         const int b = blank_of(s);
         int r = b / S, c = b % S;
@@ -861,9 +880,10 @@ struct PuzzleOps : HashedU64Set {
         else --c;
         if (r < 0 || r >= S || c < 0 || c >= S) return false;
         const int nb = r * S + c;
+        if ((uint32_t)nb + 1 == from) return false;
         const uint32_t tile = (uint32_t)(s >> (4 * nb)) & 15u;
         t = (s & ~(0xFull << (4 * nb))) | ((uint64_t)tile << (4 * b));
-        taux = 0;
+        taux = (uint32_t)b + 1;
         cost = 1;
         th = h - md(d, tile, nb) + md(d, tile, b);
         return true;
@@ -912,6 +932,15 @@ struct GridOps : DenseCellSet {
         return nx < d.width && ny < d.height;  // also catches -1 wrap-around
     }
 
+    // The stored parent cell of a node popped with its stored g: the move back
+    // to it would give that cell g + its weight, above its stored g (every
+    // weight >= 1), so it is not generated.
+    __device__ static inline uint32_t drop_parent(const InstDesc& d, uint32_t aux, int i, uint32_t w, uint32_t par) {
+        uint32_t nx, ny;
+        if (!neighbour(d, aux, i, nx, ny)) return w;
+        return (uint64_t)ny * d.width + nx == par ? 0u : w;
+    }
+
     // `w` = prefetch(d, aux, i)
     __device__ static inline bool succ(const InstDesc& d, uint64_t, uint32_t aux, uint32_t, int i, uint32_t w,
                                        uint64_t& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
@@ -928,7 +957,8 @@ struct GridOps : DenseCellSet {
 
 // 24-puzzle (5x5 board).  Board5 packs cells 0..11 into lo and 12..23 into hi,
 // 5 bits each (bits 5c..5c+4), and cell 24 into the spare bits: its low 4
-// bits in lo[60..63], its high bit in hi[60].  aux = the blank's cell.  Moves,
+// bits in lo[60..63], its high bit in hi[60].  aux = the blank's cell (and,
+// from bit 8, the cell it left to reach the state, + 1).  Moves,
 // cost and heuristic as PuzzleOps (a_star_sliding_puzzle.h:83-123, Manhattan
 // without the blank).
 struct Board5 {
@@ -979,9 +1009,11 @@ struct Puzzle5Ops : WideKeySet<Board5> {
     }
     __device__ static inline uint32_t prefetch(const InstDesc&, const HeapEntry&, int) { return 0; }
     // h of the successor from h of the parent: only the moved tile changes.
-    __device__ static inline bool succ(const InstDesc& d, const Key& k, uint32_t blank, uint32_t h, int i, uint32_t,
+    // aux = the blank's cell | (the cell it left to reach this state + 1) << 8
+    // (0 at the start): the move back to the parent is not generated.
+    __device__ static inline bool succ(const InstDesc& d, const Key& k, uint32_t aux, uint32_t h, int i, uint32_t,
                                        Key& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
-        const int b = (int)blank;
+        const int b = (int)(aux & 0xFFu);
         int r = b / 5, c = b % 5;
         if (i == 0) ++r;
         else if (i == 1) --r;
@@ -989,12 +1021,13 @@ struct Puzzle5Ops : WideKeySet<Board5> {
         else --c;
         if (r < 0 || r >= 5 || c < 0 || c >= 5) return false;
         const int nb = r * 5 + c;
+        if ((uint32_t)nb + 1 == (aux >> 8)) return false;
         Board5 s = board(k);
         const uint32_t tile = get(s, nb);
         set(s, b, tile);
         set(s, nb, 0u);
         t = to_raw(s);
-        taux = (uint32_t)nb;
+        taux = (uint32_t)nb | ((uint32_t)b + 1) << 8;
         cost = 1;
         th = h - md(d, tile, nb) + md(d, tile, b);
         return true;
@@ -1299,7 +1332,8 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     group_fence();
 
     // -- 2. first pop of this launch, unless the root lies above the f gate
-    auto load = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& gs) { Ops::load(d, t.slot, k, gs); };
+    uint32_t top_par = kNone;  // the popped node's stored parent (dense sets)
+    auto load = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& gs) { Ops::load(d, t.slot, k, gs, top_par); };
     uint32_t pre[KW];  // per successor: what the problem reads early (grid: the entered cell's weight)
     auto prefetch = [&](const HeapEntry& t) {
 #pragma unroll
@@ -1360,6 +1394,11 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             }
             if (best != kInf && f >= best) continue;
             n_expanded += lane == 0 ? 1u : 0u;
+#pragma unroll
+            for (int r = 0; r < KW; ++r) {
+                const int i = r * W + lane;
+                if (i < K) pre[r] = Ops::drop_parent(d, top.aux, i, pre[r], top_par);
+            }
             // expansion: successors, closed-set insertion (a successor survives
             // only if its atomicMin of (g << 32 | parent) lowered g)
             const uint32_t h = f - g0;
@@ -1522,8 +1561,8 @@ __device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, 
     while (slot != kNone) {
         if (n >= cap) return kNone;
         typename Ops::Key key;
-        uint32_t g;
-        Ops::load(d, slot, key, g);
+        uint32_t g, par;
+        Ops::load(d, slot, key, g, par);
         out[n++] = key;
         slot = val_parent(*Ops::val_ptr(d, slot));
     }
