@@ -222,20 +222,6 @@ struct StepParams {
 // ---------------------------------------------------------------------------
 // wave helpers (wave64)
 // ---------------------------------------------------------------------------
-__device__ inline uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        uint32_t w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-__device__ inline uint32_t wave_sum_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
 
 // ---------------------------------------------------------------------------
 // exchanges inside a heap group of W lanes
@@ -250,6 +236,36 @@ __device__ __forceinline__ uint32_t quad_perm(uint32_t v) {
 template <int CTRL>
 __device__ __forceinline__ uint64_t quad_perm64(uint64_t v) {
     return ((uint64_t)quad_perm<CTRL>((uint32_t)(v >> 32)) << 32) | quad_perm<CTRL>((uint32_t)v);
+}
+
+// A DPP move over the rows in ROWS (bit r: lanes 16r..16r+15); the other
+// rows take `identity`.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_rows(uint32_t v, uint32_t identity) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)identity, (int)v, CTRL, ROWS, 0xF, false);
+}
+// Reduction over the wave's 64 lanes on the VALU, returned to every lane.
+// Two quad permutes give each quad its value; the half-row mirror (0x141) and
+// the row mirror (0x140) each pair it with a disjoint part of its row, so that
+// every row of 16 holds its total; row_bcast:15 (0x142) brings row 0's into
+// row 1 and row 2's into row 3, row_bcast:31 (0x143) brings lane 31's into
+// rows 2 and 3, and lane 63 holds the wave's total.  Every lane must be
+// active.  (A __shfl_xor ladder takes six LDS permutes, each waited for.)
+template <class Op>
+__device__ __forceinline__ uint32_t wave_reduce(uint32_t v, uint32_t identity, Op op) {
+    v = op(v, dpp_rows<0xB1, 0xF>(v, identity));  // quad_perm [1,0,3,2]
+    v = op(v, dpp_rows<0x4E, 0xF>(v, identity));  // quad_perm [2,3,0,1]
+    v = op(v, dpp_rows<0x141, 0xF>(v, identity));
+    v = op(v, dpp_rows<0x140, 0xF>(v, identity));
+    v = op(v, dpp_rows<0x142, 0xA>(v, identity));
+    v = op(v, dpp_rows<0x143, 0xC>(v, identity));
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+    return wave_reduce(v, ~0u, [](uint32_t a, uint32_t b) { return a < b ? a : b; });
+}
+__device__ inline uint32_t wave_sum_u32(uint32_t v) {
+    return wave_reduce(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
 }
 
 // value of lane (lane ^ O) of the group, O a power of two below W
