@@ -1303,7 +1303,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             const int first = __builtin_ctzll(m);
             uint32_t base = 0;
             if ((int)__lane_id() == first) base = atomicAdd(&wl.n, (uint32_t)__popcll(m));
-            base = (uint32_t)__shfl((int)base, first, 64);
+            base = (uint32_t)__builtin_amdgcn_readlane((int)base, first);  // first: uniform (from a ballot)
             if (lead) {
                 const uint32_t k = base + (uint32_t)__popcll(m & ((1ull << __lane_id()) - 1));
                 wl.q[k] = q;
