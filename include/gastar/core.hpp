@@ -1143,15 +1143,18 @@ __device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const Heap
 
 // One launch = one GA* iteration for the heaps of one workgroup, in two phases.
 //
-// Phase 1 (every heap group) reads, in one round trip, its heap's size and
-// root and the records addressed to it.  It contributes the heap's top after
+// Phase 1 (every heap group) reads, in one round trip, the instance's best
+// and previous minimum f, its heap's size and root and the records addressed
+// to it (the instance's status was loaded by the kernel beforehand; a wave
+// that finds the instance finished takes no part).  It contributes the heap's top after
 // the pushes -- min(root f, f of the incoming records), the pushes have not
 // happened yet -- to the min-f reduction (HeapsHost::minimum, heaps.h:291-294)
 // and its size to the open count, and it decides whether the heap has work:
 // records to push, or a root within the f gate.  Idle heaps clear their
 // candidate slice.
 // Phase 2: the heaps with work are packed into the first groups of the
-// workgroup through an LDS list, and those groups push, pop and expand them;
+// workgroup through an LDS list, and those groups copy their heap's top levels
+// into LDS (HeapRef), push, pop and expand, and write the top levels back;
 // waves left without a heap are done.  A wave lives as long as its slowest
 // group, so packing spares the other waves from waiting on a few busy heaps.
 //
