@@ -371,7 +371,10 @@ struct Lane {
     uint32_t chunk_reads[2] = {0, 0};
     std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
     bool timed[2] = {false, false};      // profile: whether that chunk's steps were timed
-    hipGraphExec_t graph_exec[2] = {nullptr, nullptr};  // graph mode: the chunk without / with device counters
+    // graph mode: the captured chunk per step-kernel instantiation (all kinds,
+    // 8-puzzle, 15-puzzle, 4-connected grid) and without / with device counters;
+    // valid while the map buffer and its size stay the same
+    hipGraphExec_t graph_exec[4][2] = {};
     uint32_t running = 0;     // resident jobs not yet finished
     std::vector<int> free_slots;
 };
@@ -432,8 +435,9 @@ struct ga_engine {
     ~ga_engine() {
         for (Lane& L : lanes) {
             if (L.stream) (void)hipStreamSynchronize(L.stream);
-            for (hipGraphExec_t& ge : L.graph_exec)
-                if (ge) (void)hipGraphExecDestroy(ge);
+            for (auto& per_variant : L.graph_exec)
+                for (hipGraphExec_t& ge : per_variant)
+                    if (ge) (void)hipGraphExecDestroy(ge);
             for (auto& v : L.prof_ev)
                 for (auto ev : v) (void)hipEventDestroy(ev);
             for (int i = 0; i < 2; ++i) {
@@ -673,10 +677,7 @@ struct ga_engine {
             (kinds == kind_bit(KIND_PUZZLE3) || kinds == kind_bit(KIND_PUZZLE4) || kinds == kind_bit(KIND_GRID4))
                 ? kinds
                 : kAllKinds;
-        if (variant != L.variant) {
-            drop_graph(L);  // the captured launches are of the other instantiation
-            L.variant = variant;
-        }
+        L.variant = variant;  // its captured chunk, if any, stays valid (graphs are kept per instantiation)
         if (L.n_wg > L.map_cap) {
             if (L.d_map) {
                 hip_check(hipStreamSynchronize(L.stream), "sync");
@@ -700,12 +701,23 @@ struct ga_engine {
     }
 
     void drop_graph(Lane& L) {
-        for (hipGraphExec_t& ge : L.graph_exec) {
-            if (ge) {
-                hip_check(hipStreamSynchronize(L.stream), "sync");
+        bool synced = false;
+        for (auto& per_variant : L.graph_exec)
+            for (hipGraphExec_t& ge : per_variant) {
+                if (!ge) continue;
+                if (!synced) hip_check(hipStreamSynchronize(L.stream), "sync");
+                synced = true;
                 (void)hipGraphExecDestroy(ge);
                 ge = nullptr;
             }
+    }
+
+    static int variant_slot(uint32_t variant) {
+        switch (variant) {
+            case kind_bit(KIND_PUZZLE3): return 1;
+            case kind_bit(KIND_PUZZLE4): return 2;
+            case kind_bit(KIND_GRID4): return 3;
+            default: return 0;
         }
     }
 
@@ -770,7 +782,7 @@ struct ga_engine {
         }
         L.timed[b] = timed;
         if (cfg.use_graph) {
-            hipGraphExec_t& ge = L.graph_exec[timed ? 1 : 0];
+            hipGraphExec_t& ge = L.graph_exec[variant_slot(L.variant)][timed ? 1 : 0];
             if (!ge) {
                 hipGraph_t g;
                 hip_check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "begin capture");
