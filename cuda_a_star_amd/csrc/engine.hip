@@ -41,7 +41,7 @@ constexpr uint32_t kAllKinds =
 // The step kernel for the kinds in KINDS.  The register count of a kernel is
 // the maximum over the kinds it can run, so a lane that holds one kind runs an
 // instantiation for that kind alone: the 15-puzzle and 4-connected grid paths
-// fit 64 VGPRs (8 waves per SIMD) where the kernel for all kinds needs 70 (7).
+// fit 64 VGPRs (8 waves per SIMD) where the kernel for all kinds needs 77 (6).
 template <int W, uint32_t KINDS>
 __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict__ wgmap,
                                                       const InstDesc* __restrict__ descs, InstState* states,
