@@ -356,6 +356,7 @@ struct Lane {
     hipStream_t stream = nullptr;
     WgEntry* d_map = nullptr;
     uint32_t map_cap = 0, n_wg = 0;
+    uint32_t grid = 0;  // workgroups launched: n_wg, or in graph mode n_wg rounded up to a sixteenth of map_cap
     uint64_t n_heaps = 0;     // heaps the map covers
     uint32_t variant = 0;     // kind mask of the step-kernel instantiation the lane launches
     bool map_dirty = true;
@@ -371,10 +372,10 @@ struct Lane {
     uint32_t chunk_reads[2] = {0, 0};
     std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
     bool timed[2] = {false, false};      // profile: whether that chunk's steps were timed
-    // graph mode: the captured chunk per step-kernel instantiation (all kinds,
-    // 8-puzzle, 15-puzzle, 4-connected grid) and without / with device counters;
-    // valid while the map buffer and its size stay the same
-    hipGraphExec_t graph_exec[4][2] = {};
+    // graph mode: the captured chunks, by step-kernel instantiation (all kinds,
+    // 8-puzzle, 15-puzzle, 4-connected grid), without / with device counters,
+    // and launch grid; valid while the map buffer stays the same
+    std::map<uint64_t, hipGraphExec_t> graphs;
     uint32_t running = 0;     // resident jobs not yet finished
     std::vector<int> free_slots;
 };
@@ -435,9 +436,8 @@ struct ga_engine {
     ~ga_engine() {
         for (Lane& L : lanes) {
             if (L.stream) (void)hipStreamSynchronize(L.stream);
-            for (auto& per_variant : L.graph_exec)
-                for (hipGraphExec_t& ge : per_variant)
-                    if (ge) (void)hipGraphExecDestroy(ge);
+            for (auto& kv : L.graphs)
+                if (kv.second) (void)hipGraphExecDestroy(kv.second);
             for (auto& v : L.prof_ev)
                 for (auto ev : v) (void)hipEventDestroy(ev);
             for (int i = 0; i < 2; ++i) {
@@ -687,8 +687,13 @@ struct ga_engine {
             hip_check(hipMalloc(&L.d_map, sizeof(WgEntry) * L.map_cap), "hipMalloc map");
             drop_graph(L);  // the captured launches refer to the old map and grid size
         }
-        // graph mode launches map_cap workgroups: pad with entries that exit at once
-        const uint32_t n_up = cfg.use_graph ? L.map_cap : L.n_wg;
+        // Graph mode launches a captured grid: n_wg rounded up to a sixteenth of
+        // map_cap (one graph per such size is kept), padded with entries that
+        // exit at once.  Launching map_cap workgroups, a size that only grows,
+        // had left more than half of every launch as padding.
+        L.grid = cfg.use_graph ? std::min<uint32_t>(L.map_cap, (uint32_t)round_up(L.n_wg, std::max<uint32_t>(64, L.map_cap / 16)))
+                               : L.n_wg;
+        const uint32_t n_up = L.grid;
         map.resize(n_up, WgEntry{kNone, 0, 0, 0});
         int b = L.map_buf;
         L.map_buf ^= 1;
@@ -701,15 +706,11 @@ struct ga_engine {
     }
 
     void drop_graph(Lane& L) {
-        bool synced = false;
-        for (auto& per_variant : L.graph_exec)
-            for (hipGraphExec_t& ge : per_variant) {
-                if (!ge) continue;
-                if (!synced) hip_check(hipStreamSynchronize(L.stream), "sync");
-                synced = true;
-                (void)hipGraphExecDestroy(ge);
-                ge = nullptr;
-            }
+        if (L.graphs.empty()) return;
+        hip_check(hipStreamSynchronize(L.stream), "sync");
+        for (auto& kv : L.graphs)
+            if (kv.second) (void)hipGraphExecDestroy(kv.second);
+        L.graphs.clear();
     }
 
     static int variant_slot(uint32_t variant) {
@@ -782,11 +783,12 @@ struct ga_engine {
         }
         L.timed[b] = timed;
         if (cfg.use_graph) {
-            hipGraphExec_t& ge = L.graph_exec[variant_slot(L.variant)][timed ? 1 : 0];
+            const uint64_t key = ((uint64_t)variant_slot(L.variant) << 40) | ((uint64_t)(timed ? 1 : 0) << 32) | L.grid;
+            hipGraphExec_t& ge = L.graphs[key];
             if (!ge) {
                 hipGraph_t g;
                 hip_check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "begin capture");
-                enqueue_steps(L, L.map_cap, prm, nullptr);
+                enqueue_steps(L, L.grid, prm, nullptr);
                 hip_check(hipStreamEndCapture(L.stream, &g), "end capture");
                 hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(g);
