@@ -48,6 +48,14 @@ def test_config2_costs():
         assert res.cost == g["cost"]
 
 
+def test_config3_instance_and_cost():
+    # the 4096 x 4096 grid: the generator's cells, and Dijkstra's cost on them
+    x, g = wl.config3(0), load("config3.json")["instances"][0]
+    assert (x.side, x.start, x.goal) == (g["side"], 0, 4096 * 4096 - 1)
+    assert digest(x.cells) == g["cells_sha256"]
+    assert oracle.dial_grid(x.cells, 4096, 4096, x.start, x.goal, 4) == g["cost"]
+
+
 def test_config4_instances_and_sample_costs():
     insts = wl.config4(4, 1024, threads=8)
     gold = load("config4.json")["instances"]

@@ -3,11 +3,16 @@
 Costs must equal the oracle's exactly (integer, tolerance 0) and every path
 must pass a move-by-move replay.  Instances come from the seeded generators.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
 import cuda_a_star_amd as ga
 import oracle
+from cuda_a_star_amd import workloads as wl
 
 pytestmark = pytest.mark.gpu
 
@@ -120,3 +125,30 @@ def test_capacity_growth(engine):
     start = ga.gen_puzzle(4, 60, 1)
     sol = check_puzzle(engine, 4, start, num_queues=1024, heap_capacity=16, table_slots=1 << 12)
     assert sol.retries > 0
+
+
+def _golden(name):
+    with open(os.path.join(os.path.dirname(__file__), "golden", name)) as f:
+        return json.load(f)["instances"]
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_config2_puzzle15_matches_golden(engine, seed):
+    # 15-puzzle, 60-move walk without immediate backtracks, N = 16384 heaps,
+    # 2^26 closed-set slots (load <= 0.5): every committed instance
+    x, g = wl.config2(seed), _golden("config2.json")[seed]
+    assert x.start == int(g["start"], 16)
+    sol = engine.solve(x.request())
+    assert sol.status == ga.OK, ga.status_str(sol.status)
+    assert sol.cost == g["cost"]
+    assert oracle.replay_puzzle(4, sol.path, x.start, x.goal) == sol.cost
+
+
+def test_config3_grid4096_matches_golden(engine):
+    # 4096 x 4096 grid, weights 1..9, 20 % walls, (0,0) -> (4095,4095), N = 16384 heaps
+    x, g = wl.config3(0), _golden("config3.json")[0]
+    assert hashlib.sha256(x.cells.tobytes()).hexdigest()[:32] == g["cells_sha256"]
+    sol = engine.solve(x.request())
+    assert sol.status == ga.OK, ga.status_str(sol.status)
+    assert sol.cost == g["cost"]
+    assert oracle.replay_grid(x.cells, 4096, 4096, 4, sol.path, x.start, x.goal) == sol.cost
