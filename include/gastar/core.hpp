@@ -773,10 +773,13 @@ struct WideKeySet {
     __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
         return &as_global(d.table)[slot].val;
     }
-    // All side effects (claim, key write, publish) happen inside the loop body
-    // of the iteration that claims the slot; the loop exit carries none, so a
-    // lane of the same wave that saw the slot busy finds it published on its
-    // next iteration (divergent exits are only taken once the wave leaves).
+    // Every probe is one CAS(empty -> busy): it claims a free slot, or returns
+    // the tag already there (one round trip, where a load of the tag followed
+    // by the claim took two for every new key).  All side effects (claim, key
+    // write, publish) happen inside the loop body of the iteration that claims
+    // the slot; the loop exit carries none, so a lane of the same wave that
+    // saw the slot busy finds it published on its next iteration (divergent
+    // exits are only taken once the wave leaves).
     __device__ static inline bool find_or_insert(const InstDesc& d, const Key& key, uint32_t max_probes,
                                                  uint32_t& slot, bool& isnew, uint32_t& probes) {
         const uint64_t h = hash_words(key);
@@ -787,19 +790,14 @@ struct WideKeySet {
         for (uint32_t iter = 0; iter < kMaxIters; ++iter) {
             ++probes;
             const gptr<uint64_t> tagp = &as_global(d.table)[i].key;
-            uint64_t tag = __hip_atomic_load(tagp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (tag == kEmptyKey) {
-                uint64_t prev = kEmptyKey;  // receives the tag found in the slot
-                __hip_atomic_compare_exchange_strong(tagp, &prev, kBusyKey, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                     __HIP_MEMORY_SCOPE_AGENT);
-                if (prev == kEmptyKey) {
-                    keys(d)[i] = key;
-                    __threadfence();
-                    __hip_atomic_store(tagp, fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    outcome = 2;
-                } else {
-                    tag = prev;
-                }
+            uint64_t tag = kEmptyKey;  // receives the tag found in the slot
+            __hip_atomic_compare_exchange_strong(tagp, &tag, kBusyKey, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            if (tag == kEmptyKey) {  // claimed: write the key, then publish the tag
+                keys(d)[i] = key;
+                __threadfence();
+                __hip_atomic_store(tagp, fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                outcome = 2;
             }
             if (outcome == 0 && tag != kBusyKey) {  // a busy slot is looked at again next iteration
                 if (tag == fp) {
