@@ -1039,6 +1039,16 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
                         j->table_shift += 2;
                         if (heaps_full) j->heap_shift += 1;
                     }
+                    // Hashed sets: the heaps together hold at least as many
+                    // entries as the closed set may store states.  The open
+                    // list stays below the stored count, so a search that has
+                    // outgrown its table does not fail again, late, on a heap
+                    // (a restart repeats all the work done so far).
+                    for (ga_engine::Plan q = e->plan(*j);
+                         q.table_slots && q.heap_cap < (1u << 28) &&
+                         (uint64_t)q.n_heaps * q.heap_cap < q.table_slots * e->cfg.max_load_pct / 100;
+                         q = e->plan(*j))
+                        ++j->heap_shift;
                     j->finished = false;
                     retry.push_back(j);
                 }

@@ -86,7 +86,8 @@ def test_batch_forced_retries(engine):
 def test_overflow_grows_both_structures():
     """Searches that outgrow both their heaps and their closed set: a retry grows
     the structure that overflowed 4x and the other one 2x when it was more than a
-    quarter full, so the default retry budget still reaches the optimum."""
+    quarter full, then the heaps at least to the closed set's load limit, so the
+    default retry budget still reaches the optimum."""
     insts = [wl.puzzle15(wl.instance_seed(61, i), depth=48, num_queues=256, table_slots=0) for i in range(4)]
     reqs = []
     for x in insts:
