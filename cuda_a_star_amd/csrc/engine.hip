@@ -509,8 +509,10 @@ struct ga_engine {
         uint64_t b = (uint64_t)p.n_heaps * p.heap_stride * sizeof(HeapEntry);  // heaps
         b += (uint64_t)p.n_heaps * sizeof(uint32_t);                          // heap sizes
         b += 2ull * p.n_heaps * K * sizeof(HeapEntry);                         // candidates
-        b += p.table_slots ? p.table_slots * sizeof(Slot) : cells * sizeof(uint64_t);
-        if (is_wide(j.kind)) b += p.table_slots * sizeof(Puzzle5Ops::Key) + 2 * sizeof(Puzzle5Ops::Key);  // keys, ends
+        if (is_wide(j.kind))  // 32-byte board slots, start and goal
+            b += p.table_slots * sizeof(Board5KeySet::PairSlot) + 2 * sizeof(Puzzle5Ops::Key);
+        else
+            b += p.table_slots ? p.table_slots * sizeof(Slot) : cells * sizeof(uint64_t);
         p.bytes = b;
         return p;
     }
@@ -555,10 +557,11 @@ struct ga_engine {
         d.heap_size = (uint32_t*)take((uint64_t)p.n_heaps * sizeof(uint32_t));
         d.cand = (HeapEntry*)take(2ull * p.n_heaps * K * sizeof(HeapEntry));
         if (r.kind == GA_PUZZLE) {
-            d.table = (Slot*)take(p.table_slots * sizeof(Slot));
             if (is_wide(j.kind)) {
-                d.keys = take(p.table_slots * sizeof(Puzzle5Ops::Key));
+                d.table = (Slot*)take(p.table_slots * sizeof(Board5KeySet::PairSlot));
                 d.user = take(2 * sizeof(Puzzle5Ops::Key));
+            } else {
+                d.table = (Slot*)take(p.table_slots * sizeof(Slot));
             }
         } else {
             uint64_t cells = (uint64_t)r.width * r.height;
@@ -594,7 +597,8 @@ struct ga_engine {
                 hip_check(hipMemcpyAsync((void*)d.user, j.ends, sizeof(j.ends), hipMemcpyHostToDevice, stream),
                           "upload ends");
             }
-            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>((uint64_t*)d.table, p.table_slots * 2, ~0ull);
+            const uint64_t words = p.table_slots * (is_wide(j.kind) ? sizeof(Board5KeySet::PairSlot) : sizeof(Slot)) / 8;
+            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>((uint64_t*)d.table, words, ~0ull);
         } else {
             d.width = r.width;
             d.height = r.height;

@@ -979,7 +979,107 @@ struct Board5 {
     uint64_t lo, hi;
 };
 
-struct Puzzle5Ops : WideKeySet<Board5> {
+// Closed set of 24-puzzle boards: a 32-byte slot {lo, hi, val, unused} holds
+// the board and its g | parent in one line.  lo is claimed by CAS(empty ->
+// lo); it never has all bits set (its 5-bit fields hold tiles 0..24), so it
+// cannot be taken for an empty slot.  hi is stored right after the claim; a
+// board's hi has its top bit clear and the table is filled with all bits
+// set, so a reader that finds its own lo with hi's top bit set knows the
+// board is still being written and looks again on its next iteration.  Each
+// word is read and written whole, so no fence orders the two: a claim is one
+// atomic, a lookup one atomic and one load issued together, where the
+// tag-and-side-array protocol of WideKeySet needs an agent-scope fence
+// (an L2 write-back and invalidate on this part) per new key and per match.
+struct Board5KeySet {
+    using Key = RawKey<Board5>;
+    static constexpr bool kDense = false;
+    static constexpr uint32_t kMaxIters = 1u << 20;
+    struct alignas(32) PairSlot {
+        uint64_t lo, hi, val, unused;
+    };
+    static constexpr uint64_t kUnwritten = 1ull << 63;  // hi's top bit: the board is not yet stored
+
+    __device__ static inline gptr<PairSlot> slots(const InstDesc& d) {
+        return as_global(reinterpret_cast<PairSlot*>(d.table));
+    }
+    __host__ __device__ static inline uint64_t lo_of(const Key& k) { return (uint64_t)k.w[0] | (uint64_t)k.w[1] << 32; }
+    __host__ __device__ static inline uint64_t hi_of(const Key& k) { return (uint64_t)k.w[2] | (uint64_t)k.w[3] << 32; }
+    __device__ static inline uint32_t drop_parent(const InstDesc&, uint32_t, int, uint32_t w, uint32_t) { return w; }
+    __device__ static inline void load(const InstDesc& d, uint32_t slot, Key& key, uint32_t& g, uint32_t& par) {
+        // agent-scope loads: with several rounds per launch a board stored by
+        // another wave earlier in this launch can be popped
+        const gptr<PairSlot> s = slots(d) + slot;
+        const uint64_t lo = __hip_atomic_load(&s->lo, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t hi = __hip_atomic_load(&s->hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        g = g_of(&s->val);
+        par = kNone;
+        key.w[0] = (uint32_t)lo;
+        key.w[1] = (uint32_t)(lo >> 32);
+        key.w[2] = (uint32_t)hi;
+        key.w[3] = (uint32_t)(hi >> 32);
+    }
+    __device__ static inline gptr<uint64_t> val_ptr(const InstDesc& d, uint32_t slot) {
+        return &(slots(d) + slot)->val;
+    }
+    // As in WideKeySet, all side effects happen in the loop body of the
+    // iteration that claims the slot, so a lane that met the slot half
+    // written finds it complete on its next iteration.
+    __device__ static inline bool find_or_insert(const InstDesc& d, const Key& key, uint32_t max_probes,
+                                                 uint32_t& slot, bool& isnew, uint32_t& probes) {
+        const uint64_t lo = lo_of(key), hi = hi_of(key);
+        uint64_t i = hash_words(key) & d.table_mask;
+        uint32_t probed = 0;
+        int outcome = 0;  // 0 searching, 1 found, 2 inserted, 3 table exhausted
+        for (uint32_t iter = 0; iter < kMaxIters; ++iter) {
+            ++probes;
+            const gptr<PairSlot> s = slots(d) + i;
+            // hi is read with the claim: a board stored earlier compares at once
+            const uint64_t seen_hi = __hip_atomic_load(&s->hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            uint64_t prev = kEmptyKey;  // receives the lo found in the slot
+            __hip_atomic_compare_exchange_strong(&s->lo, &prev, lo, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            bool again = false;  // the same slot next iteration: its board is being written
+            if (prev == kEmptyKey) {
+                __hip_atomic_store(&s->hi, hi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                outcome = 2;
+            } else if (prev == lo) {
+                if (seen_hi & kUnwritten) again = true;
+                else if (seen_hi == hi) outcome = 1;
+            }
+            if (outcome == 0 && !again) {
+                if (++probed > max_probes || probed > d.table_mask) outcome = 3;
+                else i = (i + 1) & d.table_mask;
+            }
+            if (outcome != 0) break;
+        }
+        slot = (uint32_t)i;
+        isnew = outcome == 2;
+        return outcome == 1 || outcome == 2;
+    }
+    // After the search (no concurrent writers).
+    __device__ static inline bool find(const InstDesc& d, const Key& key, uint32_t& slot) {
+        const uint64_t lo = lo_of(key), hi = hi_of(key);
+        uint64_t i = hash_words(key) & d.table_mask;
+        for (uint64_t p = 0; p <= d.table_mask; ++p) {
+            const PairSlot& s = reinterpret_cast<const PairSlot*>(d.table)[i];
+            if (s.lo == kEmptyKey) return false;
+            if (s.lo == lo && s.hi == hi) {
+                slot = (uint32_t)i;
+                return true;
+            }
+            i = (i + 1) & d.table_mask;
+        }
+        return false;
+    }
+    __device__ static inline bool is_goal(const InstDesc& d, const Key& key) {
+        const Key g = goal_key(d);
+        return lo_of(key) == lo_of(g) && hi_of(key) == hi_of(g);
+    }
+    __device__ static inline Key start_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[0]; }
+    __device__ static inline Key goal_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[1]; }
+};
+
+struct Puzzle5Ops : Board5KeySet {
     static constexpr int K = 4;
     using Key = RawKey<Board5>;
 
