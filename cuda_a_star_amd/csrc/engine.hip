@@ -130,23 +130,30 @@ struct PathJob {
     uint64_t offset;  // words of the output buffer
 };
 
-__global__ void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n, uint64_t* out, uint32_t* lens) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+// One wave (one 64-thread block) per finished instance: grids walk with the
+// whole wave through an LDS window of the closed set, the hashed sets with
+// lane 0 (their parent slots have no locality to exploit).
+__global__ __launch_bounds__(64) void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n,
+                                                  uint64_t* out, uint32_t* lens) {
+    __shared__ uint64_t win[kPathWin * kPathWin];
+    const uint32_t i = blockIdx.x;
     if (i >= n) return;
     const PathJob j = jobs[i];
     const InstDesc& d = descs[j.inst];
     uint32_t len = kNone;
-    switch (d.kind) {
-        case KIND_PUZZLE3: len = walk_path<PuzzleOps<3>>(d, out + j.offset, j.cap); break;
-        case KIND_PUZZLE4: len = walk_path<PuzzleOps<4>>(d, out + j.offset, j.cap); break;
-        case KIND_GRID4: len = walk_path<GridOps<4>>(d, out + j.offset, j.cap); break;
-        case KIND_GRID8: len = walk_path<GridOps<8>>(d, out + j.offset, j.cap); break;
-        case KIND_PUZZLE5:
-            len = walk_path<Puzzle5Ops>(d, reinterpret_cast<Puzzle5Ops::Key*>(out + j.offset), j.cap);
-            break;
-        default: break;
+    if (d.kind == KIND_GRID4 || d.kind == KIND_GRID8) {
+        len = walk_grid_path(d, out + j.offset, j.cap, win);
+    } else if (threadIdx.x == 0) {
+        switch (d.kind) {
+            case KIND_PUZZLE3: len = walk_path<PuzzleOps<3>>(d, out + j.offset, j.cap); break;
+            case KIND_PUZZLE4: len = walk_path<PuzzleOps<4>>(d, out + j.offset, j.cap); break;
+            case KIND_PUZZLE5:
+                len = walk_path<Puzzle5Ops>(d, reinterpret_cast<Puzzle5Ops::Key*>(out + j.offset), j.cap);
+                break;
+            default: break;
+        }
     }
-    lens[i] = len;
+    if (threadIdx.x == 0) lens[i] = len;
 }
 
 __global__ void abort_kernel(InstState* states, uint32_t slot) {
@@ -1092,8 +1099,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         hip_check(hipEventRecord(ready, L.stream), "record");
         hip_check(hipStreamWaitEvent(e->path_stream, ready, 0), "wait");
         hip_check(hipMemcpyAsync(dj, pj, n * sizeof(PathJob), hipMemcpyHostToDevice, e->path_stream), "upload jobs");
-        path_kernel<<<(uint32_t)((n + 63) / 64), 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n,
-                                                                          dout, dl);
+        path_kernel<<<(uint32_t)n, 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n, dout, dl);
         hip_check(hipGetLastError(), "path launch");
         hip_check(hipMemcpyAsync(b->host.p, dl, lens_bytes + total * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                  e->path_stream),

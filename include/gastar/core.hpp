@@ -738,7 +738,9 @@ __host__ __device__ inline uint64_t hash_words(const RawKey<T>& k) {
 
 // States wider than 8 bytes: Slot.key holds a tag (empty / busy / a 62-bit
 // fingerprint) and d.keys[slot] the state.  A lane that claims an empty slot
-// writes the state, fences, then publishes the fingerprint in the same loop
+// writes the state, fences (release: an L2 write-back on this part, whose
+// XCDs each have their own L2; a reader that matches the fingerprint fences
+// with acquire, an L2 invalidate), then publishes the fingerprint in the same loop
 // iteration; lanes that meet a busy slot look again on their next iteration,
 // so no lane waits inside an inner loop on another lane of its own wave.
 // The loop is bounded; running out counts as a closed-set overflow.
@@ -795,13 +797,13 @@ struct WideKeySet {
                                                  __HIP_MEMORY_SCOPE_AGENT);
             if (tag == kEmptyKey) {  // claimed: write the key, then publish the tag
                 keys(d)[i] = key;
-                __threadfence();
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the key before the tag
                 __hip_atomic_store(tagp, fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 outcome = 2;
             }
             if (outcome == 0 && tag != kBusyKey) {  // a busy slot is looked at again next iteration
                 if (tag == fp) {
-                    __threadfence();
+                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the tag before the key
                     if (same(load_key(d, (uint32_t)i), key)) outcome = 1;
                 }
                 if (outcome == 0) {
@@ -1682,6 +1684,84 @@ __device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, 
         Ops::load(d, slot, key, g, par);
         out[n++] = key;
         slot = val_parent(*Ops::val_ptr(d, slot));
+    }
+    return n;
+}
+
+// walk_path for dense grids with the 64 lanes of one wave (all of them call
+// it, with `win` pointing at kPathWin * kPathWin words of LDS).  A node's
+// parent is one of its neighbours, so the wave loads the closed-set values of
+// the kPathWin x kPathWin cells around the current cell in one round trip and
+// follows the links inside that window; where the walk leaves it, the window
+// is reloaded around the new cell.  One dependent memory access per path
+// step becomes one per kPathWin / 2 steps or more.  Same output and result as
+// walk_path: cells goal..start (written by lane 0), their number, or kNone
+// past cap.
+constexpr uint32_t kPathWin = 32;
+
+__device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint32_t cap, uint64_t* win) {
+    constexpr int kPer = (int)(kPathWin * kPathWin / 64);  // window words per lane
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t W = d.width, H = d.height;
+    const uint32_t ww = W < kPathWin ? W : kPathWin, wh = H < kPathWin ? H : kPathWin;
+    const gptr<const uint64_t> val = as_global<const uint64_t>(d.dense);
+    uint32_t slot = (uint32_t)d.goal;
+    uint32_t x = slot % W, y = slot / W;
+    uint32_t n = 0;
+    while (slot != kNone) {
+        // the window: ww x wh cells inside the grid, the current cell near its
+        // middle.  (Written without a select: `x < ww/2 ? 0 : min(x - ww/2, ..)`
+        // was compiled into min(x - ww/2, ..) alone, which wraps near column 0.)
+        const uint32_t ox = min(max(x, ww / 2) - ww / 2, W - ww);
+        const uint32_t oy = min(max(y, wh / 2) - wh / 2, H - wh);
+        const uint32_t n0 = n;
+        uint64_t v[kPer];
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) {  // all loads in flight together; positions outside read the corner
+            const uint32_t k = lane + 64u * (uint32_t)r, wx = k % kPathWin, wy = k / kPathWin;
+            const bool in = wx < ww && wy < wh;
+            v[r] = val[(size_t)(oy + (in ? wy : 0u)) * W + ox + (in ? wx : 0u)];
+        }
+        __syncthreads();  // every lane is done with the previous window
+#pragma unroll
+        for (int r = 0; r < kPer; ++r) win[lane + 64u * (uint32_t)r] = v[r];
+        __syncthreads();
+        // the links inside the window; every lane follows the same chain
+        while (slot != kNone && x - ox < ww && y - oy < wh) {
+            if (n >= cap) return kNone;
+            if (lane == 0) out[n] = slot;
+            ++n;
+            const uint32_t p = val_parent(win[(y - oy) * kPathWin + (x - ox)]);
+            if (p == kNone) {
+                slot = kNone;
+                break;
+            }
+            // x, y of the parent from the index difference: the eight
+            // neighbour offsets are distinct once W >= 3
+            const int64_t dd = (int64_t)p - (int64_t)slot, w = (int64_t)W;
+            int dx = 2, dy = 0;  // dx = 2: not decoded
+            if (W >= 3) {
+                if (dd == 1) dx = 1;
+                else if (dd == -1) dx = -1;
+                else if (dd == w) dx = 0, dy = 1;
+                else if (dd == -w) dx = 0, dy = -1;
+                else if (dd == w + 1) dx = 1, dy = 1;
+                else if (dd == w - 1) dx = -1, dy = 1;
+                else if (dd == 1 - w) dx = 1, dy = -1;
+                else if (dd == -1 - w) dx = -1, dy = -1;
+            }
+            if (dx == 2) {
+                x = p % W;
+                y = p / W;
+            } else {
+                x = (uint32_t)((int)x + dx);
+                y = (uint32_t)((int)y + dy);
+            }
+            slot = p;
+        }
+        // every window holds the cell it was loaded around, so each pass takes
+        // a step; one that does not ends the walk with an error, not a loop
+        if (slot != kNone && n == n0) return kNone;
     }
     return n;
 }
