@@ -311,6 +311,14 @@ def main() -> int:
                     # (probes and lookups are counted in the sampled launches only)
                     "atomics_per_launch": round((st.probes + st.lookups) / st.timed_launches, 1),
                     "atomic_rate_G_per_s": round((st.probes + st.lookups) / st.timed_launches / (avg_ms * 1e-3) / 1e9, 3)}
+        # The device's rate for those atomics (random 64-bit, agent scope, over a 1 GiB table: no cache
+        # reuse) and the share of the average launch that the launch's CAS probes and fetch-mins would
+        # take at those rates.
+        peak_min = e.atomic_rate(1 << 30, 1 << 26, 0)
+        peak_cas = e.atomic_rate(1 << 30, 1 << 26, 1)
+        t_atomics = (st.probes / peak_cas + st.lookups / peak_min) / st.timed_launches
+        roofline["atomic_peak_G_per_s"] = {"fetch_min": round(peak_min / 1e9, 2), "cas": round(peak_cas / 1e9, 2)}
+        roofline["atomic_frac"] = round(t_atomics / (avg_ms * 1e-3), 4)
     log(f"[bench] served {ok} ok / {failed} failed in {t_total:.2f}s = {served_sps:.1f} solves/s; roofline {roofline}")
 
     # ---- batched: the 1024 mixed instances of the batch configuration in one ga_solve_batch ----

@@ -134,7 +134,7 @@ class ServeStats(ctypes.Structure):
 EXPORTS = (
     "ga_config_default", "ga_engine_create", "ga_engine_destroy", "ga_engine_get_stats",
     "ga_engine_reset_stats", "ga_solve", "ga_solve_puzzle", "ga_solve_grid", "ga_solve_batch",
-    "ga_serve_run", "ga_result_free", "ga_status_str", "ga_device_upload", "ga_device_free",
+    "ga_serve_run", "ga_result_free", "ga_status_str", "ga_device_upload", "ga_device_free", "ga_atomic_rate",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
     "ga_gen_board", "ga_board_solvable",
 )
@@ -171,6 +171,8 @@ def lib() -> ctypes.CDLL:
     L.ga_status_str.restype = ctypes.c_char_p
     L.ga_device_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, P(ctypes.c_void_p)]
     L.ga_device_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    L.ga_atomic_rate.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
+                                 P(ctypes.c_double)]
     L.ga_splitmix64.argtypes = [_u64p]
     L.ga_splitmix64.restype = ctypes.c_uint64
     L.ga_puzzle_goal.argtypes = [ctypes.c_uint32]
@@ -431,6 +433,15 @@ class Engine:
 
     def reset_stats(self):
         lib().ga_engine_reset_stats(self._h)
+
+    def atomic_rate(self, table_bytes: int = 1 << 30, ops: int = 1 << 26, kind: int = 0) -> float:
+        """Random 64-bit global atomics per second on this device (kind 0:
+        fetch-min, 1: CAS from the empty key) over a table of table_bytes."""
+        out = ctypes.c_double()
+        rc = lib().ga_atomic_rate(self._h, table_bytes, ops, kind, ctypes.byref(out))
+        if rc != OK:
+            raise GAStarError(rc, "ga_atomic_rate")
+        return out.value
 
     def upload(self, arr: np.ndarray) -> int:
         """Copies a host array into device memory owned by the caller; returns the device address."""

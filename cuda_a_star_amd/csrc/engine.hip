@@ -1264,6 +1264,73 @@ int run_checked(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, 
     return worst;
 }
 
+// Random 64-bit global atomics at agent scope, the closed sets' operations:
+// kind 0 a fetch-min (the atomicMin on a slot's g | parent), kind 1 a CAS
+// from the empty key (a probe).  Each thread issues PER of them to unrelated
+// addresses back to back and keeps every returned value, so they are all in
+// flight together: the launch measures the device's rate for them.
+template <int PER>
+__global__ __launch_bounds__(256) void atomic_rate_kernel(uint64_t* table, uint64_t mask, int kind, uint64_t* sink) {
+    const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const gptr<uint64_t> t = as_global(table);
+    uint64_t got[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const uint64_t h = mix64(tid * PER + (uint64_t)i + 1);
+        const gptr<uint64_t> p = t + (h & mask);
+        if (kind == 0) {
+            got[i] = __hip_atomic_fetch_min(p, h >> 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            uint64_t prev = kEmptyKey;
+            __hip_atomic_compare_exchange_strong(p, &prev, h >> 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT);
+            got[i] = prev;
+        }
+    }
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < PER; ++i) acc ^= got[i];
+    if (acc == 0x5A5A5A5A5A5A5A5Aull) *sink = acc;  // keeps the returned values live
+}
+
+int atomic_rate(ga_engine* e, uint64_t table_bytes, uint64_t ops, int kind, double* ops_per_sec) {
+    constexpr int kPer = 16;
+    constexpr uint32_t kThreads = 256;
+    const uint64_t words = pow2_floor(table_bytes / sizeof(uint64_t));
+    const uint64_t blocks = (ops + (uint64_t)kPer * kThreads - 1) / ((uint64_t)kPer * kThreads);
+    if (blocks > (1u << 30)) return GA_E_INVALID;
+    uint64_t* table = nullptr;
+    uint64_t* sink = nullptr;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = GA_OK;
+    try {
+        hip_check(hipSetDevice(e->device), "hipSetDevice");
+        if (hipMalloc(&table, words * sizeof(uint64_t)) != hipSuccess || hipMalloc(&sink, sizeof(uint64_t)) != hipSuccess)
+            throw GaError(GA_E_NOMEM, "atomic table");
+        hip_check(hipEventCreate(&t0), "hipEventCreate");
+        hip_check(hipEventCreate(&t1), "hipEventCreate");
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; ++rep) {  // the first pass maps the table's pages; the second is timed
+            fill_kernel<uint64_t><<<1024, kThreads>>>(table, words, kEmptyKey);
+            hip_check(hipEventRecord(t0, nullptr), "record");
+            atomic_rate_kernel<kPer><<<(uint32_t)blocks, kThreads>>>(table, words - 1, kind, sink);
+            hip_check(hipEventRecord(t1, nullptr), "record");
+            hip_check(hipEventSynchronize(t1), "sync");
+            hip_check(hipGetLastError(), "atomic_rate_kernel");
+            hip_check(hipEventElapsedTime(&ms, t0, t1), "elapsed");
+        }
+        *ops_per_sec = ms > 0.f ? (double)(blocks * kThreads * kPer) / (ms * 1e-3) : 0.0;
+    } catch (const GaError& err) {
+        e->last_error = err.what();
+        rc = err.code;
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (table) (void)hipFree(table);
+    if (sink) (void)hipFree(sink);
+    return rc;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1403,6 +1470,12 @@ int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device
 int ga_device_free(ga_engine* e, void* device) {
     if (!e) return GA_E_INVALID;
     return hipFree(device) == hipSuccess ? GA_OK : GA_E_HIP;
+}
+
+int ga_atomic_rate(ga_engine* e, uint64_t table_bytes, uint64_t ops, int kind, double* ops_per_sec) {
+    if (!e || !ops_per_sec || (kind != 0 && kind != 1) || table_bytes < sizeof(uint64_t) || ops == 0)
+        return GA_E_INVALID;
+    return atomic_rate(e, table_bytes, ops, kind, ops_per_sec);
 }
 
 }  // extern "C"

@@ -185,6 +185,13 @@ const char* ga_status_str(int status);
 int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device_out);
 int ga_device_free(ga_engine* e, void* device);
 
+/* Measurement helper, no counterpart in the reference: the device's rate for
+ * the closed sets' operations, random 64-bit global atomics at agent scope
+ * (kind 0: fetch-min, kind 1: CAS from the empty key) over a table of
+ * table_bytes (rounded down to a power of two), about `ops` of them in one
+ * timed launch.  Writes operations per second to *ops_per_sec. */
+int ga_atomic_rate(ga_engine* e, uint64_t table_bytes, uint64_t ops, int kind, double* ops_per_sec);
+
 /* ---- seeded instance generators (host only; splitmix64 streams) ---- */
 uint64_t ga_splitmix64(uint64_t* state);
 /* Goal = tiles 1..n-1 in row-major order, blank last. */

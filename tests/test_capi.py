@@ -31,6 +31,13 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_atomic_rate_rejects_bad_arguments_before_any_device_call():
+    out = ctypes.c_double()
+    L = ga.lib()
+    assert L.ga_atomic_rate(None, 1 << 20, 1 << 10, 0, ctypes.byref(out)) == -1  # no engine
+    assert L.ga_atomic_rate(None, 1 << 20, 1 << 10, 2, ctypes.byref(out)) == -1  # no such kind
+
+
 def test_status_strings():
     for code, name in [(0, "GA_OK"), (1, "GA_NO_PATH"), (-1, "GA_E_INVALID"), (-2, "GA_E_NOMEM"),
                        (-3, "GA_E_BUDGET"), (-4, "GA_E_HIP"), (-5, "GA_E_TIMEOUT"), (-6, "GA_E_OVERFLOW")]:

@@ -46,3 +46,13 @@ def test_profile_sampled_chunks_scale_to_every_launch():
     assert 0 < st4.work_groups <= st4.heap_groups
     # algorithmic bytes are scaled from the sample to all launches: same order as counting everything
     assert 0.5 * st1.algo_bytes < st4.algo_bytes < 2.0 * st1.algo_bytes
+
+
+def test_atomic_rate_calibration():
+    # random 64-bit atomics over 256 MiB: a positive rate, and bad kinds refused
+    with ga.Engine(0) as e:
+        for kind in (0, 1):
+            r = e.atomic_rate(1 << 28, 1 << 22, kind)
+            assert 1e8 < r < 1e12, (kind, r)
+        with pytest.raises(ga.GAStarError):
+            e.atomic_rate(1 << 28, 1 << 22, 7)
