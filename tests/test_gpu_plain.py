@@ -115,6 +115,21 @@ def test_grid_unreachable(engine):
     assert sol.status == ga.NO_PATH
 
 
+@pytest.mark.parametrize("w,h,conn", [(1, 64, 4), (2, 50, 4), (2, 50, 8), (64, 1, 4), (3, 40, 8), (33, 2, 4),
+                                        (40, 33, 8)])
+def test_grid_thin_and_window_edges(engine, w, h, conn):
+    # widths and heights below, at and just above the path walk's window
+    # limits (rows of 1 or 2 cells, the 32-cell window): cost, then replay
+    rng = np.random.default_rng(1000 * w + h + conn)
+    cells = rng.integers(1, 10, size=w * h).astype(np.uint8)
+    check_grid(engine, cells, w, h, conn=conn, num_queues=64)
+    # the same grid walked from the other corner (goal at cell 0)
+    sol = engine.solve_grid(cells, w, h, w * h - 1, 0, conn, num_queues=64)
+    ref = oracle.dial_grid(cells, w, h, w * h - 1, 0, conn)
+    assert sol.status == ga.OK and sol.cost == ref
+    assert oracle.replay_grid(cells, w, h, conn, sol.path, w * h - 1, 0) == sol.cost
+
+
 def test_grid_1024(engine):
     cells, _ = ga.gen_grid(1024, 1024, 0.2, 5)
     check_grid(engine, cells, 1024, 1024, num_queues=4096)
