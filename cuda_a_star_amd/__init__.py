@@ -26,7 +26,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgastar.so")
 
 # status codes (include/gastar.h)
-OK, NO_PATH = 0, 1
+OK, NO_PATH, PENDING = 0, 1, 2
 E_INVALID, E_NOMEM, E_BUDGET, E_HIP, E_TIMEOUT, E_OVERFLOW = -1, -2, -3, -4, -5, -6
 PUZZLE, GRID = 1, 2
 
@@ -134,7 +134,8 @@ class ServeStats(ctypes.Structure):
 EXPORTS = (
     "ga_config_default", "ga_engine_create", "ga_engine_destroy", "ga_engine_get_stats",
     "ga_engine_reset_stats", "ga_solve", "ga_solve_puzzle", "ga_solve_grid", "ga_solve_batch",
-    "ga_serve_run", "ga_result_free", "ga_status_str", "ga_device_upload", "ga_device_free", "ga_atomic_rate",
+    "ga_serve_run", "ga_result_free", "ga_status_str", "ga_engine_last_error", "ga_debug_fail_run",
+    "ga_device_upload", "ga_device_free", "ga_atomic_rate",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
     "ga_gen_board", "ga_board_solvable",
 )
@@ -169,6 +170,9 @@ def lib() -> ctypes.CDLL:
     L.ga_result_free.restype = None
     L.ga_status_str.argtypes = [ctypes.c_int]
     L.ga_status_str.restype = ctypes.c_char_p
+    L.ga_engine_last_error.argtypes = [ctypes.c_void_p]
+    L.ga_engine_last_error.restype = ctypes.c_char_p
+    L.ga_debug_fail_run.argtypes = [P(Request), ctypes.c_uint32, P(Result), ctypes.c_uint32, ctypes.c_int]
     L.ga_device_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, P(ctypes.c_void_p)]
     L.ga_device_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.ga_atomic_rate.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,
@@ -195,9 +199,14 @@ def status_str(code: int) -> str:
 
 
 class GAStarError(RuntimeError):
-    def __init__(self, code: int, what: str = ""):
+    """A failed call or run.  ``results`` holds the per-request results of a
+    failed run: the requests that had finished keep their status, the others
+    carry the run's error code."""
+
+    def __init__(self, code: int, what: str = "", results=None):
         super().__init__(f"{status_str(code)} ({code}) {what}".strip())
         self.code = code
+        self.results = results
 
 
 # ---------------------------------------------------------------------------
@@ -394,11 +403,22 @@ class Engine:
     def __exit__(self, *a):
         self.close()
 
+    def _check_run(self, rc: int, what: str, sols):
+        if rc != OK:
+            msg = lib().ga_engine_last_error(self._h).decode(errors="replace")
+            raise GAStarError(rc, f"{what}: {msg}", sols)
+
     # -- plain GA*: one instance per call (a_star<>(), a_star.h:223-282) --
     def solve(self, req: Request) -> Solution:
+        """One request.  The request's own outcome is in the Solution's status;
+        a run that fails (HIP error, device memory) raises GAStarError.  (Runs
+        through ga_solve_batch with one request, whose return code tells the
+        two apart; ga_solve returns the request's status for both.)"""
         res = Result()
-        lib().ga_solve(self._h, ctypes.byref(req), ctypes.byref(res))
-        return _take(res, _board_side(req))
+        rc = lib().ga_solve_batch(self._h, ctypes.byref(req), 1, ctypes.byref(res))
+        sol = _take(res, _board_side(req))
+        self._check_run(rc, "ga_solve", [sol])
+        return sol
 
     def solve_puzzle(self, side: int, start: int, goal: Optional[int] = None, num_queues: int = 0,
                      **kw) -> Solution:
@@ -414,8 +434,10 @@ class Engine:
         n = len(reqs)
         arr = (Request * n)(*reqs)
         outs = (Result * n)()
-        lib().ga_solve_batch(self._h, arr, n, outs)
-        return [_take(outs[i], _board_side(reqs[i])) for i in range(n)]
+        rc = lib().ga_solve_batch(self._h, arr, n, outs)
+        sols = [_take(outs[i], _board_side(reqs[i])) for i in range(n)]
+        self._check_run(rc, "ga_solve_batch", sols)
+        return sols
 
     def serve(self, reqs: Sequence[Request], arrival_ns: Sequence[int]):
         n = len(reqs)
@@ -423,8 +445,10 @@ class Engine:
         arrivals = (ctypes.c_uint64 * n)(*[int(a) for a in arrival_ns])
         outs = (Result * n)()
         ss = ServeStats()
-        lib().ga_serve_run(self._h, arr, arrivals, n, outs, ctypes.byref(ss))
-        return [_take(outs[i], _board_side(reqs[i])) for i in range(n)], ss
+        rc = lib().ga_serve_run(self._h, arr, arrivals, n, outs, ctypes.byref(ss))
+        sols = [_take(outs[i], _board_side(reqs[i])) for i in range(n)]
+        self._check_run(rc, "ga_serve_run", sols)
+        return sols, ss
 
     def stats(self) -> EngineStats:
         s = EngineStats()

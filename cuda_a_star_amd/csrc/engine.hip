@@ -12,9 +12,11 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <new>
 #include <stdexcept>
@@ -350,6 +352,46 @@ struct PathBatch {
 
 }  // namespace
 
+namespace {
+
+// Step-kernel instantiation a launch runs: a kind mask (kind_bit of one kind
+// that has a single-kind instantiation, or kAllKinds), or kWideVariant for
+// the 24-puzzle kernel.
+constexpr uint32_t kWideVariant = kind_bit(KIND_PUZZLE5);
+
+template <int W>
+void launch_step_w(uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map, const InstDesc* descs,
+                   InstState* states, const uint32_t* iter, uint32_t r, const StepParams& prm) {
+    const dim3 g(grid), b(kBlock);
+    switch (variant) {
+        case kind_bit(KIND_PUZZLE3):
+            step_kernel<W, kind_bit(KIND_PUZZLE3)><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            break;
+        case kind_bit(KIND_PUZZLE4):
+            step_kernel<W, kind_bit(KIND_PUZZLE4)><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            break;
+        case kind_bit(KIND_GRID4):
+            step_kernel<W, kind_bit(KIND_GRID4)><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            break;
+        case kWideVariant:
+            step_kernel_wide<W><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            break;
+        default:
+            step_kernel<W, kAllKinds><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            break;
+    }
+}
+
+void launch_step(uint32_t arity, uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map,
+                 const InstDesc* descs, InstState* states, const uint32_t* iter, uint32_t r, const StepParams& prm) {
+    if (arity == 8)
+        launch_step_w<8>(variant, s, grid, map, descs, states, iter, r, prm);
+    else
+        launch_step_w<4>(variant, s, grid, map, descs, states, iter, r, prm);
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // engine
 // ---------------------------------------------------------------------------
@@ -400,12 +442,18 @@ struct ga_engine {
     ga_engine_stats stats{};
     double algo_sampled = 0, algo_always = 0;  // algorithmic bytes: profiled-launch terms, all-launch terms
     std::string last_error;
+    bool broken = false;  // a HIP error ended a run: the device state is unknown, later runs are refused
 
     void init(int dev, const ga_config* c) {
         device = dev;
         ga_config_default(&cfg);
         if (c) cfg = *c;
-        if (cfg.heap_arity != 4 && cfg.heap_arity != 8 && cfg.heap_arity != 16) cfg.heap_arity = 4;
+        if (cfg.heap_arity == 0) cfg.heap_arity = 4;
+        // Heap arity 16 was taken out: it served slower than 4 and 8, and a
+        // served run with it (4,096 heaps per 15-puzzle) once stalled with no
+        // cause shown.
+        if (cfg.heap_arity != 4 && cfg.heap_arity != 8) throw GaError(GA_E_INVALID, "heap_arity must be 4 or 8");
+        if (cfg.lanes > 4) throw GaError(GA_E_INVALID, "lanes must be at most 4");
         if (cfg.max_instances == 0) cfg.max_instances = 1;
         if (cfg.chunk_iters == 0) cfg.chunk_iters = 1;
         if (cfg.max_load_pct == 0 || cfg.max_load_pct > 95) cfg.max_load_pct = 50;
@@ -413,7 +461,11 @@ struct ga_engine {
         if (cfg.rounds == 0) cfg.rounds = 1;
         if (cfg.grow_thresh == 0) cfg.grow_thresh = 32;
         if (cfg.lanes == 0) cfg.lanes = 1;
-        if (cfg.lanes > 4) cfg.lanes = 4;
+        // Under rocprofv3 (which exports ROCPROF_OUTPUT_PATH to the program it
+        // runs) chunks are launched one kernel at a time: a kernel trace of
+        // graph replay ended once in an HSA queue abort (malformed AQL
+        // packet) and once in a host segmentation fault inside the runtime.
+        if (cfg.use_graph && getenv("ROCPROF_OUTPUT_PATH")) cfg.use_graph = 0;
         n_lanes = cfg.lanes + 1;
         if (cfg.max_instances < n_lanes) cfg.max_instances = n_lanes;
         hip_check(hipSetDevice(device), "hipSetDevice");
@@ -665,6 +717,38 @@ struct ga_engine {
         }
     }
 
+    // After a run has failed: waits for everything launched, drops the jobs
+    // still resident (they belong to the failed run) and resets every lane's
+    // iteration clock, so that the engine holds nothing of that run.  A HIP
+    // error leaves the device state unknown: the engine is then marked
+    // broken and refuses later runs.
+    void abandon(int code) {
+        bool ok = code != GA_E_HIP;
+        for (Lane& L : lanes) ok = hipStreamSynchronize(L.stream) == hipSuccess && ok;
+        ok = hipStreamSynchronize(path_stream) == hipSuccess && ok;
+        for (Job*& j : resident) {
+            if (!j) continue;
+            for (void* a : j->allocs) arena->release(a);
+            j->allocs.clear();
+            lanes[j->lane].free_slots.push_back(j->slot);
+            j->slot = -1;
+            j = nullptr;
+        }
+        for (Lane& L : lanes) {
+            L.processed = L.launched;
+            L.running = 0;
+            L.n_wg = 0;
+            L.map_dirty = true;
+            L.host_base = 0;
+            if (ok) {
+                fill_kernel<uint32_t><<<1, 64, 0, L.stream>>>(L.d_iter, 1, 0u);
+                ok = hipGetLastError() == hipSuccess && hipStreamSynchronize(L.stream) == hipSuccess;
+            }
+        }
+        (void)hipGetLastError();
+        if (!ok) broken = true;
+    }
+
     // ---- launch ---------------------------------------------------------------
     void upload_map(Lane& L, uint32_t lane) {
         const uint32_t W = cfg.heap_arity;
@@ -733,44 +817,13 @@ struct ga_engine {
         }
     }
 
-    template <int W>
-    void launch_step(const Lane& L, uint32_t grid, const StepParams& prm, uint32_t r) {
-        const dim3 g(grid), b(kBlock);
-        switch (L.variant) {
-            case kind_bit(KIND_PUZZLE3):
-                step_kernel<W, kind_bit(KIND_PUZZLE3)><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
-                break;
-            case kind_bit(KIND_PUZZLE4):
-                step_kernel<W, kind_bit(KIND_PUZZLE4)><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
-                break;
-            case kind_bit(KIND_GRID4):
-                step_kernel<W, kind_bit(KIND_GRID4)><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
-                break;
-            default:
-                step_kernel<W, kAllKinds><<<g, b, 0, L.stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm);
-                break;
-        }
-    }
-
     void enqueue_steps(Lane& L, uint32_t grid, const StepParams& prm, std::vector<hipEvent_t>* pe) {
         const uint32_t R = cfg.chunk_iters;
         hipStream_t stream = L.stream;
-        const bool wide = &L == &lanes.back();
+        const uint32_t variant = &L == &lanes.back() ? kWideVariant : L.variant;
         for (uint32_t r = 0; r < R; ++r) {
             if (pe) hip_check(hipEventRecord((*pe)[2 * r], stream), "event");
-            if (wide) {
-                switch (cfg.heap_arity) {
-                    case 8: step_kernel_wide<8><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
-                    case 16: step_kernel_wide<16><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
-                    default: step_kernel_wide<4><<<grid, kBlock, 0, stream>>>(L.d_map, d_descs, d_states, L.d_iter, r, prm); break;
-                }
-            } else {
-                switch (cfg.heap_arity) {
-                    case 8: launch_step<8>(L, grid, prm, r); break;
-                    case 16: launch_step<16>(L, grid, prm, r); break;
-                    default: launch_step<4>(L, grid, prm, r); break;
-                }
-            }
+            launch_step(cfg.heap_arity, variant, stream, grid, L.d_map, d_descs, d_states, L.d_iter, r, prm);
             if (pe) hip_check(hipEventRecord((*pe)[2 * r + 1], stream), "event");
         }
         advance_kernel<<<1, 64, 0, stream>>>(L.d_iter, R);
@@ -1075,6 +1128,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         if (done.empty()) return;
         // walk the paths on the path stream once this lane's work so far has finished
         PathBatch* b = new PathBatch();
+        batches.push_back(b);  // freed by finalize_batch, or by the clean-up of a failed run
         b->jobs = done;
         uint64_t total = 0;
         for (Job* j : done) {
@@ -1108,9 +1162,9 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         hip_check(hipEventRecord(b->ev, e->path_stream), "record");
         (void)hipEventDestroy(ready);
         b->total = total;
-        batches.push_back(b);
     };
 
+    try {
     for (;;) {
         // -- admit: retries first, then arrivals in order
         uint64_t now = now_ns() - t0;
@@ -1187,6 +1241,16 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
             if (wait > now) std::this_thread::sleep_for(std::chrono::nanoseconds(std::min<uint64_t>(wait - now, 200000)));
         }
     }
+    } catch (...) {
+        // the path walks in flight belong to this run: wait for them and free them
+        (void)hipStreamSynchronize(e->path_stream);
+        for (PathBatch* b : batches) {
+            e->arena->release(b->dev);
+            if (b->ev) (void)hipEventDestroy(b->ev);
+            delete b;
+        }
+        throw;
+    }
     if (ss) {
         std::vector<double> lat;
         uint64_t last = 0, first = ~0ull;
@@ -1215,13 +1279,20 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
     }
 }
 
-int run_checked(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, uint32_t n, ga_result* outs,
-                ga_serve_stats* ss) {
-    if (!e || (!reqs && n) || (!outs && n)) return GA_E_INVALID;
+// Runs the live jobs of one call; throws GaError when the run fails.
+using Runner = std::function<void(std::vector<Job>&)>;
+
+// The request checks and the result contract shared by every solve and serve
+// entry point: each result is GA_PENDING until its job finishes; a run that
+// fails leaves its error code in every result still pending.  Returns GA_OK
+// once every request has its final status, or the run's error.
+int run_checked(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* outs, const uint64_t* arrivals,
+                const Runner& runner) {
+    if ((!reqs && n) || (!outs && n)) return GA_E_INVALID;
     std::vector<Job> jobs(n);
-    int worst = GA_OK;
     for (uint32_t i = 0; i < n; ++i) {
         memset(&outs[i], 0, sizeof(ga_result));
+        outs[i].status = GA_PENDING;
         outs[i].user_tag = reqs[i].user_tag;
         jobs[i].req = &reqs[i];
         jobs[i].out = &outs[i];
@@ -1230,7 +1301,6 @@ int run_checked(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, 
         if (v != GA_OK) {
             outs[i].status = v;
             jobs[i].finished = true;
-            worst = v;
             continue;
         }
         jobs[i].kind = kind_of(reqs[i]);
@@ -1244,24 +1314,38 @@ int run_checked(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, 
         }
     }
     std::vector<Job> live;
-    std::vector<uint32_t> live_idx;
     for (uint32_t i = 0; i < n; ++i)
-        if (!jobs[i].finished) {
-            live.push_back(jobs[i]);
-            live_idx.push_back(i);
-        }
+        if (!jobs[i].finished) live.push_back(jobs[i]);
+    int rc = GA_OK;
     try {
+        runner(live);
+    } catch (const GaError& err) {
+        if (e) e->last_error = err.what();
+        rc = err.code;
+    } catch (const std::bad_alloc&) {
+        if (e) e->last_error = "host memory exhausted";
+        rc = GA_E_NOMEM;
+    }
+    if (rc != GA_OK && e && !e->broken) e->abandon(rc);
+    for (uint32_t i = 0; i < n; ++i)
+        if (outs[i].status == GA_PENDING) {
+            if (rc == GA_OK) {  // a job the run loop did not finish: never expected, reported as a failure
+                rc = GA_E_HIP;
+                if (e) e->last_error = "a request was left unfinished by a completed run";
+            }
+            outs[i].status = rc;
+        }
+    return rc;
+}
+
+int run_engine(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, uint32_t n, ga_result* outs,
+               ga_serve_stats* ss) {
+    if (!e) return GA_E_INVALID;
+    return run_checked(e, reqs, n, outs, arrivals, [&](std::vector<Job>& live) {
+        if (e->broken) throw GaError(GA_E_HIP, "the engine was stopped by a HIP error in an earlier run");
         hip_check(hipSetDevice(e->device), "hipSetDevice");
         run_jobs(e, live, arrivals != nullptr, ss);
-    } catch (const GaError& err) {
-        e->last_error = err.what();
-        return err.code;
-    } catch (const std::bad_alloc&) {
-        return GA_E_NOMEM;
-    }
-    for (auto& j : live)
-        if (j.out->status != GA_OK && j.out->status != GA_NO_PATH) worst = j.out->status;
-    return worst;
+    });
 }
 
 // Random 64-bit global atomics at agent scope, the closed sets' operations:
@@ -1393,8 +1477,9 @@ void ga_engine_reset_stats(ga_engine* e) {
 }
 
 int ga_solve(ga_engine* e, const ga_request* req, ga_result* out) {
-    int rc = run_checked(e, req, nullptr, 1, out, nullptr);
-    return rc == GA_OK ? out->status : rc;
+    if (!e || !req || !out) return GA_E_INVALID;
+    (void)run_engine(e, req, nullptr, 1, out, nullptr);  // sets out->status in every case
+    return out->status;
 }
 
 int ga_solve_puzzle(ga_engine* e, uint32_t side, uint64_t start, uint64_t goal, uint32_t num_queues,
@@ -1423,14 +1508,14 @@ int ga_solve_grid(ga_engine* e, const uint8_t* cells, uint32_t width, uint32_t h
 }
 
 int ga_solve_batch(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* outs) {
-    return run_checked(e, reqs, nullptr, n, outs, nullptr);
+    return run_engine(e, reqs, nullptr, n, outs, nullptr);
 }
 
 int ga_serve_run(ga_engine* e, const ga_request* reqs, const uint64_t* arrival_ns, uint32_t n, ga_result* outs,
                  ga_serve_stats* stats) {
     if (!arrival_ns) return GA_E_INVALID;
     ga_serve_stats local{};
-    int rc = run_checked(e, reqs, arrival_ns, n, outs, stats ? stats : &local);
+    int rc = run_engine(e, reqs, arrival_ns, n, outs, stats ? stats : &local);
     return rc;
 }
 
@@ -1445,6 +1530,7 @@ const char* ga_status_str(int status) {
     switch (status) {
         case GA_OK: return "GA_OK";
         case GA_NO_PATH: return "GA_NO_PATH";
+        case GA_PENDING: return "GA_PENDING";
         case GA_E_INVALID: return "GA_E_INVALID";
         case GA_E_NOMEM: return "GA_E_NOMEM";
         case GA_E_BUDGET: return "GA_E_BUDGET";
@@ -1453,6 +1539,19 @@ const char* ga_status_str(int status) {
         case GA_E_OVERFLOW: return "GA_E_OVERFLOW";
         default: return "GA_E_UNKNOWN";
     }
+}
+
+const char* ga_engine_last_error(ga_engine* e) { return e ? e->last_error.c_str() : ""; }
+
+int ga_debug_fail_run(const ga_request* reqs, uint32_t n, ga_result* outs, uint32_t finish_first, int fail_code) {
+    if (fail_code >= 0) return GA_E_INVALID;
+    return run_checked(nullptr, reqs, n, outs, nullptr, [&](std::vector<Job>& live) {
+        for (uint32_t i = 0; i < live.size() && i < finish_first; ++i) {
+            live[i].out->status = GA_OK;
+            live[i].finished = true;
+        }
+        throw GaError(fail_code, "injected failure");
+    });
 }
 
 int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device_out) {

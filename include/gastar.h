@@ -38,6 +38,8 @@ extern "C" {
  *      helpers.h:46-50; "no path" is its empty vector, a_star.h:268-274) ---- */
 #define GA_OK 0
 #define GA_NO_PATH 1
+#define GA_PENDING 2       /* not finished yet: every result holds it while a run is in progress,
+                              and no result holds it once the run has returned */
 #define GA_E_INVALID (-1)  /* malformed request */
 #define GA_E_NOMEM (-2)    /* device memory exhausted */
 #define GA_E_BUDGET (-3)   /* request can never fit the admission budget */
@@ -88,15 +90,18 @@ typedef struct ga_result {
     uint64_t user_tag;
 } ga_result;
 
+/* Engine configuration.  An invalid value (heap_arity other than 0, 4 or 8,
+ * lanes above 4) makes ga_engine_create return GA_E_INVALID. */
 typedef struct ga_config {
     uint64_t budget_bytes;   /* admission budget for heap + closed-set bytes */
     uint64_t chunk_bytes;    /* device arena grows in chunks of this size */
     uint32_t max_instances;  /* instances resident at once */
     uint32_t chunk_iters;    /* GA* iterations launched between host checks */
-    uint32_t use_graph;      /* replay each chunk from a captured hipGraph */
+    uint32_t use_graph;      /* replay each chunk from a captured hipGraph (turned off when the
+                                process runs under rocprofv3, which sets ROCPROF_OUTPUT_PATH) */
     uint32_t max_retries;    /* capacity-overflow restarts per request */
     uint32_t max_iterations; /* per request; 0 = unlimited */
-    uint32_t heap_arity;     /* lanes per heap = heap arity: 4, 8 or 16 */
+    uint32_t heap_arity;     /* lanes per heap = heap arity: 4 or 8 (0 = 4) */
     uint32_t max_load_pct;   /* closed-set load factor limit, percent */
     uint32_t pops_per_iter;  /* heap pops tried per heap per iteration (stale skip) */
     uint32_t profile;        /* 0 off; 1 count work and time every step with HIP events;
@@ -158,6 +163,19 @@ void ga_engine_destroy(ga_engine* e);
 int ga_engine_get_stats(ga_engine* e, ga_engine_stats* out);
 void ga_engine_reset_stats(ga_engine* e);
 
+/* Return codes of the solve and serve entry points.  A run ends in one of
+ * two ways:
+ *   - it completes: every request has its final status in its ga_result
+ *     (GA_OK, GA_NO_PATH, or a per-request error such as GA_E_BUDGET), and
+ *     ga_solve_batch / ga_serve_run return GA_OK;
+ *   - it fails (a HIP error, device memory exhausted while the run holds
+ *     requests): the entry point returns that negative code, every request
+ *     that had not finished gets it as its status, and requests that had
+ *     finished keep theirs.  After a HIP error the engine refuses further
+ *     runs with GA_E_HIP; destroy it and create another.
+ * ga_solve returns its request's status in both cases (the run's error when
+ * the run failed). */
+
 /* Plain GA*, one request at a time.  Replaces a_star<>() (a_star.h:223-282)
  * as called by SlidingPuzzle::solve (a_star_sliding_puzzle.h:165-171) and
  * Pathfinding::solve (a_star_pathfinding.h:223-238). */
@@ -180,6 +198,15 @@ int ga_serve_run(ga_engine* e, const ga_request* reqs, const uint64_t* arrival_n
 
 void ga_result_free(ga_result* r);
 const char* ga_status_str(int status);
+/* Message of the last failed run or call on this engine ("" if none). */
+const char* ga_engine_last_error(ga_engine* e);
+
+/* Test support, no counterpart in the reference, no device call: applies the
+ * request checks of ga_solve_batch to reqs, finishes the first finish_first
+ * requests that pass them as completed (GA_OK, cost 0), then fails the run
+ * with fail_code (negative) as an error inside a run would, and returns what
+ * ga_solve_batch returns then. */
+int ga_debug_fail_run(const ga_request* reqs, uint32_t n, ga_result* outs, uint32_t finish_first, int fail_code);
 
 /* Device memory helpers for callers that keep grids resident in HBM. */
 int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device_out);

@@ -5,9 +5,12 @@
 //   NeighbourGenerator<T>: writes successor i of `source` into `target`, sets
 //                          `target_exists` and the move `cost` (i in [0, K)).
 //   Heuristic<T>:          admissible lower bound on the remaining cost.
-//   Hash<T>:               optional 64-bit hash of a state (the reference
-//                          declares it but hard-wires MurmurHash3,
-//                          helpers.h:10-35; here it is used when given).
+//   Hash<T>:               declared for source compatibility only.  Neither
+//                          the reference's a_star<> nor this one takes a
+//                          hash parameter: the reference hard-wires
+//                          MurmurHash3 (helpers.h:10-35), this engine hashes
+//                          a state's bytes with its own 64-bit mix
+//                          (gastar::hash_words, gastar::mix64).
 #ifndef GASTAR_FUNCTIONAL_TYPES_H
 #define GASTAR_FUNCTIONAL_TYPES_H
 

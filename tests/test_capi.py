@@ -184,3 +184,70 @@ def test_puzzle5_request_carries_both_boards():
     assert tuple(raw[:25]) == b and tuple(raw[25:]) == ga.board_goal(5)
     arr = (ga.Request * 1)(r)  # the batch array copies the pointer, the buffer stays on r
     assert ctypes.string_at(arr[0].cells, 50) == raw
+
+
+def _fail_run(reqs, finish_first, code):
+    n = len(reqs)
+    arr = (ga.Request * n)(*reqs)
+    outs = (ga.Result * n)()
+    rc = ga.lib().ga_debug_fail_run(arr, n, outs, finish_first, code)
+    return rc, [outs[i].status for i in range(n)]
+
+
+def _mixed_requests():
+    good = [ga.puzzle_request(4, ga.gen_puzzle(4, 10, s)) for s in range(4)]
+    bad = ga.puzzle_request(4, 0x1111111111111111)        # not a permutation: GA_E_INVALID
+    g = ga.puzzle_goal(4)
+    t = ga.state_to_board(g, 4)
+    t[0], t[1] = t[1], t[0]
+    unsolvable = ga.puzzle_request(4, ga.board_to_state(t))  # odd permutation: GA_NO_PATH
+    return [good[0], bad, good[1], unsolvable, good[2], good[3]]
+
+
+def test_failed_run_leaves_no_result_reading_as_success():
+    # A run that fails part-way (here: an injected HIP error after two requests
+    # finished) returns the error, and every request that had not finished
+    # carries it: none is left GA_OK with a cost of 0, none stays GA_PENDING.
+    rc, st = _fail_run(_mixed_requests(), 2, ga.E_HIP)
+    assert rc == ga.E_HIP
+    assert st == [ga.OK, ga.E_INVALID, ga.OK, ga.NO_PATH, ga.E_HIP, ga.E_HIP]
+    rc, st = _fail_run(_mixed_requests(), 0, ga.E_NOMEM)
+    assert rc == ga.E_NOMEM and st == [ga.E_NOMEM, ga.E_INVALID, ga.E_NOMEM, ga.NO_PATH, ga.E_NOMEM, ga.E_NOMEM]
+    assert ga.PENDING not in st
+    assert ga.status_str(ga.PENDING) == "GA_PENDING"
+    assert ga.lib().ga_debug_fail_run(None, 0, None, 0, 0) == ga.E_INVALID  # the injected code must be an error
+
+
+def test_python_wrappers_raise_on_a_failed_run(monkeypatch):
+    # Engine.solve / solve_batch raise GAStarError when the C run fails, with
+    # the per-request results attached (the C entry point is replaced by the
+    # fault-injecting one; no device is touched).
+    L = ga.lib()
+
+    def failing_batch(h, arr, n, outs):
+        return L.ga_debug_fail_run(arr, n, outs, 1, ga.E_HIP)
+
+    monkeypatch.setattr(L, "ga_solve_batch", failing_batch)
+    e = object.__new__(ga.Engine)
+    e._h = None
+    reqs = _mixed_requests()
+    with pytest.raises(ga.GAStarError) as ei:
+        e.solve_batch(reqs)
+    assert ei.value.code == ga.E_HIP
+    assert [s.status for s in ei.value.results] == [ga.OK, ga.E_INVALID, ga.E_HIP, ga.NO_PATH, ga.E_HIP, ga.E_HIP]
+    with pytest.raises(ga.GAStarError) as ei:
+        e.solve(reqs[0])
+    assert ei.value.code == ga.E_HIP and ei.value.results[0].status == ga.OK
+    e._h = None  # nothing to destroy
+
+
+def test_engine_create_rejects_invalid_config_before_any_device_call():
+    cfg = ga.default_config()
+    h = ctypes.c_void_p()
+    for field, value in (("heap_arity", 16), ("heap_arity", 5), ("lanes", 5)):
+        c = ga.default_config()
+        setattr(c, field, value)
+        assert ga.lib().ga_engine_create(0, ctypes.byref(c), ctypes.byref(h)) == ga.E_INVALID, field
+        assert not h.value
+    assert cfg.heap_arity == 4
+    assert ga.lib().ga_engine_last_error(None) == b""

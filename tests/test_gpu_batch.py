@@ -180,13 +180,13 @@ def test_serve_request_over_budget_is_rejected():
 VARIANTS = [
     dict(heap_arity=4, rounds=1, use_graph=1, pops_per_iter=1),
     dict(heap_arity=8, rounds=1, use_graph=0, pops_per_iter=2),
-    dict(heap_arity=16, rounds=2, use_graph=0, pops_per_iter=1),
+    dict(heap_arity=8, rounds=2, use_graph=0, pops_per_iter=1, chunk_iters=5),
     dict(heap_arity=4, rounds=4, use_graph=0, pops_per_iter=2),
     dict(heap_arity=8, rounds=8, use_graph=1, pops_per_iter=4),
     dict(heap_arity=4, rounds=2, use_graph=1, pops_per_iter=2),
     dict(heap_arity=4, rounds=1, pops_per_iter=2, adaptive_start=16, grow_thresh=2),
     dict(heap_arity=8, rounds=2, use_graph=1, adaptive_start=64, grow_thresh=4),
-    dict(heap_arity=16, rounds=1, adaptive_start=1, grow_thresh=1),
+    dict(heap_arity=8, rounds=1, adaptive_start=1, grow_thresh=1),
     dict(heap_arity=8, rounds=2, adaptive_start=256, grow_thresh=32),
     dict(heap_arity=4, rounds=2, rounds_grid=16),
     dict(heap_arity=8, rounds=1, rounds_grid=8, adaptive_start=32, grow_thresh=8),

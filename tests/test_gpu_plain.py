@@ -167,3 +167,34 @@ def test_config3_grid4096_matches_golden(engine):
     assert sol.status == ga.OK, ga.status_str(sol.status)
     assert sol.cost == g["cost"]
     assert oracle.replay_grid(x.cells, 4096, 4096, 4, sol.path, x.start, x.goal) == sol.cost
+
+
+def test_c_abi_solve_entry_points_directly(engine):
+    # ga_solve / ga_solve_puzzle / ga_solve_grid through ctypes (the Python
+    # wrapper runs through ga_solve_batch): the return code is the request's
+    # status, the result holds the cost and a path that replays
+    import ctypes
+
+    L = ga.lib()
+    start = ga.gen_puzzle(4, 40, 11)
+    res = ga.Result()
+    rc = L.ga_solve_puzzle(engine._h, 4, start, ga.puzzle_goal(4), 1024, ctypes.byref(res))
+    _, ref, _ = oracle.astar_puzzle(4, start, ga.puzzle_goal(4), want_path=False)
+    assert rc == ga.OK == res.status and res.cost == ref.cost
+    path = [res.path[i] for i in range(res.path_len)]
+    L.ga_result_free(ctypes.byref(res))
+    assert oracle.replay_puzzle(4, path, start, ga.puzzle_goal(4)) == ref.cost
+    cells, _ = ga.gen_grid(300, 200, 0.2, 5)
+    rc = L.ga_solve_grid(engine._h, cells.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8)), 300, 200, 0,
+                         300 * 200 - 1, 4, 512, ctypes.byref(res))
+    assert rc == ga.OK == res.status and res.cost == oracle.dial_grid(cells, 300, 200, 0, 300 * 200 - 1, 4)
+    path = [res.path[i] for i in range(res.path_len)]
+    L.ga_result_free(ctypes.byref(res))
+    assert oracle.replay_grid(cells, 300, 200, 4, path, 0, 300 * 200 - 1) == res.cost
+    # a per-request failure comes back as the request's status, not as a failed run
+    bad = ga.puzzle_request(4, 0x1111111111111111)
+    assert L.ga_solve(engine._h, ctypes.byref(bad), ctypes.byref(res)) == ga.E_INVALID == res.status
+    t = ga.state_to_board(ga.puzzle_goal(4), 4)
+    t[0], t[1] = t[1], t[0]
+    assert L.ga_solve_puzzle(engine._h, 4, ga.board_to_state(t), ga.puzzle_goal(4), 64,
+                             ctypes.byref(res)) == ga.NO_PATH
