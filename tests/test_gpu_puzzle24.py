@@ -81,9 +81,9 @@ def test_forced_retries():
         check(b, sol)
 
 
-@pytest.mark.parametrize("variant", [dict(heap_arity=8, rounds=2, use_graph=1), dict(heap_arity=16, rounds=1),
+@pytest.mark.parametrize("variant", [dict(heap_arity=8, rounds=2, use_graph=1), dict(heap_arity=8, rounds=1, chunk_iters=7),
                                      dict(heap_arity=4, rounds=4, slack_puzzle=3, lanes=2)],
-                         ids=["arity8-graph", "arity16", "arity4-slack-lanes2"])
+                         ids=["arity8-graph", "arity8-chunk7", "arity4-slack-lanes2"])
 def test_variants(variant):
     boards = [ga.gen_board(5, 28 + 4 * i, 300 + i) for i in range(6)]
     with ga.Engine(0, max_iterations=2_000_000, **variant) as e:
