@@ -134,7 +134,8 @@ class ServeStats(ctypes.Structure):
 EXPORTS = (
     "ga_config_default", "ga_engine_create", "ga_engine_destroy", "ga_engine_get_stats",
     "ga_engine_reset_stats", "ga_solve", "ga_solve_puzzle", "ga_solve_grid", "ga_solve_batch",
-    "ga_serve_run", "ga_result_free", "ga_status_str", "ga_engine_last_error", "ga_debug_fail_run",
+    "ga_serve_run", "ga_solve_noalloc", "ga_result_free", "ga_status_str", "ga_engine_last_error",
+    "ga_debug_fail_run",
     "ga_device_upload", "ga_device_free", "ga_atomic_rate",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
     "ga_gen_board", "ga_board_solvable",
@@ -166,6 +167,7 @@ def lib() -> ctypes.CDLL:
                                 ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, P(Result)]
     L.ga_solve_batch.argtypes = [ctypes.c_void_p, P(Request), ctypes.c_uint32, P(Result)]
     L.ga_serve_run.argtypes = [ctypes.c_void_p, P(Request), _u64p, ctypes.c_uint32, P(Result), P(ServeStats)]
+    L.ga_solve_noalloc.argtypes = [ctypes.c_void_p, P(Request), P(Result), _u64p, ctypes.c_uint32]
     L.ga_result_free.argtypes = [P(Result)]
     L.ga_result_free.restype = None
     L.ga_status_str.argtypes = [ctypes.c_int]
@@ -428,6 +430,22 @@ class Engine:
                    connectivity: int = 4, num_queues: int = 0, **kw) -> Solution:
         cells = np.ascontiguousarray(cells, dtype=np.uint8).reshape(-1)
         return self.solve(grid_request(cells, width, height, start, goal, connectivity, num_queues, **kw))
+
+    def solve_noalloc(self, req: Request, path_cap: int = 1 << 16) -> Solution:
+        """Plain GA* of one request with every device buffer in the library's
+        static scratch (ga_solve_noalloc): no device allocation."""
+        words = 4 if _board_side(req) else 1
+        buf = (ctypes.c_uint64 * (path_cap * words))()
+        res = Result()
+        lib().ga_solve_noalloc(self._h, ctypes.byref(req), ctypes.byref(res), buf, path_cap)
+        n = res.path_len if res.status == OK else 0
+        if _board_side(req):
+            raw = ctypes.string_at(buf, 32 * n)
+            path = [tuple(raw[32 * k:32 * k + 25]) for k in range(n)]
+        else:
+            path = list(buf[:n])
+        return Solution(res.status, res.cost, path, res.expanded, res.stored, res.iterations, res.retries,
+                        res.search_ns, res.latency_ns, res.user_tag)
 
     # -- batched and served forms --
     def solve_batch(self, reqs: Sequence[Request]) -> List[Solution]:

@@ -18,6 +18,7 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
@@ -156,6 +157,32 @@ __global__ __launch_bounds__(64) void path_kernel(const InstDesc* descs, const P
         }
     }
     if (threadIdx.x == 0) lens[i] = len;
+}
+
+// Launch map of a single resident instance (slot 0) with n_heaps heaps.
+__global__ void map_kernel(WgEntry* map, uint32_t n_wg, uint32_t heaps_per_wg, uint32_t n_heaps) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_wg) map[i] = WgEntry{0u, i * heaps_per_wg, n_heaps, 0u};
+}
+
+// The walked path (states goal..start in path_kernel's output, `wide`: two
+// words of Board5 per state) in start..goal order; 24-puzzle boards are
+// unpacked into 32 bytes each (25 tile bytes, 7 zero bytes), the layout of
+// ga_result paths.
+__global__ void path_order_kernel(const uint64_t* src, const uint32_t* lens, uint64_t* dst, uint32_t wide) {
+    const uint32_t n = lens[0];
+    if (n == kNone) return;
+    for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const uint32_t from = n - 1 - k;
+        if (!wide) {
+            dst[k] = src[from];
+            continue;
+        }
+        const Board5 x{src[2 * (size_t)from], src[2 * (size_t)from + 1]};
+        uint64_t w[4] = {0, 0, 0, 0};
+        for (int c = 0; c < 25; ++c) w[c >> 3] |= (uint64_t)Puzzle5Ops::get(x, c) << (8 * (c & 7));
+        for (int q = 0; q < 4; ++q) dst[4 * (size_t)k + q] = w[q];
+    }
 }
 
 __global__ void abort_kernel(InstState* states, uint32_t slot) {
@@ -392,6 +419,40 @@ void launch_step(uint32_t arity, uint32_t variant, hipStream_t s, uint32_t grid,
 
 }  // namespace
 
+namespace {
+
+// ---------------------------------------------------------------------------
+// scratch of the no-allocation entry point (ga_solve_noalloc)
+// ---------------------------------------------------------------------------
+// All device memory that ga_solve_noalloc's kernels use beyond the caller's
+// inputs and outputs -- instance descriptor and state, launch map, iteration
+// counter, heaps, candidate records, closed set, a grid's cells, the path
+// walk's buffers -- is carved out of this array.  Each GPU the library runs on
+// holds its own copy until the process ends.  Its host-side name is not a
+// device address: host code uses the address hipGetSymbolAddress returns for
+// the current device (scratch_address), never the name.
+constexpr uint64_t kScratchBytes = 256ull << 20;
+static __device__ __align__(256) uint64_t scratch[kScratchBytes / sizeof(uint64_t)];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `device`'s scratch; `device` must be current.
+char* scratch_address(int device) {
+    if (device < 0 || device >= kMaxDevices) throw GaError(GA_E_INVALID, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        if (!p) throw GaError(GA_E_HIP, "hipGetSymbolAddress(scratch) returned no address");
+        g_scratch_ptr[device] = (char*)p;
+    }
+    return g_scratch_ptr[device];
+}
+
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // engine
 // ---------------------------------------------------------------------------
@@ -443,6 +504,7 @@ struct ga_engine {
     double algo_sampled = 0, algo_always = 0;  // algorithmic bytes: profiled-launch terms, all-launch terms
     std::string last_error;
     bool broken = false;  // a HIP error ended a run: the device state is unknown, later runs are refused
+    char* scratch_dev = nullptr;  // this device's scratch (ga_solve_noalloc)
 
     void init(int dev, const ga_config* c) {
         device = dev;
@@ -469,6 +531,7 @@ struct ga_engine {
         n_lanes = cfg.lanes + 1;
         if (cfg.max_instances < n_lanes) cfg.max_instances = n_lanes;
         hip_check(hipSetDevice(device), "hipSetDevice");
+        scratch_dev = scratch_address(device);
         hip_check(hipStreamCreateWithFlags(&path_stream, hipStreamNonBlocking), "hipStreamCreate");
         arena = new Arena(cfg.chunk_bytes);
         hip_check(hipMalloc(&d_descs, sizeof(InstDesc) * cfg.max_instances), "hipMalloc descs");
@@ -577,6 +640,93 @@ struct ga_engine {
     }
 
     // ---- admission ------------------------------------------------------------
+    // Takes the device buffers of one instance from `take` (the arena, or the
+    // no-allocation scratch) and fills its descriptor; returns false when
+    // `take` returns nullptr for one of them.
+    template <class Take>
+    bool carve_instance(const Job& j, const Plan& p, Take take, InstDesc& d) const {
+        const ga_request& r = *j.req;
+        const int K = branching_of(j.kind);
+        d = InstDesc{};
+        d.kind = (uint32_t)j.kind;
+        d.start = r.start;
+        d.goal = r.goal;
+        d.heaps = (HeapEntry*)take((uint64_t)p.n_heaps * p.heap_stride * sizeof(HeapEntry));
+        d.heap_size = (uint32_t*)take((uint64_t)p.n_heaps * sizeof(uint32_t));
+        d.cand = (HeapEntry*)take(2ull * p.n_heaps * K * sizeof(HeapEntry));
+        if (!d.heaps || !d.heap_size || !d.cand) return false;
+        if (r.kind == GA_PUZZLE) {
+            if (is_wide(j.kind)) {
+                d.table = (Slot*)take(p.table_slots * sizeof(Board5KeySet::PairSlot));
+                d.user = take(2 * sizeof(Puzzle5Ops::Key));
+                if (!d.user) return false;
+            } else {
+                d.table = (Slot*)take(p.table_slots * sizeof(Slot));
+            }
+            if (!d.table) return false;
+        } else {
+            uint64_t cells = (uint64_t)r.width * r.height;
+            d.dense = (uint64_t*)take(cells * sizeof(uint64_t));
+            if (!d.dense) return false;
+            if (!r.cells_device) {
+                d.cells = (const uint8_t*)take(cells);
+                if (!d.cells) return false;
+            }
+        }
+        d.n_heaps = p.n_heaps;
+        d.heap_cap = p.heap_cap;
+        d.heap_stride = p.heap_stride;
+        d.a0 = cfg.adaptive_start ? (uint32_t)std::min<uint64_t>(p.n_heaps, pow2_floor(cfg.adaptive_start))
+                                  : p.n_heaps;
+        d.rounds = (r.kind == GA_GRID && cfg.rounds_grid) ? cfg.rounds_grid : cfg.rounds;
+        {
+            const uint32_t slack = r.kind == GA_GRID ? cfg.slack_grid : cfg.slack_puzzle;
+            d.f_slack = slack ? slack - 1 : kInf;  // 0 = no gating; k = slack k-1
+        }
+        if (r.kind == GA_PUZZLE) {
+            d.width = r.side;
+            d.height = r.side;
+            d.table_mask = p.table_slots - 1;
+            d.load_limit = p.table_slots * cfg.max_load_pct / 100;
+            for (uint32_t c = 0; c < r.side * r.side; ++c) {
+                const uint32_t t = r.side == 5 ? r.cells[25 + c] : (uint32_t)(r.goal >> (4 * c)) & 15u;
+                d.goal_rc[t] = (uint8_t)(((c / r.side) << 4) | (c % r.side));
+            }
+            if (is_wide(j.kind)) d.start = d.goal = 0;  // the keys are in d.user
+        } else {
+            d.width = r.width;
+            d.height = r.height;
+            d.w_min = j.w_min;
+            if (r.cells_device) d.cells = r.cells_device;
+        }
+        return true;
+    }
+
+    // Clears an instance's closed set and heap sizes and uploads its problem
+    // data on `stream` (the descriptor itself is written by seed_kernel).
+    // j.ends must stay in place until the stream has run the copy.
+    void init_instance(Job& j, const Plan& p, const InstDesc& d, hipStream_t stream) {
+        const ga_request& r = *j.req;
+        const uint32_t fill_grid = 1024;
+        if (r.kind == GA_PUZZLE) {
+            if (is_wide(j.kind)) {  // start and goal keys for the seed, the goal test and the path walk
+                j.ends[0] = pack_board5(r.cells);
+                j.ends[1] = pack_board5(r.cells + 25);
+                hip_check(hipMemcpyAsync((void*)d.user, j.ends, sizeof(j.ends), hipMemcpyHostToDevice, stream),
+                          "upload ends");
+            }
+            const uint64_t words = p.table_slots * (is_wide(j.kind) ? sizeof(Board5KeySet::PairSlot) : sizeof(Slot)) / 8;
+            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>((uint64_t*)d.table, words, ~0ull);
+        } else {
+            uint64_t cells = (uint64_t)r.width * r.height;
+            if (!r.cells_device)
+                hip_check(hipMemcpyAsync((void*)d.cells, r.cells, cells, hipMemcpyHostToDevice, stream), "upload cells");
+            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>(d.dense, cells, ~0ull);
+        }
+        fill_kernel<uint32_t><<<std::min<uint32_t>(fill_grid, (p.n_heaps + kBlock - 1) / kBlock), kBlock, 0,
+                                stream>>>(d.heap_size, p.n_heaps, 0u);
+    }
+
     // Returns 1 if admitted, 0 if it must wait (no slot / budget), or throws.
     int admit(Job& j) {
         const uint32_t lane = lane_of(j);
@@ -589,7 +739,6 @@ struct ga_engine {
         if (need > cfg.budget_bytes) throw GaError(GA_E_BUDGET, "request exceeds the admission budget");
         if (arena->in_use() + need > cfg.budget_bytes) return 0;
         const int W = (int)cfg.heap_arity;
-        const int K = branching_of(j.kind);
         std::vector<void*> allocs;
         bool others_resident = false;
         for (Job* o : resident)
@@ -609,70 +758,9 @@ struct ga_engine {
             return ptr;
         };
         InstDesc d{};
-        d.kind = (uint32_t)j.kind;
-        d.start = r.start;
-        d.goal = r.goal;
-        d.heaps = (HeapEntry*)take((uint64_t)p.n_heaps * p.heap_stride * sizeof(HeapEntry));
-        d.heap_size = (uint32_t*)take((uint64_t)p.n_heaps * sizeof(uint32_t));
-        d.cand = (HeapEntry*)take(2ull * p.n_heaps * K * sizeof(HeapEntry));
-        if (r.kind == GA_PUZZLE) {
-            if (is_wide(j.kind)) {
-                d.table = (Slot*)take(p.table_slots * sizeof(Board5KeySet::PairSlot));
-                d.user = take(2 * sizeof(Puzzle5Ops::Key));
-            } else {
-                d.table = (Slot*)take(p.table_slots * sizeof(Slot));
-            }
-        } else {
-            uint64_t cells = (uint64_t)r.width * r.height;
-            d.dense = (uint64_t*)take(cells * sizeof(uint64_t));
-            if (!r.cells_device) d.cells = (const uint8_t*)take(cells);
-        }
-        if (short_of_memory) return 0;
-        d.n_heaps = p.n_heaps;
-        d.heap_cap = p.heap_cap;
-        d.heap_stride = p.heap_stride;
-        d.a0 = cfg.adaptive_start ? (uint32_t)std::min<uint64_t>(p.n_heaps, pow2_floor(cfg.adaptive_start))
-                                  : p.n_heaps;
-        d.rounds = (r.kind == GA_GRID && cfg.rounds_grid) ? cfg.rounds_grid : cfg.rounds;
-        {
-            const uint32_t slack = r.kind == GA_GRID ? cfg.slack_grid : cfg.slack_puzzle;
-            d.f_slack = slack ? slack - 1 : kInf;  // 0 = no gating; k = slack k-1
-        }
-        const uint32_t fill_grid = 1024;
+        if (!carve_instance(j, p, take, d)) return 0;
         hipStream_t stream = L.stream;
-        if (r.kind == GA_PUZZLE) {
-            d.width = r.side;
-            d.height = r.side;
-            d.table_mask = p.table_slots - 1;
-            d.load_limit = p.table_slots * cfg.max_load_pct / 100;
-            for (uint32_t c = 0; c < r.side * r.side; ++c) {
-                const uint32_t t = r.side == 5 ? r.cells[25 + c] : (uint32_t)(r.goal >> (4 * c)) & 15u;
-                d.goal_rc[t] = (uint8_t)(((c / r.side) << 4) | (c % r.side));
-            }
-            if (is_wide(j.kind)) {  // start and goal keys for the seed, the goal test and the path walk
-                j.ends[0] = pack_board5(r.cells);
-                j.ends[1] = pack_board5(r.cells + 25);
-                d.start = d.goal = 0;
-                hip_check(hipMemcpyAsync((void*)d.user, j.ends, sizeof(j.ends), hipMemcpyHostToDevice, stream),
-                          "upload ends");
-            }
-            const uint64_t words = p.table_slots * (is_wide(j.kind) ? sizeof(Board5KeySet::PairSlot) : sizeof(Slot)) / 8;
-            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>((uint64_t*)d.table, words, ~0ull);
-        } else {
-            d.width = r.width;
-            d.height = r.height;
-            d.w_min = j.w_min;
-            uint64_t cells = (uint64_t)r.width * r.height;
-            if (r.cells_device) {
-                d.cells = r.cells_device;
-            } else {
-                hip_check(hipMemcpyAsync((void*)d.cells, r.cells, cells, hipMemcpyHostToDevice, stream),
-                          "upload cells");
-            }
-            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>(d.dense, cells, ~0ull);
-        }
-        fill_kernel<uint32_t><<<std::min<uint32_t>(fill_grid, (p.n_heaps + kBlock - 1) / kBlock), kBlock, 0,
-                                stream>>>(d.heap_size, p.n_heaps, 0u);
+        init_instance(j, p, d, stream);
         int slot = L.free_slots.back();
         L.free_slots.pop_back();
         d.start_iter = (uint32_t)L.host_base;
@@ -1415,6 +1503,129 @@ int atomic_rate(ga_engine* e, uint64_t table_bytes, uint64_t ops, int kind, doub
     return rc;
 }
 
+// ---------------------------------------------------------------------------
+// plain GA* of one request with no device allocation (ga_solve_noalloc)
+// ---------------------------------------------------------------------------
+// Bump allocation inside this device's scratch, 256-byte aligned.
+struct ScratchCarve {
+    char* base;
+    uint64_t off = 0;
+    void* operator()(uint64_t bytes) {
+        off = round_up(off, 256);
+        if (off > kScratchBytes || bytes > kScratchBytes - off) return nullptr;
+        void* p = base + off;
+        off += bytes;
+        return p;
+    }
+};
+
+// The search of ga_solve on one request, with every device buffer in the
+// scratch array (nothing allocated: no hipMalloc/hipFree, no event, stream or
+// pinned buffer created) on lane 0's stream, one chunk at a time.  Retries
+// after a capacity overflow as ga_solve does, as long as the larger
+// structures fit the scratch.  Throws GaError.
+void solve_noalloc(ga_engine* e, const ga_request& r, ga_result* out, uint64_t* path_buf, uint32_t path_cap) {
+    Job j;
+    j.req = &r;
+    j.out = out;
+    j.kind = kind_of(r);
+    if (r.kind == GA_GRID) j.w_min = grid_wmin(r);
+    hip_check(hipSetDevice(e->device), "hipSetDevice");
+    hipStream_t st = e->lanes[0].stream;
+    // the pinned read-back buffer of lane 0, allocated when the engine was created
+    InstState* snap = (InstState*)e->lanes[0].h_states[0].p;
+    const uint32_t W = e->cfg.heap_arity, R = e->cfg.chunk_iters;
+    const uint32_t variant = is_wide(j.kind) ? kWideVariant
+                             : j.kind == KIND_GRID8 ? kAllKinds
+                                                    : kind_bit((uint32_t)j.kind);
+    const uint64_t t0 = now_ns();
+    for (;;) {
+        const ga_engine::Plan p = e->plan(j);
+        ScratchCarve take{e->scratch_dev};
+        InstDesc* d_desc = (InstDesc*)take(sizeof(InstDesc));
+        InstState* d_state = (InstState*)take(sizeof(InstState));
+        uint32_t* d_iter = (uint32_t*)take(sizeof(uint32_t));
+        const uint32_t heaps_per_wg = kBlock / W;
+        const uint32_t n_wg = (p.n_heaps + heaps_per_wg - 1) / heaps_per_wg;
+        WgEntry* d_map = (WgEntry*)take((uint64_t)n_wg * sizeof(WgEntry));
+        InstDesc d{};
+        if (!d_map || !e->carve_instance(j, p, take, d))
+            throw GaError(GA_E_BUDGET, "request needs more device memory than the " +
+                                           std::to_string(kScratchBytes >> 20) + " MiB no-allocation scratch");
+        e->init_instance(j, p, d, st);
+        fill_kernel<uint32_t><<<1, 64, 0, st>>>(d_iter, 1, 0u);
+        map_kernel<<<(n_wg + 255) / 256, 256, 0, st>>>(d_map, n_wg, heaps_per_wg, p.n_heaps);
+        d.start_iter = 0;
+        seed_kernel<<<1, 64, 0, st>>>(d, d_desc, d_state, 0u, (int)W);
+        hip_check(hipGetLastError(), "seed launch");
+        const StepParams prm{e->cfg.pops_per_iter, 1u << 16, 0u, e->cfg.rounds, e->cfg.grow_thresh, {0, 0, 0}};
+        for (;;) {
+            for (uint32_t k = 0; k < R; ++k) launch_step(W, variant, st, n_wg, d_map, d_desc, d_state, d_iter, k, prm);
+            advance_kernel<<<1, 64, 0, st>>>(d_iter, R);
+            hip_check(hipGetLastError(), "step launch");
+            hip_check(hipMemcpyAsync(snap, d_state, sizeof(InstState), hipMemcpyDeviceToHost, st), "read back state");
+            hip_check(hipStreamSynchronize(st), "chunk");
+            if (snap->status != ST_RUNNING) break;
+            if (e->cfg.max_iterations && snap->iterations > e->cfg.max_iterations) {
+                abort_kernel<<<1, 64, 0, st>>>(d_state, 0u);
+                hip_check(hipGetLastError(), "abort launch");
+            }
+        }
+        const InstState s = *snap;
+        out->expanded = s.expanded;
+        out->stored = (j.kind == KIND_GRID4 || j.kind == KIND_GRID8) ? 0 : s.occupied;
+        out->iterations = s.iterations;
+        out->retries = j.retries;
+        if (s.status == ST_OVERFLOW_HEAP || s.status == ST_OVERFLOW_TABLE) {
+            if (j.retries >= e->cfg.max_retries) throw GaError(GA_E_OVERFLOW, "capacity exceeded after all retries");
+            ++j.retries;
+            if (s.status == ST_OVERFLOW_HEAP) j.heap_shift += 2;
+            else j.table_shift += 2;
+            continue;  // the larger structures must still fit the scratch (else GA_E_BUDGET)
+        }
+        out->search_ns = now_ns() - t0;
+        if (s.status == ST_ABORTED) throw GaError(GA_E_TIMEOUT, "iteration limit reached");
+        if (s.status != ST_DONE) throw GaError(GA_E_HIP, "unexpected instance status");
+        if (s.best == kInf) {
+            out->status = GA_NO_PATH;
+            return;
+        }
+        out->cost = s.best;
+        // walk the parent links into the scratch, then order (and unpack) the states
+        const uint32_t cap = s.best + 1;  // every move costs >= 1
+        const uint32_t words = words_of(j.kind), out_words = is_wide(j.kind) ? 4u : 1u;
+        PathJob* d_job = (PathJob*)take(sizeof(PathJob));
+        uint32_t* d_len = (uint32_t*)take(sizeof(uint32_t));
+        uint64_t* d_walk = (uint64_t*)take((uint64_t)cap * words * sizeof(uint64_t));
+        uint64_t* d_path = (uint64_t*)take((uint64_t)cap * out_words * sizeof(uint64_t));
+        if (!d_job || !d_len || !d_walk || !d_path)
+            throw GaError(GA_E_BUDGET, "the path does not fit the no-allocation scratch");
+        const PathJob job{0u, cap, 0u};
+        hip_check(hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, st), "upload path job");
+        path_kernel<<<1, 64, 0, st>>>(d_desc, d_job, 1u, d_walk, d_len);
+        path_order_kernel<<<std::max<uint32_t>(1, std::min<uint32_t>(1024, (cap + 255) / 256)), 256, 0, st>>>(
+            d_walk, d_len, d_path, is_wide(j.kind) ? 1u : 0u);
+        hip_check(hipGetLastError(), "path launch");
+        uint32_t* h_len = reinterpret_cast<uint32_t*>(snap);  // the pinned buffer again
+        hip_check(hipMemcpyAsync(h_len, d_len, sizeof(uint32_t), hipMemcpyDeviceToHost, st), "read back length");
+        hip_check(hipStreamSynchronize(st), "path");
+        const uint32_t len = *h_len;
+        if (len == kNone || len == 0) throw GaError(GA_E_OVERFLOW, "path reconstruction failed");
+        out->path_len = len;
+        if (len > path_cap) {  // the caller's buffer is too small: path_len says how many states are needed
+            out->status = GA_E_INVALID;
+            e->last_error = "path buffer too small: " + std::to_string(len) + " states needed";
+            return;
+        }
+        hip_check(hipMemcpyAsync(path_buf, d_path, (uint64_t)len * out_words * sizeof(uint64_t),
+                                 hipMemcpyDeviceToHost, st),
+                  "read back path");
+        hip_check(hipStreamSynchronize(st), "path copy");
+        out->status = GA_OK;
+        return;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1539,6 +1750,33 @@ const char* ga_status_str(int status) {
         case GA_E_OVERFLOW: return "GA_E_OVERFLOW";
         default: return "GA_E_UNKNOWN";
     }
+}
+
+int ga_solve_noalloc(ga_engine* e, const ga_request* req, ga_result* out, uint64_t* path_buf, uint32_t path_cap) {
+    if (!e || !req || !out || (!path_buf && path_cap)) return GA_E_INVALID;
+    memset(out, 0, sizeof(*out));
+    out->status = GA_PENDING;
+    out->user_tag = req->user_tag;
+    int v = validate(*req);
+    if (v != GA_OK) return out->status = v;
+    const bool solvable = req->kind != GA_PUZZLE ||
+                          (req->side == 5 ? ga_board_solvable(5, req->cells, req->cells + 25)
+                                          : ga_puzzle_solvable(req->side, req->start, req->goal));
+    if (!solvable) return out->status = GA_NO_PATH;
+    if (e->broken) {
+        e->last_error = "the engine was stopped by a HIP error in an earlier run";
+        return out->status = GA_E_HIP;
+    }
+    try {
+        solve_noalloc(e, *req, out, path_buf, path_cap);
+    } catch (const GaError& err) {
+        e->last_error = err.what();
+        out->status = err.code;
+        // nothing of this call may stay in flight on the scratch
+        if (hipStreamSynchronize(e->lanes[0].stream) != hipSuccess || err.code == GA_E_HIP) e->broken = true;
+        (void)hipGetLastError();
+    }
+    return out->status;
 }
 
 const char* ga_engine_last_error(ga_engine* e) { return e ? e->last_error.c_str() : ""; }

@@ -196,6 +196,25 @@ int ga_solve_batch(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* 
 int ga_serve_run(ga_engine* e, const ga_request* reqs, const uint64_t* arrival_ns, uint32_t n,
                  ga_result* outs, ga_serve_stats* stats);
 
+/* Plain GA* of one request with no device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, code that must not call hipMalloc).  The same search as ga_solve,
+ * with every device buffer its kernels use (descriptor, state, launch map,
+ * heaps, candidate records, closed set, a grid's cells, the path walk) taken
+ * from one static array of 256 MiB per GPU, which the library's code object
+ * holds on each GPU it runs on until the process ends; it calls no hipMalloc,
+ * hipFree, hipHostMalloc, event or stream creation.  A request whose
+ * structures (or path) do not fit 256 MiB gets GA_E_BUDGET.  The path is
+ * written start..goal into the caller's host buffer path_buf (path_cap
+ * states; 4 words per state for side-5 puzzles, else 1); out->path stays
+ * NULL and out->path_len counts the states.  If the path is longer than
+ * path_cap the status is GA_E_INVALID, out->cost holds the cost and
+ * out->path_len the states needed.  Every call on a GPU shares that GPU's
+ * array, which keeps the last call's contents (each call clears what it
+ * needs): calls that overlap on one GPU race, from any engine or thread.
+ * Returns the request's status. */
+int ga_solve_noalloc(ga_engine* e, const ga_request* req, ga_result* out, uint64_t* path_buf, uint32_t path_cap);
+
 void ga_result_free(ga_result* r);
 const char* ga_status_str(int status);
 /* Message of the last failed run or call on this engine ("" if none). */

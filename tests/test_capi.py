@@ -251,3 +251,10 @@ def test_engine_create_rejects_invalid_config_before_any_device_call():
         assert not h.value
     assert cfg.heap_arity == 4
     assert ga.lib().ga_engine_last_error(None) == b""
+
+
+def test_solve_noalloc_rejects_bad_arguments_before_any_device_call():
+    res = ga.Result()
+    req = ga.puzzle_request(3, ga.gen_puzzle(3, 10, 1))
+    buf = (ctypes.c_uint64 * 4)()
+    assert ga.lib().ga_solve_noalloc(None, ctypes.byref(req), ctypes.byref(res), buf, 4) == ga.E_INVALID
