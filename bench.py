@@ -10,17 +10,27 @@ and closed-set bytes.
 A "step" is one stream segment of --requests requests.  The W warm-up
 segments are served first (untimed); then the K timed segments are served as
 one continuous arrival stream with continuous batching (ga_serve_run), so no
-segment drains before the next one arrives.
-value = completed solves / wall seconds of the timed stream.
+segment drains before the next one arrives.  The offered load (--qps) lies
+above what the engine serves, so the timed stream measures the saturation
+throughput:  value = completed solves / wall seconds of the timed stream.
 
-Also measured (outside the timed region): the same requests solved one at a
-time by plain GA* (ga_solve), and the CPU oracle on the host cores.
+Measured outside the timed region:
+  latency    the same requests served again with their arrivals spread to
+             --latency-load x the measured saturation rate (below capacity):
+             p50/p99 of arrival -> result on the host at that offered load
+  batched    the 1024 mixed instances of the batch configuration, one call
+  24-puzzle  256 random-walk 5x5 boards, one call
+  one at a time  --plain-sample requests of the timed stream, plain GA*
+             (ga_solve) back to back on the one-at-a-time tuning
+  CPU        the oracle (sequential A*) built with -march=native on this
+             host: throughput on pinned threads, latency on one pinned core
 The step kernel's roofline comes from the sampled chunks (every
---time-every-th chunk of each launch stream): HIP events around their step
-launches (with graph replay one event pair brackets a chunk) and the device
-counters, which count in those launches only.  Its HBM traffic is this run's algorithmic bytes per launch times
-the HBM-bytes-per-algorithmic-byte ratio of the committed rocprofv3 counter
-runs (profiles/step_kernel_traffic.json).
+--time-every-th chunk of each launch lane), whose launches are issued one at a
+time between HIP events on the lane's stream and run with the device counters
+on.  The dominant kernel is the lane whose launches take the most kernel time.
+Its HBM traffic is this run's algorithmic bytes per launch times the
+HBM-bytes-per-algorithmic-byte ratio of the committed rocprofv3 counter runs
+(profiles/step_kernel_traffic.json).
 """
 from __future__ import annotations
 
@@ -52,13 +62,70 @@ def pct(xs, p):
     return xs[k]
 
 
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores(limit: int):
+    """The cores this process may run on (at most `limit`), for pinned workers."""
+    try:
+        cores = sorted(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = list(range(os.cpu_count() or 1))
+    return cores[:max(1, limit)]
+
+
+def run_pinned(cores, n_items: int, seconds: float, solve):
+    """Workers pinned one per core take items 0, 1, ... in order until `seconds`
+    have passed; solve(i) -> (done: bool, result).  Returns (per-item wall
+    seconds of the items done, results by item, elapsed seconds)."""
+    lock = threading.Lock()
+    state = {"next": 0}
+    times, results = {}, {}
+    deadline = time.perf_counter() + seconds
+    t0 = time.perf_counter()
+
+    def worker(core):
+        try:
+            os.sched_setaffinity(0, {core})  # Linux: pins the calling thread
+        except (AttributeError, OSError):
+            pass
+        while True:
+            with lock:
+                i = state["next"]
+                if i >= n_items or time.perf_counter() > deadline:
+                    return
+                state["next"] += 1
+            t = time.perf_counter()
+            done, res = solve(i)
+            dt = time.perf_counter() - t
+            with lock:
+                results[i] = res
+                if done:
+                    times[i] = dt
+
+    ts = [threading.Thread(target=worker, args=(c,)) for c in cores]
+    for th in ts:
+        th.start()
+    for th in ts:
+        th.join()
+    return times, results, time.perf_counter() - t0
+
+
 P24_N, P24_DEPTH = 256, (30, 60)
 
 
-def puzzle24_leg(e, heaps: int, threads: int, cpu_seconds: float):
+def puzzle24_leg(e, heaps: int, cores, cpu_seconds: float):
     """256 seeded 24-puzzles (walks of depth U{30..60} from the ordered board, no
     immediate backtrack) in one ga_solve_batch; the CPU oracle (sequential A*
-    on boards) takes the same instances in order on `threads` workers for
+    on boards) takes the same instances in order on pinned workers for
     `cpu_seconds`, and every instance it finishes checks the GPU cost."""
     import cuda_a_star_amd as ga
     from cuda_a_star_amd import workloads as wl
@@ -78,82 +145,110 @@ def puzzle24_leg(e, heaps: int, threads: int, cpu_seconds: float):
         return out
     import oracle
 
-    lock = threading.Lock()
-    state = {"next": 0, "done": 0, "match": 0, "capped": 0}
-    deadline = time.perf_counter() + cpu_seconds
-    t0 = time.perf_counter()
+    def solve(i):
+        rc, res, _ = oracle.astar_board(5, boards[i], ga.board_goal(5), max_states=MAX_ORACLE_STATES,
+                                        want_path=False)
+        if rc == oracle.E_LIMIT:
+            return False, None
+        return True, res.cost if rc == oracle.OK else -1
 
-    def worker():
-        while True:
-            with lock:
-                i = state["next"]
-                if i >= P24_N or time.perf_counter() > deadline:
-                    return
-                state["next"] += 1
-            rc, res, _ = oracle.astar_board(5, boards[i], ga.board_goal(5), max_states=MAX_ORACLE_STATES,
-                                            want_path=False)
-            with lock:
-                if rc == oracle.E_LIMIT:
-                    state["capped"] += 1
-                    continue
-                state["done"] += 1
-                state["match"] += int(rc == oracle.OK and sols[i].status == ga.OK and sols[i].cost == res.cost)
-
-    ts = [threading.Thread(target=worker) for _ in range(threads)]
-    for th in ts:
-        th.start()
-    for th in ts:
-        th.join()
-    el = time.perf_counter() - t0
-    out["cpu_oracle"] = {"solves_per_sec": round(state["done"] / el, 3), "cores": threads, "kind": "port",
-                         "sample": f"first {state['done']} instances in order, {el:.1f}s; {state['capped']} above "
-                                   f"{MAX_ORACLE_STATES} stored states skipped"}
-    out["gpu_cost_matches"] = f"{state['match']}/{state['done']}"
+    times, results, el = run_pinned(cores, P24_N, cpu_seconds, solve)
+    match = sum(1 for i in times if sols[i].status == ga.OK and sols[i].cost == results[i])
+    capped = sum(1 for i in results if i not in times)
+    out["cpu_oracle"] = {"solves_per_sec": round(len(times) / el, 3), "cores": len(cores), "kind": "port",
+                         "sample": f"first {len(results)} instances in order on {len(cores)} pinned threads, "
+                                   f"{el:.1f}s; {capped} above {MAX_ORACLE_STATES} stored states skipped"}
+    out["gpu_cost_matches"] = f"{match}/{len(times)}"
     return out
 
 
-def cpu_baseline(insts, seconds: float, threads: int, gpu_costs: dict):
-    """CPU oracle (sequential A*) on the host cores over a bounded sample of
-    the same stream: requests are taken in stream order by `threads` workers
-    until `seconds` have passed."""
+def cpu_baseline(insts, seconds: float, latency_seconds: float, cores, gpu_costs: dict, build: str):
+    """The oracle (sequential A*; `build`: "native" when oracle.use_native()
+    compiled it for this host) over a bounded sample of the timed stream, in
+    stream order:
+      throughput  one worker pinned per core on `cores` for `seconds`
+      latency     one worker pinned to the first core for `latency_seconds`:
+                  wall time of each request's search
+    Every request solved checks the GPU's cost for it."""
     import oracle
 
-    lock = threading.Lock()
-    state = {"next": 0, "done": 0, "match": 0, "checked": 0, "capped": 0}
-    deadline = time.perf_counter() + seconds
-    t0 = time.perf_counter()
+    def solve(i):
+        x = insts[i]
+        if x.kind == "puzzle":
+            rc, res, _ = oracle.astar_puzzle(4, x.start, x.goal, max_states=MAX_ORACLE_STATES, want_path=False)
+            if rc == oracle.E_LIMIT:
+                return False, None
+        else:
+            rc, res, _ = oracle.astar_grid(x.cells, x.side, x.side, x.start, x.goal, 4, want_path=False)
+        return True, res.cost if rc == oracle.OK else -1
 
-    def worker():
-        while True:
-            with lock:
-                i = state["next"]
-                if i >= len(insts) or time.perf_counter() > deadline:
-                    return
-                state["next"] += 1
-            x = insts[i]
-            if x.kind == "puzzle":
-                rc, res, _ = oracle.astar_puzzle(4, x.start, x.goal, max_states=MAX_ORACLE_STATES, want_path=False)
-                if rc == oracle.E_LIMIT:
-                    with lock:
-                        state["capped"] += 1
-                    continue
-                cost = res.cost if rc == oracle.OK else -1
-            else:
-                rc, res, _ = oracle.astar_grid(x.cells, x.side, x.side, x.start, x.goal, 4, want_path=False)
-                cost = res.cost if rc == oracle.OK else -1
-            with lock:
-                state["done"] += 1
-                if i in gpu_costs:
-                    state["checked"] += 1
-                    state["match"] += int(gpu_costs[i] == cost)
+    times, results, el = run_pinned(cores, len(insts), seconds, solve)
+    checked = [i for i in times if i in gpu_costs]
+    match = sum(1 for i in checked if gpu_costs[i] == results[i])
+    capped = sum(1 for i in results if i not in times)
+    lat_times, _, lat_el = run_pinned(cores[:1], len(insts), latency_seconds, solve)
+    lat_ms = [v * 1e3 for v in lat_times.values()]
+    return {
+        "value": round(len(times) / el, 3), "unit": "solves/s", "cores": len(cores), "kind": "port",
+        "cpu_model": cpu_model(), "build": f"oracle/oracle.c, gcc -O3 {'-march=native' if build == 'native' else '(portable)'}",
+        "sample": f"first {len(results)} requests of the timed stream in stream order, sequential A* (binary heap, "
+                  f"lazy deletion) per request, one thread pinned per core on {len(cores)} cores, {el:.1f}s; "
+                  f"{capped} puzzles above {MAX_ORACLE_STATES} stored states skipped",
+        "latency_mode": {"cores": 1, "requests": len(lat_ms), "seconds": round(lat_el, 1),
+                         "p50_ms": round(pct(lat_ms, 0.5), 3), "p99_ms": round(pct(lat_ms, 0.99), 3),
+                         "solves_per_sec": round(len(lat_ms) / lat_el, 3) if lat_el else None},
+        "gpu_cost_matches": f"{match}/{len(checked)}",
+    }
 
-    ts = [threading.Thread(target=worker) for _ in range(threads)]
-    for t in ts:
-        t.start()
-    for t in ts:
-        t.join()
-    el = time.perf_counter() - t0
-    return state["done"] / el, state["done"], el, state["checked"], state["match"], state["capped"]
+
+def lane_roofline(st, cfg, elapsed_s):
+    """Roofline line of the dominant step-kernel lane (most kernel time) from
+    the sampled launches of the timed stream."""
+    n = cfg.lanes + 1
+    lanes = []
+    for l in range(n):
+        if not st.lane_timed_launches[l]:
+            continue
+        avg_ms = st.lane_step_ms[l] / st.lane_timed_launches[l]
+        lanes.append({
+            "lane": l, "launches": st.lane_launches[l], "timed_launches": st.lane_timed_launches[l],
+            "avg_launch_us": round(avg_ms * 1e3, 3),
+            "kernel_ms_est": round(avg_ms * st.lane_launches[l], 1),
+            "algo_bytes_per_launch": round(st.lane_algo_bytes[l] / st.lane_launches[l], 1),
+            "achieved_GBps": round(st.lane_algo_bytes[l] / st.lane_launches[l] / (avg_ms * 1e-3) / 1e9, 2),
+        })
+    if not lanes:
+        return None
+    dom = max(lanes, key=lambda x: x["kernel_ms_est"])
+    kernel = {0: "step_kernel<4, all kinds> (lane 0: 15-puzzles and grids below deep_cells)",
+              1: "step_kernel<4, 4-connected grids> (lane 1: grids of at least deep_cells cells)"}
+    rf = {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+          "frac": round(dom["achieved_GBps"] / HBM_PEAK_GBS, 5), "traffic": None,
+          "kernel": kernel.get(dom["lane"], f"step kernel of lane {dom['lane']}"),
+          "avg_launch_us": dom["avg_launch_us"], "algo_bytes_per_launch": dom["algo_bytes_per_launch"],
+          "event_timing": "HIP events around each step launch of the sampled chunks, on the lane's stream",
+          "lanes": lanes,
+          "expansions_per_launch": round(st.expanded / st.step_launches, 1),
+          # heap-group iterations with work / launched (every mapped heap, every sampled launch)
+          "working_group_fraction": round(st.work_groups / st.heap_groups, 4) if st.heap_groups else None,
+          "sift_down_hbm_fraction": round(st.sift_down_hbm / st.sift_down, 4) if st.sift_down else None,
+          # all algorithmic bytes of the timed stream over its wall time (lanes run concurrently)
+          "achieved_device_GBps": round(st.algo_bytes / elapsed_s / 1e9, 2)}
+    tpath = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
+    if os.path.exists(tpath):  # HBM bytes per algorithmic byte from the committed rocprofv3 --pmc runs
+        with open(tpath) as f:
+            tj = json.load(f)
+        ratio = tj.get("hbm_bytes_per_algo_byte", {})
+        r = ratio.get(str(dom["lane"])) if isinstance(ratio, dict) else None
+        if r:
+            rf["traffic"] = round(r * dom["algo_bytes_per_launch"], 1)
+            rf["traffic_source"] = (f"profiles/step_kernel_traffic.json: {r:.3f} HBM bytes per algorithmic byte "
+                                    f"(lane {dom['lane']}) x this run's algorithmic bytes per launch")
+    # global atomics of the sampled launches: one CAS per closed-set probe (hashed sets), one atomicMin per lookup
+    avg_all = st.step_ms_total / st.timed_launches
+    rf["atomics_per_launch"] = round((st.probes + st.lookups) / st.timed_launches, 1)
+    rf["atomic_rate_G_per_s"] = round((st.probes + st.lookups) / st.timed_launches / (avg_all * 1e-3) / 1e9, 3)
+    return rf
 
 
 def main() -> int:
@@ -162,11 +257,14 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=512, help="requests per step (stream segment)")
-    ap.add_argument("--qps", type=float, default=1500.0, help="offered load (Poisson arrivals)")
+    ap.add_argument("--qps", type=float, default=1500.0, help="offered load of the timed stream (Poisson arrivals)")
+    ap.add_argument("--latency-load", type=float, default=0.8,
+                    help="offered load of the latency leg, as a fraction of the measured saturation rate (0 = skip)")
     ap.add_argument("--seed", type=int, default=5)
-    ap.add_argument("--plain-sample", type=int, default=40, help="requests solved one at a time")
+    ap.add_argument("--plain-sample", type=int, default=200, help="requests solved one at a time")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cpu count)")
+    ap.add_argument("--cpu-latency-seconds", type=float, default=5.0)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores available)")
     ap.add_argument("--budget-gb", type=float, default=200.0)
     ap.add_argument("--no-profile", action="store_true", help="skip per-step HIP events and counters")
     ap.add_argument("--time-every", type=int, default=8,
@@ -210,6 +308,7 @@ def main() -> int:
     for kv in filter(None, args.override.split(",")):
         k, v = kv.split("=")
         setattr(served_cfg, k, int(v))
+    under_profiler = bool(os.environ.get("ROCPROF_OUTPUT_PATH"))  # the engine then launches without graphs
     # The iteration limit only bounds a pathological request (it would count as failed); the longest
     # searches of this stream take about 6,000 iterations.
     e = ga.Engine(0, budget_bytes=int(args.budget_gb * (1 << 30)), chunk_bytes=8 << 30,
@@ -242,7 +341,6 @@ def main() -> int:
     # ---- timed: K steps of M requests, served as one continuous arrival stream ----
     launches_before = e.stats().step_launches  # step dispatches of this process before the timed segment
     e.reset_stats()
-    lat, ok, failed, served_costs = [], 0, 0, {}
     try:
         import torch
         sync = torch.cuda.synchronize if torch.cuda.is_available() else (lambda: None)
@@ -251,75 +349,50 @@ def main() -> int:
     idx, reqs, arr = segment(W * M, K * M)
     sync()
     t_start = time.perf_counter()
-    sols, ss = e.serve(reqs, arr)
+    sols, ss = e.serve(reqs, arr)  # raises GAStarError if the run fails
     sync()
     t_total = time.perf_counter() - t_start
-    for i, sol in zip(idx, sols):
-        if sol.status == ga.OK:
-            ok += 1
-            lat.append(sol.latency_ns / 1e6)
-            served_costs[i] = sol.cost
-        else:
-            failed += 1
-    log(f"[bench] timed: {ss.completed}/{len(reqs)} in {t_total:.2f}s, p50 {ss.p50_ms:.1f} ms, "
-        f"p99 {ss.p99_ms:.1f} ms, inflight max {ss.max_inflight}")
-    by_kind = {}
-    for kind in ("puzzle", "grid"):
-        kl = [sol.latency_ns / 1e6 for i, sol in zip(idx, sols) if insts[i].kind == kind and sol.status == ga.OK]
-        if kl:
-            by_kind[kind] = {"n": len(kl), "p50_ms": round(pct(kl, 0.5), 2), "p99_ms": round(pct(kl, 0.99), 2)}
-    slow = sorted(zip(idx, sols), key=lambda p: -p[1].latency_ns)[:5]
-    slowest = [{"kind": insts[i].kind, "side": insts[i].side, "depth": insts[i].depth,
-                "latency_ms": round(sol.latency_ns / 1e6, 1), "iterations": sol.iterations,
-                "expanded": sol.expanded, "retries": sol.retries} for i, sol in slow]
-    log(f"[bench] latency by kind {by_kind}; slowest {slowest}")
     st = e.stats()
+    ok = sum(1 for s in sols if s.status == ga.OK)
+    failed = len(sols) - ok
+    served_costs = {i: s.cost for i, s in zip(idx, sols) if s.status == ga.OK}
+    sat_lat = [s.latency_ns / 1e6 for s in sols if s.status == ga.OK]
     served_sps = ok / t_total
-
-    # ---- kernel roofline (timed segments) ----
+    log(f"[bench] timed (saturation, {args.qps:.0f} QPS offered): {ok}/{len(reqs)} in {t_total:.2f}s = "
+        f"{served_sps:.1f} solves/s, p50 {pct(sat_lat, 0.5):.1f} ms, p99 {pct(sat_lat, 0.99):.1f} ms, "
+        f"inflight max {ss.max_inflight}")
     roofline = None
     if not args.no_profile and st.step_launches and st.timed_launches:
-        avg_ms = st.step_ms_total / st.timed_launches  # launches of the sampled chunks
-        bytes_per_launch = st.algo_bytes / st.step_launches
-        achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-        traffic, traffic_src = None, None
-        tpath = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
-        if os.path.exists(tpath):  # HBM bytes per algorithmic byte from the committed rocprofv3 --pmc runs
-            with open(tpath) as f:
-                tj = json.load(f)
-            if "hbm_bytes_per_algo_byte" in tj:
-                traffic = round(tj["hbm_bytes_per_algo_byte"] * bytes_per_launch, 1)
-                traffic_src = (f"profiles/{os.path.basename(tpath)}: {tj['hbm_bytes_per_algo_byte']:.3f} HBM bytes "
-                               f"per algorithmic byte x this run's algorithmic bytes per launch")
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "traffic_source": traffic_src,
-                    "kernel": f"step_kernel<{served_cfg.heap_arity}>", "avg_launch_us": round(avg_ms * 1e3, 3),
-                    "algo_bytes_per_launch": round(bytes_per_launch, 1),
-                    "expansions_per_launch": round(st.expanded / st.step_launches, 1),
-                    # heap-group iterations with work / launched (every mapped heap, every launch)
-                    "working_group_fraction": round(st.work_groups / st.heap_groups, 4) if st.heap_groups else None,
-                    "launches": st.step_launches,
-                    "timed_launches": st.timed_launches,
-                    "launches_before_timed": launches_before,
-                    "event_timing": ("per chunk of step launches (graph replay): chunk time / launches"
-                                     if served_cfg.use_graph else "per step launch"),
-                    # launch lanes run concurrently, so the per-launch figure above shares the device;
-                    # this one is all algorithmic bytes of the timed stream over its wall time
-                    "achieved_device_GBps": round(st.algo_bytes / t_total / 1e9, 2),
-                    # global atomics issued by the step kernel: one CAS per closed-set probe (hashed sets)
-                    # plus one atomicMin per lookup; counters and reductions excluded
-                    # (probes and lookups are counted in the sampled launches only)
-                    "atomics_per_launch": round((st.probes + st.lookups) / st.timed_launches, 1),
-                    "atomic_rate_G_per_s": round((st.probes + st.lookups) / st.timed_launches / (avg_ms * 1e-3) / 1e9, 3)}
-        # The device's rate for those atomics (random 64-bit, agent scope, over a 1 GiB table: no cache
-        # reuse) and the share of the average launch that the launch's CAS probes and fetch-mins would
-        # take at those rates.
+        roofline = lane_roofline(st, served_cfg, t_total)
+        roofline["launches_before_timed"] = launches_before
         peak_min = e.atomic_rate(1 << 30, 1 << 26, 0)
         peak_cas = e.atomic_rate(1 << 30, 1 << 26, 1)
+        avg_all = st.step_ms_total / st.timed_launches
         t_atomics = (st.probes / peak_cas + st.lookups / peak_min) / st.timed_launches
         roofline["atomic_peak_G_per_s"] = {"fetch_min": round(peak_min / 1e9, 2), "cas": round(peak_cas / 1e9, 2)}
-        roofline["atomic_frac"] = round(t_atomics / (avg_ms * 1e-3), 4)
-    log(f"[bench] served {ok} ok / {failed} failed in {t_total:.2f}s = {served_sps:.1f} solves/s; roofline {roofline}")
+        roofline["atomic_frac"] = round(t_atomics / (avg_all * 1e-3), 4)
+    log(f"[bench] roofline {roofline}")
+
+    # ---- latency leg: the same requests at a stated offered load below saturation ----
+    latency = None
+    if args.latency_load > 0 and served_sps > 0:
+        lam = args.latency_load * served_sps
+        scale = args.qps / lam  # arrival gaps of the stream drawn at --qps, stretched to lam
+        arr2 = [int(a * scale) for a in arr]
+        sols2, ss2 = e.serve(reqs, arr2)
+        lat2 = [s.latency_ns / 1e6 for s in sols2 if s.status == ga.OK]
+        by_kind = {}
+        for kind in ("puzzle", "grid"):
+            kl = [s.latency_ns / 1e6 for i, s in zip(idx, sols2) if insts[i].kind == kind and s.status == ga.OK]
+            if kl:
+                by_kind[kind] = {"n": len(kl), "p50_ms": round(pct(kl, 0.5), 2), "p99_ms": round(pct(kl, 0.99), 2)}
+        mism = sum(1 for i, s in zip(idx, sols2) if s.status != ga.OK or served_costs.get(i) != s.cost)
+        latency = {"offered_qps": round(lam, 1), "load_vs_saturation": args.latency_load,
+                   "completed": len(lat2), "solves_per_sec": round(ss2.solves_per_sec, 3),
+                   "p50_ms": round(pct(lat2, 0.5), 3), "p99_ms": round(pct(lat2, 0.99), 3),
+                   "latency_by_kind": by_kind, "max_inflight": ss2.max_inflight,
+                   "cost_mismatches_vs_saturation_leg": mism}
+        log(f"[bench] latency leg at {lam:.0f} QPS offered: {latency}")
 
     # ---- batched: the 1024 mixed instances of the batch configuration in one ga_solve_batch ----
     batch = None
@@ -340,16 +413,22 @@ def main() -> int:
             e.free_device(a)
         log(f"[bench] batched: {b_ok}/{len(b_reqs)} in {b_t:.2f}s = {b_ok / b_t:.1f} solves/s")
 
+    cores = host_cores(args.cpu_threads or 16)
+    oracle_build = "portable"
+    if not args.skip_cpu:  # the timed CPU legs use the oracle compiled for this host
+        import oracle
+
+        oracle_build = oracle.use_native()
+
     # ---- 24-puzzle: 5x5 boards on the wide-key kernel, one batch, vs the CPU oracle ----
     p24 = None
     if not args.skip_p24:
-        p24 = puzzle24_leg(e, args.p24_heaps or served_cfg.heaps("puzzle", 5), args.cpu_threads or min(16, os.cpu_count() or 1),
-                           0.0 if args.skip_cpu else 8.0)
+        p24 = puzzle24_leg(e, args.p24_heaps or served_cfg.heaps("puzzle", 5), cores, 0.0 if args.skip_cpu else 8.0)
         log(f"[bench] 24-puzzle: {p24}")
 
     # ---- one at a time: plain GA*, same requests, sequential ----
     timed_idx = list(range(W * M, (W + K) * M))
-    plain_sps, mismatch = None, 0
+    plain_sps, mismatch, plain_n = None, 0, 0
     if args.plain_sample > 0:
         step = max(1, len(timed_idx) // args.plain_sample)
         sample = timed_idx[::step][: args.plain_sample]
@@ -363,24 +442,18 @@ def main() -> int:
             if sol.status != ga.OK or served_costs.get(i, sol.cost) != sol.cost:
                 mismatch += 1
         plain_t = time.perf_counter() - t
-        plain_sps = len(sample) / plain_t
+        plain_n = len(sample)
+        plain_sps = plain_n / plain_t
         e1.close()
-        log(f"[bench] one-at-a-time: {len(sample)} in {plain_t:.2f}s = {plain_sps:.2f} solves/s, "
+        log(f"[bench] one-at-a-time: {plain_n} in {plain_t:.2f}s = {plain_sps:.2f} solves/s, "
             f"served-vs-alone cost mismatches {mismatch}")
 
     # ---- CPU oracle on the host cores ----
     cpu = None
     if not args.skip_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        sps, n, el, checked, match, capped = cpu_baseline([insts[i] for i in timed_idx], args.cpu_seconds, threads,
-                                                  {k - W * M: v for k, v in served_costs.items()})
-        cpu = {"value": round(sps, 3), "unit": "solves/s", "cores": threads, "kind": "port",
-               "sample": f"first {n} requests of the timed stream in stream order, sequential A* "
-                         f"(binary heap, lazy deletion) per request, {threads} threads, {el:.1f}s; "
-                         f"{capped} puzzles above {MAX_ORACLE_STATES} stored states skipped",
-               "gpu_cost_matches": f"{match}/{checked}"}
-        log(f"[bench] cpu oracle: {n} in {el:.1f}s = {sps:.2f} solves/s ({threads} threads), "
-            f"GPU costs equal on {match}/{checked}")
+        cpu = cpu_baseline([insts[i] for i in timed_idx], args.cpu_seconds, args.cpu_latency_seconds, cores,
+                           {k - W * M: v for k, v in served_costs.items()}, oracle_build)
+        log(f"[bench] cpu oracle: {cpu}")
 
     for a in dev.values():
         e.free_device(a)
@@ -406,14 +479,19 @@ def main() -> int:
             "offered_qps": args.qps,
             "seed": args.seed,
             "heaps_per_instance": served_cfg.describe(),
+            "graph_replay": bool(served_cfg.use_graph) and not under_profiler,
         },
-        "p50_ms": round(pct(lat, 0.50), 3),
-        "p99_ms": round(pct(lat, 0.99), 3),
-        "latency_by_kind": by_kind,
-        "slowest_requests": slowest,
+        # latency at the stated offered load below saturation (the latency leg)
+        "p50_ms": latency["p50_ms"] if latency else None,
+        "p99_ms": latency["p99_ms"] if latency else None,
+        "latency": latency,
+        "saturation_latency": {"offered_qps": args.qps, "p50_ms": round(pct(sat_lat, 0.5), 3),
+                               "p99_ms": round(pct(sat_lat, 0.99), 3), "max_inflight": ss.max_inflight,
+                               "note": "offered load above capacity: a backlog builds over the stream"},
         "completed": ok,
         "failed": failed,
         "one_at_a_time_solves_per_sec": round(plain_sps, 3) if plain_sps else None,
+        "one_at_a_time_sample": plain_n,
         "batched": batch,
         "puzzle24": p24,
         "speedup_vs_one_at_a_time": round(served_sps / plain_sps, 2) if plain_sps else None,
@@ -422,7 +500,8 @@ def main() -> int:
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    return 0 if failed == 0 and mismatch == 0 else 1
+    bad = failed or mismatch or (latency and latency["cost_mismatches_vs_saturation_leg"])
+    return 1 if bad else 0
 
 
 if __name__ == "__main__":

@@ -29,6 +29,7 @@ LIB_PATH = os.path.join(_HERE, "libgastar.so")
 OK, NO_PATH, PENDING = 0, 1, 2
 E_INVALID, E_NOMEM, E_BUDGET, E_HIP, E_TIMEOUT, E_OVERFLOW = -1, -2, -3, -4, -5, -6
 PUZZLE, GRID = 1, 2
+MAX_LANES = 4  # GA_MAX_LANES
 
 _u8p = ctypes.POINTER(ctypes.c_uint8)
 _u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -114,6 +115,13 @@ class EngineStats(ctypes.Structure):
         ("timed_launches", ctypes.c_uint64),
         ("heap_groups", ctypes.c_uint64),
         ("work_groups", ctypes.c_uint64),
+        ("sift_down_hbm", ctypes.c_uint64),
+        ("sift_up_hbm", ctypes.c_uint64),
+        ("push_in", ctypes.c_uint64),
+        ("lane_launches", ctypes.c_uint64 * (MAX_LANES + 1)),
+        ("lane_timed_launches", ctypes.c_uint64 * (MAX_LANES + 1)),
+        ("lane_step_ms", ctypes.c_double * (MAX_LANES + 1)),
+        ("lane_algo_bytes", ctypes.c_double * (MAX_LANES + 1)),
     ]
 
 

@@ -501,7 +501,8 @@ struct ga_engine {
     uint32_t n_lanes = 1;     // cfg.lanes + 1; slots are partitioned by slot % n_lanes
     std::vector<Job*> resident;
     ga_engine_stats stats{};
-    double algo_sampled = 0, algo_always = 0;  // algorithmic bytes: profiled-launch terms, all-launch terms
+    // algorithmic bytes per lane: profiled-launch terms, all-launch terms
+    double algo_sampled[GA_MAX_LANES + 1] = {}, algo_always[GA_MAX_LANES + 1] = {};
     std::string last_error;
     bool broken = false;  // a HIP error ended a run: the device state is unknown, later runs are refused
     char* scratch_dev = nullptr;  // this device's scratch (ga_solve_noalloc)
@@ -515,7 +516,7 @@ struct ga_engine {
         // served run with it (4,096 heaps per 15-puzzle) once stalled with no
         // cause shown.
         if (cfg.heap_arity != 4 && cfg.heap_arity != 8) throw GaError(GA_E_INVALID, "heap_arity must be 4 or 8");
-        if (cfg.lanes > 4) throw GaError(GA_E_INVALID, "lanes must be at most 4");
+        if (cfg.lanes > GA_MAX_LANES) throw GaError(GA_E_INVALID, "lanes must be at most 4");
         if (cfg.max_instances == 0) cfg.max_instances = 1;
         if (cfg.chunk_iters == 0) cfg.chunk_iters = 1;
         if (cfg.max_load_pct == 0 || cfg.max_load_pct > 95) cfg.max_load_pct = 50;
@@ -934,8 +935,11 @@ struct ga_engine {
             }
         }
         L.timed[b] = timed;
-        if (cfg.use_graph) {
-            const uint64_t key = ((uint64_t)variant_slot(L.variant) << 40) | ((uint64_t)(timed ? 1 : 0) << 32) | L.grid;
+        // Sampled chunks are launched one kernel at a time, each bracketed by
+        // its own HIP events (an event pair around a replayed graph would also
+        // time the dispatch gaps and the advance kernel).
+        if (cfg.use_graph && !timed) {
+            const uint64_t key = ((uint64_t)variant_slot(L.variant) << 40) | L.grid;
             hipGraphExec_t& ge = L.graphs[key];
             if (!ge) {
                 hipGraph_t g;
@@ -945,15 +949,15 @@ struct ga_engine {
                 hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(g);
             }
-            if (timed) hip_check(hipEventRecord(pe[0], L.stream), "event");
             hip_check(hipGraphLaunch(ge, L.stream), "graph launch");
-            if (timed) hip_check(hipEventRecord(pe[1], L.stream), "event");
         } else {
             enqueue_steps(L, L.n_wg, prm, timed ? &pe : nullptr);
         }
         hip_check(hipGetLastError(), "step launch");
         L.host_base += R;
         stats.step_launches += R;
+        const uint32_t li = (uint32_t)(&L - lanes.data());
+        stats.lane_launches[li] += R;
         if (timed) stats.heap_groups += L.n_heaps * R;
         uint32_t n_read = 0;
         for (uint32_t s = 0; s < cfg.max_instances; ++s)
@@ -970,13 +974,14 @@ struct ga_engine {
     void collect_profile(Lane& L, uint64_t chunk) {
         if (!cfg.profile || !L.timed[chunk & 1]) return;
         std::vector<hipEvent_t>& pe = L.prof_ev[chunk & 1];
+        const uint32_t li = (uint32_t)(&L - lanes.data());
         stats.timed_launches += cfg.chunk_iters;
-        // graph mode times the whole chunk (R steps + the counter advance)
-        const size_t pairs = cfg.use_graph ? 1 : cfg.chunk_iters;
-        for (size_t r = 0; r < pairs && 2 * r + 1 < pe.size(); ++r) {
+        stats.lane_timed_launches[li] += cfg.chunk_iters;
+        for (size_t r = 0; r < cfg.chunk_iters && 2 * r + 1 < pe.size(); ++r) {
             float ms = 0;
             hip_check(hipEventElapsedTime(&ms, pe[2 * r], pe[2 * r + 1]), "elapsed");
             stats.step_ms_total += ms;
+            stats.lane_step_ms[li] += ms;
         }
     }
 
@@ -999,38 +1004,60 @@ struct ga_engine {
         stats.inserted += s.inserted;
         stats.lookups += s.lookups;
         stats.work_groups += s.work_groups;
-        algo_sampled += algo_bytes(j.kind, s, true);
-        algo_always += algo_bytes(j.kind, s, false);
+        stats.sift_down_hbm += s.sift_down_hbm;
+        stats.sift_up_hbm += s.sift_up_hbm;
+        stats.push_in += s.push_in;
+        algo_sampled[j.lane] += algo_bytes(j.kind, s, true);
+        algo_always[j.lane] += algo_bytes(j.kind, s, false);
     }
 
-    // Bytes the step kernel's data-structure operations must move (the
-    // per-unit figures are listed in the design notes):
-    //   per heap-group iteration  2*K*16 (candidate slice written + read) + 8 (heap size)
-    //   per pop                   32 (root + last) + closed-set entry (16 hashed, 8 dense)
-    //   per sift-down level       W*16 (children) + 16 (write)
-    //   per push                  16, per sift-up level 32
-    //   per lookup                8 (atomicMin on g|parent); dense: +K*1 per expansion (cells)
-    //   hashed only               8 per probe (CAS on the key) + 8 per new key
-    // `sampled`: the terms of the counters that only profiled launches count
-    // (every term but the cells and the new keys); otherwise those two.
+    // Algorithmic bytes of the step kernel: the bytes its data structures'
+    // operations must read or write in global memory, with E = 16 (heap entry
+    // and candidate record) and W = heap arity:
+    //   per live heap group and launch  E (its top, read for the min-f test)
+    //   per pop                         2E (root + last entry) + the closed-set
+    //                                   stale check (key 8 + g 4 hashed, g|parent
+    //                                   8 dense, board 16 + g 4 for 24-puzzles)
+    //   per sift-down level in global   W*E (children) + E (entry moved)
+    //   memory (levels inside the LDS copy of the heap's top are not counted)
+    //   per closed-set probe            8 (CAS on the key; 24-puzzle 16: key
+    //                                   CAS + board half)
+    //   per closed-set lookup           8 (atomicMin on g | parent)
+    //   per generated grid move         1 (weight of the entered cell)
+    //   per push                        E (entry written)
+    //   per record pushed from the      2E (written by its producer, read by
+    //   candidate buffer                its heap)
+    //   per sift-up level in global     2E (parent read, entry moved)
+    //   memory
+    // Idle heaps clearing their candidate slice and the copy of heap tops into
+    // LDS and back are this design's overhead, not counted here: they show in
+    // the measured traffic.  `sampled`: the terms from the counters that only
+    // profiled launches count; otherwise the always-counted term (grid moves).
     double algo_bytes(int kind, const InstState& s, bool sampled) const {
         const bool dense = kind == KIND_GRID4 || kind == KIND_GRID8;
-        const double K = branching_of(kind), W = cfg.heap_arity;
-        if (!sampled) return dense ? (double)s.expanded * K : (double)s.occupied * 8;
-        double b = (double)s.groups * (2.0 * K * 16 + 8);
-        b += (double)s.pops * (32 + (dense ? 8 : 16));
-        b += (double)s.sift_down * (W * 16 + 16);
-        b += (double)s.pushed * 16 + (double)s.sift_up * 32;
+        const double E = 16, W = cfg.heap_arity;
+        if (!sampled) return dense ? (double)s.generated : 0.0;
+        const double stale = dense ? 8 : is_wide(kind) ? 20 : 12;
+        double b = (double)s.groups * E;
+        b += (double)s.pops * (2 * E + stale);
+        b += (double)s.sift_down_hbm * (W * E + E);
+        if (!dense) b += (double)s.probes * (is_wide(kind) ? 16 : 8);
         b += (double)s.lookups * 8;
-        if (!dense) b += (double)s.probes * 8;
-        if (is_wide(kind)) b += (double)s.pops * 16 + (double)s.lookups * 16;  // 16-byte keys read / written
+        b += (double)s.pushed * E + (double)s.push_in * 2 * E + (double)s.sift_up_hbm * 2 * E;
         return b;
     }
-    // algorithmic bytes of all step launches: the sampled terms scaled from the
-    // profiled launches to all of them
+    // algorithmic bytes of all step launches of lane l: the sampled terms
+    // scaled from its profiled launches to all of them
+    double algo_lane(uint32_t l) const {
+        const double scale = stats.lane_timed_launches[l]
+                                 ? (double)stats.lane_launches[l] / (double)stats.lane_timed_launches[l]
+                                 : 0.0;
+        return algo_always[l] + algo_sampled[l] * scale;
+    }
     double algo_total() const {
-        const double scale = stats.timed_launches ? (double)stats.step_launches / (double)stats.timed_launches : 0.0;
-        return algo_always + algo_sampled * scale;
+        double t = 0;
+        for (uint32_t l = 0; l < n_lanes; ++l) t += algo_lane(l);
+        return t;
     }
 
     void finish(Job& j, int status, uint64_t t0) {
@@ -1677,6 +1704,7 @@ int ga_engine_get_stats(ga_engine* e, ga_engine_stats* out) {
     if (!e || !out) return GA_E_INVALID;
     *out = e->stats;
     out->algo_bytes = e->algo_total();
+    for (uint32_t l = 0; l < e->n_lanes; ++l) out->lane_algo_bytes[l] = e->algo_lane(l);
     out->peak_bytes = e->arena->peak();
     return GA_OK;
 }
@@ -1684,7 +1712,7 @@ int ga_engine_get_stats(ga_engine* e, ga_engine_stats* out) {
 void ga_engine_reset_stats(ga_engine* e) {
     if (!e) return;
     e->stats = ga_engine_stats{};
-    e->algo_sampled = e->algo_always = 0;
+    for (uint32_t l = 0; l <= GA_MAX_LANES; ++l) e->algo_sampled[l] = e->algo_always[l] = 0;
 }
 
 int ga_solve(ga_engine* e, const ga_request* req, ga_result* out) {

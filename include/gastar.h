@@ -47,6 +47,8 @@ extern "C" {
 #define GA_E_TIMEOUT (-5)  /* iteration limit reached */
 #define GA_E_OVERFLOW (-6) /* capacity exceeded after all retries */
 
+#define GA_MAX_LANES 4 /* ga_config.lanes at most */
+
 /* ---- problem kinds ---- */
 #define GA_PUZZLE 1 /* sliding puzzle: a_star_sliding_puzzle.h */
 #define GA_GRID 2   /* weighted grid:  a_star_pathfinding.h    */
@@ -136,10 +138,19 @@ typedef struct ga_engine_stats {
     uint64_t inserted;       /* successors that lowered a stored g (profile) */
     uint64_t lookups;        /* closed-set find-or-insert calls (profile) */
     double algo_bytes;       /* algorithmic bytes of all step launches, the profiled terms scaled
-                                from the sampled launches to all of them */
+                                from the sampled launches to all of them (per-unit figures: the
+                                design notes' kernel table) */
     uint64_t timed_launches; /* step launches sampled: step_ms_total and the (profile) counts cover them */
     uint64_t heap_groups;    /* heap-group iterations launched in the sampled launches (every mapped heap) */
     uint64_t work_groups;    /* heap-group iterations with work: pushes, or a pop within the f gate (profile) */
+    uint64_t sift_down_hbm;  /* sift-down levels whose children were read from global memory (profile) */
+    uint64_t sift_up_hbm;    /* sift-up levels whose parent was read from global memory (profile) */
+    uint64_t push_in;        /* records pushed from the candidate buffer (profile) */
+    /* per launch lane: index l < lanes for the configured lanes, index lanes for the 24-puzzle lane */
+    uint64_t lane_launches[GA_MAX_LANES + 1];        /* step launches */
+    uint64_t lane_timed_launches[GA_MAX_LANES + 1];  /* sampled step launches */
+    double lane_step_ms[GA_MAX_LANES + 1];           /* their summed kernel time from HIP events */
+    double lane_algo_bytes[GA_MAX_LANES + 1];        /* algorithmic bytes of all the lane's launches */
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {

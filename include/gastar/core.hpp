@@ -180,6 +180,9 @@ struct InstState {
     unsigned long long inserted;  // successors whose atomicMin lowered g
     unsigned long long lookups;   // closed-set find-or-insert calls (one atomicMin each)
     unsigned long long work_groups;  // heap-group iterations with work (pushes or a pop within the gate)
+    unsigned long long sift_down_hbm;  // sift-down levels whose children were read from global memory
+    unsigned long long sift_up_hbm;    // sift-up levels whose parent was read from global memory
+    unsigned long long push_in;        // records pushed from the candidate buffer (written by another heap)
 };
 
 // Per-workgroup tallies of one step, in LDS.  Lanes add to them with LDS
@@ -190,15 +193,16 @@ struct InstState {
 // counter fields of InstState from `occupied` on.
 enum : uint32_t {
     TL_OCCUPIED, TL_EXPANDED, TL_GENERATED, TL_PUSHED, TL_POPS, TL_PROBES, TL_SIFT_DOWN, TL_SIFT_UP,
-    TL_GROUPS, TL_INSERTED, TL_LOOKUPS, TL_WORK,
+    TL_GROUPS, TL_INSERTED, TL_LOOKUPS, TL_WORK, TL_SIFT_DOWN_HBM, TL_SIFT_UP_HBM, TL_PUSH_IN,
     kCounters,
     TL_MIN_TOP = kCounters,  // min f over the heap tops
     TL_OPEN,                 // heap entries after the pushes
     TL_WAVES_DONE,           // waves of the workgroup that have finished the step
     kTallies
 };
-static_assert(offsetof(InstState, work_groups) == offsetof(InstState, occupied) + 8 * (kCounters - 1),
+static_assert(offsetof(InstState, push_in) == offsetof(InstState, occupied) + 8 * (kCounters - 1),
               "InstState counters must be contiguous and in tally order");
+static_assert(kTallies <= 64, "tally_end flushes one tally per lane of a wave");
 
 // Workgroup -> (instance, first heap) map of one launch.  `cover` is the
 // number of heaps of the instance that the map covers (a power of two): the
@@ -507,8 +511,8 @@ struct HeapRef {
 // then read while the sift-down runs.  If the root's f exceeds gate_f the
 // heap is left as it is and false is returned (`top` still holds the root).
 template <int W, class Load, class Pre, class Key>
-__device__ inline bool heap_pop(const HeapRef<W>& H, uint32_t& S, int lane, uint32_t& levels, Load load, Pre pre,
-                                Key& key, uint32_t& g, uint32_t gate_f, HeapEntry& top) {
+__device__ inline bool heap_pop(const HeapRef<W>& H, uint32_t& S, int lane, uint32_t& levels, uint32_t& hbm_levels,
+                                Load load, Pre pre, Key& key, uint32_t& g, uint32_t gate_f, HeapEntry& top) {
     top = H.get(0);
     const uint32_t n = S - 1;
     HeapEntry last;
@@ -541,6 +545,7 @@ __device__ inline bool heap_pop(const HeapRef<W>& H, uint32_t& S, int lane, uint
         if (idx == mi) H.put(j, e);
         j = mi;
         ++levels;
+        if (W * j + 1 >= HeapRef<W>::kTop) ++hbm_levels;  // this level's children are not in the LDS copy
         idx = W * j + 1 + (uint32_t)lane;
         e.prio = ~0ull;
         if (idx < n) e = H.get(idx);
@@ -553,7 +558,8 @@ __device__ inline bool heap_pop(const HeapRef<W>& H, uint32_t& S, int lane, uint
 }
 
 template <int W>
-__device__ inline void heap_push(const HeapRef<W>& H, uint32_t S, const HeapEntry& x, uint32_t& levels) {
+__device__ inline void heap_push(const HeapRef<W>& H, uint32_t S, const HeapEntry& x, uint32_t& levels,
+                                 uint32_t& hbm_levels) {
     uint32_t j = S;
     while (j > 0) {
         const uint32_t p = (j - 1) / W;
@@ -562,6 +568,7 @@ __device__ inline void heap_push(const HeapRef<W>& H, uint32_t S, const HeapEntr
             H.put(j, pe);
             j = p;
             ++levels;
+            if (p >= HeapRef<W>::kTop) ++hbm_levels;  // the parent was read from global memory
         } else {
             break;
         }
@@ -1157,7 +1164,7 @@ struct Puzzle5Ops : Board5KeySet {
 // register j / W) into the group's own heap; lane 0 performs the pushes.
 template <int W, int KW>
 __device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const HeapEntry (&rec)[KW], int K, int lane,
-                                    uint32_t best, uint32_t cap, bool& overflow, const Tally& tl) {
+                                    uint32_t best, uint32_t cap, bool& overflow, const Tally& tl, bool from_cand) {
     // Record j = r*W + l sits in lane l, register r.  Valid records take the
     // positions S, S+1, ... in j order.  When every new position's parent is an
     // old entry (true unless the heap is nearly empty), the lanes check their
@@ -1196,15 +1203,19 @@ __device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const Heap
                 x.slot = grp_bcast<W>(rec[r].slot, l);
                 x.aux = grp_bcast<W>(rec[r].aux, l);
                 if (lane == 0) {
-                    uint32_t lv = 0;
-                    heap_push<W>(H, S, x, lv);
+                    uint32_t lv = 0, hl = 0;
+                    heap_push<W>(H, S, x, lv, hl);
                     tl.prof_add(TL_SIFT_UP, lv);
+                    tl.prof_add(TL_SIFT_UP_HBM, hl);
                     ++S;
                 }
             }
         }
         S = grp_bcast<W>(S, 0);
-        if (lane == 0) tl.prof_add(TL_PUSHED, c);
+        if (lane == 0) {
+            tl.prof_add(TL_PUSHED, c);
+            if (from_cand) tl.prof_add(TL_PUSH_IN, c);
+        }
         return;
     }
     bool bad[KW];
@@ -1231,13 +1242,17 @@ __device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const Heap
             x.aux = grp_bcast<W>(rec[r].aux, l);
             const uint32_t pos = S + grp_bcast<W>(rank[r], l);
             if (lane == 0) {  // sift up from its leaf position
-                uint32_t lv = 0;
-                heap_push<W>(H, pos, x, lv);
+                uint32_t lv = 0, hl = 0;
+                heap_push<W>(H, pos, x, lv, hl);
                 tl.prof_add(TL_SIFT_UP, lv);
+                tl.prof_add(TL_SIFT_UP_HBM, hl);
             }
         }
     }
-    if (lane == 0) tl.prof_add(TL_PUSHED, c);
+    if (lane == 0) {
+        tl.prof_add(TL_PUSHED, c);
+        if (from_cand) tl.prof_add(TL_PUSH_IN, c);
+    }
     S += c;
 }
 
@@ -1444,7 +1459,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
 
     // -- 1. push the records addressed to this heap
     if (mine && it > 0 && q < Aw) {
-        push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl);
+        push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl, true);
 #pragma unroll
         for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
     }
@@ -1469,12 +1484,13 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     uint32_t top_g = 0;  // g stored in the closed set for the popped node
     bool popped = false, gated = false;
     if (mine && S > 0) {
-        uint32_t lv = 0;
-        popped = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_g, gate, top);
+        uint32_t lv = 0, hl = 0;
+        popped = heap_pop<W>(H, S, lane, lv, hl, load, prefetch, top_key, top_g, gate, top);
         gated = !popped;
         group_fence();
         if (lane == 0) {
             tl.prof_add(TL_SIFT_DOWN, lv);
+            tl.prof_add(TL_SIFT_DOWN_HBM, hl);
             tl.prof_add(TL_POPS, popped ? 1u : 0u);
         }
     }
@@ -1488,9 +1504,12 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             for (uint32_t t = 0; t < prm.max_pops; ++t) {
                 if (!popped) {
                     if (S == 0) break;
-                    uint32_t lv = 0;
-                    const bool ok = heap_pop<W>(H, S, lane, lv, load, prefetch, top_key, top_g, gate, top);
-                    if (lane == 0) tl.prof_add(TL_SIFT_DOWN, lv);
+                    uint32_t lv = 0, hl = 0;
+                    const bool ok = heap_pop<W>(H, S, lane, lv, hl, load, prefetch, top_key, top_g, gate, top);
+                    if (lane == 0) {
+                        tl.prof_add(TL_SIFT_DOWN, lv);
+                        tl.prof_add(TL_SIFT_DOWN_HBM, hl);
+                    }
                     if (!ok) {
                         gated = true;
                         break;
@@ -1559,7 +1578,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 }
             }
             if (round + 1 == rounds) break;  // last round: records go to the candidate buffer
-            push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl);
+            push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl, false);
 #pragma unroll
             for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
             group_fence();
@@ -1650,6 +1669,7 @@ __device__ inline bool seed_instance(const InstDesc& d, InstState* st, int W) {
     z.occupied = Ops::kDense ? 0 : 1;
     z.expanded = z.generated = z.pushed = z.pops = z.probes = z.sift_down = z.sift_up = 0;
     z.groups = z.inserted = z.lookups = z.work_groups = 0;
+    z.sift_down_hbm = z.sift_up_hbm = z.push_in = 0;
     uint32_t slot, probes = 0;
     bool isnew;
     const typename Ops::Key start = Ops::start_key(d);

@@ -40,12 +40,30 @@ def build() -> str:
     return _LIB_PATH
 
 
-def lib() -> ctypes.CDLL:
+def use_native(out_dir: str = "/tmp/gastar_oracle") -> str:
+    """For the timed CPU baseline only: compile the oracle with -march=native
+    for the host this runs on (the ``native`` target of the Makefile, output
+    outside the repository) and route every call of this module through it.
+    Returns "native", or "portable" when that build fails (liboracle.so is
+    then used as is)."""
+    global _lib
+    out = os.path.join(out_dir, "liboracle_native.so")
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", _HERE, "native", f"NATIVE_OUT={out}"], check=True,
+                       capture_output=True, timeout=120)
+    except (subprocess.SubprocessError, OSError):
+        return "portable"
+    _lib = None
+    lib(out)
+    return "native"
+
+
+def lib(path: str = _LIB_PATH) -> ctypes.CDLL:
     global _lib
     if _lib is None:
-        if not os.path.exists(_LIB_PATH):
+        if path == _LIB_PATH and not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        L = ctypes.CDLL(path)
         u8p = ctypes.POINTER(ctypes.c_uint8)
         u32p = ctypes.POINTER(ctypes.c_uint32)
         u64p = ctypes.POINTER(ctypes.c_uint64)

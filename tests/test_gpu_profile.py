@@ -56,3 +56,19 @@ def test_atomic_rate_calibration():
             assert 1e8 < r < 1e12, (kind, r)
         with pytest.raises(ga.GAStarError):
             e.atomic_rate(1 << 28, 1 << 22, 7)
+
+
+def test_profile_counts_hbm_levels_and_lanes():
+    # the algorithmic-byte counters: levels served from the LDS copy of a heap's
+    # top are not HBM levels, records from the candidate buffer are a part of all
+    # pushes, and the per-lane figures add up to the engine's totals
+    x = wl.puzzle15(5, depth=46, num_queues=256, table_slots=1 << 20)
+    with ga.Engine(0, profile=1, chunk_iters=4, lanes=1) as e:
+        sol = e.solve(x.request())
+        st = e.stats()
+    assert sol.status == ga.OK
+    assert 0 < st.sift_down_hbm <= st.sift_down and st.sift_up_hbm <= st.sift_up
+    assert 0 < st.push_in <= st.pushed
+    assert st.lane_launches[0] == st.step_launches and st.lane_timed_launches[0] == st.timed_launches
+    assert sum(st.lane_algo_bytes) == pytest.approx(st.algo_bytes, rel=1e-9)
+    assert st.lane_step_ms[0] == pytest.approx(st.step_ms_total, rel=1e-9) and st.step_ms_total > 0
