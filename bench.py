@@ -24,10 +24,12 @@ Measured outside the timed region:
              (ga_solve) back to back on the one-at-a-time tuning
   CPU        the oracle (sequential A*) built with -march=native on this
              host: throughput on pinned threads, latency on one pinned core
-The step kernel's roofline comes from the sampled chunks (every
---time-every-th chunk of each launch lane), whose launches are issued one at a
-time between HIP events on the lane's stream and run with the device counters
-on.  The dominant kernel is the lane whose launches take the most kernel time.
+The step kernel's roofline comes from sampled chunks of each launch lane:
+every --time-every-th chunk is timed (its launches issued one by one between
+one pair of HIP events on the lane's stream, device counters off), and the
+chunks half-way between are counted (device counters on: the algorithmic
+bytes, scaled to every launch).  The dominant kernel is the lane whose
+launches take the most kernel time.
 Its HBM traffic is this run's algorithmic bytes per launch times the
 HBM-bytes-per-algorithmic-byte ratio of the committed rocprofv3 counter runs
 (profiles/step_kernel_traffic.json).
@@ -207,11 +209,12 @@ def lane_roofline(st, cfg, elapsed_s):
     n = cfg.lanes + 1
     lanes = []
     for l in range(n):
-        if not st.lane_timed_launches[l]:
+        if not st.lane_timed_launches[l] or not st.lane_counted_launches[l]:
             continue
         avg_ms = st.lane_step_ms[l] / st.lane_timed_launches[l]
         lanes.append({
             "lane": l, "launches": st.lane_launches[l], "timed_launches": st.lane_timed_launches[l],
+            "counted_launches": st.lane_counted_launches[l],
             "avg_launch_us": round(avg_ms * 1e3, 3),
             "kernel_ms_est": round(avg_ms * st.lane_launches[l], 1),
             "algo_bytes_per_launch": round(st.lane_algo_bytes[l] / st.lane_launches[l], 1),
@@ -226,7 +229,8 @@ def lane_roofline(st, cfg, elapsed_s):
           "frac": round(dom["achieved_GBps"] / HBM_PEAK_GBS, 5), "traffic": None,
           "kernel": kernel.get(dom["lane"], f"step kernel of lane {dom['lane']}"),
           "avg_launch_us": dom["avg_launch_us"], "algo_bytes_per_launch": dom["algo_bytes_per_launch"],
-          "event_timing": "HIP events around each step launch of the sampled chunks, on the lane's stream",
+          "event_timing": "one HIP event pair around each timed chunk of step launches (issued one by one, "
+                          "device counters off), on the lane's stream; time / launches",
           "lanes": lanes,
           "expansions_per_launch": round(st.expanded / st.step_launches, 1),
           # heap-group iterations with work / launched (every mapped heap, every sampled launch)
@@ -246,8 +250,8 @@ def lane_roofline(st, cfg, elapsed_s):
                                     f"(lane {dom['lane']}) x this run's algorithmic bytes per launch")
     # global atomics of the sampled launches: one CAS per closed-set probe (hashed sets), one atomicMin per lookup
     avg_all = st.step_ms_total / st.timed_launches
-    rf["atomics_per_launch"] = round((st.probes + st.lookups) / st.timed_launches, 1)
-    rf["atomic_rate_G_per_s"] = round((st.probes + st.lookups) / st.timed_launches / (avg_all * 1e-3) / 1e9, 3)
+    rf["atomics_per_launch"] = round((st.probes + st.lookups) / st.counted_launches, 1)
+    rf["atomic_rate_G_per_s"] = round((st.probes + st.lookups) / st.counted_launches / (avg_all * 1e-3) / 1e9, 3)
     return rf
 
 
@@ -362,13 +366,13 @@ def main() -> int:
         f"{served_sps:.1f} solves/s, p50 {pct(sat_lat, 0.5):.1f} ms, p99 {pct(sat_lat, 0.99):.1f} ms, "
         f"inflight max {ss.max_inflight}")
     roofline = None
-    if not args.no_profile and st.step_launches and st.timed_launches:
+    if not args.no_profile and st.step_launches and st.timed_launches and st.counted_launches:
         roofline = lane_roofline(st, served_cfg, t_total)
         roofline["launches_before_timed"] = launches_before
         peak_min = e.atomic_rate(1 << 30, 1 << 26, 0)
         peak_cas = e.atomic_rate(1 << 30, 1 << 26, 1)
         avg_all = st.step_ms_total / st.timed_launches
-        t_atomics = (st.probes / peak_cas + st.lookups / peak_min) / st.timed_launches
+        t_atomics = (st.probes / peak_cas + st.lookups / peak_min) / st.counted_launches
         roofline["atomic_peak_G_per_s"] = {"fetch_min": round(peak_min / 1e9, 2), "cas": round(peak_cas / 1e9, 2)}
         roofline["atomic_frac"] = round(t_atomics / (avg_all * 1e-3), 4)
     log(f"[bench] roofline {roofline}")

@@ -118,8 +118,10 @@ class EngineStats(ctypes.Structure):
         ("sift_down_hbm", ctypes.c_uint64),
         ("sift_up_hbm", ctypes.c_uint64),
         ("push_in", ctypes.c_uint64),
+        ("counted_launches", ctypes.c_uint64),
         ("lane_launches", ctypes.c_uint64 * (MAX_LANES + 1)),
         ("lane_timed_launches", ctypes.c_uint64 * (MAX_LANES + 1)),
+        ("lane_counted_launches", ctypes.c_uint64 * (MAX_LANES + 1)),
         ("lane_step_ms", ctypes.c_double * (MAX_LANES + 1)),
         ("lane_algo_bytes", ctypes.c_double * (MAX_LANES + 1)),
     ]

@@ -906,59 +906,65 @@ struct ga_engine {
         }
     }
 
-    void enqueue_steps(Lane& L, uint32_t grid, const StepParams& prm, std::vector<hipEvent_t>* pe) {
+    void enqueue_steps(Lane& L, uint32_t grid, const StepParams& prm) {
         const uint32_t R = cfg.chunk_iters;
         hipStream_t stream = L.stream;
         const uint32_t variant = &L == &lanes.back() ? kWideVariant : L.variant;
-        for (uint32_t r = 0; r < R; ++r) {
-            if (pe) hip_check(hipEventRecord((*pe)[2 * r], stream), "event");
+        for (uint32_t r = 0; r < R; ++r)
             launch_step(cfg.heap_arity, variant, stream, grid, L.d_map, d_descs, d_states, L.d_iter, r, prm);
-            if (pe) hip_check(hipEventRecord((*pe)[2 * r + 1], stream), "event");
-        }
         advance_kernel<<<1, 64, 0, stream>>>(L.d_iter, R);
     }
 
     void launch_chunk(Lane& L) {
         const uint32_t R = cfg.chunk_iters;
-        // profile n > 1 samples every n-th chunk of the lane: HIP events around
-        // it, and the device counters that only profiling needs (their ballots
-        // and LDS atomics slow every expansion) in its steps only
-        const bool timed = cfg.profile && (cfg.profile == 1 || L.launched % cfg.profile == 0);
-        StepParams prm{cfg.pops_per_iter, 1u << 16, timed ? 1u : 0u, cfg.rounds, cfg.grow_thresh, {0, 0, 0}};
+        // profile n samples two kinds of chunks of the lane: chunk c is timed
+        // when c % n == 0 (one pair of HIP events around its R launches, which
+        // are issued one by one with nothing between them: no graph node, no
+        // event) and counted when c % n == n / 2 (the device counters that only
+        // profiling needs are on in its steps; their ballots and LDS atomics
+        // slow an expansion, so timed chunks run without them).  n == 1: every
+        // chunk is both.
+        const uint64_t ph = cfg.profile ? L.launched % cfg.profile : 1;
+        const bool timed = cfg.profile && ph == 0;
+        const bool counted = cfg.profile && ph == cfg.profile / 2;
+        StepParams prm{cfg.pops_per_iter, 1u << 16, counted ? 1u : 0u, cfg.rounds, cfg.grow_thresh, {0, 0, 0}};
         const int b = (int)(L.launched & 1);
         std::vector<hipEvent_t>& pe = L.prof_ev[b];
-        if (cfg.profile && pe.size() < 2 * R) {
-            for (size_t i = pe.size(); i < 2 * R; ++i) {
+        if (timed && pe.size() < 2) {
+            for (size_t i = pe.size(); i < 2; ++i) {
                 hipEvent_t ev;
                 hip_check(hipEventCreate(&ev), "hipEventCreate");
                 pe.push_back(ev);
             }
         }
         L.timed[b] = timed;
-        // Sampled chunks are launched one kernel at a time, each bracketed by
-        // its own HIP events (an event pair around a replayed graph would also
-        // time the dispatch gaps and the advance kernel).
         if (cfg.use_graph && !timed) {
-            const uint64_t key = ((uint64_t)variant_slot(L.variant) << 40) | L.grid;
+            const uint64_t key = ((uint64_t)variant_slot(L.variant) << 40) | ((uint64_t)(counted ? 1 : 0) << 32) | L.grid;
             hipGraphExec_t& ge = L.graphs[key];
             if (!ge) {
                 hipGraph_t g;
                 hip_check(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal), "begin capture");
-                enqueue_steps(L, L.grid, prm, nullptr);
+                enqueue_steps(L, L.grid, prm);
                 hip_check(hipStreamEndCapture(L.stream, &g), "end capture");
                 hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "instantiate");
                 (void)hipGraphDestroy(g);
             }
             hip_check(hipGraphLaunch(ge, L.stream), "graph launch");
         } else {
-            enqueue_steps(L, L.n_wg, prm, timed ? &pe : nullptr);
+            if (timed) hip_check(hipEventRecord(pe[0], L.stream), "event");
+            enqueue_steps(L, L.n_wg, prm);
+            if (timed) hip_check(hipEventRecord(pe[1], L.stream), "event");
         }
         hip_check(hipGetLastError(), "step launch");
         L.host_base += R;
         stats.step_launches += R;
         const uint32_t li = (uint32_t)(&L - lanes.data());
         stats.lane_launches[li] += R;
-        if (timed) stats.heap_groups += L.n_heaps * R;
+        if (counted) {
+            stats.heap_groups += L.n_heaps * R;
+            stats.counted_launches += R;
+            stats.lane_counted_launches[li] += R;
+        }
         uint32_t n_read = 0;
         for (uint32_t s = 0; s < cfg.max_instances; ++s)
             if (resident[s]) n_read = s + 1;
@@ -970,19 +976,18 @@ struct ga_engine {
         ++L.launched;
     }
 
-    // Called once the chunk's read-back event has completed.
+    // Called once the chunk's read-back event has completed: the time of a
+    // timed chunk's R launches (and its one-block counter advance).
     void collect_profile(Lane& L, uint64_t chunk) {
         if (!cfg.profile || !L.timed[chunk & 1]) return;
         std::vector<hipEvent_t>& pe = L.prof_ev[chunk & 1];
         const uint32_t li = (uint32_t)(&L - lanes.data());
+        float ms = 0;
+        hip_check(hipEventElapsedTime(&ms, pe[0], pe[1]), "elapsed");
         stats.timed_launches += cfg.chunk_iters;
         stats.lane_timed_launches[li] += cfg.chunk_iters;
-        for (size_t r = 0; r < cfg.chunk_iters && 2 * r + 1 < pe.size(); ++r) {
-            float ms = 0;
-            hip_check(hipEventElapsedTime(&ms, pe[2 * r], pe[2 * r + 1]), "elapsed");
-            stats.step_ms_total += ms;
-            stats.lane_step_ms[li] += ms;
-        }
+        stats.step_ms_total += ms;
+        stats.lane_step_ms[li] += ms;
     }
 
     // ---- results -------------------------------------------------------------
@@ -1047,10 +1052,10 @@ struct ga_engine {
         return b;
     }
     // algorithmic bytes of all step launches of lane l: the sampled terms
-    // scaled from its profiled launches to all of them
+    // scaled from its counted launches to all of them
     double algo_lane(uint32_t l) const {
-        const double scale = stats.lane_timed_launches[l]
-                                 ? (double)stats.lane_launches[l] / (double)stats.lane_timed_launches[l]
+        const double scale = stats.lane_counted_launches[l]
+                                 ? (double)stats.lane_launches[l] / (double)stats.lane_counted_launches[l]
                                  : 0.0;
         return algo_always[l] + algo_sampled[l] * scale;
     }

@@ -119,9 +119,11 @@ typedef struct ga_config {
     uint32_t slack_grid;     /* the same for grids */
 } ga_config;
 
-/* Engine statistics.  Kernel timing and the counts marked (profile) cover the
-   sampled step launches only (timed_launches of them; all launches when
-   cfg.profile == 1), and are zero when cfg.profile == 0. */
+/* Engine statistics.  With cfg.profile = n, kernel time covers the timed
+   chunks (every n-th chunk of a lane: timed_launches launches) and the counts
+   marked (profile) the counted chunks (those half-way between: counted_launches
+   launches); n = 1 times and counts every chunk; both are zero when
+   cfg.profile == 0. */
 typedef struct ga_engine_stats {
     uint64_t step_launches;  /* step kernels launched */
     double step_ms_total;    /* summed step-kernel time from HIP events (profile) */
@@ -140,16 +142,18 @@ typedef struct ga_engine_stats {
     double algo_bytes;       /* algorithmic bytes of all step launches, the profiled terms scaled
                                 from the sampled launches to all of them (per-unit figures: the
                                 design notes' kernel table) */
-    uint64_t timed_launches; /* step launches sampled: step_ms_total and the (profile) counts cover them */
-    uint64_t heap_groups;    /* heap-group iterations launched in the sampled launches (every mapped heap) */
+    uint64_t timed_launches; /* step launches of the timed chunks: step_ms_total covers them */
+    uint64_t heap_groups;    /* heap-group iterations launched in the counted launches (every mapped heap) */
     uint64_t work_groups;    /* heap-group iterations with work: pushes, or a pop within the f gate (profile) */
     uint64_t sift_down_hbm;  /* sift-down levels whose children were read from global memory (profile) */
     uint64_t sift_up_hbm;    /* sift-up levels whose parent was read from global memory (profile) */
     uint64_t push_in;        /* records pushed from the candidate buffer (profile) */
+    uint64_t counted_launches; /* step launches of the counted chunks: the (profile) counts cover them */
     /* per launch lane: index l < lanes for the configured lanes, index lanes for the 24-puzzle lane */
     uint64_t lane_launches[GA_MAX_LANES + 1];        /* step launches */
-    uint64_t lane_timed_launches[GA_MAX_LANES + 1];  /* sampled step launches */
-    double lane_step_ms[GA_MAX_LANES + 1];           /* their summed kernel time from HIP events */
+    uint64_t lane_timed_launches[GA_MAX_LANES + 1];  /* step launches of timed chunks */
+    uint64_t lane_counted_launches[GA_MAX_LANES + 1]; /* step launches of counted chunks */
+    double lane_step_ms[GA_MAX_LANES + 1];           /* timed chunks' time from HIP events (one pair per chunk) */
     double lane_algo_bytes[GA_MAX_LANES + 1];        /* algorithmic bytes of all the lane's launches */
 } ga_engine_stats;
 

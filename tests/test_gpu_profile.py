@@ -29,7 +29,7 @@ def test_profile_off_counts_nothing():
 def test_profile_every_chunk_counts_every_launch():
     g, sol, st = _solve(1)
     assert sol.status == ga.OK
-    assert st.timed_launches == st.step_launches
+    assert st.timed_launches == st.counted_launches == st.step_launches
     assert st.pops >= st.expanded > 0
     assert 0 < st.work_groups <= st.heap_groups
 
@@ -38,10 +38,14 @@ def test_profile_sampled_chunks_scale_to_every_launch():
     _, full, st1 = _solve(1)
     _, samp, st4 = _solve(4)
     assert samp.status == ga.OK and samp.cost == full.cost
-    # a chunk in four is sampled (the first chunk of the lane always is)
+    # a chunk in four is timed (the first chunk of the lane always is), and the
+    # chunk half-way between two timed ones is counted
     assert 0 < st4.timed_launches < st4.step_launches
-    assert st4.timed_launches % 4 == 0  # whole chunks of chunk_iters = 4 launches
-    # counters of the sampled launches only, yet covering real work
+    assert 0 < st4.counted_launches < st4.step_launches
+    assert st4.timed_launches + st4.counted_launches <= st4.step_launches
+    assert st4.timed_launches % 4 == 0 and st4.counted_launches % 4 == 0  # whole chunks of 4 launches
+    assert st4.step_ms_total > 0
+    # counters of the counted launches only, yet covering real work
     assert 0 < st4.pops < st1.pops
     assert 0 < st4.work_groups <= st4.heap_groups
     # algorithmic bytes are scaled from the sample to all launches: same order as counting everything
@@ -70,5 +74,6 @@ def test_profile_counts_hbm_levels_and_lanes():
     assert 0 < st.sift_down_hbm <= st.sift_down and st.sift_up_hbm <= st.sift_up
     assert 0 < st.push_in <= st.pushed
     assert st.lane_launches[0] == st.step_launches and st.lane_timed_launches[0] == st.timed_launches
+    assert st.lane_counted_launches[0] == st.counted_launches == st.step_launches
     assert sum(st.lane_algo_bytes) == pytest.approx(st.algo_bytes, rel=1e-9)
     assert st.lane_step_ms[0] == pytest.approx(st.step_ms_total, rel=1e-9) and st.step_ms_total > 0
