@@ -144,7 +144,8 @@ class ServeStats(ctypes.Structure):
 EXPORTS = (
     "ga_config_default", "ga_engine_create", "ga_engine_destroy", "ga_engine_get_stats",
     "ga_engine_reset_stats", "ga_solve", "ga_solve_puzzle", "ga_solve_grid", "ga_solve_batch",
-    "ga_serve_run", "ga_solve_noalloc", "ga_result_free", "ga_status_str", "ga_engine_last_error",
+    "ga_serve_run", "ga_serve_start", "ga_serve_submit", "ga_serve_poll", "ga_serve_drain", "ga_serve_status",
+    "ga_serve_stop", "ga_solve_noalloc", "ga_result_free", "ga_status_str", "ga_engine_last_error",
     "ga_debug_fail_run",
     "ga_device_upload", "ga_device_free", "ga_atomic_rate",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
@@ -177,6 +178,12 @@ def lib() -> ctypes.CDLL:
                                 ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint32, P(Result)]
     L.ga_solve_batch.argtypes = [ctypes.c_void_p, P(Request), ctypes.c_uint32, P(Result)]
     L.ga_serve_run.argtypes = [ctypes.c_void_p, P(Request), _u64p, ctypes.c_uint32, P(Result), P(ServeStats)]
+    L.ga_serve_start.argtypes = [ctypes.c_void_p, P(ctypes.c_void_p)]
+    L.ga_serve_submit.argtypes = [ctypes.c_void_p, P(Request), P(ctypes.c_uint64)]
+    L.ga_serve_poll.argtypes = [ctypes.c_void_p, ctypes.c_uint64, P(Result)]
+    L.ga_serve_drain.argtypes = [ctypes.c_void_p]
+    L.ga_serve_status.argtypes = [ctypes.c_void_p, P(ServeStats)]
+    L.ga_serve_stop.argtypes = [ctypes.c_void_p]
     L.ga_solve_noalloc.argtypes = [ctypes.c_void_p, P(Request), P(Result), _u64p, ctypes.c_uint32]
     L.ga_result_free.argtypes = [P(Result)]
     L.ga_result_free.restype = None
@@ -478,6 +485,10 @@ class Engine:
         self._check_run(rc, "ga_serve_run", sols)
         return sols, ss
 
+    def server(self) -> "Server":
+        """An online server on this engine (ga_serve_start): submit, poll, drain."""
+        return Server(self)
+
     def stats(self) -> EngineStats:
         s = EngineStats()
         lib().ga_engine_get_stats(self._h, ctypes.byref(s))
@@ -506,3 +517,58 @@ class Engine:
 
     def free_device(self, addr: int):
         lib().ga_device_free(self._h, addr)
+
+
+class Server:
+    """Online serving on one engine (ga_serve_start .. ga_serve_stop): requests
+    are submitted while others run and are admitted as slots and the budget
+    free up.  Usable as a context manager (stop on exit)."""
+
+    def __init__(self, engine: Engine):
+        h = ctypes.c_void_p()
+        rc = lib().ga_serve_start(engine._h, ctypes.byref(h))
+        if rc != OK:
+            raise GAStarError(rc, "ga_serve_start")
+        self._h = h
+        self._sides = {}
+
+    def submit(self, req: Request) -> int:
+        t = ctypes.c_uint64()
+        rc = lib().ga_serve_submit(self._h, ctypes.byref(req), ctypes.byref(t))
+        if rc != OK:
+            raise GAStarError(rc, "ga_serve_submit")
+        self._sides[t.value] = _board_side(req)
+        return t.value
+
+    def poll(self, ticket: int) -> Optional[Solution]:
+        """The Solution once the request has finished, else None."""
+        res = Result()
+        rc = lib().ga_serve_poll(self._h, ticket, ctypes.byref(res))
+        if rc == PENDING:
+            return None
+        if rc != OK:
+            raise GAStarError(rc, "ga_serve_poll")
+        return _take(res, self._sides.pop(ticket, 0))
+
+    def drain(self):
+        rc = lib().ga_serve_drain(self._h)
+        if rc != OK:
+            raise GAStarError(rc, "ga_serve_drain")
+
+    def status(self) -> ServeStats:
+        ss = ServeStats()
+        lib().ga_serve_status(self._h, ctypes.byref(ss))
+        return ss
+
+    def stop(self):
+        if getattr(self, "_h", None):
+            rc = lib().ga_serve_stop(self._h)
+            self._h = None
+            if rc != OK:
+                raise GAStarError(rc, "ga_serve_stop")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.stop()

@@ -11,18 +11,22 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <condition_variable>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <deque>
 #include <functional>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "gastar.h"
@@ -365,6 +369,7 @@ struct Job {
     uint32_t lane = 0;   // launch lane
     bool finished = false;
     Puzzle5Ops::Key ends[2] = {};  // 24-puzzle: start and goal in the device packing (uploaded at admission)
+    void* owner = nullptr;         // the online server's entry holding this job
 };
 
 struct PathBatch {
@@ -505,6 +510,7 @@ struct ga_engine {
     double algo_sampled[GA_MAX_LANES + 1] = {}, algo_always[GA_MAX_LANES + 1] = {};
     std::string last_error;
     bool broken = false;  // a HIP error ended a run: the device state is unknown, later runs are refused
+    std::atomic<bool> serving{false};  // an online server owns the engine (ga_serve_start .. ga_serve_stop)
     char* scratch_dev = nullptr;  // this device's scratch (ga_solve_noalloc)
 
     void init(int dev, const ga_config* c) {
@@ -1131,161 +1137,29 @@ uint32_t grid_wmin(const ga_request& r) {
     return m == 256 ? 1 : m;
 }
 
-void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* ss) {
-    const uint64_t t0 = now_ns();
-    std::deque<Job*> pending, retry;
-    std::vector<Job*> order;
-    for (auto& j : jobs) order.push_back(&j);
-    std::stable_sort(order.begin(), order.end(), [](Job* a, Job* b) { return a->arrival_ns < b->arrival_ns; });
-    for (Job* j : order) pending.push_back(j);
-    std::vector<PathBatch*> batches;
+// The run loop shared by plain, batched and served GA* and by the online
+// server.  Jobs are added in arrival order; each step() admits what has
+// arrived and fits (retries first), retires finished path walks, and lets
+// every lane handle its chunk read-backs and launch its next chunk.  With
+// `timed` arrivals a job is admitted once its arrival_ns (relative to the
+// loop's start) has passed; otherwise at once.  Throws GaError when the run
+// fails (the caller then calls discard() and ga_engine::abandon()).
+class RunLoop {
+   public:
+    RunLoop(ga_engine* e, bool timed, uint64_t t0 = now_ns()) : e(e), timed(timed), t0(t0) {}
+    RunLoop(const RunLoop&) = delete;
+    RunLoop& operator=(const RunLoop&) = delete;
+
+    ga_engine* const e;
+    const bool timed;
+    const uint64_t t0;                // the loop's start (steady clock, ns)
     uint32_t inflight = 0, max_inflight = 0;
+    std::vector<Job*> finished;       // jobs completed since the owner last cleared this list
 
-    auto finalize_batch = [&](PathBatch* b) {
-        hip_check(hipEventSynchronize(b->ev), "path event");
-        size_t n = b->jobs.size();
-        const uint32_t* lens = (const uint32_t*)b->host.p;
-        const uint64_t* out = (const uint64_t*)((char*)b->host.p + round_up(n * sizeof(uint32_t), 16));
-        for (size_t i = 0; i < n; ++i) {
-            Job& j = *b->jobs[i];
-            uint32_t len = lens[i];
-            e->fill_counters(j, b->snaps[i]);
-            j.out->cost = b->snaps[i].best;
-            if (len == kNone || len == 0) {
-                e->finish(j, GA_E_OVERFLOW, t0);
-            } else if (is_wide(j.kind)) {  // 24-puzzle: 32-byte boards (4 words) per state
-                j.out->path = (uint64_t*)calloc((size_t)len * 4, sizeof(uint64_t));
-                const uint64_t* src = out + b->offsets[i];
-                for (uint32_t k = 0; k < len; ++k) {
-                    Board5 x;
-                    std::memcpy(&x, src + 2 * (size_t)(len - 1 - k), sizeof(x));
-                    uint8_t* dst = (uint8_t*)(j.out->path + 4 * (size_t)k);
-                    for (int c = 0; c < 25; ++c) dst[c] = (uint8_t)Puzzle5Ops::get(x, c);
-                }
-                j.out->path_len = len;
-                e->finish(j, GA_OK, t0);
-            } else {
-                j.out->path = (uint64_t*)malloc(sizeof(uint64_t) * len);
-                for (uint32_t k = 0; k < len; ++k) j.out->path[k] = out[b->offsets[i] + (len - 1 - k)];
-                j.out->path_len = len;
-                e->finish(j, GA_OK, t0);
-            }
-            e->release(j);
-            --inflight;
-        }
-        e->arena->release(b->dev);
-        (void)hipEventDestroy(b->ev);
-        delete b;
-    };
+    void add(Job* j) { pending.push_back(j); }
 
-    auto handle_snapshot = [&](uint32_t lane, uint64_t chunk) {
-        Lane& L = e->lanes[lane];
-        e->collect_profile(L, chunk);
-        const InstState* snap = (const InstState*)L.h_states[chunk & 1].p;
-        std::vector<Job*> done;
-        for (uint32_t s = lane; s < L.chunk_reads[chunk & 1]; s += e->n_lanes) {
-            Job* j = e->resident[s];
-            if (!j || j->finished || j->first_chunk > chunk) continue;
-            const InstState& st = snap[s];
-            if (st.status == ST_RUNNING) {
-                j->last_iterations = st.iterations;
-                if (e->cfg.max_iterations && st.iterations > e->cfg.max_iterations) {
-                    abort_kernel<<<1, 64, 0, L.stream>>>(e->d_states, (uint32_t)s);
-                }
-                continue;
-            }
-            if (st.status == ST_DONE && st.best != kInf) {
-                e->retire(*j);  // leaves the launch map; memory kept for the path walk
-                done.push_back(j);
-                continue;
-            }
-            e->retire(*j);
-            if (st.status == ST_OVERFLOW_HEAP || st.status == ST_OVERFLOW_TABLE) {
-                e->fill_counters(*j, st);
-                e->release(*j);
-                --inflight;
-                if (j->retries >= e->cfg.max_retries) {
-                    e->finish(*j, GA_E_OVERFLOW, t0);
-                } else {
-                    ++j->retries;
-                    // 4x the structure that overflowed; 2x the other one if it was
-                    // more than a quarter full (open entries per heap, stored states),
-                    // so that a search which outgrows both does not use up its retries
-                    // one structure at a time
-                    const ga_engine::Plan p = e->plan(*j);
-                    const uint32_t open = std::max({st.open_sum[0], st.open_sum[1], st.open_sum[2]});
-                    const bool heaps_full = (uint64_t)open * 4 > (uint64_t)p.heap_cap * p.n_heaps;
-                    const bool table_full = p.table_slots && st.occupied * 4 > p.table_slots;
-                    if (st.status == ST_OVERFLOW_HEAP) {
-                        j->heap_shift += 2;
-                        if (table_full) j->table_shift += 1;
-                    } else {
-                        j->table_shift += 2;
-                        if (heaps_full) j->heap_shift += 1;
-                    }
-                    // Hashed sets: the heaps together hold at least as many
-                    // entries as the closed set may store states.  The open
-                    // list stays below the stored count, so a search that has
-                    // outgrown its table does not fail again, late, on a heap
-                    // (a restart repeats all the work done so far).
-                    for (ga_engine::Plan q = e->plan(*j);
-                         q.table_slots && q.heap_cap < (1u << 28) &&
-                         (uint64_t)q.n_heaps * q.heap_cap < q.table_slots * e->cfg.max_load_pct / 100;
-                         q = e->plan(*j))
-                        ++j->heap_shift;
-                    j->finished = false;
-                    retry.push_back(j);
-                }
-                continue;
-            }
-            // DONE without a goal = search space exhausted; ABORTED = iteration limit
-            e->fill_counters(*j, st);
-            j->out->cost = 0;
-            e->finish(*j, st.status == ST_DONE ? GA_NO_PATH : GA_E_TIMEOUT, t0);
-            e->release(*j);
-            --inflight;
-        }
-        if (done.empty()) return;
-        // walk the paths on the path stream once this lane's work so far has finished
-        PathBatch* b = new PathBatch();
-        batches.push_back(b);  // freed by finalize_batch, or by the clean-up of a failed run
-        b->jobs = done;
-        uint64_t total = 0;
-        for (Job* j : done) {
-            b->offsets.push_back(total);
-            b->snaps.push_back(snap[j->slot]);
-            total += ((uint64_t)snap[j->slot].best + 1) * words_of(j->kind);  // every move costs >= 1
-        }
-        size_t n = done.size();
-        size_t lens_bytes = round_up(n * sizeof(uint32_t), 16);
-        size_t jobs_bytes = round_up(n * sizeof(PathJob), 16);
-        b->dev = e->arena->alloc(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
-        if (!b->dev) throw GaError(GA_E_NOMEM, "device arena exhausted (paths)");
-        b->host.ensure(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
-        PathJob* pj = (PathJob*)((char*)b->host.p + lens_bytes + total * sizeof(uint64_t));
-        for (size_t i = 0; i < n; ++i)
-            pj[i] = PathJob{(uint32_t)done[i]->slot, snap[done[i]->slot].best + 1, b->offsets[i]};
-        char* dj = (char*)b->dev;
-        uint32_t* dl = (uint32_t*)(dj + jobs_bytes);
-        uint64_t* dout = (uint64_t*)(dj + jobs_bytes + lens_bytes);
-        hipEvent_t ready;
-        hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
-        hip_check(hipEventRecord(ready, L.stream), "record");
-        hip_check(hipStreamWaitEvent(e->path_stream, ready, 0), "wait");
-        hip_check(hipMemcpyAsync(dj, pj, n * sizeof(PathJob), hipMemcpyHostToDevice, e->path_stream), "upload jobs");
-        path_kernel<<<(uint32_t)n, 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n, dout, dl);
-        hip_check(hipGetLastError(), "path launch");
-        hip_check(hipMemcpyAsync(b->host.p, dl, lens_bytes + total * sizeof(uint64_t), hipMemcpyDeviceToHost,
-                                 e->path_stream),
-                  "read back paths");
-        hip_check(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming), "hipEventCreate");
-        hip_check(hipEventRecord(b->ev, e->path_stream), "record");
-        (void)hipEventDestroy(ready);
-        b->total = total;
-    };
-
-    try {
-    for (;;) {
+    // One pass; returns false once nothing is pending, running or walking.
+    bool step() {
         // -- admit: retries first, then arrivals in order
         uint64_t now = now_ns() - t0;
         for (std::deque<Job*>* q : {&retry, &pending}) {
@@ -1298,7 +1172,7 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
                 } catch (const GaError& err) {
                     if (err.code == GA_E_HIP) throw;
                     q->pop_front();
-                    e->finish(*j, err.code, t0);
+                    complete(*j, err.code);
                     continue;
                 }
                 if (!rc) break;
@@ -1347,28 +1221,196 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         }
         if (busy) {
             if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
-            continue;
+            return true;
         }
         if (!batches.empty()) {
             finalize_batch(batches.front());
             batches.erase(batches.begin());
-            continue;
+            return true;
         }
-        if (pending.empty() && retry.empty()) break;
+        if (pending.empty() && retry.empty()) return false;
         if (retry.empty() && timed) {
             uint64_t wait = pending.front()->arrival_ns;
             now = now_ns() - t0;
             if (wait > now) std::this_thread::sleep_for(std::chrono::nanoseconds(std::min<uint64_t>(wait - now, 200000)));
         }
+        return true;
     }
-    } catch (...) {
-        // the path walks in flight belong to this run: wait for them and free them
+
+    // After a failure: waits for the path walks in flight and frees them (the
+    // lanes and resident jobs are ga_engine::abandon's).
+    void discard() {
         (void)hipStreamSynchronize(e->path_stream);
         for (PathBatch* b : batches) {
             e->arena->release(b->dev);
             if (b->ev) (void)hipEventDestroy(b->ev);
             delete b;
         }
+        batches.clear();
+    }
+
+   private:
+    std::deque<Job*> pending, retry;
+    std::vector<PathBatch*> batches;
+
+    void complete(Job& j, int status) {
+        e->finish(j, status, t0);
+        finished.push_back(&j);
+    }
+    void finalize_batch(PathBatch* b) {
+        hip_check(hipEventSynchronize(b->ev), "path event");
+        size_t n = b->jobs.size();
+        const uint32_t* lens = (const uint32_t*)b->host.p;
+        const uint64_t* out = (const uint64_t*)((char*)b->host.p + round_up(n * sizeof(uint32_t), 16));
+        for (size_t i = 0; i < n; ++i) {
+            Job& j = *b->jobs[i];
+            uint32_t len = lens[i];
+            e->fill_counters(j, b->snaps[i]);
+            j.out->cost = b->snaps[i].best;
+            if (len == kNone || len == 0) {
+                complete(j, GA_E_OVERFLOW);
+            } else if (is_wide(j.kind)) {  // 24-puzzle: 32-byte boards (4 words) per state
+                j.out->path = (uint64_t*)calloc((size_t)len * 4, sizeof(uint64_t));
+                const uint64_t* src = out + b->offsets[i];
+                for (uint32_t k = 0; k < len; ++k) {
+                    Board5 x;
+                    std::memcpy(&x, src + 2 * (size_t)(len - 1 - k), sizeof(x));
+                    uint8_t* dst = (uint8_t*)(j.out->path + 4 * (size_t)k);
+                    for (int c = 0; c < 25; ++c) dst[c] = (uint8_t)Puzzle5Ops::get(x, c);
+                }
+                j.out->path_len = len;
+                complete(j, GA_OK);
+            } else {
+                j.out->path = (uint64_t*)malloc(sizeof(uint64_t) * len);
+                for (uint32_t k = 0; k < len; ++k) j.out->path[k] = out[b->offsets[i] + (len - 1 - k)];
+                j.out->path_len = len;
+                complete(j, GA_OK);
+            }
+            e->release(j);
+            --inflight;
+        }
+        e->arena->release(b->dev);
+        (void)hipEventDestroy(b->ev);
+        delete b;
+    }
+
+    void handle_snapshot(uint32_t lane, uint64_t chunk) {
+        Lane& L = e->lanes[lane];
+        e->collect_profile(L, chunk);
+        const InstState* snap = (const InstState*)L.h_states[chunk & 1].p;
+        std::vector<Job*> done;
+        for (uint32_t s = lane; s < L.chunk_reads[chunk & 1]; s += e->n_lanes) {
+            Job* j = e->resident[s];
+            if (!j || j->finished || j->first_chunk > chunk) continue;
+            const InstState& st = snap[s];
+            if (st.status == ST_RUNNING) {
+                j->last_iterations = st.iterations;
+                if (e->cfg.max_iterations && st.iterations > e->cfg.max_iterations) {
+                    abort_kernel<<<1, 64, 0, L.stream>>>(e->d_states, (uint32_t)s);
+                }
+                continue;
+            }
+            if (st.status == ST_DONE && st.best != kInf) {
+                e->retire(*j);  // leaves the launch map; memory kept for the path walk
+                done.push_back(j);
+                continue;
+            }
+            e->retire(*j);
+            if (st.status == ST_OVERFLOW_HEAP || st.status == ST_OVERFLOW_TABLE) {
+                e->fill_counters(*j, st);
+                e->release(*j);
+                --inflight;
+                if (j->retries >= e->cfg.max_retries) {
+                    complete(*j, GA_E_OVERFLOW);
+                } else {
+                    ++j->retries;
+                    // 4x the structure that overflowed; 2x the other one if it was
+                    // more than a quarter full (open entries per heap, stored states),
+                    // so that a search which outgrows both does not use up its retries
+                    // one structure at a time
+                    const ga_engine::Plan p = e->plan(*j);
+                    const uint32_t open = std::max({st.open_sum[0], st.open_sum[1], st.open_sum[2]});
+                    const bool heaps_full = (uint64_t)open * 4 > (uint64_t)p.heap_cap * p.n_heaps;
+                    const bool table_full = p.table_slots && st.occupied * 4 > p.table_slots;
+                    if (st.status == ST_OVERFLOW_HEAP) {
+                        j->heap_shift += 2;
+                        if (table_full) j->table_shift += 1;
+                    } else {
+                        j->table_shift += 2;
+                        if (heaps_full) j->heap_shift += 1;
+                    }
+                    // Hashed sets: the heaps together hold at least as many
+                    // entries as the closed set may store states.  The open
+                    // list stays below the stored count, so a search that has
+                    // outgrown its table does not fail again, late, on a heap
+                    // (a restart repeats all the work done so far).
+                    for (ga_engine::Plan q = e->plan(*j);
+                         q.table_slots && q.heap_cap < (1u << 28) &&
+                         (uint64_t)q.n_heaps * q.heap_cap < q.table_slots * e->cfg.max_load_pct / 100;
+                         q = e->plan(*j))
+                        ++j->heap_shift;
+                    j->finished = false;
+                    retry.push_back(j);
+                }
+                continue;
+            }
+            // DONE without a goal = search space exhausted; ABORTED = iteration limit
+            e->fill_counters(*j, st);
+            j->out->cost = 0;
+            complete(*j, st.status == ST_DONE ? GA_NO_PATH : GA_E_TIMEOUT);
+            e->release(*j);
+            --inflight;
+        }
+        if (done.empty()) return;
+        // walk the paths on the path stream once this lane's work so far has finished
+        PathBatch* b = new PathBatch();
+        batches.push_back(b);  // freed by finalize_batch, or by the clean-up of a failed run
+        b->jobs = done;
+        uint64_t total = 0;
+        for (Job* j : done) {
+            b->offsets.push_back(total);
+            b->snaps.push_back(snap[j->slot]);
+            total += ((uint64_t)snap[j->slot].best + 1) * words_of(j->kind);  // every move costs >= 1
+        }
+        size_t n = done.size();
+        size_t lens_bytes = round_up(n * sizeof(uint32_t), 16);
+        size_t jobs_bytes = round_up(n * sizeof(PathJob), 16);
+        b->dev = e->arena->alloc(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
+        if (!b->dev) throw GaError(GA_E_NOMEM, "device arena exhausted (paths)");
+        b->host.ensure(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
+        PathJob* pj = (PathJob*)((char*)b->host.p + lens_bytes + total * sizeof(uint64_t));
+        for (size_t i = 0; i < n; ++i)
+            pj[i] = PathJob{(uint32_t)done[i]->slot, snap[done[i]->slot].best + 1, b->offsets[i]};
+        char* dj = (char*)b->dev;
+        uint32_t* dl = (uint32_t*)(dj + jobs_bytes);
+        uint64_t* dout = (uint64_t*)(dj + jobs_bytes + lens_bytes);
+        hipEvent_t ready;
+        hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventRecord(ready, L.stream), "record");
+        hip_check(hipStreamWaitEvent(e->path_stream, ready, 0), "wait");
+        hip_check(hipMemcpyAsync(dj, pj, n * sizeof(PathJob), hipMemcpyHostToDevice, e->path_stream), "upload jobs");
+        path_kernel<<<(uint32_t)n, 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n, dout, dl);
+        hip_check(hipGetLastError(), "path launch");
+        hip_check(hipMemcpyAsync(b->host.p, dl, lens_bytes + total * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                 e->path_stream),
+                  "read back paths");
+        hip_check(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming), "hipEventCreate");
+        hip_check(hipEventRecord(b->ev, e->path_stream), "record");
+        (void)hipEventDestroy(ready);
+        b->total = total;
+    }
+};
+
+void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* ss) {
+    RunLoop loop(e, timed);
+    std::vector<Job*> order;
+    for (auto& j : jobs) order.push_back(&j);
+    std::stable_sort(order.begin(), order.end(), [](Job* a, Job* b) { return a->arrival_ns < b->arrival_ns; });
+    for (Job* j : order) loop.add(j);
+    try {
+        while (loop.step()) loop.finished.clear();
+    } catch (...) {
+        loop.discard();
         throw;
     }
     if (ss) {
@@ -1395,7 +1437,34 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         ss->solves_per_sec = ss->wall_ns ? ss->completed / (ss->wall_ns / 1e9) : 0.0;
         ss->p50_ms = pct(0.50);
         ss->p99_ms = pct(0.99);
-        ss->max_inflight = max_inflight;
+        ss->max_inflight = loop.max_inflight;
+    }
+}
+
+
+// The checks every request passes before it reaches the device: the result
+// starts as GA_PENDING; a malformed request gets GA_E_INVALID and an
+// unsolvable puzzle GA_NO_PATH at once (the job is then marked finished).
+void prepare_job(Job& j, const ga_request& r, ga_result* out, uint64_t arrival_ns) {
+    memset(out, 0, sizeof(ga_result));
+    out->status = GA_PENDING;
+    out->user_tag = r.user_tag;
+    j.req = &r;
+    j.out = out;
+    j.arrival_ns = arrival_ns;
+    const int v = validate(r);
+    if (v != GA_OK) {
+        out->status = v;
+        j.finished = true;
+        return;
+    }
+    j.kind = kind_of(r);
+    if (r.kind == GA_GRID) j.w_min = grid_wmin(r);
+    const bool solvable = r.kind != GA_PUZZLE || (r.side == 5 ? ga_board_solvable(5, r.cells, r.cells + 25)
+                                                              : ga_puzzle_solvable(r.side, r.start, r.goal));
+    if (!solvable) {
+        out->status = GA_NO_PATH;
+        j.finished = true;
     }
 }
 
@@ -1410,29 +1479,7 @@ int run_checked(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* out
                 const Runner& runner) {
     if ((!reqs && n) || (!outs && n)) return GA_E_INVALID;
     std::vector<Job> jobs(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        memset(&outs[i], 0, sizeof(ga_result));
-        outs[i].status = GA_PENDING;
-        outs[i].user_tag = reqs[i].user_tag;
-        jobs[i].req = &reqs[i];
-        jobs[i].out = &outs[i];
-        jobs[i].arrival_ns = arrivals ? arrivals[i] : 0;
-        int v = validate(reqs[i]);
-        if (v != GA_OK) {
-            outs[i].status = v;
-            jobs[i].finished = true;
-            continue;
-        }
-        jobs[i].kind = kind_of(reqs[i]);
-        if (reqs[i].kind == GA_GRID) jobs[i].w_min = grid_wmin(reqs[i]);
-        const bool solvable = reqs[i].kind != GA_PUZZLE ||
-                              (reqs[i].side == 5 ? ga_board_solvable(5, reqs[i].cells, reqs[i].cells + 25)
-                                                 : ga_puzzle_solvable(reqs[i].side, reqs[i].start, reqs[i].goal));
-        if (!solvable) {
-            outs[i].status = GA_NO_PATH;
-            jobs[i].finished = true;
-        }
-    }
+    for (uint32_t i = 0; i < n; ++i) prepare_job(jobs[i], reqs[i], &outs[i], arrivals ? arrivals[i] : 0);
     std::vector<Job> live;
     for (uint32_t i = 0; i < n; ++i)
         if (!jobs[i].finished) live.push_back(jobs[i]);
@@ -1461,12 +1508,139 @@ int run_checked(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* out
 int run_engine(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, uint32_t n, ga_result* outs,
                ga_serve_stats* ss) {
     if (!e) return GA_E_INVALID;
+    if (e->serving) {  // refused before anything runs: every result says so
+        e->last_error = "an online server owns the engine";
+        if (!reqs || !outs) return GA_E_INVALID;
+        for (uint32_t i = 0; i < n; ++i) {
+            Job j;
+            prepare_job(j, reqs[i], &outs[i], 0);
+            if (outs[i].status == GA_PENDING) outs[i].status = GA_E_INVALID;
+        }
+        return GA_E_INVALID;
+    }
     return run_checked(e, reqs, n, outs, arrivals, [&](std::vector<Job>& live) {
         if (e->broken) throw GaError(GA_E_HIP, "the engine was stopped by a HIP error in an earlier run");
         hip_check(hipSetDevice(e->device), "hipSetDevice");
         run_jobs(e, live, arrivals != nullptr, ss);
     });
 }
+
+// Percentile p of sorted values (the first value at or above p of them).
+double percentile(const std::vector<double>& sorted, double p) {
+    if (sorted.empty()) return 0.0;
+    const size_t k = (size_t)std::min<double>(sorted.size() - 1, std::ceil(p * sorted.size()) - 1);
+    return sorted[k];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// online server: requests submitted while the engine runs
+// ---------------------------------------------------------------------------
+// One engine-owned thread drives the RunLoop; submissions reach it through
+// a queue.  A request is admitted as soon as a slot and the budget allow,
+// into the same launches as the requests already resident.
+struct ga_server {
+    struct Entry {
+        ga_request req{};
+        std::vector<uint8_t> data;  // copy of the host grid cells, or of the two side-5 boards
+        ga_result res{};
+        Job job;
+        bool done = false;
+    };
+
+    ga_engine* e = nullptr;
+    uint64_t t0 = 0;  // the loop's clock origin: arrival times are submission times after it
+    std::thread th;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stop = false;
+    int error = GA_OK;
+    uint64_t next_ticket = 1;
+    std::unordered_map<uint64_t, std::unique_ptr<Entry>> entries;  // by ticket, until polled
+    std::deque<Entry*> incoming;  // submitted, not yet given to the loop
+    uint32_t outstanding = 0;     // submitted and not finished
+    uint32_t completed = 0, failed = 0, max_inflight = 0;
+    std::vector<double> lat_ms;  // latencies of the completed requests
+    uint64_t first_arrival = ~0ull, last_done = 0;
+
+    // with mu held
+    void mark_done(Entry& en) {
+        en.done = true;
+        --outstanding;
+        const int st = en.res.status;
+        if (st == GA_OK || st == GA_NO_PATH) {
+            ++completed;
+            lat_ms.push_back(en.res.latency_ns / 1e6);
+            last_done = std::max(last_done, en.job.arrival_ns + en.res.latency_ns);
+        } else {
+            ++failed;
+        }
+    }
+
+    // with mu held: the run failed with `code`; every request not finished gets it
+    void fail_all(int code) {
+        error = code;
+        for (auto& kv : entries)
+            if (!kv.second->done) {
+                kv.second->res.status = code;
+                mark_done(*kv.second);
+            }
+        incoming.clear();
+    }
+
+    void run() {
+        RunLoop loop(e, false, t0);
+        try {
+            hip_check(hipSetDevice(e->device), "hipSetDevice");
+        } catch (const GaError& err) {
+            std::lock_guard<std::mutex> lk(mu);
+            e->last_error = err.what();
+            fail_all(err.code);
+            cv_done.notify_all();
+            return;
+        }
+        for (;;) {
+            std::deque<Entry*> fresh;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                fresh.swap(incoming);
+            }
+            for (Entry* en : fresh) loop.add(&en->job);
+            bool more = false;
+            int code = GA_OK;
+            try {
+                more = loop.step();
+            } catch (const GaError& err) {
+                e->last_error = err.what();
+                code = err.code;
+            } catch (const std::bad_alloc&) {
+                e->last_error = "host memory exhausted";
+                code = GA_E_NOMEM;
+            }
+            std::unique_lock<std::mutex> lk(mu);
+            for (Job* j : loop.finished) mark_done(*static_cast<Entry*>(j->owner));
+            loop.finished.clear();
+            max_inflight = std::max(max_inflight, loop.max_inflight);
+            if (code != GA_OK) {
+                lk.unlock();
+                loop.discard();
+                if (!e->broken) e->abandon(code);
+                lk.lock();
+                fail_all(code);
+                cv_done.notify_all();
+                return;
+            }
+            cv_done.notify_all();
+            if (!more) {
+                if (stop && incoming.empty()) return;
+                cv_work.wait_for(lk, std::chrono::milliseconds(5), [&] { return stop || !incoming.empty(); });
+            }
+        }
+    }
+};
+
+namespace {
 
 // Random 64-bit global atomics at agent scope, the closed sets' operations:
 // kind 0 a fetch-min (the atomicMin on a slot's g | parent), kind 1 a CAS
@@ -1706,7 +1880,7 @@ int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {
 void ga_engine_destroy(ga_engine* e) { delete e; }
 
 int ga_engine_get_stats(ga_engine* e, ga_engine_stats* out) {
-    if (!e || !out) return GA_E_INVALID;
+    if (!e || !out || e->serving) return GA_E_INVALID;
     *out = e->stats;
     out->algo_bytes = e->algo_total();
     for (uint32_t l = 0; l < e->n_lanes; ++l) out->lane_algo_bytes[l] = e->algo_lane(l);
@@ -1800,6 +1974,10 @@ int ga_solve_noalloc(ga_engine* e, const ga_request* req, ga_result* out, uint64
         e->last_error = "the engine was stopped by a HIP error in an earlier run";
         return out->status = GA_E_HIP;
     }
+    if (e->serving) {
+        e->last_error = "an online server owns the engine";
+        return out->status = GA_E_INVALID;
+    }
     try {
         solve_noalloc(e, *req, out, path_buf, path_cap);
     } catch (const GaError& err) {
@@ -1810,6 +1988,122 @@ int ga_solve_noalloc(ga_engine* e, const ga_request* req, ga_result* out, uint64
         (void)hipGetLastError();
     }
     return out->status;
+}
+
+int ga_serve_start(ga_engine* e, ga_server** out) {
+    if (!e || !out) return GA_E_INVALID;
+    *out = nullptr;
+    if (e->broken) return GA_E_HIP;
+    bool expected = false;
+    if (!e->serving.compare_exchange_strong(expected, true)) {
+        e->last_error = "an online server already owns the engine";
+        return GA_E_INVALID;
+    }
+    ga_server* sv = new (std::nothrow) ga_server();
+    if (!sv) {
+        e->serving = false;
+        return GA_E_NOMEM;
+    }
+    sv->e = e;
+    sv->t0 = now_ns();
+    try {
+        sv->th = std::thread([sv] { sv->run(); });
+    } catch (const std::system_error&) {
+        delete sv;
+        e->serving = false;
+        return GA_E_NOMEM;
+    }
+    *out = sv;
+    return GA_OK;
+}
+
+int ga_serve_submit(ga_server* sv, const ga_request* req, uint64_t* ticket) {
+    if (!sv || !req || !ticket) return GA_E_INVALID;
+    std::unique_ptr<ga_server::Entry> en(new (std::nothrow) ga_server::Entry());
+    if (!en) return GA_E_NOMEM;
+    en->req = *req;
+    if (validate(*req) == GA_OK) {  // copy the host data the request points to
+        const ga_request& r = *req;
+        size_t bytes = 0;
+        if (r.kind == GA_GRID && r.cells && !r.cells_device) bytes = (size_t)r.width * r.height;
+        if (r.kind == GA_PUZZLE && r.side == 5) bytes = 50;
+        if (bytes) {
+            try {
+                en->data.assign(r.cells, r.cells + bytes);
+            } catch (const std::bad_alloc&) {
+                return GA_E_NOMEM;
+            }
+            en->req.cells = en->data.data();
+        }
+    }
+    const uint64_t t = now_ns();
+    prepare_job(en->job, en->req, &en->res, t > sv->t0 ? t - sv->t0 : 0);  // host checks only
+    en->job.owner = en.get();
+    std::lock_guard<std::mutex> lk(sv->mu);
+    if (sv->error != GA_OK) return sv->error;
+    if (sv->stop) return GA_E_INVALID;
+    sv->first_arrival = std::min(sv->first_arrival, en->job.arrival_ns);
+    ++sv->outstanding;
+    ga_server::Entry* raw = en.get();
+    *ticket = sv->next_ticket++;
+    sv->entries.emplace(*ticket, std::move(en));
+    if (raw->job.finished) {  // failed its checks: final at once
+        sv->mark_done(*raw);
+        sv->cv_done.notify_all();
+    } else {
+        sv->incoming.push_back(raw);
+        sv->cv_work.notify_one();
+    }
+    return GA_OK;
+}
+
+int ga_serve_poll(ga_server* sv, uint64_t ticket, ga_result* out) {
+    if (!sv || !out) return GA_E_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    auto it = sv->entries.find(ticket);
+    if (it == sv->entries.end()) return GA_E_INVALID;
+    if (!it->second->done) return GA_PENDING;
+    *out = it->second->res;  // the path now belongs to the caller (ga_result_free)
+    sv->entries.erase(it);
+    return GA_OK;
+}
+
+int ga_serve_drain(ga_server* sv) {
+    if (!sv) return GA_E_INVALID;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    sv->cv_done.wait(lk, [&] { return sv->outstanding == 0; });
+    return sv->error;
+}
+
+int ga_serve_status(ga_server* sv, ga_serve_stats* out) {
+    if (!sv || !out) return GA_E_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    std::vector<double> lat = sv->lat_ms;
+    std::sort(lat.begin(), lat.end());
+    memset(out, 0, sizeof(*out));
+    out->completed = sv->completed;
+    out->failed = sv->failed;
+    out->wall_ns = sv->last_done > sv->first_arrival && sv->first_arrival != ~0ull ? sv->last_done - sv->first_arrival : 0;
+    out->solves_per_sec = out->wall_ns ? sv->completed / (out->wall_ns / 1e9) : 0.0;
+    out->p50_ms = percentile(lat, 0.50);
+    out->p99_ms = percentile(lat, 0.99);
+    out->max_inflight = sv->max_inflight;
+    return GA_OK;
+}
+
+int ga_serve_stop(ga_server* sv) {
+    if (!sv) return GA_E_INVALID;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_work.notify_all();
+    if (sv->th.joinable()) sv->th.join();
+    const int rc = sv->error;
+    for (auto& kv : sv->entries) ga_result_free(&kv.second->res);  // results never polled
+    sv->e->serving = false;
+    delete sv;
+    return rc;
 }
 
 const char* ga_engine_last_error(ga_engine* e) { return e ? e->last_error.c_str() : ""; }

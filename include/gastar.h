@@ -211,6 +211,36 @@ int ga_solve_batch(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* 
 int ga_serve_run(ga_engine* e, const ga_request* reqs, const uint64_t* arrival_ns, uint32_t n,
                  ga_result* outs, ga_serve_stats* stats);
 
+/* ---- online serving: requests submitted while the engine runs ----
+ * No reference counterpart (the reference solves one instance per a_star<>()
+ * call, a_star.h:223-224).  ga_serve_start hands the engine to a thread of
+ * its own, which admits every submitted request as soon as a slot and the
+ * admission budget (cfg.budget_bytes) allow, into the same launches as the
+ * requests already resident (continuous batching); each result is the same
+ * as the request's result alone.  While a server runs, the engine's other
+ * entry points return GA_E_INVALID; stop the server before destroying the
+ * engine.  The server functions may be called from any thread. */
+typedef struct ga_server ga_server;
+int ga_serve_start(ga_engine* e, ga_server** out);
+/* Queues one request and returns its ticket.  Host data the request points
+ * to (grid cells, side-5 boards) is copied; device cells (cells_device) must
+ * stay valid until the request has finished.  Latency is measured from this
+ * call.  After a failed run every call returns that run's error. */
+int ga_serve_submit(ga_server* s, const ga_request* req, uint64_t* ticket);
+/* GA_PENDING while the request runs; once it has finished, fills *out (the
+ * request's own outcome is out->status; the path belongs to the caller, see
+ * ga_result_free), forgets the ticket and returns GA_OK.  GA_E_INVALID for an
+ * unknown ticket. */
+int ga_serve_poll(ga_server* s, uint64_t ticket, ga_result* out);
+/* Waits until every submitted request has finished; returns GA_OK, or the
+ * error of a failed run (whose unfinished requests then carry it). */
+int ga_serve_drain(ga_server* s);
+/* Counts and latency percentiles of the requests finished so far. */
+int ga_serve_status(ga_server* s, ga_serve_stats* out);
+/* Waits for every submitted request, ends the thread, frees the results
+ * never polled and the server; returns as ga_serve_drain. */
+int ga_serve_stop(ga_server* s);
+
 /* Plain GA* of one request with no device allocation.  No counterpart in the
  * reference: for callers that cannot allocate device memory (tight serving
  * loops, code that must not call hipMalloc).  The same search as ga_solve,

@@ -258,3 +258,18 @@ def test_solve_noalloc_rejects_bad_arguments_before_any_device_call():
     req = ga.puzzle_request(3, ga.gen_puzzle(3, 10, 1))
     buf = (ctypes.c_uint64 * 4)()
     assert ga.lib().ga_solve_noalloc(None, ctypes.byref(req), ctypes.byref(res), buf, 4) == ga.E_INVALID
+
+
+def test_server_entry_points_reject_bad_arguments_before_any_device_call():
+    L = ga.lib()
+    h = ctypes.c_void_p()
+    assert L.ga_serve_start(None, ctypes.byref(h)) == ga.E_INVALID and not h.value
+    t = ctypes.c_uint64()
+    req = ga.puzzle_request(3, ga.gen_puzzle(3, 10, 1))
+    res = ga.Result()
+    ss = ga.ServeStats()
+    assert L.ga_serve_submit(None, ctypes.byref(req), ctypes.byref(t)) == ga.E_INVALID
+    assert L.ga_serve_poll(None, 1, ctypes.byref(res)) == ga.E_INVALID
+    assert L.ga_serve_drain(None) == ga.E_INVALID
+    assert L.ga_serve_status(None, ctypes.byref(ss)) == ga.E_INVALID
+    assert L.ga_serve_stop(None) == ga.E_INVALID
