@@ -143,6 +143,20 @@ def puzzle24_leg(e, heaps: int, cores, cpu_seconds: float):
     out = {"solves_per_sec": round(ok / gpu_t, 3), "instances": P24_N, "ok": ok, "seconds": round(gpu_t, 3),
            "workload": f"24-puzzle walks of depth U{{{P24_DEPTH[0]}..{P24_DEPTH[1]}}} from the ordered board, "
                        f"{heaps} heaps each, one batch"}
+    # walks of depth U{80..120}: out of reach of Manhattan; the pattern databases (ga_config.pdb24,
+    # built on the device at the first such request) bring them within reach
+    deep = wl.puzzle24_deep(25, 24)
+    d_reqs = [ga.puzzle5_request(b, num_queues=heaps) for _, _, b in deep]
+    t = time.perf_counter()
+    d_sols = e.solve_batch(d_reqs)
+    d_t = time.perf_counter() - t
+    d_ok = sum(s.status == ga.OK for s in d_sols)
+    out["deep_pdb"] = {"solves_per_sec": round(d_ok / d_t, 3), "instances": len(deep), "ok": d_ok,
+                       "seconds": round(d_t, 3),
+                       "expanded_mean": round(sum(s.expanded for s in d_sols) / len(d_sols), 1),
+                       "workload": "24-puzzle walks of depth U{80..120}, additive pattern databases (5-5-5-5-4), "
+                                   f"{heaps} heaps each, one batch; costs checked in the GPU tests against "
+                                   "the pattern-database oracle"}
     if cpu_seconds <= 0:
         return out
     import oracle

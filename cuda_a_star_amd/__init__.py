@@ -92,6 +92,8 @@ class Config(ctypes.Structure):
         ("deep_cells", ctypes.c_uint64),
         ("slack_puzzle", ctypes.c_uint32),
         ("slack_grid", ctypes.c_uint32),
+        ("pdb24", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -146,6 +148,7 @@ EXPORTS = (
     "ga_engine_reset_stats", "ga_solve", "ga_solve_puzzle", "ga_solve_grid", "ga_solve_batch",
     "ga_serve_run", "ga_serve_start", "ga_serve_submit", "ga_serve_poll", "ga_serve_drain", "ga_serve_status",
     "ga_serve_stop", "ga_solve_noalloc", "ga_result_free", "ga_status_str", "ga_engine_last_error",
+    "ga_pdb24_tables",
     "ga_debug_fail_run",
     "ga_device_upload", "ga_device_free", "ga_atomic_rate",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
@@ -184,6 +187,7 @@ def lib() -> ctypes.CDLL:
     L.ga_serve_drain.argtypes = [ctypes.c_void_p]
     L.ga_serve_status.argtypes = [ctypes.c_void_p, P(ServeStats)]
     L.ga_serve_stop.argtypes = [ctypes.c_void_p]
+    L.ga_pdb24_tables.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t]
     L.ga_solve_noalloc.argtypes = [ctypes.c_void_p, P(Request), P(Result), _u64p, ctypes.c_uint32]
     L.ga_result_free.argtypes = [P(Result)]
     L.ga_result_free.restype = None
@@ -488,6 +492,14 @@ class Engine:
     def server(self) -> "Server":
         """An online server on this engine (ga_serve_start): submit, poll, drain."""
         return Server(self)
+
+    def pdb24_tables(self) -> np.ndarray:
+        """The device's 24-puzzle pattern databases (ga_pdb24_tables)."""
+        out = np.empty(5 * 25 ** 5, dtype=np.uint8)
+        rc = lib().ga_pdb24_tables(self._h, out.ctypes.data_as(_u8p), out.nbytes)
+        if rc != OK:
+            raise GAStarError(rc, "ga_pdb24_tables")
+        return out
 
     def stats(self) -> EngineStats:
         s = EngineStats()

@@ -163,6 +163,49 @@ __global__ __launch_bounds__(64) void path_kernel(const InstDesc* descs, const P
     if (threadIdx.x == 0) lens[i] = len;
 }
 
+// One level of the breadth-first search that builds a pattern database of
+// NT tiles (core.hpp, kPdbTiles): every placement at distance d marks the
+// unvisited placements one tile move away with d + 1.  Several threads may
+// mark the same byte with the same value, which is harmless; a level whose
+// predecessor marked nothing returns at once.
+template <int NT>
+__global__ void pdb_level_kernel(uint8_t* dist, uint32_t d, uint32_t* changed) {
+    if (d > 0 && __hip_atomic_load(&changed[d - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
+    uint32_t n = 1;
+    for (int i = 0; i < NT; ++i) n *= 25u;
+    bool any = false;
+    for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
+        if (dist[idx] != d) continue;
+        int pos[NT];
+        uint32_t rest = idx;
+        for (int i = 0; i < NT; ++i) {
+            pos[i] = (int)(rest % 25u);
+            rest /= 25u;
+        }
+        for (int i = 0; i < NT; ++i) {
+            const int r = pos[i] / 5, c = pos[i] % 5;
+            for (int m = 0; m < 4; ++m) {  // row +1, row -1, column +1, column -1
+                const int nr = r + (m == 0) - (m == 1), nc = c + (m == 2) - (m == 3);
+                if (nr < 0 || nr > 4 || nc < 0 || nc > 4) continue;
+                const int np = nr * 5 + nc;
+                bool taken = false;
+                for (int k = 0; k < NT; ++k) taken |= (k != i && pos[k] == np);
+                if (taken) continue;
+                const uint32_t nidx = (uint32_t)((int)idx + (np - pos[i]) * (int)pow25((uint32_t)i));
+                if (dist[nidx] == 0xFF) {
+                    dist[nidx] = (uint8_t)(d + 1);
+                    any = true;
+                }
+            }
+        }
+    }
+    if (any) __hip_atomic_store(&changed[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ void put_byte_kernel(uint8_t* p, uint8_t v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
 // Launch map of a single resident instance (slot 0) with n_heaps heaps.
 __global__ void map_kernel(WgEntry* map, uint32_t n_wg, uint32_t heaps_per_wg, uint32_t n_heaps) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -512,6 +555,7 @@ struct ga_engine {
     bool broken = false;  // a HIP error ended a run: the device state is unknown, later runs are refused
     std::atomic<bool> serving{false};  // an online server owns the engine (ga_serve_start .. ga_serve_stop)
     char* scratch_dev = nullptr;  // this device's scratch (ga_solve_noalloc)
+    uint8_t* d_pdb = nullptr;     // 24-puzzle pattern databases (kPdbBytes + level flags), built on first use
 
     void init(int dev, const ga_config* c) {
         device = dev;
@@ -580,6 +624,7 @@ struct ga_engine {
         if (path_stream) (void)hipStreamSynchronize(path_stream);
         if (d_descs) (void)hipFree(d_descs);
         if (d_states) (void)hipFree(d_states);
+        if (d_pdb) (void)hipFree(d_pdb);
         delete arena;
         if (path_stream) (void)hipStreamDestroy(path_stream);
     }
@@ -644,6 +689,56 @@ struct ga_engine {
             b += p.table_slots ? p.table_slots * sizeof(Slot) : cells * sizeof(uint64_t);
         p.bytes = b;
         return p;
+    }
+
+    // ---- 24-puzzle pattern databases -------------------------------------------
+    static constexpr uint32_t kPdbLevels = 64;  // the deepest table (5 tiles on 5x5) has 30 levels
+    static bool ordered_goal5(const ga_request& r) {
+        if (r.kind != GA_PUZZLE || r.side != 5 || !r.cells) return false;
+        for (int c = 0; c < 25; ++c)
+            if (r.cells[25 + c] != (c < 24 ? c + 1 : 0)) return false;
+        return true;
+    }
+    // Builds the tables on `stream` (breadth-first search from each pattern's
+    // goal placement, one launch per level) and waits for them; one hipMalloc
+    // per engine lifetime.
+    const uint8_t* ensure_pdb(hipStream_t stream) {
+        if (d_pdb) return d_pdb;
+        uint8_t* p = nullptr;
+        const uint64_t words = (kPdbBytes + 7) / 8;
+        const uint64_t bytes = words * 8 + (uint64_t)kPdbPatterns * kPdbLevels * sizeof(uint32_t);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            throw GaError(GA_E_NOMEM, "pattern databases");
+        }
+        uint32_t* changed = (uint32_t*)(p + words * 8);
+        fill_kernel<uint64_t><<<1024, kBlock, 0, stream>>>((uint64_t*)p, words, ~0ull);
+        fill_kernel<uint32_t><<<1, kBlock, 0, stream>>>(changed, (uint64_t)kPdbPatterns * kPdbLevels, 0u);
+        for (int q = 0; q < kPdbPatterns; ++q) {
+            uint32_t goal = 0;
+            for (int i = 0; i < kPdbCount[q]; ++i) goal += (kPdbTiles[q][i] - 1) * pow25((uint32_t)i);
+            uint8_t* dist = p + (uint64_t)q * kPdbTable;
+            put_byte_kernel<<<1, 64, 0, stream>>>(dist + goal, 0);
+            for (uint32_t d = 0; d < kPdbLevels; ++d) {
+                uint32_t* ch = changed + (uint64_t)q * kPdbLevels;
+                if (kPdbCount[q] == 5)
+                    pdb_level_kernel<5><<<4096, kBlock, 0, stream>>>(dist, d, ch);
+                else
+                    pdb_level_kernel<4><<<512, kBlock, 0, stream>>>(dist, d, ch);
+            }
+        }
+        hip_check(hipGetLastError(), "pattern database build");
+        std::vector<uint32_t> flags((size_t)kPdbPatterns * kPdbLevels);
+        hip_check(hipMemcpyAsync(flags.data(), changed, flags.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+                  "read back level flags");
+        hip_check(hipStreamSynchronize(stream), "pattern database build");
+        for (int q = 0; q < kPdbPatterns; ++q)
+            if (flags[(size_t)q * kPdbLevels + kPdbLevels - 1]) {  // still growing at the last level
+                (void)hipFree(p);
+                throw GaError(GA_E_HIP, "pattern database deeper than the level bound");
+            }
+        d_pdb = p;
+        return d_pdb;
     }
 
     // ---- admission ------------------------------------------------------------
@@ -767,6 +862,7 @@ struct ga_engine {
         InstDesc d{};
         if (!carve_instance(j, p, take, d)) return 0;
         hipStream_t stream = L.stream;
+        if (cfg.pdb24 && ordered_goal5(r)) d.pdb = ensure_pdb(stream);
         init_instance(j, p, d, stream);
         int slot = L.free_slots.back();
         L.free_slots.pop_back();
@@ -1758,6 +1854,8 @@ void solve_noalloc(ga_engine* e, const ga_request& r, ga_result* out, uint64_t* 
         if (!d_map || !e->carve_instance(j, p, take, d))
             throw GaError(GA_E_BUDGET, "request needs more device memory than the " +
                                            std::to_string(kScratchBytes >> 20) + " MiB no-allocation scratch");
+        // the pattern databases when this engine has built them (building them would allocate)
+        if (e->cfg.pdb24 && e->d_pdb && ga_engine::ordered_goal5(r)) d.pdb = e->d_pdb;
         e->init_instance(j, p, d, st);
         fill_kernel<uint32_t><<<1, 64, 0, st>>>(d_iter, 1, 0u);
         map_kernel<<<(n_wg + 255) / 256, 256, 0, st>>>(d_map, n_wg, heaps_per_wg, p.n_heaps);
@@ -1860,6 +1958,7 @@ void ga_config_default(ga_config* cfg) {
     cfg->deep_cells = 4ull << 20;
     cfg->slack_puzzle = 0;
     cfg->slack_grid = 0;
+    cfg->pdb24 = 1;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {
@@ -2104,6 +2203,20 @@ int ga_serve_stop(ga_server* sv) {
     sv->e->serving = false;
     delete sv;
     return rc;
+}
+
+int ga_pdb24_tables(ga_engine* e, uint8_t* out, size_t bytes) {
+    if (!e || !out || bytes < kPdbBytes || e->serving) return GA_E_INVALID;
+    if (e->broken) return GA_E_HIP;
+    try {
+        hip_check(hipSetDevice(e->device), "hipSetDevice");
+        const uint8_t* p = e->ensure_pdb(e->lanes.back().stream);
+        hip_check(hipMemcpy(out, p, kPdbBytes, hipMemcpyDeviceToHost), "copy pattern databases");
+    } catch (const GaError& err) {
+        e->last_error = err.what();
+        return err.code;
+    }
+    return GA_OK;
 }
 
 const char* ga_engine_last_error(ga_engine* e) { return e ? e->last_error.c_str() : ""; }

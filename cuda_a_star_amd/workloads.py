@@ -13,6 +13,10 @@ libgastar.so), so a (config, seed) pair names the same instances everywhere.
             weights 1..9, wall probability U[0.10, 0.25]
   config 5  Poisson arrival stream: 60 % 15-puzzles (depth U{30..70}),
             40 % grids (side U{512..4096}, weights 1..9, 20 % walls)
+
+  deep 24-puzzles  5x5 boards from walks of depth U{80..120} (no immediate
+            backtrack) from the ordered board: out of reach of Manhattan,
+            solved with the additive pattern databases
 """
 from __future__ import annotations
 
@@ -148,3 +152,17 @@ def config5(master: int = 5, n: int = 1024, qps: float = 200.0, threads: int = 1
         inst.arrival_ns = int(t * 1e9)
         out.append(inst)
     return _materialise(out, threads)
+
+
+def puzzle24_deep(master: int = 25, n: int = 16, depth=(80, 120)):
+    """(depth, seed, board) of n 24-puzzles: walks of depth U{80..120} from the
+    ordered board, no immediate backtrack."""
+    from . import gen_board
+
+    st = Stream(master)
+    out = []
+    for i in range(n):
+        d = st.uniform_int(*depth)
+        seed = instance_seed(master, i)
+        out.append((d, seed, gen_board(5, d, seed)))
+    return out

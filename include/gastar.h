@@ -117,6 +117,10 @@ typedef struct ga_config {
     uint64_t deep_cells;     /* grids with at least this many cells run on the last lane */
     uint32_t slack_puzzle;   /* 0 = off; k: expand a heap top only if f <= previous min f + k - 1 */
     uint32_t slack_grid;     /* the same for grids */
+    uint32_t pdb24;          /* 1: 24-puzzles towards the ordered goal use additive pattern databases
+                                (built on the device at the first such request, 48.8 MB), others
+                                Manhattan; 0: Manhattan for every 24-puzzle */
+    uint32_t reserved;
 } ga_config;
 
 /* Engine statistics.  With cfg.profile = n, kernel time covers the timed
@@ -264,6 +268,12 @@ void ga_result_free(ga_result* r);
 const char* ga_status_str(int status);
 /* Message of the last failed run or call on this engine ("" if none). */
 const char* ga_engine_last_error(ga_engine* e);
+
+/* The 24-puzzle pattern databases of the engine's device (ga_config.pdb24),
+ * built first if need be, copied to `out` (bytes must be at least
+ * 48,828,125: five tables of 25^5 bytes, the fifth using its first 25^4, the
+ * rest 255).  No counterpart in the reference; for tests and inspection. */
+int ga_pdb24_tables(ga_engine* e, uint8_t* out, size_t bytes);
 
 /* Test support, no counterpart in the reference, no device call: applies the
  * request checks of ga_solve_batch to reqs, finishes the first finish_first

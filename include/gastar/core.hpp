@@ -153,6 +153,7 @@ struct InstDesc {
     uint8_t goal_rc[32];   // puzzle: row << 4 | col of each tile in the goal
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
+    const uint8_t* pdb;    // 24-puzzle: the pattern databases (kPdbBytes), or null for Manhattan
 };
 static_assert(offsetof(InstDesc, goal_rc) % 4 == 0, "goal_rc is read as 32-bit words");
 
@@ -1088,6 +1089,45 @@ struct Board5KeySet {
     __device__ static inline Key goal_key(const InstDesc& d) { return as_global(reinterpret_cast<const Key*>(d.user))[1]; }
 };
 
+// Additive pattern databases of the 24-puzzle towards the ordered goal (tile
+// t on cell t - 1, blank on cell 24).  The 24 tiles form five disjoint
+// patterns; pattern p's table (kPdbTable bytes at p * kPdbTable) holds, for
+// each placement of its tiles (index: sum over its tiles i of cell * 25^i),
+// the fewest moves of those tiles that bring them to their goal cells when a
+// tile may step into any neighbouring cell not held by a tile of the same
+// pattern.  That relaxes the puzzle, and each real move moves one tile of one
+// pattern, so the sum over the patterns is admissible and changes by at most
+// 1 per move (consistent).
+constexpr int kPdbPatterns = 5;
+constexpr uint32_t kPdbTiles[kPdbPatterns][5] = {
+    {1, 2, 3, 6, 7}, {4, 5, 8, 9, 10}, {11, 12, 16, 17, 21}, {13, 14, 18, 19, 22}, {15, 20, 23, 24, 0}};
+constexpr int kPdbCount[kPdbPatterns] = {5, 5, 5, 5, 4};
+constexpr uint64_t kPdbTable = 9765625;  // 25^5
+constexpr uint64_t kPdbBytes = kPdbPatterns * kPdbTable;
+
+// Pattern (high 3 bits) and position in the pattern (low 3 bits) of tile t;
+// 63 for the blank.
+constexpr uint32_t pdb_code_of(uint32_t t) {
+    for (int p = 0; p < kPdbPatterns; ++p)
+        for (int i = 0; i < kPdbCount[p]; ++i)
+            if (kPdbTiles[p][i] == t) return (uint32_t)(p << 3 | i);
+    return 63u;
+}
+// the codes of tiles 10k .. 10k+9, 6 bits each
+constexpr uint64_t pdb_code_word(int k) {
+    uint64_t w = 0;
+    for (int j = 0; j < 10; ++j) w |= (uint64_t)pdb_code_of((uint32_t)(10 * k + j)) << (6 * j);
+    return w;
+}
+__host__ __device__ inline uint32_t pdb_code(uint32_t t) {
+    constexpr uint64_t w0 = pdb_code_word(0), w1 = pdb_code_word(1), w2 = pdb_code_word(2);
+    const uint64_t w = t < 10 ? w0 : t < 20 ? w1 : w2;
+    return (uint32_t)(w >> (6 * (t % 10))) & 63u;
+}
+__host__ __device__ inline uint32_t pow25(uint32_t i) {
+    return i == 0 ? 1u : i == 1 ? 25u : i == 2 ? 625u : i == 3 ? 15625u : 390625u;
+}
+
 struct Puzzle5Ops : Board5KeySet {
     static constexpr int K = 4;
     using Key = RawKey<Board5>;
@@ -1115,9 +1155,25 @@ struct Puzzle5Ops : Board5KeySet {
         const int r = cell / 5, c = cell % 5;
         return (uint32_t)((r > gr ? r - gr : gr - r) + (c > gc ? c - gc : gc - c));
     }
+    // placement index of pattern p's tiles on board b
+    __device__ static inline uint32_t pdb_index(const Board5& b, uint32_t p) {
+        uint32_t idx = 0;
+        for (int c = 0; c < 25; ++c) {
+            const uint32_t code = pdb_code(get(b, c));
+            if ((code >> 3) == p) idx += (uint32_t)c * pow25(code & 7u);
+        }
+        return idx;
+    }
+    __device__ static inline uint32_t pdb_value(const InstDesc& d, uint32_t p, uint32_t idx) {
+        return as_global(d.pdb)[(uint64_t)p * kPdbTable + idx];
+    }
     __device__ static inline uint32_t heuristic(const InstDesc& d, const Key& k, uint32_t) {
         const Board5 b = board(k);
         uint32_t h = 0;
+        if (d.pdb) {
+            for (uint32_t p = 0; p < kPdbPatterns; ++p) h += pdb_value(d, p, pdb_index(b, p));
+            return h;
+        }
         for (int c = 0; c < 25; ++c) {
             const uint32_t t = get(b, c);
             if (t) h += md(d, t, c);
@@ -1147,12 +1203,19 @@ struct Puzzle5Ops : Board5KeySet {
         if ((uint32_t)nb + 1 == (aux >> 8)) return false;
         Board5 s = board(k);
         const uint32_t tile = get(s, nb);
+        if (d.pdb) {  // only the moved tile's pattern changes: its placement index moves by (b - nb) * 25^i
+            const uint32_t code = pdb_code(tile), p = code >> 3;
+            const uint32_t before = pdb_index(s, p);
+            const uint32_t after = (uint32_t)((int)before + (b - nb) * (int)pow25(code & 7u));
+            th = h - pdb_value(d, p, before) + pdb_value(d, p, after);
+        } else {
+            th = h - md(d, tile, nb) + md(d, tile, b);
+        }
         set(s, b, tile);
         set(s, nb, 0u);
         t = to_raw(s);
         taux = (uint32_t)nb | ((uint32_t)b + 1) << 8;
         cost = 1;
-        th = h - md(d, tile, nb) + md(d, tile, b);
         return true;
     }
 };

@@ -95,6 +95,13 @@ def lib(path: str = _LIB_PATH) -> ctypes.CDLL:
         L.oracle_board_manhattan.restype = ctypes.c_uint32
         L.oracle_bfs_board.argtypes = [ctypes.c_int, u8p, u8p, ctypes.c_uint32, ctypes.c_uint64]
         L.oracle_bfs_board.restype = ctypes.c_int64
+        L.oracle_pdb24_build.argtypes = [u8p]
+        L.oracle_pdb24_build.restype = ctypes.c_int
+        L.oracle_pdb24_h.argtypes = [u8p, u8p]
+        L.oracle_pdb24_h.restype = ctypes.c_uint32
+        L.oracle_astar_board_pdb.argtypes = [u8p, u8p, ctypes.c_uint64, ctypes.POINTER(OracleResult), u8p,
+                                             ctypes.c_uint32]
+        L.oracle_astar_board_pdb.restype = ctypes.c_int
         L.oracle_replay_board.argtypes = [ctypes.c_int, u8p, ctypes.c_uint32, u8p, u8p]
         L.oracle_replay_board.restype = ctypes.c_int64
         _lib = L
@@ -226,3 +233,37 @@ def walk_board(side: int, depth: int, seed: int, no_backtrack: bool = True):
         b[p], b[nb] = b[nb], 0
         prev = p
     return tuple(b), tuple(goal)
+
+
+PDB24_BYTES = 5 * 25 ** 5
+_pdb24 = None
+
+
+def pdb24() -> np.ndarray:
+    """The 24-puzzle pattern databases (built once per process by BFS, ~1 s per table)."""
+    global _pdb24
+    if _pdb24 is None:
+        t = np.empty(PDB24_BYTES, dtype=np.uint8)
+        rc = lib().oracle_pdb24_build(_u8(t))
+        if rc != OK:
+            raise RuntimeError(f"oracle_pdb24_build failed ({rc})")
+        _pdb24 = t
+    return _pdb24
+
+
+def pdb24_h(board) -> int:
+    b = np.ascontiguousarray(np.asarray(board, dtype=np.uint8))
+    return lib().oracle_pdb24_h(_u8(pdb24()), _u8(b))
+
+
+def astar_board_pdb(start, max_states: int = 0, want_path: bool = True):
+    """oracle_astar_board towards the ordered 5x5 goal with the pattern databases."""
+    b = np.ascontiguousarray(np.asarray(start, dtype=np.uint8))
+    res = OracleResult()
+    cap = 4096 if want_path else 0
+    buf = np.zeros(max(cap, 1) * 25, dtype=np.uint8)
+    rc = lib().oracle_astar_board_pdb(_u8(b), _u8(pdb24()), max_states, ctypes.byref(res),
+                                      _u8(buf) if want_path else None, cap)
+    path = ([tuple(buf[25 * k:25 * k + 25]) for k in range(res.path_len)]
+            if (want_path and rc == OK and res.path_len <= cap) else [])
+    return rc, res, path

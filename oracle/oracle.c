@@ -655,14 +655,115 @@ static void bstore_free(bstore* st) {
     free(st->slot);
 }
 
+/* ---- additive pattern databases for 5x5 boards with the ordered goal ---- */
+/* The 24 tiles are split into five disjoint patterns.  A pattern's table
+ * holds, for every placement of its tiles, the fewest moves of those tiles
+ * that bring them to their goal cells when a tile may step into any
+ * neighbouring cell not held by another tile of the same pattern (the other
+ * tiles and the blank are ignored).  That is a relaxation of the puzzle, and
+ * every real move moves one tile of one pattern, so the sum over the patterns
+ * never exceeds the true distance and changes by at most 1 per move.
+ * Placement index: sum over the pattern's tiles i of cell(tile i) * 25^i.
+ * Goal: tile t on cell t - 1, blank on cell 24. */
+static const uint8_t PDB_TILES[ORACLE_PDB_PATTERNS][5] = {
+    {1, 2, 3, 6, 7}, {4, 5, 8, 9, 10}, {11, 12, 16, 17, 21}, {13, 14, 18, 19, 22}, {15, 20, 23, 24, 0}};
+static const int PDB_NT[ORACLE_PDB_PATTERNS] = {5, 5, 5, 5, 4};
+static const uint32_t POW25[6] = {1u, 25u, 625u, 15625u, 390625u, 9765625u};
+
+static size_t pdb_offset(int p) { return (size_t)p * POW25[5]; }
+
+int oracle_pdb24_build(uint8_t* out) {
+    if (!out) return ORACLE_E_INVALID;
+    for (int p = 0; p < ORACLE_PDB_PATTERNS; ++p) {
+        const int nt = PDB_NT[p];
+        const uint32_t n = POW25[nt];
+        uint8_t* dist = out + pdb_offset(p);
+        memset(dist, 0xFF, POW25[5]);  /* the 4-tile table's unused tail too */
+        uint32_t* queue = (uint32_t*)malloc((size_t)n * sizeof(uint32_t));
+        if (!queue) return ORACLE_E_NOMEM;
+        uint32_t goal = 0;
+        for (int i = 0; i < nt; ++i) goal += (uint32_t)(PDB_TILES[p][i] - 1) * POW25[i];
+        size_t head = 0, tail = 0;
+        dist[goal] = 0;
+        queue[tail++] = goal;
+        while (head < tail) { /* FIFO breadth-first search from the goal placement */
+            const uint32_t idx = queue[head++];
+            int pos[5];
+            for (int i = 0; i < nt; ++i) pos[i] = (int)(idx / POW25[i] % 25u);
+            for (int i = 0; i < nt; ++i) {
+                const int r = pos[i] / 5, c = pos[i] % 5;
+                const int nr[4] = {r + 1, r - 1, r, r}, nc[4] = {c, c, c + 1, c - 1};
+                for (int m = 0; m < 4; ++m) {
+                    if (nr[m] < 0 || nr[m] > 4 || nc[m] < 0 || nc[m] > 4) continue;
+                    const int np = nr[m] * 5 + nc[m];
+                    int taken = 0;
+                    for (int k = 0; k < nt; ++k) taken |= (k != i && pos[k] == np);
+                    if (taken) continue;
+                    const uint32_t nidx = idx + (uint32_t)(np - pos[i]) * POW25[i];
+                    if (dist[nidx] != 0xFF) continue;
+                    dist[nidx] = (uint8_t)(dist[idx] + 1);
+                    queue[tail++] = nidx;
+                }
+            }
+        }
+        free(queue);
+    }
+    return ORACLE_OK;
+}
+
+static uint32_t pdb_h128(const uint8_t* pdb, u128 s) {
+    uint32_t idx[ORACLE_PDB_PATTERNS] = {0, 0, 0, 0, 0};
+    for (int c = 0; c < 25; ++c) {
+        const uint32_t t = cell5(s, c);
+        for (int p = 0; p < ORACLE_PDB_PATTERNS; ++p)
+            for (int i = 0; i < PDB_NT[p]; ++i)
+                if (PDB_TILES[p][i] == t && t) idx[p] += (uint32_t)c * POW25[i];
+    }
+    uint32_t h = 0;
+    for (int p = 0; p < ORACLE_PDB_PATTERNS; ++p) h += pdb[pdb_offset(p) + idx[p]];
+    return h;
+}
+
+uint32_t oracle_pdb24_h(const uint8_t* pdb, const uint8_t* board) { return pdb_h128(pdb, board_pack(5, board)); }
+
+/* the heuristic of one board search: Manhattan without the blank, or the sum of the pattern databases */
+typedef struct {
+    int side;
+    int gr[32], gc[32];
+    const uint8_t* pdb;
+} board_h;
+
+static uint32_t board_h_of(const board_h* h, u128 s) {
+    return h->pdb ? pdb_h128(h->pdb, s) : manh128(h->side, s, h->gr, h->gc);
+}
+
+static int astar_board_h(int side, const uint8_t* start_b, const uint8_t* goal_b, const uint8_t* pdb,
+                         uint64_t max_states, oracle_result* res, uint8_t* path, uint32_t path_cap);
+
 int oracle_astar_board(int side, const uint8_t* start_b, const uint8_t* goal_b, uint64_t max_states,
                        oracle_result* res, uint8_t* path, uint32_t path_cap) {
+    return astar_board_h(side, start_b, goal_b, NULL, max_states, res, path, path_cap);
+}
+
+int oracle_astar_board_pdb(const uint8_t* start_b, const uint8_t* pdb, uint64_t max_states, oracle_result* res,
+                           uint8_t* path, uint32_t path_cap) {
+    uint8_t goal_b[25];
+    for (int c = 0; c < 24; ++c) goal_b[c] = (uint8_t)(c + 1);
+    goal_b[24] = 0;
+    if (!pdb) return ORACLE_E_INVALID;
+    return astar_board_h(5, start_b, goal_b, pdb, max_states, res, path, path_cap);
+}
+
+static int astar_board_h(int side, const uint8_t* start_b, const uint8_t* goal_b, const uint8_t* pdb,
+                         uint64_t max_states, oracle_result* res, uint8_t* path, uint32_t path_cap) {
     memset(res, 0, sizeof(*res));
     res->cost = -1;
     if (!board_valid(side, start_b) || !board_valid(side, goal_b)) return ORACLE_E_INVALID;
     const u128 start = board_pack(side, start_b), goal = board_pack(side, goal_b);
-    int gr[32], gc[32];
-    goal_rc(side, goal, gr, gc);
+    board_h H;
+    H.side = side;
+    H.pdb = pdb;
+    goal_rc(side, goal, H.gr, H.gc);
     bstore st;
     memset(&st, 0, sizeof(st));
     pq q;
@@ -671,7 +772,7 @@ int oracle_astar_board(int side, const uint8_t* start_b, const uint8_t* goal_b, 
     int64_t s0 = bstore_get(&st, start);
     if (s0 < 0) { rc = ORACLE_E_NOMEM; goto out; }
     st.g[s0] = 0;
-    if (pq_push(&q, (pq_item){prio_of(manh128(side, start, gr, gc), 0), (uint32_t)s0, 0})) { rc = ORACLE_E_NOMEM; goto out; }
+    if (pq_push(&q, (pq_item){prio_of(board_h_of(&H, start), 0), (uint32_t)s0, 0})) { rc = ORACLE_E_NOMEM; goto out; }
     int64_t goal_idx = -1;
     while (q.n) {
         pq_item it = pq_pop(&q);
@@ -691,7 +792,7 @@ int oracle_astar_board(int side, const uint8_t* start_b, const uint8_t* goal_b, 
             if (g2 >= st.g[ti]) continue; /* fancy_hashmap.h:56-61 */
             st.g[ti] = g2;
             st.parent[ti] = it.idx;
-            if (pq_push(&q, (pq_item){prio_of(g2 + manh128(side, t, gr, gc), g2), (uint32_t)ti, g2})) { rc = ORACLE_E_NOMEM; goto out; }
+            if (pq_push(&q, (pq_item){prio_of(g2 + board_h_of(&H, t), g2), (uint32_t)ti, g2})) { rc = ORACLE_E_NOMEM; goto out; }
         }
     }
     res->stored = st.n;

@@ -42,6 +42,13 @@
  *                          blank slid k cells along a row or column; parity
  *                          and bound invariants on deeper 5x5 walks.
  *   oracle_bfs_board     - equal to the 8-puzzle BFS table on 3x3 boards.
+ *   oracle_pdb24_build   - h = 0 at the goal; h >= Manhattan and h <= the
+ *   oracle_pdb24_h         BFS depth on sampled boards; |h(s) - h(t)| <= 1
+ *                          across every move; a one-tile and a two-tile
+ *                          closed form; equal to the engine's device-built
+ *                          tables byte for byte (a GPU test).
+ *   oracle_astar_board_pdb - equal to oracle_astar_board (Manhattan) wherever
+ *                          both finish; equal to oracle_bfs_board on walks.
  *
  * State encodings (shared with the engine's C ABI):
  *   puzzle: uint64, nibble i (bits 4i..4i+3) = tile at cell i in row-major
@@ -122,6 +129,18 @@ uint32_t oracle_board_manhattan(int side, const uint8_t* state, const uint8_t* g
  * -2 for an invalid board, -3 if max_states (0 = none) is exceeded. */
 int64_t oracle_bfs_board(int side, const uint8_t* start, const uint8_t* goal,
                          uint32_t max_depth, uint64_t max_states);
+/* Additive pattern databases of 5x5 boards with the ordered goal (tile t on
+ * cell t-1, blank last): ORACLE_PDB_BYTES bytes, five tables of 25^5 bytes
+ * (the fifth uses its first 25^4), built by breadth-first search. */
+#define ORACLE_PDB_PATTERNS 5
+#define ORACLE_PDB_BYTES (5ull * 9765625ull)
+int oracle_pdb24_build(uint8_t* out);
+/* Sum of the five tables for a 5x5 board (an admissible, consistent h). */
+uint32_t oracle_pdb24_h(const uint8_t* pdb, const uint8_t* board);
+/* oracle_astar_board on 5x5 boards towards the ordered goal with the
+ * pattern-database heuristic instead of Manhattan. */
+int oracle_astar_board_pdb(const uint8_t* start, const uint8_t* pdb, uint64_t max_states, oracle_result* res,
+                           uint8_t* path, uint32_t path_cap);
 int64_t oracle_replay_board(int side, const uint8_t* path, uint32_t n,
                             const uint8_t* start, const uint8_t* goal);
 

@@ -8,6 +8,8 @@ the CPU oracle (both built from this repository).
   config3.json     the 4096x4096 grid instance: cell digest and cost
   config4.json     the 1024 mixed batch instances and their costs
   config5.json     the first requests of the serving stream and their costs
+  puzzle24_pdb.json 24-puzzles of walk depth 80..120 that the pattern-database
+                   A* solves, with their costs (Manhattan A* stops at its cap)
 
 Run: python tests/golden/make_golden.py [--threads 8]
 """
@@ -94,6 +96,22 @@ def main() -> int:
         for o, x in zip(out, insts):
             o["arrival_ns_at_1500qps"] = x.arrival_ns
         write("config5.json", {"master": 5, "n": 256, "instances": out})
+    if not only or "puzzle24_pdb" in only:
+        oracle.pdb24()
+        cand = wl.puzzle24_deep(25, 24)
+
+        def solve24(c):
+            d, seed, b = c
+            rc, res, _ = oracle.astar_board_pdb(b, max_states=15_000_000, want_path=False)
+            if rc != oracle.OK:
+                return None
+            rm, rman, _ = oracle.astar_board(5, b, ga.board_goal(5), max_states=3_000_000, want_path=False)
+            return {"index": cand.index(c), "depth": d, "seed": seed, "board": list(b), "cost": int(res.cost),
+                    "oracle_pdb_expanded": int(res.expanded),
+                    "manhattan_within_3M_states": rm == oracle.OK}
+
+        out = [o for o in pool.map(solve24, cand) if o]
+        write("puzzle24_pdb.json", {"master": 25, "depth": [80, 120], "tried": len(cand), "instances": out})
     return 0
 
 

@@ -87,3 +87,18 @@ def test_config5_stream_instances():
             assert x.start == int(g["start"], 16)
         else:
             assert digest(x.cells) == g["cells_sha256"]
+
+
+def test_puzzle24_pdb_fixture_boards_and_cheap_costs():
+    # the deep 24-puzzle boards regenerate from the workload's stream, and the two
+    # cheapest fixture costs reproduce with the pattern-database A*
+    gold = load("puzzle24_pdb.json")
+    cand = wl.puzzle24_deep(gold["master"], gold["tried"])
+    for o in gold["instances"]:
+        d, seed, b = cand[o["index"]]
+        assert (d, seed, list(b)) == (o["depth"], o["seed"], o["board"])
+        assert 80 <= d <= 120
+    cheap = sorted(gold["instances"], key=lambda o: o["oracle_pdb_expanded"])[:2]
+    for o in cheap:
+        rc, res, _ = oracle.astar_board_pdb(o["board"], want_path=False)
+        assert rc == oracle.OK and res.cost == o["cost"]

@@ -2,6 +2,10 @@
 through the C ABI: 128-bit keys on their own kernel and launch lane.  Every
 cost equals the CPU oracle's (oracle_astar_board, pinned to BFS and to the
 64-bit puzzle search), every path replays move by move."""
+import json
+import os
+
+import numpy as np
 import pytest
 
 import cuda_a_star_amd as ga
@@ -9,6 +13,7 @@ import oracle
 from cuda_a_star_amd import workloads as wl
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 
 
 def check(start, sol, goal=None):
@@ -106,3 +111,40 @@ def test_serve_stream_mixed():
         check(boards[i], sols[2 * i])
         assert sols[2 * i + 1].cost == oracle.astar_puzzle(4, puzzles[i].start, puzzles[i].goal,
                                                            want_path=False)[1].cost
+
+
+# ---- additive pattern databases (ga_config.pdb24) ----
+def test_pdb_tables_built_on_device_equal_the_oracle():
+    with ga.Engine(0) as e:
+        dev = e.pdb24_tables()
+    assert np.array_equal(dev, oracle.pdb24())
+
+
+def test_deep_walks_with_pdb_match_golden():
+    # walks of depth 80..120 that Manhattan cannot bring within budget; served heap count
+    gold = json.load(open(os.path.join(GOLDEN, "puzzle24_pdb.json")))["instances"]
+    with ga.Engine(0, max_iterations=2_000_000) as e:
+        sols = e.solve_batch([ga.puzzle5_request(tuple(o["board"]), num_queues=8192) for o in gold])
+    for o, sol in zip(gold, sols):
+        assert sol.status == ga.OK, (o["index"], ga.status_str(sol.status))
+        assert sol.cost == o["cost"], o["index"]
+        assert oracle.replay_board(5, sol.path, tuple(o["board"]), ga.board_goal(5)) == sol.cost
+
+
+def test_pdb_and_manhattan_agree_and_pdb_expands_less():
+    gold = json.load(open(os.path.join(GOLDEN, "puzzle24_pdb.json")))["instances"]
+    o = next(x for x in gold if x["manhattan_within_3M_states"] and x["oracle_pdb_expanded"] > 100_000)
+    req = lambda: ga.puzzle5_request(tuple(o["board"]), num_queues=4096)
+    with ga.Engine(0, max_iterations=2_000_000, pdb24=1) as e:
+        p = e.solve(req())
+    with ga.Engine(0, max_iterations=2_000_000, pdb24=0) as e:
+        m = e.solve(req())
+    assert p.status == m.status == ga.OK and p.cost == m.cost == o["cost"]
+    assert p.expanded < m.expanded
+    # a goal other than the ordered board has no tables: Manhattan, still exact
+    b = ga.gen_board(5, 20, 77)
+    goal = tuple(ga.gen_board(5, 6, 78))
+    with ga.Engine(0, max_iterations=2_000_000) as e:
+        sol = e.solve(ga.puzzle5_request(b, goal, num_queues=1024))
+    rc, ref, _ = oracle.astar_board(5, b, goal, want_path=False)
+    assert rc == oracle.OK and sol.status == ga.OK and sol.cost == ref.cost

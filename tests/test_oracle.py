@@ -236,3 +236,102 @@ def test_board_rejects_bad_inputs_and_paths():
     jump = [path[0]] + path[2:]
     if len(path) > 2:
         assert oracle.replay_board(5, jump, start, goal) == -1  # two moves at once
+
+
+# ---- 24-puzzle pattern databases (oracle_pdb24_*) ----
+GOAL5 = tuple(range(1, 25)) + (0,)
+
+
+def _neighbours5(b):
+    b = list(b)
+    z = b.index(0)
+    r, c = divmod(z, 5)
+    for nr, nc in ((r + 1, c), (r - 1, c), (r, c + 1), (r, c - 1)):
+        if 0 <= nr < 5 and 0 <= nc < 5:
+            t = b[:]
+            t[z], t[nr * 5 + nc] = t[nr * 5 + nc], 0
+            yield tuple(t)
+
+
+def test_pdb24_tables_structure():
+    t = oracle.pdb24()
+    for p, nt in enumerate((5, 5, 5, 5, 4)):
+        seg = t[p * 25 ** 5:(p + 1) * 25 ** 5]
+        reach = seg[seg != 255]
+        # every placement of nt distinct cells is reached (the blank-free moves connect them all)
+        assert len(reach) == int(np.prod(range(25 - nt + 1, 26)))
+        assert reach.min() == 0 and (reach == 0).sum() == 1 and reach.max() <= 8 * nt + 16
+    assert oracle.pdb24_h(GOAL5) == 0
+
+
+def test_pdb24_dominates_manhattan_and_is_consistent():
+    boards = [oracle.walk_board(5, d, 500 + d)[0] for d in range(4, 120, 3)]
+    rng = np.random.default_rng(7)
+    for _ in range(40):  # random solvable permutations
+        b = list(rng.permutation(25))
+        if not ga.board_solvable(5, b, GOAL5):
+            i, j = [k for k in range(25) if b[k]][:2]
+            b[i], b[j] = b[j], b[i]
+        boards.append(tuple(int(x) for x in b))
+    for b in boards:
+        h = oracle.pdb24_h(b)
+        assert h >= oracle.board_manhattan(5, b, GOAL5)
+        for t in _neighbours5(b):  # one move changes h by exactly one tile step, up or down
+            assert abs(oracle.pdb24_h(t) - h) == 1
+
+
+@pytest.mark.parametrize("depth,seed", [(8, 21), (12, 22), (14, 23), (16, 24)])
+def test_pdb24_below_bfs_depth_and_astar_exact(depth, seed):
+    start, goal = oracle.walk_board(5, depth, seed)
+    d = oracle.bfs_board(5, start, goal, depth, max_states=20_000_000)
+    assert 0 <= oracle.pdb24_h(start) <= d
+    rc, res, path = oracle.astar_board_pdb(start)
+    assert rc == oracle.OK and res.cost == d
+    assert oracle.replay_board(5, path, start, goal) == d
+
+
+def test_pdb24_closed_forms():
+    # the blank slid k cells: every displaced tile is one free step from home
+    for k in (1, 3, 4):
+        row = list(GOAL5)
+        for _ in range(k):
+            p = row.index(0)
+            row[p], row[p - 1] = row[p - 1], 0
+        assert oracle.pdb24_h(row) == k
+    # tiles 1 and 2 (one pattern) swapped in the top row: they cannot pass each other
+    # in the relaxation, so the table adds at least a detour of two moves to Manhattan's 2
+    b = list(GOAL5)
+    b[0], b[1] = 2, 1
+    assert oracle.pdb24_h(b) >= oracle.board_manhattan(5, b, GOAL5) + 2
+    # tiles 19 and 20 belong to different patterns and step home through each other's cell,
+    # which neither pattern holds: exactly their Manhattan distance, 2
+    b = list(GOAL5)
+    b[18], b[19] = 20, 19
+    assert oracle.pdb24_h(b) == oracle.board_manhattan(5, b, GOAL5) == 2
+
+
+@pytest.mark.parametrize("depth,seed", [(30, 31), (40, 32), (46, 33)])
+def test_pdb24_astar_equals_manhattan_astar(depth, seed):
+    start, goal = oracle.walk_board(5, depth, seed)
+    rc_m, m, _ = oracle.astar_board(5, start, goal, max_states=20_000_000, want_path=False)
+    rc_p, p, path = oracle.astar_board_pdb(start)
+    assert rc_m == rc_p == oracle.OK and p.cost == m.cost
+    assert p.expanded <= m.expanded
+    assert oracle.replay_board(5, path, start, goal) == p.cost
+
+
+def test_pdb24_planted_mistake_is_caught():
+    # counting one pattern twice (an overestimate) breaks h <= the true distance
+    # on the boards of the BFS check above: that check would catch it
+    t = oracle.pdb24()
+
+    def pattern_a(b):
+        cell = {tile: c for c, tile in enumerate(b)}
+        return int(t[sum(cell[tile] * 25 ** i for i, tile in enumerate((1, 2, 3, 6, 7)))])
+
+    over = 0
+    for depth, seed in [(8, 21), (12, 22), (14, 23), (16, 24)]:
+        start, goal = oracle.walk_board(5, depth, seed)
+        d = oracle.bfs_board(5, start, goal, depth, max_states=20_000_000)
+        over += oracle.pdb24_h(start) + pattern_a(start) > d
+    assert over > 0
