@@ -63,7 +63,7 @@ class Result(ctypes.Structure):
         ("iterations", ctypes.c_uint32),
         ("retries", ctypes.c_uint32),
         ("path_len", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("grows", ctypes.c_uint32),
         ("path", _u64p),
         ("search_ns", ctypes.c_uint64),
         ("latency_ns", ctypes.c_uint64),
@@ -93,7 +93,7 @@ class Config(ctypes.Structure):
         ("slack_puzzle", ctypes.c_uint32),
         ("slack_grid", ctypes.c_uint32),
         ("pdb24", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("grow_in_place", ctypes.c_uint32),
     ]
 
 
@@ -320,6 +320,7 @@ class Solution:
     search_ns: int = 0
     latency_ns: int = 0
     user_tag: int = 0
+    grows: int = 0
 
     @property
     def ok(self) -> bool:
@@ -335,7 +336,7 @@ def _take(res: Result, board_side: int = 0) -> Solution:
     else:
         path = [res.path[i] for i in range(res.path_len)] if res.path_len else []
     sol = Solution(res.status, res.cost, path, res.expanded, res.stored, res.iterations, res.retries,
-                   res.search_ns, res.latency_ns, res.user_tag)
+                   res.search_ns, res.latency_ns, res.user_tag, res.grows)
     lib().ga_result_free(ctypes.byref(res))
     return sol
 
@@ -466,7 +467,7 @@ class Engine:
         else:
             path = list(buf[:n])
         return Solution(res.status, res.cost, path, res.expanded, res.stored, res.iterations, res.retries,
-                        res.search_ns, res.latency_ns, res.user_tag)
+                        res.search_ns, res.latency_ns, res.user_tag, res.grows)
 
     # -- batched and served forms --
     def solve_batch(self, reqs: Sequence[Request]) -> List[Solution]:

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
         // it stall (set_status changes only a running instance)
         if (!ran && threadIdx.x == 0) set_status(states + inst, ST_ABORTED);
     }
-    tally_end(sh.tally, states + inst, it);
+    tally_end(sh.tally, states + inst, it, d.grow_table_at);
 }
 
 // Step kernel of the wide-key kinds (24-puzzle): its own instantiation so that
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel_wide(const WgEntry* __rest
     if (d.kind == KIND_PUZZLE5)
         run_group<W, Puzzle5Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm, tl, sh.work,
                                  sh.heap_top);
-    tally_end(sh.tally, states + inst, it);
+    tally_end(sh.tally, states + inst, it, d.grow_table_at);
 }
 
 __global__ void seed_kernel(InstDesc d, InstDesc* descs, InstState* states, uint32_t slot, int W) {
@@ -204,6 +204,83 @@ __global__ void pdb_level_kernel(uint8_t* dist, uint32_t d, uint32_t* changed) {
 
 __global__ void put_byte_kernel(uint8_t* p, uint8_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ---- growth of a paused instance's structures (ga_engine::grow_in_place) ----
+// Inserts every key of the old closed set `od` into the cleared larger one
+// `nd`, with its value, and records where it went: map[old slot] = new slot
+// (kNone for empty slots).  Keys are unique in the old set, so every insert
+// claims a fresh slot.
+template <class Set>
+__global__ void rehash_kernel(InstDesc od, InstDesc nd, uint32_t* map, uint32_t* failed) {
+    const uint64_t n = od.table_mask + 1, stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        typename Set::Key key;
+        uint32_t g, par;
+        Set::load(od, (uint32_t)i, key, g, par);
+        const uint64_t first = *reinterpret_cast<const uint64_t*>(&key);
+        if (first == kEmptyKey) {  // the first key word of an empty slot (hashed and board sets alike)
+            map[i] = kNone;
+            continue;
+        }
+        uint32_t slot, probes = 0;
+        bool isnew;
+        if (!Set::find_or_insert(nd, key, 1u << 20, slot, isnew, probes)) {
+            map[i] = kNone;
+            atomicAdd(failed, 1u);
+            continue;
+        }
+        *Set::val_ptr(nd, slot) = *Set::val_ptr(od, (uint32_t)i);
+        map[i] = slot;
+    }
+}
+
+// Parent links of the rehashed set: old slots to new slots.
+template <class Set>
+__global__ void remap_parents_kernel(InstDesc nd, const uint32_t* map) {
+    const uint64_t n = nd.table_mask + 1, stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += stride) {
+        const gptr<uint64_t> vp = Set::val_ptr(nd, (uint32_t)j);
+        const uint64_t v = *vp;
+        if (v == kInfVal) continue;  // empty slot
+        const uint32_t par = val_parent(v);
+        if (par != kNone) *vp = (v & ~0xFFFFFFFFull) | map[par];
+    }
+}
+
+// Copies heap q's entries 0 .. size[q]-1 from the old layout to the new one
+// (map, if given, renumbers their closed-set slots); one thread per entry
+// position of the old stride.
+__global__ void copy_heaps_kernel(const HeapEntry* from, HeapEntry* to, const uint32_t* size, uint32_t n_heaps,
+                                  uint32_t old_stride, uint32_t new_stride, uint32_t W, const uint32_t* map,
+                                  uint64_t n_map) {
+    const uint64_t total = (uint64_t)n_heaps * old_stride, stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += stride) {
+        const uint32_t q = (uint32_t)(k / old_stride), pos = (uint32_t)(k % old_stride);
+        if (pos < W - 1 || pos - (W - 1) >= size[q]) continue;  // logical entry j sits at physical j + W - 1
+        HeapEntry e = from[k];
+        if (map && e.slot < n_map) e.slot = map[e.slot];
+        to[(uint64_t)q * new_stride + pos] = e;
+    }
+}
+
+// Renumbers the closed-set slots of the candidate records the next
+// iteration reads (the first n of the buffer).
+__global__ void remap_records_kernel(HeapEntry* rec, uint64_t n, const uint32_t* map, uint64_t n_map) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride)
+        if (rec[k].prio != kNoRecord && rec[k].slot < n_map) rec[k].slot = map[rec[k].slot];
+}
+
+// Resumes a paused instance with its grown descriptor: clears its grow
+// requests and sets it running again (its heaps, records and closed set are
+// those of the iteration it paused at).
+__global__ void resume_kernel(InstDesc d, InstDesc* descs, InstState* states, uint32_t slot) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    descs[slot] = d;
+    InstState* st = states + slot;
+    st->grow[0] = st->grow[1] = st->grow[2] = 0;
+    st->status = ST_RUNNING;
 }
 
 // Launch map of a single resident instance (slot 0) with n_heaps heaps.
@@ -413,6 +490,7 @@ struct Job {
     bool finished = false;
     Puzzle5Ops::Key ends[2] = {};  // 24-puzzle: start and goal in the device packing (uploaded at admission)
     void* owner = nullptr;         // the online server's entry holding this job
+    uint32_t grows = 0;            // in-place growths of its heaps or closed set
 };
 
 struct PathBatch {
@@ -863,6 +941,7 @@ struct ga_engine {
         if (!carve_instance(j, p, take, d)) return 0;
         hipStream_t stream = L.stream;
         if (cfg.pdb24 && ordered_goal5(r)) d.pdb = ensure_pdb(stream);
+        set_grow_limits(j, p, d);
         init_instance(j, p, d, stream);
         int slot = L.free_slots.back();
         L.free_slots.pop_back();
@@ -885,6 +964,125 @@ struct ga_engine {
         L.map_dirty = true;
         stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena->in_use());
         return 1;
+    }
+
+    // The grow thresholds of an instance (InstDesc::grow_heap_at, grow_table_at):
+    // a heap may gain K records and K more per expansion round in one
+    // iteration, and the closed set K new keys per heap and round, so a
+    // request raised at the end of an iteration is acted on before any
+    // structure can overflow.  A structure already at its largest size has
+    // none (it then overflows, and the request restarts).
+    void set_grow_limits(const Job& j, const Plan& p, InstDesc& d) const {
+        d.grow_heap_at = 0;
+        d.grow_table_at = 0;
+        if (!cfg.grow_in_place) return;
+        const uint64_t K = (uint64_t)branching_of(j.kind), rounds = d.rounds ? d.rounds : cfg.rounds;
+        const uint64_t per_iter = K * (rounds + 1);
+        d.grow_heap_at = p.heap_cap < (1u << 26) ? (uint32_t)std::max<int64_t>(1, (int64_t)p.heap_cap - (int64_t)per_iter) : 0u;
+        d.grow_table_at = 0;
+        if (p.table_slots && p.table_slots < (1ull << 29)) {  // at most 2^31 slots after a growth
+            const uint64_t new_keys = (uint64_t)p.n_heaps * rounds * K;
+            d.grow_table_at = d.load_limit > new_keys ? d.load_limit - new_keys : 1;
+        }
+    }
+
+    // Grows the structures a paused instance asked for, in place: the heaps
+    // are copied into four times their capacity and/or the closed set
+    // rehashed into four times its slots (heap entries, the candidate
+    // records the next iteration reads and the parent links follow the new
+    // slot numbers), then the instance resumes at the iteration it paused at
+    // with all its search state.  Returns false (nothing changed) when the
+    // arena cannot hold the larger structures.
+    bool grow_in_place(Job& j, const InstState& st, Lane& L) {
+        const uint32_t req = st.grow[(st.paused_iter + 2) % 3];
+        const Plan op = plan(j);
+        const bool dense = j.kind == KIND_GRID4 || j.kind == KIND_GRID8;
+        const bool heap = (req & GROW_HEAP) != 0, table = (req & GROW_TABLE) != 0 && !dense;
+        Job g = j;
+        if (heap) g.heap_shift += 2;
+        if (table) g.table_shift += 2;
+        const Plan np = plan(g);
+        if (np.bytes > op.bytes && arena->in_use() + (np.bytes - op.bytes) > cfg.budget_bytes) return false;
+        InstDesc d = j.desc;
+        std::vector<void*> fresh;
+        auto take = [&](uint64_t bytes) -> void* {
+            void* ptr = arena->alloc(bytes);
+            if (ptr) fresh.push_back(ptr);
+            return ptr;
+        };
+        HeapEntry* heaps = heap ? (HeapEntry*)take((uint64_t)np.n_heaps * np.heap_stride * sizeof(HeapEntry)) : d.heaps;
+        const uint64_t slot_bytes = is_wide(j.kind) ? sizeof(Board5KeySet::PairSlot) : sizeof(Slot);
+        Slot* tab = table ? (Slot*)take(np.table_slots * slot_bytes) : d.table;
+        const uint64_t n_old = op.table_slots;
+        uint32_t* map = table ? (uint32_t*)take(n_old * sizeof(uint32_t) + 256) : nullptr;
+        if (!heaps || !tab || (table && !map)) {
+            for (void* f : fresh) arena->release(f);
+            return false;
+        }
+        hipStream_t s = L.stream;
+        const uint32_t W = cfg.heap_arity;
+        uint32_t* failed = map ? map + n_old : nullptr;  // one word past the map
+        if (table) {
+            InstDesc nd = d;
+            nd.table = tab;
+            nd.table_mask = np.table_slots - 1;
+            fill_kernel<uint64_t><<<1024, kBlock, 0, s>>>((uint64_t*)tab, np.table_slots * slot_bytes / 8, ~0ull);
+            fill_kernel<uint32_t><<<1, 64, 0, s>>>(failed, 1, 0u);
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (n_old + kBlock - 1) / kBlock);
+            if (is_wide(j.kind)) {
+                rehash_kernel<Board5KeySet><<<grid, kBlock, 0, s>>>(d, nd, map, failed);
+                remap_parents_kernel<Board5KeySet><<<4096, kBlock, 0, s>>>(nd, map);
+            } else {
+                rehash_kernel<HashedU64Set><<<grid, kBlock, 0, s>>>(d, nd, map, failed);
+                remap_parents_kernel<HashedU64Set><<<4096, kBlock, 0, s>>>(nd, map);
+            }
+            // the records iteration paused_iter reads: the first (width of the iteration before) * K
+            const uint32_t K = (uint32_t)branching_of(j.kind);
+            const uint64_t width = st.active[(st.paused_iter + 2) % 3];
+            HeapEntry* cin = d.cand + (uint64_t)((st.paused_iter - 1) & 1) * d.n_heaps * K;
+            remap_records_kernel<<<1024, kBlock, 0, s>>>(cin, width * K, map, n_old);
+            d.table = tab;
+            d.table_mask = np.table_slots - 1;
+            d.load_limit = np.table_slots * cfg.max_load_pct / 100;
+        }
+        if (heap || table) {
+            const uint64_t total = (uint64_t)op.n_heaps * op.heap_stride;
+            const uint32_t grid = (uint32_t)std::min<uint64_t>(4096, (total + kBlock - 1) / kBlock);
+            if (heap)
+                copy_heaps_kernel<<<grid, kBlock, 0, s>>>(d.heaps, heaps, d.heap_size, op.n_heaps, op.heap_stride,
+                                                          np.heap_stride, W, map, n_old);
+            else  // same layout: renumber in place
+                copy_heaps_kernel<<<grid, kBlock, 0, s>>>(d.heaps, d.heaps, d.heap_size, op.n_heaps, op.heap_stride,
+                                                          op.heap_stride, W, map, n_old);
+            d.heaps = heaps;
+            d.heap_cap = np.heap_cap;
+            d.heap_stride = np.heap_stride;
+        }
+        set_grow_limits(g, np, d);
+        // the next launch of this lane runs iteration paused_iter of the instance
+        d.start_iter = (uint32_t)(L.host_base - st.paused_iter);
+        resume_kernel<<<1, 64, 0, s>>>(d, d_descs, d_states, (uint32_t)j.slot);
+        hip_check(hipGetLastError(), "grow in place");
+        uint32_t fails = 0;
+        if (failed) hip_check(hipMemcpyAsync(&fails, failed, sizeof(fails), hipMemcpyDeviceToHost, s), "read back");
+        hip_check(hipStreamSynchronize(s), "grow in place");  // the old structures are unused from here on
+        if (fails) throw GaError(GA_E_HIP, "closed-set rehash lost keys");
+        auto drop = [&](void* old) {
+            arena->release(old);
+            j.allocs.erase(std::remove(j.allocs.begin(), j.allocs.end(), old), j.allocs.end());
+        };
+        if (heap) drop(j.desc.heaps);
+        if (table) drop(j.desc.table);
+        if (map) arena->release(map);
+        for (void* f : fresh)
+            if (f != map) j.allocs.push_back(f);
+        j.heap_shift = g.heap_shift;
+        j.table_shift = g.table_shift;
+        j.bytes += np.bytes - op.bytes;
+        j.desc = d;
+        ++j.grows;
+        stats.peak_bytes = std::max<uint64_t>(stats.peak_bytes, arena->in_use());
+        return true;
     }
 
     // Marks a running job as no longer launched (its lane's map drops it).
@@ -1100,6 +1298,7 @@ struct ga_engine {
         o->stored = dense ? 0 : s.occupied;
         o->iterations = s.iterations;
         o->retries = j.retries;
+        o->grows = j.grows;
         stats.expanded += s.expanded;
         stats.generated += s.generated;
         stats.pushed += s.pushed;
@@ -1411,45 +1610,22 @@ class RunLoop {
                 done.push_back(j);
                 continue;
             }
-            e->retire(*j);
-            if (st.status == ST_OVERFLOW_HEAP || st.status == ST_OVERFLOW_TABLE) {
-                e->fill_counters(*j, st);
-                e->release(*j);
-                --inflight;
-                if (j->retries >= e->cfg.max_retries) {
-                    complete(*j, GA_E_OVERFLOW);
-                } else {
-                    ++j->retries;
-                    // 4x the structure that overflowed; 2x the other one if it was
-                    // more than a quarter full (open entries per heap, stored states),
-                    // so that a search which outgrows both does not use up its retries
-                    // one structure at a time
-                    const ga_engine::Plan p = e->plan(*j);
-                    const uint32_t open = std::max({st.open_sum[0], st.open_sum[1], st.open_sum[2]});
-                    const bool heaps_full = (uint64_t)open * 4 > (uint64_t)p.heap_cap * p.n_heaps;
-                    const bool table_full = p.table_slots && st.occupied * 4 > p.table_slots;
-                    if (st.status == ST_OVERFLOW_HEAP) {
-                        j->heap_shift += 2;
-                        if (table_full) j->table_shift += 1;
-                    } else {
-                        j->table_shift += 2;
-                        if (heaps_full) j->heap_shift += 1;
-                    }
-                    // Hashed sets: the heaps together hold at least as many
-                    // entries as the closed set may store states.  The open
-                    // list stays below the stored count, so a search that has
-                    // outgrown its table does not fail again, late, on a heap
-                    // (a restart repeats all the work done so far).
-                    for (ga_engine::Plan q = e->plan(*j);
-                         q.table_slots && q.heap_cap < (1u << 28) &&
-                         (uint64_t)q.n_heaps * q.heap_cap < q.table_slots * e->cfg.max_load_pct / 100;
-                         q = e->plan(*j))
-                        ++j->heap_shift;
-                    j->finished = false;
-                    retry.push_back(j);
+            if (st.status == ST_PAUSED) {  // grow in place and resume; snapshots already in flight still show the pause
+                if (e->grow_in_place(*j, st, L)) {
+                    j->first_chunk = L.launched;
+                    continue;
                 }
+                // no room for the larger structures: restart as after an overflow
+                InstState ov = st;
+                ov.status = (st.grow[(st.paused_iter + 2) % 3] & GROW_TABLE) ? ST_OVERFLOW_TABLE : ST_OVERFLOW_HEAP;
+                restart_or_fail(j, ov);
                 continue;
             }
+            if (st.status == ST_OVERFLOW_HEAP || st.status == ST_OVERFLOW_TABLE) {
+                restart_or_fail(j, st);
+                continue;
+            }
+            e->retire(*j);
             // DONE without a goal = search space exhausted; ABORTED = iteration limit
             e->fill_counters(*j, st);
             j->out->cost = 0;
@@ -1457,6 +1633,52 @@ class RunLoop {
             e->release(*j);
             --inflight;
         }
+        launch_paths(lane, snap, done);
+    }
+
+    // A request that overflowed a heap or its closed set (st.status) restarts
+    // from scratch with larger structures, or fails after max_retries.
+    void restart_or_fail(Job* j, const InstState& st) {
+        e->retire(*j);
+        e->fill_counters(*j, st);
+        e->release(*j);
+        --inflight;
+        if (j->retries >= e->cfg.max_retries) {
+            complete(*j, GA_E_OVERFLOW);
+            return;
+        }
+        ++j->retries;
+        // 4x the structure that overflowed; 2x the other one if it was more than
+        // a quarter full (open entries per heap, stored states), so that a search
+        // which outgrows both does not use up its retries one structure at a time
+        const ga_engine::Plan p = e->plan(*j);
+        const uint32_t open = std::max({st.open_sum[0], st.open_sum[1], st.open_sum[2]});
+        const bool heaps_full = (uint64_t)open * 4 > (uint64_t)p.heap_cap * p.n_heaps;
+        const bool table_full = p.table_slots && st.occupied * 4 > p.table_slots;
+        if (st.status == ST_OVERFLOW_HEAP) {
+            j->heap_shift += 2;
+            if (table_full) j->table_shift += 1;
+        } else {
+            j->table_shift += 2;
+            if (heaps_full) j->heap_shift += 1;
+        }
+        // Hashed sets: the heaps together hold at least as many entries as the
+        // closed set may store states.  The open list stays below the stored
+        // count, so a search that has outgrown its table does not fail again,
+        // late, on a heap (a restart repeats all the work done so far).
+        for (ga_engine::Plan q = e->plan(*j);
+             q.table_slots && q.heap_cap < (1u << 28) &&
+             (uint64_t)q.n_heaps * q.heap_cap < q.table_slots * e->cfg.max_load_pct / 100;
+             q = e->plan(*j))
+            ++j->heap_shift;
+        j->finished = false;
+        retry.push_back(j);
+    }
+
+    // Walks the paths of the instances that finished in a chunk (one batch, on
+    // the path stream, once this lane's work so far has finished).
+    void launch_paths(uint32_t lane, const InstState* snap, const std::vector<Job*>& done) {
+        Lane& L = e->lanes[lane];
         if (done.empty()) return;
         // walk the paths on the path stream once this lane's work so far has finished
         PathBatch* b = new PathBatch();
@@ -1959,6 +2181,7 @@ void ga_config_default(ga_config* cfg) {
     cfg->slack_puzzle = 0;
     cfg->slack_grid = 0;
     cfg->pdb24 = 1;
+    cfg->grow_in_place = 1;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {

@@ -83,9 +83,9 @@ typedef struct ga_result {
     uint64_t expanded;     /* nodes expanded on the device */
     uint64_t stored;       /* distinct states inserted in the closed set */
     uint32_t iterations;   /* GA* iterations */
-    uint32_t retries;      /* restarts with larger capacities */
+    uint32_t retries;      /* restarts with larger capacities (the search repeated from the start) */
     uint32_t path_len;     /* states in path */
-    uint32_t reserved;
+    uint32_t grows;        /* in-place growths of the heaps or closed set (the search carried over) */
     uint64_t* path;        /* start..goal (side 5: 4 words per state); owned by the library, see ga_result_free */
     uint64_t search_ns;    /* admission to completion */
     uint64_t latency_ns;   /* arrival to result on the host (serving) */
@@ -120,7 +120,10 @@ typedef struct ga_config {
     uint32_t pdb24;          /* 1: 24-puzzles towards the ordered goal use additive pattern databases
                                 (built on the device at the first such request, 48.8 MB), others
                                 Manhattan; 0: Manhattan for every 24-puzzle */
-    uint32_t reserved;
+    uint32_t grow_in_place;  /* 1: a search whose heaps or closed set fill up pauses at an iteration
+                                boundary, the structure grows 4x with the search state carried over
+                                (ga_result.grows), and it resumes; 0: it overflows and restarts from
+                                scratch with larger structures (ga_result.retries) */
 } ga_config;
 
 /* Engine statistics.  With cfg.profile = n, kernel time covers the timed

@@ -74,6 +74,14 @@ enum : uint32_t {
     ST_OVERFLOW_TABLE = 3,
     ST_OVERFLOW_HEAP = 4,
     ST_ABORTED = 5,
+    ST_PAUSED = 6,  // stopped at an iteration boundary so that the host grows its heaps or closed set
+};
+
+// Grow requests (InstState::grow): raised during an iteration, acted on at the
+// next one, whose heaps all see them (they were written in the previous launch).
+enum : uint32_t {
+    GROW_HEAP = 1,   // a heap ended the iteration within one iteration's pushes of its capacity
+    GROW_TABLE = 2,  // the closed set passed its soft load limit
 };
 
 // Problem kinds compiled into the C ABI library.
@@ -154,6 +162,9 @@ struct InstDesc {
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
     const uint8_t* pdb;    // 24-puzzle: the pattern databases (kPdbBytes), or null for Manhattan
+    uint32_t grow_heap_at; // request GROW_HEAP when a heap ends an iteration above this many entries (0 = never)
+    uint32_t pad3;
+    uint64_t grow_table_at;  // hashed: request GROW_TABLE when more slots than this are occupied (0 = never)
 };
 static_assert(offsetof(InstDesc, goal_rc) % 4 == 0, "goal_rc is read as 32-bit words");
 
@@ -169,6 +180,8 @@ struct InstState {
     uint32_t open_sum[3]; // heap entries after the pushes, by iteration mod 3
     uint32_t active_now;  // latest frontier width (for the host)
     uint32_t pad2;
+    uint32_t grow[3];     // grow requests raised during an iteration, by iteration mod 3
+    uint32_t paused_iter; // the iteration at which the instance paused (ST_PAUSED)
     unsigned long long occupied;  // closed-set slots in use
     unsigned long long expanded;
     unsigned long long generated;
@@ -414,7 +427,7 @@ __device__ inline void step_begin(StepShared& sh) {
 // Adds the tallies to the instance's state for iteration `it`; every thread
 // of the workgroup calls it.  The last wave to arrive flushes, so waves that
 // finish early leave at once instead of waiting at a barrier.
-__device__ inline void tally_end(uint32_t* c, InstState* st, uint32_t it) {
+__device__ inline void tally_end(uint32_t* c, InstState* st, uint32_t it, uint64_t grow_table_at = 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // this wave's tally atomics first
     uint32_t arrived = 0;
     if (__lane_id() == 0) arrived = atomicAdd(&c[TL_WAVES_DONE], 1u);
@@ -427,7 +440,9 @@ __device__ inline void tally_end(uint32_t* c, InstState* st, uint32_t it) {
     if (i < kCounters) {
         if (v) {
             auto* ctr = reinterpret_cast<unsigned long long*>(reinterpret_cast<char*>(st) + offsetof(InstState, occupied));
-            atomicAdd(ctr + i, (unsigned long long)v);
+            const unsigned long long before = atomicAdd(ctr + i, (unsigned long long)v);
+            // the closed set passed its soft limit during this iteration: grow it before the next
+            if (i == TL_OCCUPIED && grow_table_at && before + v > grow_table_at) atomicOr(&st->grow[it % 3], GROW_TABLE);
         }
     } else if (i == TL_MIN_TOP) {
         if (v != kInf) atomicMin(&st->min_top[it % 3], v);
@@ -1377,7 +1392,20 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     const uint32_t mt_ld = load_relaxed(&st->min_top[(it + 2) % 3]);  // (it - 1) mod 3; unused at it = 0
     uint64_t occupied = 0;
     if constexpr (!Ops::kDense) occupied = load_relaxed(&st->occupied);
-    const bool running = __builtin_amdgcn_readfirstlane(status) == ST_RUNNING;
+    bool running = __builtin_amdgcn_readfirstlane(status) == ST_RUNNING;
+    // grow requests of the previous iteration (every workgroup of this launch reads the same
+    // value: it was written in the previous launch): the instance pauses here, before any of
+    // its heaps does anything in this iteration, and the host resumes it at this iteration
+    if (running && it > 0) {
+        const uint32_t req = __builtin_amdgcn_readfirstlane(load_relaxed(&st->grow[(it + 2) % 3]));
+        if (req) {
+            if (threadIdx.x == 0) {
+                store_relaxed(&st->paused_iter, it);
+                set_status(st, ST_PAUSED);
+            }
+            running = false;
+        }
+    }
     active = active && running && q < A;
     const gptr<const HeapEntry> cin = as_global<const HeapEntry>(d.cand) + (size_t)((it - 1) & 1) * N * K;
     uint32_t S = 0;
@@ -1429,6 +1457,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         store_relaxed(&st->active[(it + 1) % 3], an);
         store_relaxed(&st->active_now, an);
         store_relaxed(&st->open_sum[(it + 1) % 3], 0u);
+        store_relaxed(&st->grow[(it + 1) % 3], 0u);  // raised during the next iteration, read at the one after
     }
     if constexpr (!Ops::kDense) {
         if (live && occupied > d.load_limit) {
@@ -1648,7 +1677,11 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         }
         group_fence();
         H.write_back(S, lane);
-        if (lane == 0) as_global(d.heap_size)[q] = S;
+        if (lane == 0) {
+            as_global(d.heap_size)[q] = S;
+            // one more iteration could bring this heap past its capacity: grow before it
+            if (d.grow_heap_at && S > d.grow_heap_at) atomicOr(&st->grow[it % 3], GROW_HEAP);
+        }
         // candidate slice of this heap: K records
         const gptr<HeapEntry> cout = as_global(d.cand) + (size_t)(it & 1) * N * K + (size_t)q * K;
 #pragma unroll
@@ -1729,6 +1762,8 @@ __device__ inline bool seed_instance(const InstDesc& d, InstState* st, int W) {
     z.open_sum[0] = z.open_sum[1] = z.open_sum[2] = 0;
     z.active_now = a0;
     z.pad2 = 0;
+    z.grow[0] = z.grow[1] = z.grow[2] = 0;
+    z.paused_iter = 0;
     z.occupied = Ops::kDense ? 0 : 1;
     z.expanded = z.generated = z.pushed = z.pops = z.probes = z.sift_down = z.sift_up = 0;
     z.groups = z.inserted = z.lookups = z.work_groups = 0;

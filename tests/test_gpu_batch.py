@@ -69,7 +69,10 @@ def test_batch_config4_full_matches_golden(engine):
         assert replay_ok(insts[i], sols[i])
 
 
-def test_batch_forced_retries(engine):
+@pytest.mark.parametrize("grow", [1, 0], ids=["in-place", "restart"])
+def test_batch_forced_retries(grow):
+    # tiny heaps and closed sets: with in-place growth every search pauses, grows
+    # and resumes with its state (no restart); without, it restarts larger
     insts = small_mix(seed=9, n_puz=8, n_grid=4)
     reqs = []
     for x in insts:
@@ -77,10 +80,15 @@ def test_batch_forced_retries(engine):
         if x.kind == "puzzle":
             r.table_slots = 1 << 12
         reqs.append(r)
-    sols = engine.solve_batch(reqs)
-    assert any(s.retries > 0 for s in sols)
+    with ga.Engine(0, grow_in_place=grow, max_iterations=2_000_000) as e:
+        sols = e.solve_batch(reqs)
+    if grow:
+        assert any(s.grows > 0 for s in sols) and all(s.retries == 0 for s in sols)
+    else:
+        assert any(s.retries > 0 for s in sols) and all(s.grows == 0 for s in sols)
     for x, sol in zip(insts, sols):
         assert sol.status == ga.OK and sol.cost == oracle_cost(x)
+        assert replay_ok(x, sol)
 
 
 def test_overflow_grows_both_structures():
@@ -94,13 +102,40 @@ def test_overflow_grows_both_structures():
         r = x.request(heap_capacity=16)
         r.table_slots = 1 << 10
         reqs.append(r)
-    with ga.Engine(0, max_retries=6, max_iterations=2_000_000) as e:
+    with ga.Engine(0, max_retries=6, max_iterations=2_000_000, grow_in_place=0) as e:
         sols = e.solve_batch(reqs)
     for x, sol in zip(insts, sols):
         assert sol.status == ga.OK, ga.status_str(sol.status)
         assert sol.cost == oracle_cost(x)
         assert sol.retries >= 2
         assert replay_ok(x, sol)
+
+
+def test_in_place_growth_carries_the_search_over():
+    """Heaps of 16 entries and a closed set of 1024 slots grow in place several
+    times (both structures, hence the closed-set rehash with renumbered heap
+    entries, records and parent links): the cost is exact, the path replays,
+    and the search does about the work of one with large structures from the
+    start, where a restart repeats it."""
+    insts = [wl.puzzle15(wl.instance_seed(61, i), depth=48, num_queues=256, table_slots=0) for i in range(4)]
+    small, big = [], []
+    for x in insts:
+        r = x.request(heap_capacity=16)
+        r.table_slots = 1 << 10
+        small.append(r)
+        big.append(x.request(heap_capacity=1 << 14, table_slots=1 << 24))
+    with ga.Engine(0, max_iterations=2_000_000) as e:
+        grown = e.solve_batch(small)
+        ref = e.solve_batch(big)
+    with ga.Engine(0, max_iterations=2_000_000, grow_in_place=0) as e:
+        restarted = e.solve_batch(small)
+    for x, g, r, rs in zip(insts, grown, ref, restarted):
+        assert g.status == r.status == rs.status == ga.OK
+        assert g.cost == r.cost == rs.cost == oracle_cost(x)
+        assert g.grows >= 2 and g.retries == 0 and r.grows == 0
+        assert replay_ok(x, g)
+        assert g.expanded <= 1.1 * r.expanded
+        assert rs.expanded > g.expanded
 
 
 def test_batch_handles_invalid_and_unsolvable(engine):

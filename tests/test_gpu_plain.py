@@ -136,10 +136,18 @@ def test_grid_1024(engine):
 
 
 def test_capacity_growth(engine):
-    # tiny heaps and closed set force the overflow -> retry path
+    # tiny heaps and closed set: grown in place (paused, grown, resumed), never restarted
     start = ga.gen_puzzle(4, 60, 1)
     sol = check_puzzle(engine, 4, start, num_queues=1024, heap_capacity=16, table_slots=1 << 12)
-    assert sol.retries > 0
+    assert sol.grows > 0 and sol.retries == 0
+
+
+def test_capacity_growth_by_restart():
+    # the same with growth in place off: the overflow -> restart path
+    start = ga.gen_puzzle(4, 60, 1)
+    with ga.Engine(0, grow_in_place=0, max_iterations=2_000_000) as e:
+        sol = check_puzzle(e, 4, start, num_queues=1024, heap_capacity=16, table_slots=1 << 12)
+    assert sol.retries > 0 and sol.grows == 0
 
 
 def _golden(name):

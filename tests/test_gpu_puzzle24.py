@@ -72,18 +72,24 @@ def test_invalid_and_unsolvable(engine):
     check(tuple(b), sols[2])
 
 
-def test_forced_retries():
+@pytest.mark.parametrize("grow", [1, 0], ids=["in-place", "restart"])
+def test_forced_retries(grow):
+    # 32-byte board slots rehashed in place, or the restart path
     boards = [ga.gen_board(5, 36, 200 + i) for i in range(4)]
     reqs = []
     for b in boards:
         r = ga.puzzle5_request(b, num_queues=256, heap_capacity=16)
         r.table_slots = 1 << 10
         reqs.append(r)
-    with ga.Engine(0, max_iterations=2_000_000) as e:
+    with ga.Engine(0, max_iterations=2_000_000, grow_in_place=grow, pdb24=0) as e:
         sols = e.solve_batch(reqs)
-    assert any(s.retries > 0 for s in sols)
+    if grow:
+        assert any(s.grows > 0 for s in sols) and all(s.retries == 0 for s in sols)
+    else:
+        assert any(s.retries > 0 for s in sols)
     for b, sol in zip(boards, sols):
         check(b, sol)
+        assert oracle.replay_board(5, sol.path, b, ga.board_goal(5)) == sol.cost
 
 
 @pytest.mark.parametrize("variant", [dict(heap_arity=8, rounds=2, use_graph=1), dict(heap_arity=8, rounds=1, chunk_iters=7),
