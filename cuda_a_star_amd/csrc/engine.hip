@@ -1015,7 +1015,7 @@ struct ga_engine {
         Slot* tab = table ? (Slot*)take(np.table_slots * slot_bytes) : d.table;
         const uint64_t n_old = op.table_slots;
         uint32_t* map = table ? (uint32_t*)take(n_old * sizeof(uint32_t) + 256) : nullptr;
-        if (!heaps || !tab || (table && !map)) {
+        if (!heaps || (table && (!tab || !map))) {  // (grids have no table: d.table is null)
             for (void* f : fresh) arena->release(f);
             return false;
         }
