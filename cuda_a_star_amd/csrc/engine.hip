@@ -491,6 +491,7 @@ struct Job {
     Puzzle5Ops::Key ends[2] = {};  // 24-puzzle: start and goal in the device packing (uploaded at admission)
     void* owner = nullptr;         // the online server's entry holding this job
     uint32_t grows = 0;            // in-place growths of its heaps or closed set
+    uint64_t expanded_before = 0;  // expansions of the attempts given up by restarts
 };
 
 struct PathBatch {
@@ -1294,7 +1295,7 @@ struct ga_engine {
     void fill_counters(Job& j, const InstState& s) {
         ga_result* o = j.out;
         const bool dense = j.kind == KIND_GRID4 || j.kind == KIND_GRID8;
-        o->expanded = s.expanded;
+        o->expanded = j.expanded_before + s.expanded;
         o->stored = dense ? 0 : s.occupied;
         o->iterations = s.iterations;
         o->retries = j.retries;
@@ -1641,6 +1642,7 @@ class RunLoop {
     void restart_or_fail(Job* j, const InstState& st) {
         e->retire(*j);
         e->fill_counters(*j, st);
+        j->expanded_before += st.expanded;  // the next attempt starts from nothing
         e->release(*j);
         --inflight;
         if (j->retries >= e->cfg.max_retries) {

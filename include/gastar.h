@@ -80,7 +80,7 @@ typedef struct ga_request {
 typedef struct ga_result {
     int32_t status;        /* GA_OK, GA_NO_PATH or an error */
     uint32_t cost;         /* optimal path cost */
-    uint64_t expanded;     /* nodes expanded on the device */
+    uint64_t expanded;     /* nodes expanded on the device, over every attempt (restarts included) */
     uint64_t stored;       /* distinct states inserted in the closed set */
     uint32_t iterations;   /* GA* iterations */
     uint32_t retries;      /* restarts with larger capacities (the search repeated from the start) */

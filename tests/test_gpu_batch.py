@@ -135,7 +135,7 @@ def test_in_place_growth_carries_the_search_over():
         assert g.grows >= 2 and g.retries == 0 and r.grows == 0
         assert replay_ok(x, g)
         assert g.expanded <= 1.1 * r.expanded
-        assert rs.expanded > g.expanded
+        assert rs.expanded > g.expanded  # a result counts the expansions of every attempt
 
 
 def test_batch_handles_invalid_and_unsolvable(engine):
