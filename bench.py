@@ -127,8 +127,9 @@ P24_N, P24_DEPTH = 256, (30, 60)
 def puzzle24_leg(e, heaps: int, cores, cpu_seconds: float):
     """256 seeded 24-puzzles (walks of depth U{30..60} from the ordered board, no
     immediate backtrack) in one ga_solve_batch; the CPU oracle (sequential A*
-    on boards) takes the same instances in order on pinned workers for
-    `cpu_seconds`, and every instance it finishes checks the GPU cost."""
+    on boards with the same pattern databases) takes the same instances in
+    order on pinned workers for `cpu_seconds`, and every instance it finishes
+    checks the GPU cost.  Then 24 walks of depth U{80..120} in one batch."""
     import cuda_a_star_amd as ga
     from cuda_a_star_amd import workloads as wl
 
@@ -157,13 +158,19 @@ def puzzle24_leg(e, heaps: int, cores, cpu_seconds: float):
                        "workload": "24-puzzle walks of depth U{80..120}, additive pattern databases (5-5-5-5-4), "
                                    f"{heaps} heaps each, one batch; costs checked in the GPU tests against "
                                    "the pattern-database oracle"}
+    d_status = {}
+    for s in d_sols:
+        d_status[ga.status_str(s.status)] = d_status.get(ga.status_str(s.status), 0) + 1
+    out["deep_pdb"]["statuses"] = d_status
+    out["deep_pdb"]["grows_max"] = max(s.grows for s in d_sols)
     if cpu_seconds <= 0:
         return out
     import oracle
 
-    def solve(i):
-        rc, res, _ = oracle.astar_board(5, boards[i], ga.board_goal(5), max_states=MAX_ORACLE_STATES,
-                                        want_path=False)
+    oracle.pdb24()  # the oracle's own tables (built once, outside the timed window)
+
+    def solve(i):  # the same heuristic as the GPU: A* with the pattern databases
+        rc, res, _ = oracle.astar_board_pdb(boards[i], max_states=MAX_ORACLE_STATES, want_path=False)
         if rc == oracle.E_LIMIT:
             return False, None
         return True, res.cost if rc == oracle.OK else -1
@@ -172,6 +179,7 @@ def puzzle24_leg(e, heaps: int, cores, cpu_seconds: float):
     match = sum(1 for i in times if sols[i].status == ga.OK and sols[i].cost == results[i])
     capped = sum(1 for i in results if i not in times)
     out["cpu_oracle"] = {"solves_per_sec": round(len(times) / el, 3), "cores": len(cores), "kind": "port",
+                         "heuristic": "additive pattern databases (the GPU's)",
                          "sample": f"first {len(results)} instances in order on {len(cores)} pinned threads, "
                                    f"{el:.1f}s; {capped} above {MAX_ORACLE_STATES} stored states skipped"}
     out["gpu_cost_matches"] = f"{match}/{len(times)}"
