@@ -94,6 +94,8 @@ class Config(ctypes.Structure):
         ("slack_grid", ctypes.c_uint32),
         ("pdb24", ctypes.c_uint32),
         ("grow_in_place", ctypes.c_uint32),
+        ("block", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 

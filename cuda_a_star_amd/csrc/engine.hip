@@ -49,12 +49,13 @@ constexpr uint32_t kAllKinds =
 // the maximum over the kinds it can run, so a lane that holds one kind runs an
 // instantiation for that kind alone: the 15-puzzle and 4-connected grid paths
 // fit 64 VGPRs (8 waves per SIMD) where the kernel for all kinds needs 77 (6).
-template <int W, uint32_t KINDS>
-__global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict__ wgmap,
-                                                      const InstDesc* __restrict__ descs, InstState* states,
-                                                      const uint32_t* __restrict__ iter_base, uint32_t r,
-                                                      StepParams prm) {
-    __shared__ StepShared sh;
+template <int W, uint32_t KINDS, int B>
+__global__ __launch_bounds__(B) void step_kernel(const WgEntry* __restrict__ wgmap,
+                                                 const InstDesc* __restrict__ descs, InstState* states,
+                                                 const uint32_t* __restrict__ iter_base, uint32_t r,
+                                                 StepParams prm) {
+    static_assert((B / W) * HeapRef<W>::kTop <= StepSharedT<B>::kHeapTopEntries && W >= 4, "LDS heap tops");
+    __shared__ StepSharedT<B> sh;
     uint32_t inst, it, first, cover;
     if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
     const uint32_t status = load_relaxed(&states[inst].status);
@@ -99,12 +100,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(const WgEntry* __restrict_
 // Step kernel of the wide-key kinds (24-puzzle): its own instantiation so that
 // the 128-bit keys do not raise the register count of step_kernel.  Its
 // instances run on a launch lane of their own.
-template <int W>
-__global__ __launch_bounds__(kBlock) void step_kernel_wide(const WgEntry* __restrict__ wgmap,
-                                                           const InstDesc* __restrict__ descs, InstState* states,
-                                                           const uint32_t* __restrict__ iter_base, uint32_t r,
-                                                           StepParams prm) {
-    __shared__ StepShared sh;
+template <int W, int B>
+__global__ __launch_bounds__(B) void step_kernel_wide(const WgEntry* __restrict__ wgmap,
+                                                      const InstDesc* __restrict__ descs, InstState* states,
+                                                      const uint32_t* __restrict__ iter_base, uint32_t r,
+                                                      StepParams prm) {
+    static_assert((B / W) * HeapRef<W>::kTop <= StepSharedT<B>::kHeapTopEntries && W >= 4, "LDS heap tops");
+    __shared__ StepSharedT<B> sh;
     uint32_t inst, it, first, cover;
     if (!step_prologue(wgmap, descs, iter_base, r, inst, it, first, cover)) return;
     const uint32_t status = load_relaxed(&states[inst].status);
@@ -513,35 +515,47 @@ namespace {
 // the 24-puzzle kernel.
 constexpr uint32_t kWideVariant = kind_bit(KIND_PUZZLE5);
 
-template <int W>
+template <int W, int B>
 void launch_step_w(uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map, const InstDesc* descs,
                    InstState* states, const uint32_t* iter, uint32_t r, const StepParams& prm) {
-    const dim3 g(grid), b(kBlock);
+    const dim3 g(grid), b(B);
     switch (variant) {
         case kind_bit(KIND_PUZZLE3):
-            step_kernel<W, kind_bit(KIND_PUZZLE3)><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            step_kernel<W, kind_bit(KIND_PUZZLE3), B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             break;
         case kind_bit(KIND_PUZZLE4):
-            step_kernel<W, kind_bit(KIND_PUZZLE4)><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            step_kernel<W, kind_bit(KIND_PUZZLE4), B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             break;
         case kind_bit(KIND_GRID4):
-            step_kernel<W, kind_bit(KIND_GRID4)><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            step_kernel<W, kind_bit(KIND_GRID4), B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             break;
         case kWideVariant:
-            step_kernel_wide<W><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            step_kernel_wide<W, B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             break;
         default:
-            step_kernel<W, kAllKinds><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            step_kernel<W, kAllKinds, B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             break;
     }
 }
 
-void launch_step(uint32_t arity, uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map,
-                 const InstDesc* descs, InstState* states, const uint32_t* iter, uint32_t r, const StepParams& prm) {
+template <int B>
+void launch_step_b(uint32_t arity, uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map,
+                   const InstDesc* descs, InstState* states, const uint32_t* iter, uint32_t r, const StepParams& prm) {
     if (arity == 8)
-        launch_step_w<8>(variant, s, grid, map, descs, states, iter, r, prm);
+        launch_step_w<8, B>(variant, s, grid, map, descs, states, iter, r, prm);
     else
-        launch_step_w<4>(variant, s, grid, map, descs, states, iter, r, prm);
+        launch_step_w<4, B>(variant, s, grid, map, descs, states, iter, r, prm);
+}
+
+// `block`: threads per step workgroup (ga_config.block: 128, 256 or 512)
+void launch_step(uint32_t arity, uint32_t block, uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map,
+                 const InstDesc* descs, InstState* states, const uint32_t* iter, uint32_t r, const StepParams& prm) {
+    if (block == 128)
+        launch_step_b<128>(arity, variant, s, grid, map, descs, states, iter, r, prm);
+    else if (block == 512)
+        launch_step_b<512>(arity, variant, s, grid, map, descs, states, iter, r, prm);
+    else
+        launch_step_b<256>(arity, variant, s, grid, map, descs, states, iter, r, prm);
 }
 
 }  // namespace
@@ -645,6 +659,9 @@ struct ga_engine {
         // served run with it (4,096 heaps per 15-puzzle) once stalled with no
         // cause shown.
         if (cfg.heap_arity != 4 && cfg.heap_arity != 8) throw GaError(GA_E_INVALID, "heap_arity must be 4 or 8");
+        if (cfg.block == 0) cfg.block = kStepBlock;
+        if (cfg.block != 128 && cfg.block != 256 && cfg.block != 512)
+            throw GaError(GA_E_INVALID, "block must be 128, 256 or 512");
         if (cfg.lanes > GA_MAX_LANES) throw GaError(GA_E_INVALID, "lanes must be at most 4");
         if (cfg.max_instances == 0) cfg.max_instances = 1;
         if (cfg.chunk_iters == 0) cfg.chunk_iters = 1;
@@ -1142,7 +1159,7 @@ struct ga_engine {
     // ---- launch ---------------------------------------------------------------
     void upload_map(Lane& L, uint32_t lane) {
         const uint32_t W = cfg.heap_arity;
-        const uint32_t heaps_per_wg = kBlock / W;
+        const uint32_t heaps_per_wg = cfg.block / W;
         std::vector<WgEntry> map;
         L.n_heaps = 0;
         uint32_t kinds = 0;
@@ -1212,7 +1229,7 @@ struct ga_engine {
         hipStream_t stream = L.stream;
         const uint32_t variant = &L == &lanes.back() ? kWideVariant : L.variant;
         for (uint32_t r = 0; r < R; ++r)
-            launch_step(cfg.heap_arity, variant, stream, grid, L.d_map, d_descs, d_states, L.d_iter, r, prm);
+            launch_step(cfg.heap_arity, cfg.block, variant, stream, grid, L.d_map, d_descs, d_states, L.d_iter, r, prm);
         advance_kernel<<<1, 64, 0, stream>>>(L.d_iter, R);
     }
 
@@ -2071,7 +2088,7 @@ void solve_noalloc(ga_engine* e, const ga_request& r, ga_result* out, uint64_t* 
         InstDesc* d_desc = (InstDesc*)take(sizeof(InstDesc));
         InstState* d_state = (InstState*)take(sizeof(InstState));
         uint32_t* d_iter = (uint32_t*)take(sizeof(uint32_t));
-        const uint32_t heaps_per_wg = kBlock / W;
+        const uint32_t heaps_per_wg = e->cfg.block / W;
         const uint32_t n_wg = (p.n_heaps + heaps_per_wg - 1) / heaps_per_wg;
         WgEntry* d_map = (WgEntry*)take((uint64_t)n_wg * sizeof(WgEntry));
         InstDesc d{};
@@ -2088,7 +2105,8 @@ void solve_noalloc(ga_engine* e, const ga_request& r, ga_result* out, uint64_t* 
         hip_check(hipGetLastError(), "seed launch");
         const StepParams prm{e->cfg.pops_per_iter, 1u << 16, 0u, e->cfg.rounds, e->cfg.grow_thresh, {0, 0, 0}};
         for (;;) {
-            for (uint32_t k = 0; k < R; ++k) launch_step(W, variant, st, n_wg, d_map, d_desc, d_state, d_iter, k, prm);
+            for (uint32_t k = 0; k < R; ++k)
+                launch_step(W, e->cfg.block, variant, st, n_wg, d_map, d_desc, d_state, d_iter, k, prm);
             advance_kernel<<<1, 64, 0, st>>>(d_iter, R);
             hip_check(hipGetLastError(), "step launch");
             hip_check(hipMemcpyAsync(snap, d_state, sizeof(InstState), hipMemcpyDeviceToHost, st), "read back state");
@@ -2184,6 +2202,7 @@ void ga_config_default(ga_config* cfg) {
     cfg->slack_grid = 0;
     cfg->pdb24 = 1;
     cfg->grow_in_place = 1;
+    cfg->block = kStepBlock;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {

@@ -3,7 +3,8 @@
 Tuned per mode ("served": many instances per launch; "plain": one instance
 per launch):
   heaps per instance   by problem kind and grid size class
-  heap arity           lanes per heap group = heap arity (4, 8, 16)
+  heap arity           lanes per heap group = heap arity (4, 8)
+  block size           threads per step-kernel workgroup (128, 256, 512)
   chunk iterations     GA* iterations launched between host checks
   pops per iteration   heap pops tried while entries are stale
   hash load factor     closed-set load limit before a capacity retry
@@ -57,6 +58,7 @@ class TuneConfig:
     deep_cells: int = 4 << 20   # grids with at least this many cells take the last lane
     slack_puzzle: int = 0       # 0 = off; k: expand a heap top only if f <= previous min f + k - 1
     slack_grid: int = 0
+    block: int = 256            # threads per step-kernel workgroup
 
     def heaps(self, kind: str, side: int) -> int:
         if kind == "puzzle":
@@ -74,7 +76,7 @@ class TuneConfig:
                 "rounds": self.rounds, "use_graph": self.use_graph,
                 "adaptive_start": self.adaptive_start, "grow_thresh": self.grow_thresh,
                 "rounds_grid": self.rounds_grid, "lanes": self.lanes, "deep_cells": self.deep_cells,
-                "slack_puzzle": self.slack_puzzle, "slack_grid": self.slack_grid}
+                "slack_puzzle": self.slack_puzzle, "slack_grid": self.slack_grid, "block": self.block}
 
     def describe(self) -> Dict[str, int]:
         return asdict(self)
@@ -154,6 +156,7 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
             "deep_cells": [1 << 20, 2 << 20, 4 << 20, 8 << 20],
             "slack_puzzle": [0, 1, 3, 5, 9],
             "slack_grid": [0, 8, 16, 24, 32, 48],
+            "block": [128, 256, 512],
         }
     else:
         axes = {
@@ -172,6 +175,7 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
             "rounds_grid": [0, 1, 4, 8],
             "slack_puzzle": [0, 1, 3, 5, 9],
             "slack_grid": [0, 1, 2, 4, 8, 16],
+            "block": [128, 256, 512],
         }
     # coordinate descent: one axis at a time, two passes
     for _ in range(2):

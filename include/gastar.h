@@ -93,7 +93,8 @@ typedef struct ga_result {
 } ga_result;
 
 /* Engine configuration.  An invalid value (heap_arity other than 0, 4 or 8,
- * lanes above 4) makes ga_engine_create return GA_E_INVALID. */
+ * lanes above 4, block other than 0, 128, 256 or 512) makes ga_engine_create
+ * return GA_E_INVALID. */
 typedef struct ga_config {
     uint64_t budget_bytes;   /* admission budget for heap + closed-set bytes */
     uint64_t chunk_bytes;    /* device arena grows in chunks of this size */
@@ -124,6 +125,8 @@ typedef struct ga_config {
                                 boundary, the structure grows 4x with the search state carried over
                                 (ga_result.grows), and it resumes; 0: it overflows and restarts from
                                 scratch with larger structures (ga_result.retries) */
+    uint32_t block;          /* threads per step-kernel workgroup: 128, 256 or 512 (0 = 256) */
+    uint32_t reserved;
 } ga_config;
 
 /* Engine statistics.  With cfg.profile = n, kernel time covers the timed

@@ -392,33 +392,39 @@ struct Tally {
 };
 
 // Heaps of one workgroup with work this launch: filled in phase 1 of
-// run_group, read in phase 2 (at most 64 heap groups per workgroup).
-struct WorkList {
+// run_group, read in phase 2 (at most G heap groups per workgroup).
+template <int G>
+struct WorkListT {
     uint32_t n;
-    uint32_t q[64];  // heap
-    uint32_t S[64];  // its size before the pushes
+    uint32_t q[G];  // heap
+    uint32_t S[G];  // its size before the pushes
 };
 
-// Threads per step-kernel workgroup (the work list holds one entry per heap
-// group, so W >= 4).
+// Threads per step-kernel workgroup: the default, and the sizes the step
+// kernels are instantiated for (ga_config.block).  The work list holds one
+// entry per heap group, so W >= 4 gives at most B / 4 entries.
 constexpr int kStepBlock = 256;
-// LDS heap tops of a workgroup: (kStepBlock / W) groups of 1 + 3W entries,
-// most for W = 4.
-constexpr uint32_t kHeapTopEntries = (kStepBlock / 4) * (1 + 3 * 4);
 
-// LDS of one step-kernel workgroup.
-struct StepShared {
+// LDS of one step-kernel workgroup of B threads: the heap tops of its
+// (B / W) groups, 1 + 3W entries each (most for W = 4), the tallies and the
+// work list.
+template <int B>
+struct StepSharedT {
+    static constexpr uint32_t kHeapTopEntries = (B / 4) * (1 + 3 * 4);
     HeapEntry heap_top[kHeapTopEntries];
     uint32_t tally[kTallies];
-    WorkList work;
+    WorkListT<B / 4> work;
 };
+using StepShared = StepSharedT<kStepBlock>;
+using WorkList = WorkListT<kStepBlock / 4>;
 
 // Clears the tallies and the work list; every thread of the workgroup calls
 // it.  The kernels read the instance's status just before (each wave its own
 // load, issued as soon as the instance is known, so that it travels with the
 // instance descriptor's loads) and pass it to run_group, which every wave
 // calls whatever the status (it holds a barrier).
-__device__ inline void step_begin(StepShared& sh) {
+template <int B>
+__device__ inline void step_begin(StepSharedT<B>& sh) {
     if (threadIdx.x < kTallies) sh.tally[threadIdx.x] = threadIdx.x == TL_MIN_TOP ? kInf : 0u;
     if (threadIdx.x == 0) sh.work.n = 0;
     __syncthreads();
@@ -1362,11 +1368,10 @@ __device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const Heap
 // still enters the reduction.
 //
 // Every thread of the workgroup calls run_group (it holds a barrier).
-template <int W, class Ops>
+template <int W, class Ops, class WL>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t status, uint32_t it, uint32_t q,
                                           bool active, uint32_t cover, const StepParams& prm, const Tally& tl,
-                                          WorkList& wl, HeapEntry* heap_top) {
-    static_assert(W >= 4 && (kStepBlock / W) * HeapRef<W>::kTop <= kHeapTopEntries, "LDS heap tops");
+                                          WL& wl, HeapEntry* heap_top) {
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));

@@ -54,7 +54,7 @@ int main(void) {
   S(ga_request) F(ga_request, start) F(ga_request, cells) F(ga_request, cells_device)
   F(ga_request, num_queues) F(ga_request, table_slots) F(ga_request, user_tag)
   S(ga_result) F(ga_result, expanded) F(ga_result, path) F(ga_result, latency_ns) F(ga_result, user_tag)
-  S(ga_config) F(ga_config, max_instances) F(ga_config, profile) F(ga_config, rounds) F(ga_config, grow_thresh) F(ga_config, rounds_grid) F(ga_config, lanes) F(ga_config, deep_cells) F(ga_config, slack_grid)
+  S(ga_config) F(ga_config, max_instances) F(ga_config, profile) F(ga_config, rounds) F(ga_config, grow_thresh) F(ga_config, rounds_grid) F(ga_config, lanes) F(ga_config, deep_cells) F(ga_config, slack_grid) F(ga_config, pdb24) F(ga_config, grow_in_place) F(ga_config, block)
   S(ga_engine_stats) F(ga_engine_stats, step_ms_total) F(ga_engine_stats, peak_bytes) F(ga_engine_stats, timed_launches) F(ga_engine_stats, heap_groups) F(ga_engine_stats, work_groups) F(ga_engine_stats, push_in) F(ga_engine_stats, counted_launches) F(ga_engine_stats, lane_launches) F(ga_engine_stats, lane_counted_launches) F(ga_engine_stats, lane_step_ms) F(ga_engine_stats, lane_algo_bytes)
   S(ga_serve_stats) F(ga_serve_stats, solves_per_sec) F(ga_serve_stats, max_inflight)
   return 0;
@@ -244,7 +244,7 @@ def test_python_wrappers_raise_on_a_failed_run(monkeypatch):
 def test_engine_create_rejects_invalid_config_before_any_device_call():
     cfg = ga.default_config()
     h = ctypes.c_void_p()
-    for field, value in (("heap_arity", 16), ("heap_arity", 5), ("lanes", 5)):
+    for field, value in (("heap_arity", 16), ("heap_arity", 5), ("lanes", 5), ("block", 100), ("block", 1024)):
         c = ga.default_config()
         setattr(c, field, value)
         assert ga.lib().ga_engine_create(0, ctypes.byref(c), ctypes.byref(h)) == ga.E_INVALID, field

@@ -231,6 +231,10 @@ VARIANTS = [
     dict(heap_arity=4, rounds=2, slack_puzzle=1, slack_grid=1),
     dict(heap_arity=8, rounds=1, slack_puzzle=3, slack_grid=5, lanes=2, deep_cells=1 << 14),
     dict(heap_arity=4, rounds=2, slack_grid=2, adaptive_start=16, grow_thresh=4),
+    dict(heap_arity=4, rounds=1, block=128, use_graph=1),
+    dict(heap_arity=4, rounds=2, block=512, lanes=2, deep_cells=1 << 14),
+    dict(heap_arity=8, rounds=1, block=512, use_graph=1, slack_puzzle=3),
+    dict(heap_arity=8, rounds=2, block=128, adaptive_start=16, grow_thresh=4),
 ]
 
 
