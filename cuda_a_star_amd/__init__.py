@@ -95,7 +95,7 @@ class Config(ctypes.Structure):
         ("pdb24", ctypes.c_uint32),
         ("grow_in_place", ctypes.c_uint32),
         ("block", ctypes.c_uint32),
-        ("reserved", ctypes.c_uint32),
+        ("step_flags", ctypes.c_uint32),
     ]
 
 

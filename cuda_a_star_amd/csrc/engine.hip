@@ -69,25 +69,25 @@ __global__ __launch_bounds__(B) void step_kernel(const WgEntry* __restrict__ wgm
         if constexpr ((KINDS & kind_bit(KIND_PUZZLE3)) != 0)
             if (d.kind == KIND_PUZZLE3) {
                 run_group<W, PuzzleOps<3>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
-                                           sh.heap_top);
+                                           sh.heap_top, sh.dirty);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_PUZZLE4)) != 0)
             if (d.kind == KIND_PUZZLE4) {
                 run_group<W, PuzzleOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
-                                           sh.heap_top);
+                                           sh.heap_top, sh.dirty);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID4)) != 0)
             if (d.kind == KIND_GRID4) {
                 run_group<W, GridOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
-                                         sh.heap_top);
+                                         sh.heap_top, sh.dirty);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID8)) != 0)
             if (d.kind == KIND_GRID8) {
                 run_group<W, GridOps<8>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
-                                         sh.heap_top);
+                                         sh.heap_top, sh.dirty);
                 ran = true;
             }
         // a kind this instantiation cannot run: stop the instance rather than let
@@ -116,7 +116,7 @@ __global__ __launch_bounds__(B) void step_kernel_wide(const WgEntry* __restrict_
     const Tally tl{sh.tally, prm.profile != 0};
     if (d.kind == KIND_PUZZLE5)
         run_group<W, Puzzle5Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm, tl, sh.work,
-                                 sh.heap_top);
+                                 sh.heap_top, sh.dirty);
     tally_end(sh.tally, states + inst, it, d.grow_table_at);
 }
 
@@ -1245,7 +1245,7 @@ struct ga_engine {
         const uint64_t ph = cfg.profile ? L.launched % cfg.profile : 1;
         const bool timed = cfg.profile && ph == 0;
         const bool counted = cfg.profile && ph == cfg.profile / 2;
-        StepParams prm{cfg.pops_per_iter, 1u << 16, counted ? 1u : 0u, cfg.rounds, cfg.grow_thresh, {0, 0, 0}};
+        StepParams prm{cfg.pops_per_iter, 1u << 16, counted ? 1u : 0u, cfg.rounds, cfg.grow_thresh, cfg.step_flags, {0, 0}};
         const int b = (int)(L.launched & 1);
         std::vector<hipEvent_t>& pe = L.prof_ev[b];
         if (timed && pe.size() < 2) {
@@ -2103,7 +2103,8 @@ void solve_noalloc(ga_engine* e, const ga_request& r, ga_result* out, uint64_t* 
         d.start_iter = 0;
         seed_kernel<<<1, 64, 0, st>>>(d, d_desc, d_state, 0u, (int)W);
         hip_check(hipGetLastError(), "seed launch");
-        const StepParams prm{e->cfg.pops_per_iter, 1u << 16, 0u, e->cfg.rounds, e->cfg.grow_thresh, {0, 0, 0}};
+        const StepParams prm{e->cfg.pops_per_iter, 1u << 16, 0u, e->cfg.rounds, e->cfg.grow_thresh, e->cfg.step_flags,
+                             {0, 0}};
         for (;;) {
             for (uint32_t k = 0; k < R; ++k)
                 launch_step(W, e->cfg.block, variant, st, n_wg, d_map, d_desc, d_state, d_iter, k, prm);

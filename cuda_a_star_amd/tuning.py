@@ -59,6 +59,7 @@ class TuneConfig:
     slack_puzzle: int = 0       # 0 = off; k: expand a heap top only if f <= previous min f + k - 1
     slack_grid: int = 0
     block: int = 256            # threads per step-kernel workgroup
+    step_flags: int = 0         # step-kernel variants (ga_config.step_flags)
 
     def heaps(self, kind: str, side: int) -> int:
         if kind == "puzzle":
@@ -76,7 +77,8 @@ class TuneConfig:
                 "rounds": self.rounds, "use_graph": self.use_graph,
                 "adaptive_start": self.adaptive_start, "grow_thresh": self.grow_thresh,
                 "rounds_grid": self.rounds_grid, "lanes": self.lanes, "deep_cells": self.deep_cells,
-                "slack_puzzle": self.slack_puzzle, "slack_grid": self.slack_grid, "block": self.block}
+                "slack_puzzle": self.slack_puzzle, "slack_grid": self.slack_grid, "block": self.block,
+                "step_flags": self.step_flags}
 
     def describe(self) -> Dict[str, int]:
         return asdict(self)
@@ -157,6 +159,7 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
             "slack_puzzle": [0, 1, 3, 5, 9],
             "slack_grid": [0, 8, 16, 24, 32, 48],
             "block": [128, 256, 512],
+            "step_flags": [0, 1, 2, 3],
         }
     else:
         axes = {

@@ -126,7 +126,10 @@ typedef struct ga_config {
                                 (ga_result.grows), and it resumes; 0: it overflows and restarts from
                                 scratch with larger structures (ga_result.retries) */
     uint32_t block;          /* threads per step-kernel workgroup: 128, 256 or 512 (0 = 256) */
-    uint32_t reserved;
+    uint32_t step_flags;     /* step-kernel variants (each keeps results exact): 1 = write back only
+                                the LDS heap-top entries a launch changed; 2 = grids read each entered
+                                cell's stored g with its weight and skip the atomicMin that could not
+                                lower it */
 } ga_config;
 
 /* Engine statistics.  With cfg.profile = n, kernel time covers the timed

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(kBlock) void user_step_kernel(const WgEntry* __rest
     const InstDesc& d = descs[inst];
     const uint32_t q = first + threadIdx.x / kW;
     run_group<kW, Ops>(d, states + inst, status, it, q, q < d.n_heaps, cover, prm,
-                       Tally{sh.tally, prm.profile != 0}, sh.work, sh.heap_top);
+                       Tally{sh.tally, prm.profile != 0}, sh.work, sh.heap_top, sh.dirty);
     tally_end(sh.tally, states + inst, it);
 }
 
@@ -181,7 +181,7 @@ std::pair<std::vector<T>, std::chrono::nanoseconds> solve(const T& start, const 
         user_seed_kernel<Ops><<<1, 64>>>(d, descs, st);
         check(hipGetLastError(), "seed");
         check(hipDeviceSynchronize(), "seed");
-        StepParams prm{2, 1u << 16, 0, 1, 32, {0, 0, 0}};
+        StepParams prm{2, 1u << 16, 0, 1, 32, 0, {0, 0}};
         InstState hs{};
         auto t0 = std::chrono::steady_clock::now();
         for (;;) {

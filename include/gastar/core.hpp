@@ -228,13 +228,20 @@ struct WgEntry {
     uint32_t pad;
 };
 
+// StepParams::flags
+enum : uint32_t {
+    STEP_DIRTY_TOPS = 1,     // write back only the LDS heap-top entries a launch changed
+    STEP_SKIP_SETTLED = 2,   // grids: no atomicMin for a successor whose cell's stored g is already no larger
+};
+
 struct StepParams {
     uint32_t max_pops;     // pops tried per expansion round while entries are stale
     uint32_t max_probes;   // closed-set probe limit (overflow guard)
     uint32_t profile;      // count probes and sift levels
     uint32_t rounds;       // expansion rounds per launch (>= 1)
     uint32_t grow_thresh;  // double the frontier width when open entries exceed this many per heap
-    uint32_t pad[3];
+    uint32_t flags;        // STEP_* switches
+    uint32_t pad[2];
 };
 
 // ---------------------------------------------------------------------------
@@ -412,6 +419,7 @@ template <int B>
 struct StepSharedT {
     static constexpr uint32_t kHeapTopEntries = (B / 4) * (1 + 3 * 4);
     HeapEntry heap_top[kHeapTopEntries];
+    uint32_t dirty[B / 4];  // per heap group: the heap-top entries its launch changed
     uint32_t tally[kTallies];
     WorkListT<B / 4> work;
 };
@@ -490,6 +498,8 @@ struct HeapRef {
     static constexpr int kPer = (int)((kTop + W - 1) / W);  // LDS entries per lane in fill/write_back
     gptr<HeapEntry> g;  // logical entry 0 in global memory
     lptr<HeapEntry> t;  // logical entry 0 in LDS
+    lptr<uint32_t> dirty;  // bit j: LDS entry j changed since the fill
+    bool track;            // keep `dirty` (else every entry is written back)
 
     __device__ __forceinline__ HeapEntry get(uint32_t j) const {
         if (j < kTop) return t[j];
@@ -497,10 +507,12 @@ struct HeapRef {
     }
     __device__ __forceinline__ uint64_t prio(uint32_t j) const { return j < kTop ? t[j].prio : g[j].prio; }
     __device__ __forceinline__ void put(uint32_t j, const HeapEntry& e) const {
-        if (j < kTop)
+        if (j < kTop) {
             t[j] = e;
-        else
+            if (track) __hip_atomic_fetch_or(dirty, 1u << j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
             g[j] = e;
+        }
     }
     // Copies entries [0, min(S, kTop)) into LDS (lane l: l, l+W, ...).  The
     // loads carry no branch (positions at or past S read entry 0, which every
@@ -518,15 +530,18 @@ struct HeapRef {
             if (j < S && j < kTop) t[j] = v[i];
         }
     }
-    // Writes entries [0, min(S, kTop)) back to global memory.
+    // Writes entries [0, min(S, kTop)) back to global memory: all of them, or
+    // (with a dirty mask) those the launch changed.
     __device__ __forceinline__ void write_back(uint32_t S, int lane) const {
+        const uint32_t mask = track ? *dirty : ~0u;
 #pragma unroll
         for (int i = 0; i < kPer; ++i) {
             const uint32_t j = (uint32_t)lane + (uint32_t)i * W;
-            if (j < S && j < kTop) g[j] = t[j];
+            if (j < S && j < kTop && ((mask >> j) & 1u)) g[j] = t[j];
         }
     }
 };
+static_assert(HeapRef<8>::kTop <= 32, "dirty masks are 32-bit");
 
 // Removes the minimum.  The root, the last entry and the root's W children
 // are read together; the popped state's key and stored g (via `load`) are
@@ -968,6 +983,13 @@ struct GridOps : DenseCellSet {
         return as_global(d.cells)[(uint64_t)ny * d.width + nx];
     }
 
+    // Stored g of the cell that move i enters (kInf off the grid), read with its weight.
+    __device__ static inline uint32_t prefetch_g(const InstDesc& d, const HeapEntry& e, int i) {
+        uint32_t nx, ny;
+        if (!neighbour(d, e.aux, i, nx, ny)) return kInf;
+        return g_of(as_global(d.dense) + (uint64_t)ny * d.width + nx);
+    }
+
     __device__ static inline bool neighbour(const InstDesc& d, uint32_t aux, int i, uint32_t& nx, uint32_t& ny) {
         // move i: dx = {1,-1,0,0,1,1,-1,-1}[i], dy = {0,0,1,-1,1,-1,1,-1}[i],
         // each stored +1 in nibble i of an immediate (no table in memory)
@@ -1371,7 +1393,7 @@ __device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const Heap
 template <int W, class Ops, class WL>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t status, uint32_t it, uint32_t q,
                                           bool active, uint32_t cover, const StepParams& prm, const Tally& tl,
-                                          WL& wl, HeapEntry* heap_top) {
+                                          WL& wl, HeapEntry* heap_top, uint32_t* dirty_masks) {
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
@@ -1535,8 +1557,9 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     const bool mine = g < n_work;
     q = mine ? wl.q[g] : 0u;
     S = mine ? wl.S[g] : 0u;
+    const bool track = (prm.flags & STEP_DIRTY_TOPS) != 0;
     const HeapRef<W> H{as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1),
-                       as_local(heap_top) + (size_t)g * HeapRef<W>::kTop};
+                       as_local(heap_top) + (size_t)g * HeapRef<W>::kTop, as_local(dirty_masks) + g, track};
 #pragma unroll
     for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
     if (mine && it > 0 && q < Aw) {  // the records phase 1 read for this heap (cached)
@@ -1547,7 +1570,10 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             if (j < K) rec[r] = cin[base + (size_t)j * Aw];
         }
     }
-    if (mine) H.fill(S, lane);  // the heap's top levels into LDS, with the records' loads
+    if (mine) {
+        if (track && lane == 0) *H.dirty = 0u;
+        H.fill(S, lane);  // the heap's top levels into LDS, with the records' loads
+    }
     group_fence();
     bool overflow_heap = false, overflow_table = false;
     // always-on counts, per lane: one wave reduction and one LDS atomic each at
@@ -1566,11 +1592,16 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     uint32_t top_par = kNone;  // the popped node's stored parent (dense sets)
     auto load = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& gs) { Ops::load(d, t.slot, k, gs, top_par); };
     uint32_t pre[KW];  // per successor: what the problem reads early (grid: the entered cell's weight)
+    uint32_t pre_g[KW];  // dense sets with STEP_SKIP_SETTLED: the entered cell's stored g, read as early
+    const bool skip_settled = Ops::kDense && (prm.flags & STEP_SKIP_SETTLED) != 0;
     auto prefetch = [&](const HeapEntry& t) {
 #pragma unroll
         for (int r = 0; r < KW; ++r) {
             const int i = r * W + lane;
             pre[r] = i < K ? Ops::prefetch(d, t, i) : 0u;
+            pre_g[r] = kInf;
+            if constexpr (Ops::kDense)
+                if (skip_settled && i < K) pre_g[r] = Ops::prefetch_g(d, t, i);
         }
     };
     HeapEntry top;
@@ -1647,7 +1678,9 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                 n_generated += gen ? 1u : 0u;
                 if (gen) {
                     const uint32_t g2 = g0 + cost, f2 = g2 + th;
-                    if (f2 < best) {
+                    // a cell's stored g only decreases, so a read of it that is already no larger
+                    // than g2 means the atomicMin below could not succeed
+                    if (f2 < best && pre_g[r] > g2) {
                         uint32_t slot;
                         bool isnew;
                         uint32_t np = 0;
