@@ -1503,15 +1503,20 @@ class RunLoop {
         }
         // -- every lane keeps up to two chunks in flight and handles read-backs in order
         bool busy = false;
-        uint32_t lanes_busy = 0;  // a lone busy lane may block on its chunk instead of polling
+        // A lone busy lane may block on its chunk instead of polling, but only
+        // when nothing else waits for the host: a block lasts a whole chunk, and
+        // requests arriving meanwhile would not be admitted (another lane that
+        // ran dry would stay idle for that long).
+        uint32_t lanes_busy = 0;
         for (const Lane& L : e->lanes) lanes_busy += (L.running > 0 || L.launched > L.processed) ? 1u : 0u;
+        const bool may_block = lanes_busy == 1 && pending.empty() && retry.empty() && batches.empty();
         for (uint32_t l = 0; l < e->n_lanes; ++l) {
             Lane& L = e->lanes[l];
             if (L.launched > L.processed) {
                 const uint64_t c = L.processed;
                 const bool must_wait = L.launched - L.processed >= 2 || L.running == 0;
                 hipError_t q = hipEventQuery(L.ev_chunk[c & 1]);
-                if (q == hipErrorNotReady && must_wait && lanes_busy == 1) {
+                if (q == hipErrorNotReady && must_wait && may_block) {
                     hip_check(hipEventSynchronize(L.ev_chunk[c & 1]), "chunk event");
                     q = hipSuccess;
                 }
