@@ -15,7 +15,7 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libgastar.so")
 CLI = os.path.join(HERE, "gastar_cli")
 
-SOURCES = [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "instances.cpp")]
+SOURCES = [os.path.join(CSRC, "engine.hip"), os.path.join(CSRC, "checks.hip"), os.path.join(CSRC, "instances.cpp")]
 CLI_SOURCE = os.path.join(CSRC, "cli.hip")
 HEADERS = [os.path.join(INCLUDE, "gastar.h"), os.path.join(INCLUDE, "gastar", "core.hpp")]
 

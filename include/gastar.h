@@ -291,6 +291,64 @@ int ga_pdb24_tables(ga_engine* e, uint8_t* out, size_t bytes);
  * ga_solve_batch returns then. */
 int ga_debug_fail_run(const ga_request* reqs, uint32_t n, ga_result* outs, uint32_t finish_first, int fail_code);
 
+/* ---- kernel-level checks of the device data structures (tests) ----
+ * Neither call has a counterpart in the reference.  Each runs the engine's
+ * own device functions on caller data, on GPU `device`, in device memory it
+ * allocates and frees itself, so that the closed sets and the heaps can be
+ * checked against host models one operation at a time instead of only
+ * through whole searches.  Both return GA_OK, GA_E_INVALID (arguments,
+ * checked before any device call), GA_E_NOMEM or GA_E_HIP. */
+
+/* Closed-set layouts: 64-bit keys (the 15- and 8-puzzles; 2 words per key,
+ * low word first), 24-puzzle boards in 32-byte slots (4 words: lo, hi), and
+ * 12-byte states behind a fingerprint tag (the header drop-in's layout for
+ * states wider than 8 bytes, e.g. Position<3>; 3 words). */
+enum { GA_SET_U64 = 0, GA_SET_BOARD = 1, GA_SET_WIDE12 = 2 };
+/* Outcomes: the contract of the reference's mass insert, fancy_hashmap.h:4-8,
+ * 56-61 (INSERTED: the state was new; UPDATED: it was present and its g was
+ * lowered; REJECTED: neither), plus FULL: no slot found within max_probes. */
+enum { GA_CS_REJECTED = 0, GA_CS_INSERTED = 1, GA_CS_UPDATED = 2, GA_CS_FULL = 3 };
+/* Inserts n successors (key i = words [i*kw, (i+1)*kw), kw = 2, 4 or 3 by
+ * layout; value g[i] << 32 | parent[i]) into an empty set of 2^log2_slots
+ * slots (1 <= log2_slots <= 24), each as one step of the search does
+ * (find or claim the slot, then atomicMin of the value): all n at once, one
+ * device thread each (serial = 0), or one after another in index order on
+ * one thread (serial = 1).  Writes outcome[i] and slot[i] (0xFFFFFFFF for
+ * FULL) and, where not null, the table after the last insert:
+ * tag_out[s] = the word a probe claims (the key; a board's lo; a 12-byte
+ * state's tag: all ones for empty, else its hash >> 2), key_out[s*kw ..] =
+ * the state stored in slot s, val_out[s] = its value (all ones if empty). */
+int ga_debug_closed_set(int device, int layout, const uint32_t* keys, const uint32_t* g, const uint32_t* parent,
+                        uint32_t n, uint32_t log2_slots, uint32_t max_probes, int serial, int32_t* outcome,
+                        uint32_t* slot, uint64_t* tag_out, uint32_t* key_out, uint64_t* val_out);
+
+/* Heap operations, one value per operation; any other value v, with
+ * v < 0xFFFFFFFF00000000 (f below the infinite cost), is a push of a record
+ * with priority v.  Pushes are collected, up to 2 * arity per batch, and
+ * pushed as one batch of candidate records (the batch is flushed before a
+ * pop, a FLUSH, at the end of a launch and when full); GAP leaves an empty
+ * record position in the batch, as an expansion without a survivor does. */
+#define GA_HOP_POP 0xFFFFFFFFFFFFFFFFull
+#define GA_HOP_FLUSH 0xFFFFFFFFFFFFFFFEull
+#define GA_HOP_NOP 0xFFFFFFFFFFFFFFFDull
+#define GA_HOP_GAP 0xFFFFFFFFFFFFFFFCull
+/* Runs n_heaps (1 <= n_heaps <= 64 / arity) heaps of `arity` (4 or 8) and
+ * capacity cap (1 <= cap <= 2^20) in one wave, heap h executing
+ * ops[h*n_ops .. (h+1)*n_ops) (n_ops <= 2^20), ops_per_launch operations per
+ * kernel launch, each launch copying the heaps' top levels into LDS and back
+ * as a search step does (track_dirty: write back only the changed entries).
+ * Writes, per heap and operation, popped[h*n_ops + i] (the priority a POP
+ * returned; all ones for an empty heap or another operation) and
+ * popped_tag[h*n_ops + i] (the index, within the heap's operations, of the
+ * push that entry came from; 0xFFFFFFFE if it does not carry its own heap's
+ * mark), then the heap after the last operation: size_out[h], heap_out
+ * [h*cap + j] = the priority of logical entry j < size_out[h], and
+ * overflow_out[h] = the number of batches refused because they would have
+ * passed the capacity (such a batch pushes nothing). */
+int ga_debug_heap(int device, uint32_t arity, uint32_t n_heaps, const uint64_t* ops, uint32_t n_ops,
+                  uint32_t ops_per_launch, uint32_t cap, int track_dirty, uint64_t* popped, uint32_t* popped_tag,
+                  uint64_t* heap_out, uint32_t* size_out, uint32_t* overflow_out);
+
 /* Device memory helpers for callers that keep grids resident in HBM. */
 int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device_out);
 int ga_device_free(ga_engine* e, void* device);

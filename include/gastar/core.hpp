@@ -851,7 +851,7 @@ struct WideKeySet {
                     if (same(load_key(d, (uint32_t)i), key)) outcome = 1;
                 }
                 if (outcome == 0) {
-                    if (++probed > max_probes || probed > d.table_mask) outcome = 3;
+                    if (++probed >= max_probes || probed > d.table_mask) outcome = 3;
                     else i = (i + 1) & d.table_mask;
                 }
             }
@@ -1100,7 +1100,7 @@ struct Board5KeySet {
                 else if (seen_hi == hi) outcome = 1;
             }
             if (outcome == 0 && !again) {
-                if (++probed > max_probes || probed > d.table_mask) outcome = 3;
+                if (++probed >= max_probes || probed > d.table_mask) outcome = 3;
                 else i = (i + 1) & d.table_mask;
             }
             if (outcome != 0) break;
@@ -1262,6 +1262,35 @@ struct Puzzle5Ops : Board5KeySet {
         return true;
     }
 };
+
+// ---------------------------------------------------------------------------
+// closed-set update of one successor
+// ---------------------------------------------------------------------------
+// The contract of the reference's mass insert (fancy_hashmap.h:4-8,56-61):
+// a successor is INSERTED when its state was not in the set, UPDATED when it
+// was and the successor lowers its g, and REJECTED otherwise; the first two
+// survive and are pushed.  Here the state is found or claimed (find_or_insert)
+// and then atomicMin(val, g << 32 | parent) decides: the successor survives
+// iff the minimum it replaced had a larger g.  Successors of one state that
+// meet in the same launch are ordered by the atomic, so exactly one claim per
+// state reports it new, the stored value ends as the smallest (g, parent) of
+// all of them, and at least the one that stored it survives.  A claim whose
+// atomicMin found a smaller or equal g already there (another successor of
+// the same state came first) is REJECTED.  CS_FULL: the probe limit was
+// reached without finding or claiming a slot (a closed-set overflow).
+enum : int { CS_REJECTED = 0, CS_INSERTED = 1, CS_UPDATED = 2, CS_FULL = 3 };
+
+template <class Ops>
+__device__ __forceinline__ int closed_set_update(const InstDesc& d, const typename Ops::Key& key, uint32_t g,
+                                                 uint32_t parent, uint32_t max_probes, uint32_t& slot, bool& isnew,
+                                                 uint32_t& probes) {
+    if (!Ops::find_or_insert(d, key, max_probes, slot, isnew, probes)) return CS_FULL;
+    const uint64_t mine = ((uint64_t)g << 32) | parent;
+    const uint64_t old =
+        __hip_atomic_fetch_min(Ops::val_ptr(d, slot), mine, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (g >= val_g(old)) return CS_REJECTED;
+    return isnew ? CS_INSERTED : CS_UPDATED;
+}
 
 // ---------------------------------------------------------------------------
 // one GA* iteration of one heap group
@@ -1684,21 +1713,15 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                         uint32_t slot;
                         bool isnew;
                         uint32_t np = 0;
-                        const bool found = Ops::find_or_insert(d, skey, prm.max_probes, slot, isnew, np);
-                        bool ins = false;
-                        if (found) {
-                            const uint64_t mine_val = ((uint64_t)g2 << 32) | top.slot;
-                            const uint64_t old = __hip_atomic_fetch_min(Ops::val_ptr(d, slot), mine_val,
-                                                                        __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            ins = g2 < val_g(old);
-                            if (ins) {
-                                rec[r].prio = make_prio(f2, g2);
-                                rec[r].slot = slot;
-                                rec[r].aux = taux;
-                            }
-                        } else {
-                            overflow_table = true;
+                        const int cs = closed_set_update<Ops>(d, skey, g2, top.slot, prm.max_probes, slot, isnew, np);
+                        const bool found = cs != CS_FULL;
+                        const bool ins = cs == CS_INSERTED || cs == CS_UPDATED;
+                        if (ins) {
+                            rec[r].prio = make_prio(f2, g2);
+                            rec[r].slot = slot;
+                            rec[r].aux = taux;
                         }
+                        overflow_table |= !found;
                         if constexpr (!Ops::kDense) n_occupied += (found && isnew) ? 1u : 0u;
                         tl.prof_count(TL_PROBES, true);  // first probe of every lookup
                         if (np > 1) tl.prof_add(TL_PROBES, np - 1);

@@ -38,6 +38,38 @@ def test_atomic_rate_rejects_bad_arguments_before_any_device_call():
     assert L.ga_atomic_rate(None, 1 << 20, 1 << 10, 2, ctypes.byref(out)) == -1  # no such kind
 
 
+def test_structure_checks_reject_bad_arguments_before_any_device_call():
+    from cuda_a_star_amd import checks as ck
+    L = ga.lib()
+    keys = np.ones((4, 4), dtype=np.uint32)
+    g = np.zeros(4, dtype=np.uint32)
+    out = np.zeros(4, dtype=np.int32)
+    slot = np.zeros(4, dtype=np.uint32)
+    p = lambda a: a.ctypes.data_as(ctypes.c_void_p)
+    cs = lambda layout, log2=4, probes=8, serial=0, k=keys, n=4: L.ga_debug_closed_set(
+        0, layout, p(k), p(g), p(g), n, log2, probes, serial, p(out), p(slot), None, None, None)
+    assert cs(3) == -1  # no such layout
+    assert cs(ck.SET_U64, log2=0) == -1 and cs(ck.SET_U64, log2=25) == -1
+    assert cs(ck.SET_U64, probes=0) == -1 and cs(ck.SET_U64, serial=2) == -1
+    assert L.ga_debug_closed_set(0, 0, None, None, None, 4, 4, 8, 0, None, None, None, None, None) == -1
+    empty = np.full((4, 2), 0xFFFFFFFF, dtype=np.uint32)
+    assert cs(ck.SET_U64, k=empty) == -1  # the empty key
+    unwritten = keys.copy()
+    unwritten[2, 3] = 0x80000000
+    assert cs(ck.SET_BOARD, k=unwritten) == -1  # a board's hi never has its top bit set
+    ops = np.array([1 << 32, ck.HOP_POP], dtype=np.uint64)
+    buf = np.zeros(64, dtype=np.uint64)
+    b32 = np.zeros(64, dtype=np.uint32)
+    hp = lambda arity=4, nh=1, o=ops, opl=1, cap=8, track=0: L.ga_debug_heap(
+        0, arity, nh, p(o), len(o), opl, cap, track, p(buf), p(b32), p(buf), p(b32), p(b32))
+    assert hp(arity=16) == -1 and hp(arity=2) == -1
+    assert hp(nh=0) == -1 and hp(nh=17) == -1 and hp(arity=8, nh=9) == -1
+    assert hp(cap=0) == -1 and hp(cap=(1 << 20) + 1) == -1 and hp(opl=0) == -1 and hp(track=2) == -1
+    assert hp(o=np.array([0xFFFFFFFF << 32], dtype=np.uint64)) == -1  # f = the infinite cost
+    with pytest.raises(ga.GAStarError):
+        ck.heap(16, ops, 1, 8)
+
+
 def test_status_strings():
     for code, name in [(0, "GA_OK"), (1, "GA_NO_PATH"), (-1, "GA_E_INVALID"), (-2, "GA_E_NOMEM"),
                        (-3, "GA_E_BUDGET"), (-4, "GA_E_HIP"), (-5, "GA_E_TIMEOUT"), (-6, "GA_E_OVERFLOW")]:

@@ -1,0 +1,328 @@
+"""Kernel-level tests of the device data structures against host models,
+through the C ABI (ga_debug_closed_set, ga_debug_heap), one operation at a
+time rather than through whole searches.
+
+Closed sets (all three hashed layouts): the outcome of every insert follows
+the reference's INSERTED / UPDATED / REJECTED contract (fancy_hashmap.h:4-8,
+56-61); inserted one after another, outcomes, slots (linear probing from the
+layout's hash) and the final table equal a host model exactly; inserted all
+at once, the invariants that any order of the atomics keeps hold.  Heaps
+(arity 4 and 8, top levels in LDS for a launch): every pop returns what a
+host priority queue returns, batched pushes with gaps and refused batches
+included, and the heap left behind satisfies the W-ary heap order."""
+import heapq
+
+import numpy as np
+import pytest
+
+from cuda_a_star_amd import checks as ck
+
+pytestmark = pytest.mark.gpu
+M64 = (1 << 64) - 1
+
+
+# ---------------------------------------------------------------- host models
+def mix64(k):
+    k ^= k >> 33
+    k = (k * 0xFF51AFD7ED558CCD) & M64
+    k ^= k >> 33
+    k = (k * 0xC4CEB9FE1A85EC53) & M64
+    k ^= k >> 33
+    return k
+
+
+def hash_words(words, size):
+    h = 0x9E3779B97F4A7C15 ^ size
+    for i, w in enumerate(words):
+        h = mix64(h ^ ((int(w) << 7) & M64) ^ i)
+    return h
+
+
+def home(layout, words):
+    if layout == ck.SET_U64:
+        return mix64(int(words[0]) | int(words[1]) << 32)
+    return hash_words(words, 4 * len(words))
+
+
+def model_serial(layout, keys, g, parent, log2, max_probes):
+    """Inserts in index order; returns outcomes, slots, {slot: (key, val)}."""
+    mask = (1 << log2) - 1
+    table = {}
+    outcome, slots = [], []
+    for k, gi, pi in zip(keys, g, parent):
+        k = tuple(int(x) for x in k)
+        s = home(layout, k) & mask
+        found = None
+        if layout == ck.SET_U64:  # at most max_probes slots, wrapping
+            for p in range(max_probes):
+                t = (s + p) & mask
+                if t not in table or table[t][0] == k:
+                    found = t
+                    break
+        else:  # at most max_probes slots and never more than the table
+            probed = 0
+            while True:
+                if s not in table or table[s][0] == k:
+                    found = s
+                    break
+                probed += 1
+                if probed >= max_probes or probed > mask:
+                    break
+                s = (s + 1) & mask
+        if found is None:
+            outcome.append(ck.CS_FULL)
+            slots.append(ck.NONE)
+            continue
+        mine = int(gi) << 32 | int(pi)
+        new = found not in table
+        old = M64 if new else table[found][1]
+        table[found] = (k, min(old, mine))
+        outcome.append((ck.CS_INSERTED if new else ck.CS_UPDATED) if int(gi) < old >> 32 else ck.CS_REJECTED)
+        slots.append(found)
+    return np.array(outcome), np.array(slots, dtype=np.uint64), table
+
+
+def random_keys(layout, n, rng, distinct):
+    kw = ck.KEY_WORDS[layout]
+    base = rng.integers(0, 1 << 32, size=(distinct, kw), dtype=np.uint64).astype(np.uint32)
+    if layout == ck.SET_BOARD:
+        base[:, 3] &= 0x7FFFFFFF  # a board's hi has its top bit clear
+    return base[rng.integers(0, distinct, size=n)]
+
+
+def check_table(layout, res, table, log2):
+    for s in range(1 << log2):
+        if s in table:
+            k, v = table[s]
+            assert tuple(int(x) for x in res["key"][s]) == k, s
+            assert int(res["val"][s]) == v, s
+            if layout == ck.SET_WIDE12:
+                assert int(res["tag"][s]) == home(layout, k) >> 2  # the fingerprint tag
+        else:
+            assert int(res["tag"][s]) == M64 and int(res["val"][s]) == M64, s
+
+
+# ---------------------------------------------------------------- closed sets
+LAYOUTS = [ck.SET_U64, ck.SET_BOARD, ck.SET_WIDE12]
+LAYOUT_IDS = ["u64", "board", "wide12"]
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
+def test_closed_set_serial_equals_model(layout):
+    rng = np.random.default_rng(10 + layout)
+    n, log2 = 3000, 12
+    keys = random_keys(layout, n, rng, distinct=1200)
+    g = rng.integers(0, 40, size=n).astype(np.uint32)
+    parent = rng.integers(0, 1 << 20, size=n).astype(np.uint32)
+    res = ck.closed_set(layout, keys, g, parent, log2, serial=True)
+    out, slots, table = model_serial(layout, keys, g, parent, log2, 1 << 16)
+    assert np.array_equal(res["outcome"], out)
+    assert np.array_equal(res["slot"].astype(np.uint64), slots)
+    check_table(layout, res, table, log2)
+    # every outcome kind occurs, equal g with a smaller parent included
+    assert {ck.CS_INSERTED, ck.CS_UPDATED, ck.CS_REJECTED} <= set(out.tolist())
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
+def test_closed_set_contract_by_hand(layout):
+    kw = ck.KEY_WORDS[layout]
+    a = np.arange(1, kw + 1, dtype=np.uint32)
+    b = a + 100
+    keys = np.stack([a, a, a, b, a, a, b])
+    g = np.array([9, 12, 5, 3, 5, 5, 3], dtype=np.uint32)
+    parent = np.array([7, 1, 6, 2, 4, 9, 1], dtype=np.uint32)
+    res = ck.closed_set(layout, keys, g, parent, 6, serial=True)
+    # new; worse g; lower g; new; equal g, smaller parent (stored, not pushed); equal g, larger parent; same
+    assert res["outcome"].tolist() == [ck.CS_INSERTED, ck.CS_REJECTED, ck.CS_UPDATED, ck.CS_INSERTED,
+                                       ck.CS_REJECTED, ck.CS_REJECTED, ck.CS_REJECTED]
+    sa, sb = int(res["slot"][0]), int(res["slot"][3])
+    assert sa != sb and all(int(s) == sa for s in res["slot"][[1, 2, 4, 5]]) and int(res["slot"][6]) == sb
+    assert int(res["val"][sa]) == (5 << 32 | 4)  # the smallest (g, parent)
+    assert int(res["val"][sb]) == (3 << 32 | 1)
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
+def test_closed_set_full(layout):
+    # 8 slots, 12 distinct states: the first 8 fill the table, the rest find no slot
+    kw = ck.KEY_WORDS[layout]
+    keys = np.arange(1, 12 * kw + 1, dtype=np.uint32).reshape(12, kw)
+    g = np.full(12, 3, dtype=np.uint32)
+    parent = np.arange(12, dtype=np.uint32)
+    for max_probes in (1, 3, 64):
+        res = ck.closed_set(layout, keys, g, parent, 3, max_probes=max_probes, serial=True)
+        out, slots, table = model_serial(layout, keys, g, parent, 3, max_probes)
+        assert np.array_equal(res["outcome"], out), max_probes
+        assert np.array_equal(res["slot"].astype(np.uint64), slots), max_probes
+        check_table(layout, res, table, 3)
+        if max_probes == 64:
+            assert out.tolist() == [ck.CS_INSERTED] * 8 + [ck.CS_FULL] * 4
+        assert ck.CS_FULL in out.tolist()
+
+
+@pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
+def test_closed_set_concurrent_invariants(layout):
+    # all inserts at once: hot states met by hundreds of threads, and many cold ones
+    rng = np.random.default_rng(20 + layout)
+    n, log2 = 1 << 17, 17
+    hot = random_keys(layout, 1 << 14, rng, distinct=16)
+    cold = random_keys(layout, n - len(hot), rng, distinct=40_000)
+    keys = np.concatenate([hot, cold])
+    perm = rng.permutation(n)
+    keys = keys[perm]
+    g = rng.integers(0, 1000, size=n).astype(np.uint32)
+    parent = rng.integers(0, 1 << 30, size=n).astype(np.uint32)
+    res = ck.closed_set(layout, keys, g, parent, log2, serial=False)
+    out, slot = res["outcome"], res["slot"]
+    assert not (out == ck.CS_FULL).any()
+    groups = {}
+    for i, k in enumerate(map(tuple, keys.tolist())):
+        groups.setdefault(k, []).append(i)
+    used = set()
+    for k, idx in groups.items():
+        idx = np.array(idx)
+        s = {int(x) for x in slot[idx]}
+        assert len(s) == 1, k  # one slot per state ...
+        s = s.pop()
+        assert s not in used  # ... and one state per slot
+        used.add(s)
+        assert tuple(int(x) for x in res["key"][s]) == k
+        vals = (g[idx].astype(np.uint64) << np.uint64(32)) | parent[idx].astype(np.uint64)
+        assert int(res["val"][s]) == int(vals.min())  # the smallest (g, parent) is stored
+        o = out[idx]
+        surv = idx[(o == ck.CS_INSERTED) | (o == ck.CS_UPDATED)]
+        assert (o == ck.CS_INSERTED).sum() <= 1  # one claim per state reports it new
+        assert len(surv) >= 1 and g[surv].min() == g[idx].min()  # the best g survives
+        assert len(set(g[surv].tolist())) == len(surv)  # each survivor lowered g strictly
+        if (o == ck.CS_UPDATED).any() or (o == ck.CS_REJECTED).any():
+            assert len(idx) > 1  # found present: another insert of the state came first
+    # every other slot is still empty
+    empty = np.ones(1 << log2, dtype=bool)
+    empty[list(used)] = False
+    assert (res["tag"][empty] == np.uint64(M64)).all() and (res["val"][empty] == np.uint64(M64)).all()
+
+
+# ---------------------------------------------------------------- heaps
+def model_heap(ops, opl, cap, arity):
+    """Pops, final multiset and refused batches of one heap's operation list."""
+    k = 2 * arity
+    h, batch, popped, refused = [], [], {}, 0
+
+    def flush():
+        nonlocal batch, refused
+        recs = [r for r in batch if r is not None]
+        if recs:
+            if len(h) + len(recs) > cap:
+                refused += 1
+            else:
+                for r in recs:
+                    heapq.heappush(h, r)
+        batch = []
+
+    for i, op in enumerate(ops):
+        op = int(op)
+        if op == ck.HOP_NOP:
+            pass
+        elif op == ck.HOP_FLUSH:
+            flush()
+        elif op == ck.HOP_POP:
+            flush()
+            popped[i] = heapq.heappop(h) if h else (M64, ck.NONE)
+        else:
+            if len(batch) == k:
+                flush()
+            batch.append(None if op == ck.HOP_GAP else (op, i))
+        if (i + 1) % opl == 0 or i + 1 == len(ops):
+            flush()  # the end of a launch
+    return popped, sorted(p for p, _ in h), refused
+
+
+def random_ops(rng, n_ops, push_p, gap_p=0.1, flush_p=0.03, distinct=True, f_range=60):
+    ops = np.empty(n_ops, dtype=np.uint64)
+    used = set()
+    for i in range(n_ops):
+        u = rng.random()
+        if u < push_p:
+            while True:
+                f, g = int(rng.integers(0, f_range)), int(rng.integers(0, 1 << 20))
+                v = f << 32 | (0xFFFFFFFF ^ g)
+                if not distinct or v not in used:
+                    break
+            used.add(v)
+            ops[i] = v
+        elif u < push_p + gap_p:
+            ops[i] = ck.HOP_GAP
+        elif u < push_p + gap_p + flush_p:
+            ops[i] = ck.HOP_FLUSH
+        else:
+            ops[i] = ck.HOP_POP
+    return ops
+
+
+def check_heaps(res, ops, opl, cap, arity, exact=True):
+    for h in range(ops.shape[0]):
+        popped, rest, refused = model_heap(ops[h], opl, cap, arity)
+        for i, (p, tag) in popped.items():
+            assert int(res["popped"][h, i]) == p, (h, i)
+            if exact:
+                assert int(res["popped_tag"][h, i]) == tag, (h, i)
+            elif p != M64:
+                assert int(ops[h, int(res["popped_tag"][h, i])]) == p, (h, i)  # the entry moved whole
+        others = np.ones(ops.shape[1], dtype=bool)
+        others[list(popped)] = False
+        assert (res["popped"][h, others] == np.uint64(M64)).all()
+        assert int(res["overflow"][h]) == refused, h
+        got = res["heap"][h]
+        assert sorted(got.tolist()) == rest, h
+        for j in range(1, len(got)):  # W-ary heap order, logical parent (j - 1) / W
+            assert got[(j - 1) // arity] <= got[j], (h, j)
+
+
+@pytest.mark.parametrize("arity", [4, 8])
+@pytest.mark.parametrize("opl", [1, 7, 64, 100_000], ids=["launch1", "launch7", "launch64", "onelaunch"])
+@pytest.mark.parametrize("track", [0, 1], ids=["writeall", "dirty"])
+def test_heap_matches_priority_queue(arity, opl, track):
+    rng = np.random.default_rng(arity * 1000 + opl % 997 + 7 * track)
+    n_heaps = 64 // arity
+    # heaps that grow deep (more pushes than pops), then drain
+    grow = np.stack([random_ops(rng, 1500, 0.62) for _ in range(n_heaps)])
+    drain = np.stack([random_ops(rng, 1000, 0.25) for _ in range(n_heaps)])
+    ops = np.concatenate([grow, drain], axis=1)
+    res = ck.heap(arity, ops, opl, cap=4096, track_dirty=bool(track))
+    check_heaps(res, ops, opl, 4096, arity)
+    # the heaps did grow past the LDS top levels (1 + 3W entries)
+    grown = [model_heap(ops[h, :1500], opl, 4096, arity)[1] for h in range(n_heaps)]
+    assert max(len(x) for x in grown) > 1 + 3 * arity
+
+
+@pytest.mark.parametrize("arity", [4, 8])
+def test_heap_capacity_refuses_whole_batches(arity):
+    rng = np.random.default_rng(50 + arity)
+    n_heaps = 64 // arity
+    ops = np.stack([random_ops(rng, 1200, 0.7) for _ in range(n_heaps)])
+    res = ck.heap(arity, ops, 16, cap=40)
+    check_heaps(res, ops, 16, 40, arity)
+    assert res["overflow"].sum() > 0
+
+
+@pytest.mark.parametrize("arity", [4, 8])
+def test_heap_equal_priorities(arity):
+    # many records of equal priority (equal f and g): pops return the model's priorities,
+    # each popped entry carries its own push's priority
+    rng = np.random.default_rng(60 + arity)
+    n_heaps = 64 // arity
+    ops = np.stack([random_ops(rng, 1500, 0.55, distinct=False, f_range=3) for _ in range(n_heaps)])
+    for i in range(ops.shape[1]):
+        for h in range(n_heaps):
+            if ops[h, i] < np.uint64(ck.HOP_GAP):
+                ops[h, i] = np.uint64(int(ops[h, i]) >> 32 << 32 | 0xFFFFFFF0 | int(rng.integers(0, 4)))
+    res = ck.heap(arity, ops, 5, cap=4096)
+    check_heaps(res, ops, 5, 4096, arity, exact=False)
+
+
+def test_heap_single_heap_and_empty_pops():
+    ops = np.array([ck.HOP_POP, 5 << 32, ck.HOP_GAP, 3 << 32, ck.HOP_POP, ck.HOP_POP, ck.HOP_POP, 1 << 32],
+                   dtype=np.uint64)
+    res = ck.heap(4, ops, 3, cap=8)
+    assert [int(x) for x in res["popped"][0]] == [M64, M64, M64, M64, 3 << 32, 5 << 32, M64, M64]
+    assert res["popped_tag"][0].tolist()[4:6] == [3, 1]
+    assert res["size"].tolist() == [1] and res["heap"][0].tolist() == [1 << 32]
