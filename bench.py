@@ -24,6 +24,9 @@ Measured outside the timed region:
              (ga_solve) back to back on the one-at-a-time tuning
   CPU        the oracle (sequential A*) built with -march=native on this
              host: throughput on pinned threads, latency on one pinned core
+With --host-cells the grids stay in host memory and are copied to the device
+at admission, inside the timed region (the PCIe-inclusive rate; not the
+default).
 The step kernel's roofline comes from sampled chunks of each launch lane:
 every --time-every-th chunk is timed (its launches issued one by one between
 one pair of HIP events on the lane's stream, device counters off), and the
@@ -32,7 +35,9 @@ bytes, scaled to every launch).  The dominant kernel is the lane whose
 launches take the most kernel time.
 Its HBM traffic is this run's algorithmic bytes per launch times the
 HBM-bytes-per-algorithmic-byte ratio of the committed rocprofv3 counter runs
-(profiles/step_kernel_traffic.json).
+(profiles/step_kernel_traffic.json), and its scattered accesses (L2 misses
+per launch, by the same scaling from the committed TCC counter run) are set
+against the device's rate for scattered 64-bit loads, measured in this run.
 """
 from __future__ import annotations
 
@@ -241,6 +246,7 @@ def lane_roofline(st, cfg, elapsed_s):
             "kernel_ms_est": round(avg_ms * st.lane_launches[l], 1),
             "algo_bytes_per_launch": round(st.lane_algo_bytes[l] / st.lane_launches[l], 1),
             "achieved_GBps": round(st.lane_algo_bytes[l] / st.lane_launches[l] / (avg_ms * 1e-3) / 1e9, 2),
+            "atomics_per_launch": round(st.lane_atomics[l] / st.lane_counted_launches[l], 1),
         })
     if not lanes:
         return None
@@ -249,7 +255,7 @@ def lane_roofline(st, cfg, elapsed_s):
               1: "step_kernel<4, 4-connected grids> (lane 1: grids of at least deep_cells cells)"}
     rf = {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": round(dom["achieved_GBps"] / HBM_PEAK_GBS, 5), "traffic": None,
-          "kernel": kernel.get(dom["lane"], f"step kernel of lane {dom['lane']}"),
+          "kernel": kernel.get(dom["lane"], f"step kernel of lane {dom['lane']}"), "lane": dom["lane"],
           "avg_launch_us": dom["avg_launch_us"], "algo_bytes_per_launch": dom["algo_bytes_per_launch"],
           "event_timing": "one HIP event pair around each timed chunk of step launches (issued one by one, "
                           "device counters off), on the lane's stream; time / launches",
@@ -270,6 +276,13 @@ def lane_roofline(st, cfg, elapsed_s):
             rf["traffic"] = round(r * dom["algo_bytes_per_launch"], 1)
             rf["traffic_source"] = (f"profiles/step_kernel_traffic.json: {r:.3f} HBM bytes per algorithmic byte "
                                     f"(lane {dom['lane']}) x this run's algorithmic bytes per launch")
+        m = tj.get("l2_misses_per_algo_byte", {}).get(str(dom["lane"]))
+        if m:  # scattered lines served by the memory side per launch (L2 misses, from the TCC counter run)
+            misses = m * dom["algo_bytes_per_launch"]
+            rf["scattered"] = {"l2_misses_per_launch": round(misses, 1),
+                               "achieved_G_per_s": round(misses / (dom["avg_launch_us"] * 1e-6) / 1e9, 3),
+                               "source": f"profiles/step_kernel_traffic.json: {m * 1e3:.4f} L2 misses per algorithmic "
+                                         f"KB (lane {dom['lane']}) x this run's algorithmic bytes per launch"}
     # global atomics of the sampled launches: one CAS per closed-set probe (hashed sets), one atomicMin per lookup
     avg_all = st.step_ms_total / st.timed_launches
     rf["atomics_per_launch"] = round((st.probes + st.lookups) / st.counted_launches, 1)
@@ -300,8 +313,10 @@ def main() -> int:
     ap.add_argument("--skip-p24", action="store_true", help="skip the 24-puzzle leg")
     ap.add_argument("--p24-heaps", type=int, default=0, help="heaps per 24-puzzle (0 = the served puzzle count)")
     ap.add_argument("--override", default="", help="served tuning overrides, e.g. lanes=3,rounds=2")
+    ap.add_argument("--host-cells", action="store_true",
+                    help="pass grids in host memory (copied over PCIe at admission) instead of resident in HBM")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="only run a plain 1 GiB device copy (for scaling HBM byte counters)")
+                    help="only run a plain 1 GiB device copy and the scattered-load probe (for scaling counters)")
     args = ap.parse_args()
 
     if args.copy_calibration:
@@ -312,7 +327,13 @@ def main() -> int:
         for _ in range(5):
             dst.copy_(src)
         torch.cuda.synchronize()
-        print(json.dumps({"copy_bytes": 1 << 30, "copies": 5}), flush=True)
+        del src, dst
+        import cuda_a_star_amd as ga
+
+        with ga.Engine(0) as e:  # 2^26 scattered 64-bit loads over 1 GiB per dispatch
+            rate = e.atomic_rate(1 << 30, 1 << 26, 2)
+        print(json.dumps({"copy_bytes": 1 << 30, "copies": 5, "scattered_loads": 1 << 26,
+                          "scattered_load_G_per_s": round(rate / 1e9, 2)}), flush=True)
         return 0
 
     if args.gpus != 1:
@@ -343,9 +364,11 @@ def main() -> int:
     dev = {}
     t = time.perf_counter()
     for i, x in enumerate(insts):
-        if x.kind == "grid":
+        if x.kind == "grid" and not args.host_cells:
             dev[i] = e.upload(x.cells)
-    log(f"[bench] uploaded {len(dev)} grids in {time.perf_counter() - t:.1f}s")
+    log(f"[bench] uploaded {len(dev)} grids in {time.perf_counter() - t:.1f}s"
+        + (" (--host-cells: grids are copied from host memory at admission, inside the timed region)"
+           if args.host_cells else ""))
 
     def served_request(i):
         x = insts[i]
@@ -397,6 +420,20 @@ def main() -> int:
         t_atomics = (st.probes / peak_cas + st.lookups / peak_min) / st.counted_launches
         roofline["atomic_peak_G_per_s"] = {"fetch_min": round(peak_min / 1e9, 2), "cas": round(peak_cas / 1e9, 2)}
         roofline["atomic_frac"] = round(t_atomics / (avg_all * 1e-3), 4)
+        # scattered 64-bit loads over 1 GiB: the rate at which the memory system serves random lines
+        peak_load = e.atomic_rate(1 << 30, 1 << 26, 2)
+        roofline["scattered_load_peak_G_per_s"] = round(peak_load / 1e9, 2)
+        if "scattered" in roofline:
+            sc = roofline["scattered"]
+            sc["peak_G_per_s"] = round(peak_load / 1e9, 2)
+            sc["frac"] = round(sc["achieved_G_per_s"] * 1e9 / peak_load, 4)
+            # the launch's misses priced at the measured rates: its closed-set atomics at the atomic
+            # rate, the other misses at the scattered-load rate, as a fraction of the launch time
+            dom = next(x for x in roofline["lanes"] if x["lane"] == roofline["lane"])
+            atom = min(dom["atomics_per_launch"], sc["l2_misses_per_launch"])
+            t_priced = atom / ((peak_min + peak_cas) / 2) + (sc["l2_misses_per_launch"] - atom) / peak_load
+            sc["atomics_per_launch"] = dom["atomics_per_launch"]
+            sc["priced_frac"] = round(t_priced / (roofline["avg_launch_us"] * 1e-6), 4)
     log(f"[bench] roofline {roofline}")
 
     # ---- latency leg: the same requests at a stated offered load below saturation ----
@@ -506,6 +543,8 @@ def main() -> int:
             "seed": args.seed,
             "heaps_per_instance": served_cfg.describe(),
             "graph_replay": bool(served_cfg.use_graph) and not under_profiler,
+            "grids": "copied from host memory at admission (PCIe inside the timed region)" if args.host_cells
+                     else "resident in HBM before the timed region",
         },
         # latency at the stated offered load below saturation (the latency leg)
         "p50_ms": latency["p50_ms"] if latency else None,

@@ -128,6 +128,7 @@ class EngineStats(ctypes.Structure):
         ("lane_counted_launches", ctypes.c_uint64 * (MAX_LANES + 1)),
         ("lane_step_ms", ctypes.c_double * (MAX_LANES + 1)),
         ("lane_algo_bytes", ctypes.c_double * (MAX_LANES + 1)),
+        ("lane_atomics", ctypes.c_uint64 * (MAX_LANES + 1)),
     ]
 
 
@@ -519,7 +520,8 @@ class Engine:
 
     def atomic_rate(self, table_bytes: int = 1 << 30, ops: int = 1 << 26, kind: int = 0) -> float:
         """Random 64-bit global atomics per second on this device (kind 0:
-        fetch-min, 1: CAS from the empty key) over a table of table_bytes."""
+        fetch-min, 1: CAS from the empty key), or random 64-bit plain loads
+        (kind 2), over a table of table_bytes."""
         out = ctypes.c_double()
         rc = lib().ga_atomic_rate(self._h, table_bytes, ops, kind, ctypes.byref(out))
         if rc != OK:

@@ -367,7 +367,14 @@ class Arena {
         char* base = nullptr;
         if (hipMalloc(&base, sz) != hipSuccess) {
             (void)hipGetLastError();
-            return nullptr;
+            // Chunks are kept once made, so the device can be held by chunks with
+            // nothing in them, each too small for this request (a grown closed set
+            // is larger than one chunk).  Give those back and try once more: the
+            // only hipFree calls outside engine destruction, on this path alone.
+            if (!trim() || hipMalloc(&base, sz) != hipSuccess) {
+                (void)hipGetLastError();
+                return nullptr;
+            }
         }
         chunks_.push_back(Chunk{base, sz, {}});
         chunks_.back().free[0] = sz;
@@ -403,6 +410,20 @@ class Arena {
 
     uint64_t in_use() const { return in_use_; }
     uint64_t peak() const { return peak_; }
+
+    // Frees the chunks that hold no allocation; returns whether any was freed.
+    bool trim() {
+        bool freed = false;
+        for (size_t i = chunks_.size(); i-- > 0;) {
+            const Chunk& c = chunks_[i];
+            if (c.free.size() == 1 && c.free.begin()->first == 0 && c.free.begin()->second == c.size) {
+                (void)hipFree(c.base);
+                chunks_.erase(chunks_.begin() + (ptrdiff_t)i);
+                freed = true;
+            }
+        }
+        return freed;
+    }
 
    private:
     static constexpr uint64_t kAlign = 256;
@@ -1333,6 +1354,7 @@ struct ga_engine {
         stats.push_in += s.push_in;
         algo_sampled[j.lane] += algo_bytes(j.kind, s, true);
         algo_always[j.lane] += algo_bytes(j.kind, s, false);
+        stats.lane_atomics[j.lane] += s.probes + s.lookups;
     }
 
     // Algorithmic bytes of the step kernel: the bytes its data structures'
@@ -2000,6 +2022,8 @@ __global__ __launch_bounds__(256) void atomic_rate_kernel(uint64_t* table, uint6
         const gptr<uint64_t> p = t + (h & mask);
         if (kind == 0) {
             got[i] = __hip_atomic_fetch_min(p, h >> 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        } else if (kind == 2) {
+            got[i] = *p;  // a plain load: the random-line rate of the memory system
         } else {
             uint64_t prev = kEmptyKey;
             __hip_atomic_compare_exchange_strong(p, &prev, h >> 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
@@ -2500,7 +2524,7 @@ int ga_device_free(ga_engine* e, void* device) {
 }
 
 int ga_atomic_rate(ga_engine* e, uint64_t table_bytes, uint64_t ops, int kind, double* ops_per_sec) {
-    if (!e || !ops_per_sec || (kind != 0 && kind != 1) || table_bytes < sizeof(uint64_t) || ops == 0)
+    if (!e || !ops_per_sec || kind < 0 || kind > 2 || table_bytes < sizeof(uint64_t) || ops == 0)
         return GA_E_INVALID;
     return atomic_rate(e, table_bytes, ops, kind, ops_per_sec);
 }

@@ -124,6 +124,8 @@ def _measure_plain(insts, dev, cfg: TuneConfig) -> float:
             s = e.solve(x.request(cells_device=dev.get(i), num_queues=cfg.heaps(x.kind, x.side)))
             if s.status != ga.OK:
                 return 0.0
+            if i % 10 == 9:  # a line every ten requests: a long point is visibly alive
+                print(f"[tune] plain {i + 1}/{len(insts)} in {time.perf_counter() - t:.1f}s", flush=True)
         dt = time.perf_counter() - t
     return len(insts) / dt
 
@@ -132,9 +134,12 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
     import cuda_a_star_amd as ga
     from cuda_a_star_amd import workloads as wl
 
+    t = time.perf_counter()
     insts = wl.config5(seed, n, qps, threads=16)
     up = ga.Engine(0)
     dev = {i: up.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
+    print(f"[tune] {mode}: {n} requests generated and {len(dev)} grids uploaded in "
+          f"{time.perf_counter() - t:.1f}s", flush=True)
     measure = _measure_served if mode == "served" else _measure_plain
     best = load(mode)
     best_v = measure(insts, dev, best)

@@ -168,6 +168,8 @@ typedef struct ga_engine_stats {
     uint64_t lane_counted_launches[GA_MAX_LANES + 1]; /* step launches of counted chunks */
     double lane_step_ms[GA_MAX_LANES + 1];           /* timed chunks' time from HIP events (one pair per chunk) */
     double lane_algo_bytes[GA_MAX_LANES + 1];        /* algorithmic bytes of all the lane's launches */
+    uint64_t lane_atomics[GA_MAX_LANES + 1];         /* closed-set atomics (probes + lookups) of the lane's
+                                                        counted launches (profile) */
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {
@@ -355,9 +357,10 @@ int ga_device_free(ga_engine* e, void* device);
 
 /* Measurement helper, no counterpart in the reference: the device's rate for
  * the closed sets' operations, random 64-bit global atomics at agent scope
- * (kind 0: fetch-min, kind 1: CAS from the empty key) over a table of
- * table_bytes (rounded down to a power of two), about `ops` of them in one
- * timed launch.  Writes operations per second to *ops_per_sec. */
+ * (kind 0: fetch-min, kind 1: CAS from the empty key), or random 64-bit plain
+ * loads (kind 2: the rate at which the memory system serves scattered lines),
+ * over a table of table_bytes (rounded down to a power of two), about `ops`
+ * of them in one timed launch.  Writes operations per second to *ops_per_sec. */
 int ga_atomic_rate(ga_engine* e, uint64_t table_bytes, uint64_t ops, int kind, double* ops_per_sec);
 
 /* ---- seeded instance generators (host only; splitmix64 streams) ---- */
