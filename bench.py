@@ -247,6 +247,9 @@ def lane_roofline(st, cfg, elapsed_s):
             "algo_bytes_per_launch": round(st.lane_algo_bytes[l] / st.lane_launches[l], 1),
             "achieved_GBps": round(st.lane_algo_bytes[l] / st.lane_launches[l] / (avg_ms * 1e-3) / 1e9, 2),
             "atomics_per_launch": round(st.lane_atomics[l] / st.lane_counted_launches[l], 1),
+            # the lane's chunks on its stream (one event pair around every chunk) over the timed stream's wall time
+            "busy_frac": round(st.lane_busy_ms[l] / (elapsed_s * 1e3), 4),
+
         })
     if not lanes:
         return None
@@ -265,7 +268,10 @@ def lane_roofline(st, cfg, elapsed_s):
           "working_group_fraction": round(st.work_groups / st.heap_groups, 4) if st.heap_groups else None,
           "sift_down_hbm_fraction": round(st.sift_down_hbm / st.sift_down, 4) if st.sift_down else None,
           # all algorithmic bytes of the timed stream over its wall time (lanes run concurrently)
-          "achieved_device_GBps": round(st.algo_bytes / elapsed_s / 1e9, 2)}
+          "achieved_device_GBps": round(st.algo_bytes / elapsed_s / 1e9, 2),
+          # the run loop's host time by phase, as fractions of the timed stream's wall time
+          "host_frac": {k: round(getattr(st, f"host_{k}_ms") / (elapsed_s * 1e3), 4)
+                        for k in ("admit", "snapshot", "launch", "path")}}
     tpath = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
     if os.path.exists(tpath):  # HBM bytes per algorithmic byte from the committed rocprofv3 --pmc runs
         with open(tpath) as f:
@@ -283,10 +289,10 @@ def lane_roofline(st, cfg, elapsed_s):
                                "achieved_G_per_s": round(misses / (dom["avg_launch_us"] * 1e-6) / 1e9, 3),
                                "source": f"profiles/step_kernel_traffic.json: {m * 1e3:.4f} L2 misses per algorithmic "
                                          f"KB (lane {dom['lane']}) x this run's algorithmic bytes per launch"}
-    # global atomics of the sampled launches: one CAS per closed-set probe (hashed sets), one atomicMin per lookup
-    avg_all = st.step_ms_total / st.timed_launches
-    rf["atomics_per_launch"] = round((st.probes + st.lookups) / st.counted_launches, 1)
-    rf["atomic_rate_G_per_s"] = round((st.probes + st.lookups) / st.counted_launches / (avg_all * 1e-3) / 1e9, 3)
+    # the dominant lane's global atomics (counted launches): one CAS per closed-set probe (hashed sets),
+    # one atomicMin per lookup
+    rf["atomics_per_launch"] = dom["atomics_per_launch"]
+    rf["atomic_rate_G_per_s"] = round(dom["atomics_per_launch"] / (dom["avg_launch_us"] * 1e-6) / 1e9, 3)
     return rf
 
 
@@ -388,7 +394,9 @@ def main() -> int:
             f"({ss.solves_per_sec:.1f}/s, p50 {ss.p50_ms:.1f} ms, p99 {ss.p99_ms:.1f} ms)")
 
     # ---- timed: K steps of M requests, served as one continuous arrival stream ----
-    launches_before = e.stats().step_launches  # step dispatches of this process before the timed segment
+    st_before = e.stats()
+    launches_before = st_before.step_launches  # step dispatches of this process before the timed segment
+    lane_before = list(st_before.lane_launches)
     e.reset_stats()
     try:
         import torch
@@ -414,26 +422,22 @@ def main() -> int:
     if not args.no_profile and st.step_launches and st.timed_launches and st.counted_launches:
         roofline = lane_roofline(st, served_cfg, t_total)
         roofline["launches_before_timed"] = launches_before
+        for ln in roofline["lanes"]:  # per lane, so that a kernel trace can pick out the timed dispatches
+            ln["launches_before_timed"] = lane_before[ln["lane"]]
         peak_min = e.atomic_rate(1 << 30, 1 << 26, 0)
         peak_cas = e.atomic_rate(1 << 30, 1 << 26, 1)
-        avg_all = st.step_ms_total / st.timed_launches
-        t_atomics = (st.probes / peak_cas + st.lookups / peak_min) / st.counted_launches
         roofline["atomic_peak_G_per_s"] = {"fetch_min": round(peak_min / 1e9, 2), "cas": round(peak_cas / 1e9, 2)}
-        roofline["atomic_frac"] = round(t_atomics / (avg_all * 1e-3), 4)
+        # the dominant lane's atomics at the lower of the two random-atomic rates, per launch time
+        roofline["atomic_frac"] = round(roofline["atomic_rate_G_per_s"] * 1e9 / min(peak_min, peak_cas), 4)
         # scattered 64-bit loads over 1 GiB: the rate at which the memory system serves random lines
         peak_load = e.atomic_rate(1 << 30, 1 << 26, 2)
         roofline["scattered_load_peak_G_per_s"] = round(peak_load / 1e9, 2)
         if "scattered" in roofline:
             sc = roofline["scattered"]
             sc["peak_G_per_s"] = round(peak_load / 1e9, 2)
+            # (a lower bound on the memory side's load: some of the misses are atomics, whose rate is lower;
+            # atomics on neighbouring cells share lines, so they are not priced one by one)
             sc["frac"] = round(sc["achieved_G_per_s"] * 1e9 / peak_load, 4)
-            # the launch's misses priced at the measured rates: its closed-set atomics at the atomic
-            # rate, the other misses at the scattered-load rate, as a fraction of the launch time
-            dom = next(x for x in roofline["lanes"] if x["lane"] == roofline["lane"])
-            atom = min(dom["atomics_per_launch"], sc["l2_misses_per_launch"])
-            t_priced = atom / ((peak_min + peak_cas) / 2) + (sc["l2_misses_per_launch"] - atom) / peak_load
-            sc["atomics_per_launch"] = dom["atomics_per_launch"]
-            sc["priced_frac"] = round(t_priced / (roofline["avg_launch_us"] * 1e-6), 4)
     log(f"[bench] roofline {roofline}")
 
     # ---- latency leg: the same requests at a stated offered load below saturation ----
@@ -552,6 +556,8 @@ def main() -> int:
         "latency": latency,
         "saturation_latency": {"offered_qps": args.qps, "p50_ms": round(pct(sat_lat, 0.5), 3),
                                "p99_ms": round(pct(sat_lat, 0.99), 3), "max_inflight": ss.max_inflight,
+                               "arena_peak_GB": round(st.peak_bytes / 2**30, 1), "budget_GB": args.budget_gb,
+                               "grows": sum(s.grows for s in sols), "retries": sum(s.retries for s in sols),
                                "note": "offered load above capacity: a backlog builds over the stream"},
         "completed": ok,
         "failed": failed,

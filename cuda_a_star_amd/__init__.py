@@ -129,6 +129,11 @@ class EngineStats(ctypes.Structure):
         ("lane_step_ms", ctypes.c_double * (MAX_LANES + 1)),
         ("lane_algo_bytes", ctypes.c_double * (MAX_LANES + 1)),
         ("lane_atomics", ctypes.c_uint64 * (MAX_LANES + 1)),
+        ("lane_busy_ms", ctypes.c_double * (MAX_LANES + 1)),
+        ("host_admit_ms", ctypes.c_double),
+        ("host_snapshot_ms", ctypes.c_double),
+        ("host_launch_ms", ctypes.c_double),
+        ("host_path_ms", ctypes.c_double),
     ]
 
 

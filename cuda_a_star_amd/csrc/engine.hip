@@ -459,9 +459,9 @@ struct Pinned {
     void ensure(size_t n) {
         if (n <= bytes) return;
         if (p) (void)hipHostFree(p);
+        size_t sz = std::max<size_t>({n, bytes + bytes / 2, 4096});  // geometric: few reallocations
         p = nullptr;
         bytes = 0;
-        size_t sz = std::max<size_t>(n, 4096);
         hip_check(hipHostMalloc(&p, sz, hipHostMallocDefault), "hipHostMalloc");
         bytes = sz;
     }
@@ -522,7 +522,7 @@ struct PathBatch {
     std::vector<uint64_t> offsets;
     std::vector<InstState> snaps;
     void* dev = nullptr;   // PathJob[n] | lens[n] | out[total]
-    Pinned host;           // lens[n] | out[total]
+    Pinned* host = nullptr;  // lens[n] | out[total]; from the engine's pool
     hipEvent_t ev = nullptr;
     uint64_t total = 0;
 };
@@ -643,6 +643,7 @@ struct Lane {
     hipEvent_t ev_chunk[2] = {nullptr, nullptr};
     uint32_t chunk_reads[2] = {0, 0};
     std::vector<hipEvent_t> prof_ev[2];  // profile: 2 events per step, per chunk parity
+    hipEvent_t busy_ev[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // profile: around every chunk
     bool timed[2] = {false, false};      // profile: whether that chunk's steps were timed
     // graph mode: the captured chunks, by step-kernel instantiation (all kinds,
     // 8-puzzle, 15-puzzle, 4-connected grid), without / with device counters,
@@ -724,12 +725,24 @@ struct ga_engine {
     }
 
     ~ga_engine() {
+        if (path_stream) (void)hipStreamSynchronize(path_stream);
+        for (Lane& L : lanes)
+            if (L.stream) (void)hipStreamSynchronize(L.stream);
+        for (Deferred& df : deferred) {  // their memory goes with the arena
+            delete df.fail;
+            (void)hipEventDestroy(df.ev);
+        }
+        for (Pinned* p : pinned_pool) delete p;
+        for (hipEvent_t ev : event_pool) (void)hipEventDestroy(ev);
         for (Lane& L : lanes) {
             if (L.stream) (void)hipStreamSynchronize(L.stream);
             for (auto& kv : L.graphs)
                 if (kv.second) (void)hipGraphExecDestroy(kv.second);
             for (auto& v : L.prof_ev)
                 for (auto ev : v) (void)hipEventDestroy(ev);
+            for (auto& pair : L.busy_ev)
+                for (hipEvent_t ev : pair)
+                    if (ev) (void)hipEventDestroy(ev);
             for (int i = 0; i < 2; ++i) {
                 if (L.ev_map[i]) (void)hipEventDestroy(L.ev_map[i]);
                 if (L.ev_chunk[i]) (void)hipEventDestroy(L.ev_chunk[i]);
@@ -1102,19 +1115,26 @@ struct ga_engine {
         d.start_iter = (uint32_t)(L.host_base - st.paused_iter);
         resume_kernel<<<1, 64, 0, s>>>(d, d_descs, d_states, (uint32_t)j.slot);
         hip_check(hipGetLastError(), "grow in place");
-        uint32_t fails = 0;
-        if (failed) hip_check(hipMemcpyAsync(&fails, failed, sizeof(fails), hipMemcpyDeviceToHost, s), "read back");
-        hip_check(hipStreamSynchronize(s), "grow in place");  // the old structures are unused from here on
-        if (fails) throw GaError(GA_E_HIP, "closed-set rehash lost keys");
+        // The old structures (and the slot map) are freed once the growth's
+        // kernels have run, without the host waiting for them: a wait here held
+        // the run loop for the rest of the lane's chunk in flight.
+        Deferred df;
+        if (failed) {
+            df.fail = take_pinned(sizeof(uint32_t));
+            hip_check(hipMemcpyAsync(df.fail->p, failed, sizeof(uint32_t), hipMemcpyDeviceToHost, s), "read back");
+        }
         auto drop = [&](void* old) {
-            arena->release(old);
+            df.frees.push_back(old);
             j.allocs.erase(std::remove(j.allocs.begin(), j.allocs.end(), old), j.allocs.end());
         };
         if (heap) drop(j.desc.heaps);
         if (table) drop(j.desc.table);
-        if (map) arena->release(map);
+        if (map) df.frees.push_back(map);
         for (void* f : fresh)
             if (f != map) j.allocs.push_back(f);
+        df.ev = take_event();
+        hip_check(hipEventRecord(df.ev, s), "record");
+        deferred.push_back(df);
         j.heap_shift = g.heap_shift;
         j.table_shift = g.table_shift;
         j.bytes += np.bytes - op.bytes;
@@ -1131,6 +1151,76 @@ struct ga_engine {
         Lane& L = lanes[j.lane];
         --L.running;
         L.map_dirty = true;
+    }
+
+    // Pinned host buffers and timing-free events of the path walks, reused from
+    // batch to batch: hipHostMalloc / hipHostFree per batch had taken most of
+    // the run loop's time (a host buffer's free is slow), starving the lanes.
+    std::vector<Pinned*> pinned_pool;
+    std::vector<hipEvent_t> event_pool;
+    Pinned* take_pinned(size_t bytes) {
+        // the smallest pooled buffer that holds `bytes`, else the largest (grown to fit)
+        long fit = -1, largest = -1;
+        for (size_t i = 0; i < pinned_pool.size(); ++i) {
+            const size_t b = pinned_pool[i]->bytes;
+            if (b >= bytes && (fit < 0 || b < pinned_pool[(size_t)fit]->bytes)) fit = (long)i;
+            if (largest < 0 || b > pinned_pool[(size_t)largest]->bytes) largest = (long)i;
+        }
+        const long k = fit >= 0 ? fit : largest;
+        Pinned* p = nullptr;
+        if (k >= 0) {
+            p = pinned_pool[(size_t)k];
+            pinned_pool.erase(pinned_pool.begin() + k);
+        } else {
+            p = new Pinned();
+        }
+        p->ensure(bytes);
+        return p;
+    }
+    void give_pinned(Pinned* p) { pinned_pool.push_back(p); }
+    hipEvent_t take_event() {
+        if (!event_pool.empty()) {
+            hipEvent_t ev = event_pool.back();
+            event_pool.pop_back();
+            return ev;
+        }
+        hipEvent_t ev = nullptr;
+        hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+        return ev;
+    }
+    void give_event(hipEvent_t ev) { event_pool.push_back(ev); }
+
+    // Memory whose last readers are kernels still queued (an in-place
+    // growth's old structures): freed once the event after them completes.
+    struct Deferred {
+        hipEvent_t ev = nullptr;
+        std::vector<void*> frees;
+        Pinned* fail = nullptr;  // the rehash's lost-key count, read back with the event
+    };
+    std::vector<Deferred> deferred;
+    // Frees what has become free (wait: all of it, after the queued work).
+    // Throws when a rehash reports lost keys (never expected: a new table is
+    // four times larger than the old).
+    void reap_deferred(bool wait) {
+        bool lost = false;
+        hipError_t err = hipSuccess;
+        size_t k = 0;
+        for (size_t i = 0; i < deferred.size(); ++i) {
+            Deferred& df = deferred[i];
+            const hipError_t q = wait ? hipEventSynchronize(df.ev) : hipEventQuery(df.ev);
+            if (q == hipErrorNotReady) {
+                deferred[k++] = df;
+                continue;
+            }
+            if (q != hipSuccess) err = q;
+            else if (df.fail && *(const uint32_t*)df.fail->p) lost = true;
+            for (void* f : df.frees) arena->release(f);
+            if (df.fail) give_pinned(df.fail);
+            give_event(df.ev);
+        }
+        deferred.resize(k);
+        hip_check(err, "growth event");
+        if (lost) throw GaError(GA_E_HIP, "closed-set rehash lost keys");
     }
 
     void release(Job& j) {
@@ -1154,6 +1244,12 @@ struct ga_engine {
         bool ok = code != GA_E_HIP;
         for (Lane& L : lanes) ok = hipStreamSynchronize(L.stream) == hipSuccess && ok;
         ok = hipStreamSynchronize(path_stream) == hipSuccess && ok;
+        for (Deferred& df : deferred) {  // the streams are idle: their memory is free
+            for (void* f : df.frees) arena->release(f);
+            if (df.fail) give_pinned(df.fail);
+            give_event(df.ev);
+        }
+        deferred.clear();
         for (Job*& j : resident) {
             if (!j) continue;
             for (void* a : j->allocs) arena->release(a);
@@ -1277,6 +1373,11 @@ struct ga_engine {
             }
         }
         L.timed[b] = timed;
+        if (cfg.profile) {
+            for (hipEvent_t& ev : L.busy_ev[b])
+                if (!ev) hip_check(hipEventCreate(&ev), "hipEventCreate");
+            hip_check(hipEventRecord(L.busy_ev[b][0], L.stream), "event");
+        }
         if (cfg.use_graph && !timed) {
             const uint64_t key = ((uint64_t)variant_slot(L.variant) << 40) | ((uint64_t)(counted ? 1 : 0) << 32) | L.grid;
             hipGraphExec_t& ge = L.graphs[key];
@@ -1295,6 +1396,7 @@ struct ga_engine {
             if (timed) hip_check(hipEventRecord(pe[1], L.stream), "event");
         }
         hip_check(hipGetLastError(), "step launch");
+        if (cfg.profile) hip_check(hipEventRecord(L.busy_ev[b][1], L.stream), "event");
         L.host_base += R;
         stats.step_launches += R;
         const uint32_t li = (uint32_t)(&L - lanes.data());
@@ -1318,10 +1420,14 @@ struct ga_engine {
     // Called once the chunk's read-back event has completed: the time of a
     // timed chunk's R launches (and its one-block counter advance).
     void collect_profile(Lane& L, uint64_t chunk) {
-        if (!cfg.profile || !L.timed[chunk & 1]) return;
-        std::vector<hipEvent_t>& pe = L.prof_ev[chunk & 1];
+        if (!cfg.profile) return;
         const uint32_t li = (uint32_t)(&L - lanes.data());
         float ms = 0;
+        // every chunk: from the GPU reaching it (or the previous chunk's end) to its last launch's end
+        hip_check(hipEventElapsedTime(&ms, L.busy_ev[chunk & 1][0], L.busy_ev[chunk & 1][1]), "elapsed");
+        stats.lane_busy_ms[li] += ms;
+        if (!L.timed[chunk & 1]) return;
+        std::vector<hipEvent_t>& pe = L.prof_ev[chunk & 1];
         hip_check(hipEventElapsedTime(&ms, pe[0], pe[1]), "elapsed");
         stats.timed_launches += cfg.chunk_iters;
         stats.lane_timed_launches[li] += cfg.chunk_iters;
@@ -1495,8 +1601,10 @@ class RunLoop {
 
     // One pass; returns false once nothing is pending, running or walking.
     bool step() {
+        e->reap_deferred(false);  // memory an in-place growth no longer needs
         // -- admit: retries first, then arrivals in order
-        uint64_t now = now_ns() - t0;
+        const uint64_t t_admit = now_ns();
+        uint64_t now = t_admit - t0;
         for (std::deque<Job*>* q : {&retry, &pending}) {
             while (!q->empty()) {
                 Job* j = q->front();
@@ -1516,6 +1624,14 @@ class RunLoop {
             }
         }
         max_inflight = std::max(max_inflight, inflight);
+        uint64_t t_mark = now_ns();
+        e->stats.host_admit_ms += (double)(t_mark - t_admit) * 1e-6;
+        // host time since t_mark into `acc`, in ms
+        auto lap = [&t_mark](double& acc) {
+            const uint64_t t = now_ns();
+            acc += (double)(t - t_mark) * 1e-6;
+            t_mark = t;
+        };
         // -- retire path walks that have completed
         bool progressed = false;
         while (!batches.empty() && hipEventQuery(batches.front()->ev) == hipSuccess) {
@@ -1523,6 +1639,7 @@ class RunLoop {
             batches.erase(batches.begin());
             progressed = true;
         }
+        lap(e->stats.host_path_ms);
         // -- every lane keeps up to two chunks in flight and handles read-backs in order
         bool busy = false;
         // A lone busy lane may block on its chunk instead of polling, but only
@@ -1543,9 +1660,11 @@ class RunLoop {
                     q = hipSuccess;
                 }
                 if (q == hipSuccess) {
+                    lap(e->stats.host_launch_ms);  // (a blocking wait above counts as launch time)
                     handle_snapshot(l, c);
                     ++L.processed;
                     progressed = true;
+                    lap(e->stats.host_snapshot_ms);
                 } else if (q != hipErrorNotReady) {
                     hip_check(q, "chunk event");
                 }
@@ -1557,6 +1676,7 @@ class RunLoop {
                     progressed = true;
                 }
             }
+            lap(e->stats.host_launch_ms);
             if (L.running > 0 || L.launched > L.processed) busy = true;
         }
         if (busy) {
@@ -1566,9 +1686,13 @@ class RunLoop {
         if (!batches.empty()) {
             finalize_batch(batches.front());
             batches.erase(batches.begin());
+            lap(e->stats.host_path_ms);
             return true;
         }
-        if (pending.empty() && retry.empty()) return false;
+        if (pending.empty() && retry.empty()) {
+            e->reap_deferred(true);
+            return false;
+        }
         if (retry.empty() && timed) {
             uint64_t wait = pending.front()->arrival_ns;
             now = now_ns() - t0;
@@ -1583,7 +1707,8 @@ class RunLoop {
         (void)hipStreamSynchronize(e->path_stream);
         for (PathBatch* b : batches) {
             e->arena->release(b->dev);
-            if (b->ev) (void)hipEventDestroy(b->ev);
+            if (b->ev) e->give_event(b->ev);
+            if (b->host) e->give_pinned(b->host);
             delete b;
         }
         batches.clear();
@@ -1600,8 +1725,8 @@ class RunLoop {
     void finalize_batch(PathBatch* b) {
         hip_check(hipEventSynchronize(b->ev), "path event");
         size_t n = b->jobs.size();
-        const uint32_t* lens = (const uint32_t*)b->host.p;
-        const uint64_t* out = (const uint64_t*)((char*)b->host.p + round_up(n * sizeof(uint32_t), 16));
+        const uint32_t* lens = (const uint32_t*)b->host->p;
+        const uint64_t* out = (const uint64_t*)((char*)b->host->p + round_up(n * sizeof(uint32_t), 16));
         for (size_t i = 0; i < n; ++i) {
             Job& j = *b->jobs[i];
             uint32_t len = lens[i];
@@ -1630,7 +1755,8 @@ class RunLoop {
             --inflight;
         }
         e->arena->release(b->dev);
-        (void)hipEventDestroy(b->ev);
+        e->give_event(b->ev);
+        e->give_pinned(b->host);
         delete b;
     }
 
@@ -1741,26 +1867,25 @@ class RunLoop {
         size_t jobs_bytes = round_up(n * sizeof(PathJob), 16);
         b->dev = e->arena->alloc(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
         if (!b->dev) throw GaError(GA_E_NOMEM, "device arena exhausted (paths)");
-        b->host.ensure(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
-        PathJob* pj = (PathJob*)((char*)b->host.p + lens_bytes + total * sizeof(uint64_t));
+        b->host = e->take_pinned(jobs_bytes + lens_bytes + total * sizeof(uint64_t));
+        PathJob* pj = (PathJob*)((char*)b->host->p + lens_bytes + total * sizeof(uint64_t));
         for (size_t i = 0; i < n; ++i)
             pj[i] = PathJob{(uint32_t)done[i]->slot, snap[done[i]->slot].best + 1, b->offsets[i]};
         char* dj = (char*)b->dev;
         uint32_t* dl = (uint32_t*)(dj + jobs_bytes);
         uint64_t* dout = (uint64_t*)(dj + jobs_bytes + lens_bytes);
-        hipEvent_t ready;
-        hip_check(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "hipEventCreate");
+        hipEvent_t ready = e->take_event();
         hip_check(hipEventRecord(ready, L.stream), "record");
         hip_check(hipStreamWaitEvent(e->path_stream, ready, 0), "wait");
         hip_check(hipMemcpyAsync(dj, pj, n * sizeof(PathJob), hipMemcpyHostToDevice, e->path_stream), "upload jobs");
         path_kernel<<<(uint32_t)n, 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n, dout, dl);
         hip_check(hipGetLastError(), "path launch");
-        hip_check(hipMemcpyAsync(b->host.p, dl, lens_bytes + total * sizeof(uint64_t), hipMemcpyDeviceToHost,
+        hip_check(hipMemcpyAsync(b->host->p, dl, lens_bytes + total * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                  e->path_stream),
                   "read back paths");
-        hip_check(hipEventCreateWithFlags(&b->ev, hipEventDisableTiming), "hipEventCreate");
+        b->ev = e->take_event();
         hip_check(hipEventRecord(b->ev, e->path_stream), "record");
-        (void)hipEventDestroy(ready);
+        e->give_event(ready);  // the wait above holds the record it was made against
         b->total = total;
     }
 };

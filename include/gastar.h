@@ -170,6 +170,11 @@ typedef struct ga_engine_stats {
     double lane_algo_bytes[GA_MAX_LANES + 1];        /* algorithmic bytes of all the lane's launches */
     uint64_t lane_atomics[GA_MAX_LANES + 1];         /* closed-set atomics (probes + lookups) of the lane's
                                                         counted launches (profile) */
+    double lane_busy_ms[GA_MAX_LANES + 1];           /* time the lane's chunks occupied its stream, from one
+                                                        HIP event pair around every chunk (profile) */
+    /* host time of the run loop: admitting requests, handling chunk read-backs (finished requests, path
+       walks launched, growth), uploading launch maps and launching chunks, finishing path walks */
+    double host_admit_ms, host_snapshot_ms, host_launch_ms, host_path_ms;
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {

@@ -87,7 +87,7 @@ int main(void) {
   F(ga_request, num_queues) F(ga_request, table_slots) F(ga_request, user_tag)
   S(ga_result) F(ga_result, expanded) F(ga_result, path) F(ga_result, latency_ns) F(ga_result, user_tag)
   S(ga_config) F(ga_config, max_instances) F(ga_config, profile) F(ga_config, rounds) F(ga_config, grow_thresh) F(ga_config, rounds_grid) F(ga_config, lanes) F(ga_config, deep_cells) F(ga_config, slack_grid) F(ga_config, pdb24) F(ga_config, grow_in_place) F(ga_config, block)
-  S(ga_engine_stats) F(ga_engine_stats, step_ms_total) F(ga_engine_stats, peak_bytes) F(ga_engine_stats, timed_launches) F(ga_engine_stats, heap_groups) F(ga_engine_stats, work_groups) F(ga_engine_stats, push_in) F(ga_engine_stats, counted_launches) F(ga_engine_stats, lane_launches) F(ga_engine_stats, lane_counted_launches) F(ga_engine_stats, lane_step_ms) F(ga_engine_stats, lane_algo_bytes) F(ga_engine_stats, lane_atomics)
+  S(ga_engine_stats) F(ga_engine_stats, step_ms_total) F(ga_engine_stats, peak_bytes) F(ga_engine_stats, timed_launches) F(ga_engine_stats, heap_groups) F(ga_engine_stats, work_groups) F(ga_engine_stats, push_in) F(ga_engine_stats, counted_launches) F(ga_engine_stats, lane_launches) F(ga_engine_stats, lane_counted_launches) F(ga_engine_stats, lane_step_ms) F(ga_engine_stats, lane_algo_bytes) F(ga_engine_stats, lane_atomics) F(ga_engine_stats, lane_busy_ms) F(ga_engine_stats, host_admit_ms) F(ga_engine_stats, host_path_ms)
   S(ga_serve_stats) F(ga_serve_stats, solves_per_sec) F(ga_serve_stats, max_inflight)
   return 0;
 }
