@@ -302,7 +302,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--requests", type=int, default=512, help="requests per step (stream segment)")
-    ap.add_argument("--qps", type=float, default=1500.0, help="offered load of the timed stream (Poisson arrivals)")
+    ap.add_argument("--qps", type=float, default=4000.0,
+                    help="offered load of the timed stream (Poisson arrivals); above capacity, so that the timed "
+                         "stream measures the saturation throughput")
     ap.add_argument("--latency-load", type=float, default=0.8,
                     help="offered load of the latency leg, as a fraction of the measured saturation rate (0 = skip)")
     ap.add_argument("--seed", type=int, default=5)
@@ -317,7 +319,8 @@ def main() -> int:
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
     ap.add_argument("--skip-p24", action="store_true", help="skip the 24-puzzle leg")
-    ap.add_argument("--p24-heaps", type=int, default=0, help="heaps per 24-puzzle (0 = the served puzzle count)")
+    ap.add_argument("--p24-heaps", type=int, default=8192,
+                    help="heaps per 24-puzzle (their own launch lane; 0 = the served 15-puzzle count)")
     ap.add_argument("--override", default="", help="served tuning overrides, e.g. lanes=3,rounds=2")
     ap.add_argument("--host-cells", action="store_true",
                     help="pass grids in host memory (copied over PCIe at admission) instead of resident in HBM")

@@ -130,7 +130,7 @@ FLAGS = "--skip-batch --skip-p24 --plain-sample 0 --skip-cpu --latency-load 0"
 LANE_KERNELS = {0: "step_kernel<4, 30u", 1: "step_kernel<4, 8u"}
 LANE_NAMES = {0: "lane 0 (15-puzzles, grids below 2 M cells): `step_kernel<4, 30u, 256>`",
               1: "lane 1 (grids of at least 2 M cells): `step_kernel<4, 8u, 256>`"}
-WORKLOAD = ("the serving stream (seed 5, 1500 QPS offered): one warm-up segment of 512 requests, then 1024 timed "
+WORKLOAD = ("the serving stream (seed 5, 4000 QPS offered): one warm-up segment of 512 requests, then 1024 timed "
             "requests on two launch lanes")
 
 
