@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import ctypes
 import os
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
 import numpy as np
@@ -323,9 +323,14 @@ def board_solvable(side: int, start, goal) -> bool:
 # ---------------------------------------------------------------------------
 @dataclass
 class Solution:
+    """One request's result.  The path's states start..goal are copied out of
+    the library's result into `path_states` (a uint64 array; 24-puzzles: a
+    list of board tuples); `path` gives them as a list, made on first use (a
+    Python int per state costs far more than the copy, and a stream of large
+    grids returns millions of states)."""
     status: int
     cost: int
-    path: List[int] = field(default_factory=list)
+    path_states: object = None
     expanded: int = 0
     stored: int = 0
     iterations: int = 0
@@ -334,6 +339,15 @@ class Solution:
     latency_ns: int = 0
     user_tag: int = 0
     grows: int = 0
+
+    @property
+    def path(self) -> list:
+        p = self.path_states
+        if p is None:
+            return []
+        if isinstance(p, np.ndarray):
+            p = self.path_states = p.tolist()
+        return p
 
     @property
     def ok(self) -> bool:
@@ -347,7 +361,7 @@ def _take(res: Result, board_side: int = 0) -> Solution:
         n = board_side * board_side
         path = [tuple(raw[32 * k:32 * k + n]) for k in range(res.path_len)]
     else:
-        path = [res.path[i] for i in range(res.path_len)] if res.path_len else []
+        path = np.ctypeslib.as_array(res.path, shape=(res.path_len,)).copy() if res.path_len else None
     sol = Solution(res.status, res.cost, path, res.expanded, res.stored, res.iterations, res.retries,
                    res.search_ns, res.latency_ns, res.user_tag, res.grows)
     lib().ga_result_free(ctypes.byref(res))

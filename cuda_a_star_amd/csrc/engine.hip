@@ -204,6 +204,28 @@ __global__ void pdb_level_kernel(uint8_t* dist, uint32_t d, uint32_t* changed) {
     if (any) __hip_atomic_store(&changed[d], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Copies a grid's row-major cell weights into the tiled layout of grid_index,
+// one 4 x 4 tile (16 bytes) per thread; padding cells beyond the grid's width
+// or height become walls.
+__global__ void tile_cells_kernel(const uint8_t* src, uint8_t* dst, uint32_t w, uint32_t h, uint32_t tile_cols,
+                                  uint64_t tiles) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tiles; t += stride) {
+        const uint32_t x0 = (uint32_t)(t % tile_cols) * 4u, y0 = (uint32_t)(t / tile_cols) * 4u;
+        uint32_t row[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4; ++r) {
+            uint32_t v = 0;
+            const uint32_t y = y0 + r;
+#pragma unroll
+            for (uint32_t c = 0; c < 4; ++c)
+                if (x0 + c < w && y < h) v |= (uint32_t)src[(uint64_t)y * w + x0 + c] << (8 * c);
+            row[r] = v;
+        }
+        reinterpret_cast<uint4*>(dst)[t] = make_uint4(row[0], row[1], row[2], row[3]);
+    }
+}
+
 __global__ void put_byte_kernel(uint8_t* p, uint8_t v) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
@@ -469,6 +491,12 @@ struct Pinned {
         if (p) (void)hipHostFree(p);
     }
 };
+
+// Device bytes of a grid's cell weights: the tiled copy the kernels read and,
+// for cells in host memory, the row-major upload it is made from.
+uint64_t grid_cell_bytes(const ga_request& r) {
+    return round_up(grid_tiled_cells(r.width, r.height), 256) + (r.cells_device ? 0 : (uint64_t)r.width * r.height);
+}
 
 int kind_of(const ga_request& r) {
     if (r.kind == GA_PUZZLE) return r.side == 3 ? KIND_PUZZLE3 : r.side == 4 ? KIND_PUZZLE4 : KIND_PUZZLE5;
@@ -816,7 +844,7 @@ struct ga_engine {
         if (is_wide(j.kind))  // 32-byte board slots, start and goal
             b += p.table_slots * sizeof(Board5KeySet::PairSlot) + 2 * sizeof(Puzzle5Ops::Key);
         else
-            b += p.table_slots ? p.table_slots * sizeof(Slot) : cells * sizeof(uint64_t);
+            b += p.table_slots ? p.table_slots * sizeof(Slot) : grid_tiled_cells(r.width, r.height) * sizeof(uint64_t);
         p.bytes = b;
         return p;
     }
@@ -897,13 +925,11 @@ struct ga_engine {
             }
             if (!d.table) return false;
         } else {
-            uint64_t cells = (uint64_t)r.width * r.height;
-            d.dense = (uint64_t*)take(cells * sizeof(uint64_t));
+            d.dense = (uint64_t*)take(grid_tiled_cells(r.width, r.height) * sizeof(uint64_t));
             if (!d.dense) return false;
-            if (!r.cells_device) {
-                d.cells = (const uint8_t*)take(cells);
-                if (!d.cells) return false;
-            }
+            // the weights in the tiled layout, then (host cells) the row-major copy they are tiled from
+            d.cells = (const uint8_t*)take(grid_cell_bytes(r));
+            if (!d.cells) return false;
         }
         d.n_heaps = p.n_heaps;
         d.heap_cap = p.heap_cap;
@@ -929,7 +955,13 @@ struct ga_engine {
             d.width = r.width;
             d.height = r.height;
             d.w_min = j.w_min;
-            if (r.cells_device) d.cells = r.cells_device;
+            d.tile_cols = grid_tile_cols(r.width);
+            const uint32_t sx = (uint32_t)(r.start % r.width), sy = (uint32_t)(r.start / r.width);
+            const uint32_t gx = (uint32_t)(r.goal % r.width), gy = (uint32_t)(r.goal / r.width);
+            d.start = grid_index(sx, sy, d.tile_cols);
+            d.goal = grid_index(gx, gy, d.tile_cols);
+            d.start_xy = (sy << 16) | sx;
+            d.goal_xy = (gy << 16) | gx;
         }
         return true;
     }
@@ -950,31 +982,47 @@ struct ga_engine {
             const uint64_t words = p.table_slots * (is_wide(j.kind) ? sizeof(Board5KeySet::PairSlot) : sizeof(Slot)) / 8;
             fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>((uint64_t*)d.table, words, ~0ull);
         } else {
-            uint64_t cells = (uint64_t)r.width * r.height;
-            if (!r.cells_device)
-                hip_check(hipMemcpyAsync((void*)d.cells, r.cells, cells, hipMemcpyHostToDevice, stream), "upload cells");
-            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>(d.dense, cells, ~0ull);
+            const uint64_t tiled = grid_tiled_cells(r.width, r.height);
+            const uint8_t* src = r.cells_device;
+            if (!src) {  // host cells: uploaded row-major behind the tiled weights
+                uint8_t* stage = (uint8_t*)d.cells + round_up(tiled, 256);
+                hip_check(hipMemcpyAsync(stage, r.cells, (uint64_t)r.width * r.height, hipMemcpyHostToDevice, stream),
+                          "upload cells");
+                src = stage;
+            }
+            const uint64_t tiles = tiled / 16;
+            tile_cells_kernel<<<(uint32_t)std::min<uint64_t>(4096, (tiles + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+                src, (uint8_t*)d.cells, r.width, r.height, d.tile_cols, tiles);
+            fill_kernel<uint64_t><<<fill_grid, kBlock, 0, stream>>>(d.dense, tiled, ~0ull);
         }
         fill_kernel<uint32_t><<<std::min<uint32_t>(fill_grid, (p.n_heaps + kBlock - 1) / kBlock), kBlock, 0,
                                 stream>>>(d.heap_size, p.n_heaps, 0u);
     }
 
+    // Device bytes an admission of j takes from the budget.
+    uint64_t need_of(const Job& j) const {
+        return plan(j).bytes + (j.req->kind == GA_GRID ? grid_cell_bytes(*j.req) : 0);
+    }
+
     // Returns 1 if admitted, 0 if it must wait (no slot / budget), or throws.
-    int admit(Job& j) {
+    // `reserve`: budget bytes held back for requests that wait on other lanes.
+    int admit(Job& j, uint64_t reserve = 0) {
         const uint32_t lane = lane_of(j);
         Lane& L = lanes[lane];
         if (L.free_slots.empty()) return 0;
         const ga_request& r = *j.req;
         Plan p = plan(j);
-        uint64_t cell_bytes = (r.kind == GA_GRID && !r.cells_device) ? (uint64_t)r.width * r.height : 0;
-        uint64_t need = p.bytes + cell_bytes;
+        uint64_t need = p.bytes + (r.kind == GA_GRID ? grid_cell_bytes(r) : 0);
         if (need > cfg.budget_bytes) throw GaError(GA_E_BUDGET, "request exceeds the admission budget");
-        if (arena->in_use() + need > cfg.budget_bytes) return 0;
+        if (arena->in_use() + need + reserve > cfg.budget_bytes) return 0;
         const int W = (int)cfg.heap_arity;
         std::vector<void*> allocs;
         bool others_resident = false;
         for (Job* o : resident)
-            if (o) others_resident = true;
+            if (o) {
+                others_resident = true;
+                break;
+            }
         bool short_of_memory = false;
         auto take = [&](uint64_t bytes) -> void* {
             if (short_of_memory) return nullptr;
@@ -1558,6 +1606,7 @@ int validate(const ga_request& r) {
     if (r.kind == GA_GRID) {
         if (!r.width || !r.height || r.width > 65535 || r.height > 65535) return GA_E_INVALID;
         if ((uint64_t)r.width * r.height >= 0xFFFFFFFFull) return GA_E_INVALID;
+        if (grid_tiled_cells(r.width, r.height) >= 0xFFFFFFFFull) return GA_E_INVALID;  // tiled indices are 32-bit
         if (r.connectivity != 4 && r.connectivity != 8) return GA_E_INVALID;
         if (!r.cells && !r.cells_device) return GA_E_INVALID;
         uint64_t n = (uint64_t)r.width * r.height;
@@ -1587,6 +1636,9 @@ uint32_t grid_wmin(const ga_request& r) {
 // fails (the caller then calls discard() and ga_engine::abandon()).
 class RunLoop {
    public:
+    // requests an admission pass looks past while their lanes wait
+    static constexpr uint32_t kAdmitScan = 512;
+
     RunLoop(ga_engine* e, bool timed, uint64_t t0 = now_ns()) : e(e), timed(timed), t0(t0) {}
     RunLoop(const RunLoop&) = delete;
     RunLoop& operator=(const RunLoop&) = delete;
@@ -1602,25 +1654,48 @@ class RunLoop {
     // One pass; returns false once nothing is pending, running or walking.
     bool step() {
         e->reap_deferred(false);  // memory an in-place growth no longer needs
-        // -- admit: retries first, then arrivals in order
+        // -- admit: retries first, then arrivals in order.  Each lane admits
+        // its requests first come, first served; a request that must wait
+        // holds up only the requests of its own lane behind it (a big grid
+        // waiting for memory on its lane no longer keeps puzzles off a lane
+        // that has run dry), and while it waits the other lanes leave the
+        // budget bytes it needs free, so that it is not starved.
         const uint64_t t_admit = now_ns();
         uint64_t now = t_admit - t0;
-        for (std::deque<Job*>* q : {&retry, &pending}) {
-            while (!q->empty()) {
-                Job* j = q->front();
-                if (timed && j->arrival_ns > now) break;
-                int rc;
-                try {
-                    rc = e->admit(*j);
-                } catch (const GaError& err) {
-                    if (err.code == GA_E_HIP) throw;
-                    q->pop_front();
-                    complete(*j, err.code);
-                    continue;
+        {
+            const uint32_t all = (1u << e->n_lanes) - 1;
+            uint32_t blocked = 0;  // lanes whose first waiting request could not be admitted
+            uint64_t reserve = 0;  // bytes those requests need
+            uint32_t skipped = 0;  // requests passed over in this scan (bounded)
+            for (std::deque<Job*>* q : {&retry, &pending}) {
+                for (auto it = q->begin(); it != q->end() && blocked != all && skipped < kAdmitScan;) {
+                    Job* j = *it;
+                    if (timed && j->arrival_ns > now) break;
+                    const uint32_t lane = e->lane_of(*j);
+                    if ((blocked >> lane) & 1u) {
+                        ++it;
+                        ++skipped;
+                        continue;
+                    }
+                    int rc;
+                    try {
+                        rc = e->admit(*j, reserve);
+                    } catch (const GaError& err) {
+                        if (err.code == GA_E_HIP) throw;
+                        it = q->erase(it);
+                        complete(*j, err.code);
+                        continue;
+                    }
+                    if (!rc) {
+                        blocked |= 1u << lane;
+                        if (!e->lanes[lane].free_slots.empty()) reserve += e->need_of(*j);
+                        ++it;
+                        ++skipped;
+                        continue;
+                    }
+                    it = q->erase(it);
+                    ++inflight;
                 }
-                if (!rc) break;
-                q->pop_front();
-                ++inflight;
             }
         }
         max_inflight = std::max(max_inflight, inflight);

@@ -134,6 +134,22 @@ __host__ __device__ inline uint64_t mix64(uint64_t k) {
     return k;
 }
 
+// Tiled layout of a grid's closed set and cell weights: 4 x 4 tiles of cells,
+// the tiles in row-major order (tile_cols per row), the cells of a tile
+// row-major inside it.  A tile of closed-set values (8 B each) fills one
+// 128-byte L2 line and a line of weights (1 B each) holds a 32 x 4 strip, so
+// the four neighbours of a node lie on the node's own line or one more (with
+// rows laid out whole, the cells above and below are on lines of their own).
+// Grid states, parent links and heap slots are tiled indices; the host API
+// speaks row-major cell indices, converted at admission and in the path walk.
+__host__ __device__ inline uint32_t grid_tile_cols(uint32_t width) { return (width + 3u) >> 2; }
+__host__ __device__ inline uint64_t grid_tiled_cells(uint32_t width, uint32_t height) {
+    return (uint64_t)grid_tile_cols(width) * 16u * ((height + 3u) >> 2);
+}
+__host__ __device__ inline uint32_t grid_index(uint32_t x, uint32_t y, uint32_t tile_cols) {
+    return (((y >> 2) * tile_cols + (x >> 2)) << 4) | ((y & 3u) << 2) | (x & 3u);
+}
+
 // Read-only description of one resident instance (written once at admission).
 struct InstDesc {
     uint32_t kind;
@@ -157,7 +173,7 @@ struct InstDesc {
     uint32_t a0;           // initial frontier width (heaps in use), a power of two <= n_heaps
     uint32_t rounds;       // expansion rounds per launch for this instance (0 = the launch's default)
     uint32_t f_slack;      // expand a heap's top only if f <= previous min f + f_slack (kInf = always)
-    uint32_t pad1;
+    uint32_t tile_cols;    // grid: 4x4 tiles per row of the tiled layout (grid_index)
     uint8_t goal_rc[32];   // puzzle: row << 4 | col of each tile in the goal
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
@@ -165,6 +181,8 @@ struct InstDesc {
     uint32_t grow_heap_at; // request GROW_HEAP when a heap ends an iteration above this many entries (0 = never)
     uint32_t pad3;
     uint64_t grow_table_at;  // hashed: request GROW_TABLE when more slots than this are occupied (0 = never)
+    uint32_t start_xy;     // grid: start cell as y << 16 | x (start and goal hold tiled indices)
+    uint32_t goal_xy;      // grid: goal cell as y << 16 | x
 };
 static_assert(offsetof(InstDesc, goal_rc) % 4 == 0, "goal_rc is read as 32-bit words");
 
@@ -714,7 +732,8 @@ __device__ inline uint32_t goal_of(const InstDesc& d, uint32_t t) {
     return (word >> (8 * (t & 3))) & 255u;
 }
 
-// Direct-indexed set over grid cells: slot = cell index, val in d.dense.
+// Direct-indexed set over grid cells: slot = the cell's tiled index
+// (grid_index), val in d.dense.
 struct DenseCellSet {
     using Key = uint64_t;
     static constexpr bool kDense = true;
@@ -950,26 +969,26 @@ struct PuzzleOps : HashedU64Set {
     }
 };
 
-// 4- or 8-connected grid, state = cell index y*width + x, aux = y << 16 | x.
-// Moves in the reference order (a_star_pathfinding.h:150-182); cost = weight
-// of the entered cell (:203-208), 0 = wall; h = Manhattan (4) or Chebyshev (8)
-// distance times the minimum cell weight, both admissible (the reference's
-// plain Manhattan with unit diagonals, :219-221, is not).
+// 4- or 8-connected grid, state = the cell's tiled index (grid_index), aux =
+// y << 16 | x.  Moves in the reference order (a_star_pathfinding.h:150-182);
+// cost = weight of the entered cell (:203-208), 0 = wall; h = Manhattan (4) or
+// Chebyshev (8) distance times the minimum cell weight, both admissible (the
+// reference's plain Manhattan with unit diagonals, :219-221, is not).  The
+// closed set and the weights are both in the tiled layout.
 template <int CONN>
 struct GridOps : DenseCellSet {
     static constexpr int K = CONN;
 
     __device__ static inline uint32_t hxy(const InstDesc& d, uint32_t x, uint32_t y) {
-        const uint32_t gx = (uint32_t)(d.goal % d.width), gy = (uint32_t)(d.goal / d.width);
+        const uint32_t gx = d.goal_xy & 0xFFFFu, gy = d.goal_xy >> 16;
         const uint32_t dx = x > gx ? x - gx : gx - x;
         const uint32_t dy = y > gy ? y - gy : gy - y;
         if (CONN == 8) return (dx > dy ? dx : dy) * d.w_min;
         return (dx + dy) * d.w_min;
     }
 
-    __device__ static inline uint32_t aux_of(const InstDesc& d, uint64_t s) {
-        return ((uint32_t)(s / d.width) << 16) | (uint32_t)(s % d.width);
-    }
+    // aux of the start state (the only state whose aux is not made by succ)
+    __device__ static inline uint32_t aux_of(const InstDesc& d, uint64_t) { return d.start_xy; }
 
     __device__ static inline uint32_t heuristic(const InstDesc& d, uint64_t s, uint32_t aux) {
         return hxy(d, aux & 0xFFFFu, aux >> 16);
@@ -980,14 +999,14 @@ struct GridOps : DenseCellSet {
     __device__ static inline uint32_t prefetch(const InstDesc& d, const HeapEntry& e, int i) {
         uint32_t nx, ny;
         if (!neighbour(d, e.aux, i, nx, ny)) return 0;
-        return as_global(d.cells)[(uint64_t)ny * d.width + nx];
+        return as_global(d.cells)[grid_index(nx, ny, d.tile_cols)];
     }
 
     // Stored g of the cell that move i enters (kInf off the grid), read with its weight.
     __device__ static inline uint32_t prefetch_g(const InstDesc& d, const HeapEntry& e, int i) {
         uint32_t nx, ny;
         if (!neighbour(d, e.aux, i, nx, ny)) return kInf;
-        return g_of(as_global(d.dense) + (uint64_t)ny * d.width + nx);
+        return g_of(as_global(d.dense) + grid_index(nx, ny, d.tile_cols));
     }
 
     __device__ static inline bool neighbour(const InstDesc& d, uint32_t aux, int i, uint32_t& nx, uint32_t& ny) {
@@ -1005,7 +1024,7 @@ struct GridOps : DenseCellSet {
     __device__ static inline uint32_t drop_parent(const InstDesc& d, uint32_t aux, int i, uint32_t w, uint32_t par) {
         uint32_t nx, ny;
         if (!neighbour(d, aux, i, nx, ny)) return w;
-        return (uint64_t)ny * d.width + nx == par ? 0u : w;
+        return grid_index(nx, ny, d.tile_cols) == par ? 0u : w;
     }
 
     // `w` = prefetch(d, aux, i)
@@ -1013,8 +1032,7 @@ struct GridOps : DenseCellSet {
                                        uint64_t& t, uint32_t& taux, uint32_t& cost, uint32_t& th) {
         uint32_t nx, ny;
         if (w == 0 || !neighbour(d, aux, i, nx, ny)) return false;
-        const uint64_t c = (uint64_t)ny * d.width + nx;
-        t = c;
+        t = grid_index(nx, ny, d.tile_cols);
         taux = (ny << 16) | nx;
         cost = w;
         th = hxy(d, nx, ny);
@@ -1873,19 +1891,19 @@ __device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, 
 // the kPathWin x kPathWin cells around the current cell in one round trip and
 // follows the links inside that window; where the walk leaves it, the window
 // is reloaded around the new cell.  One dependent memory access per path
-// step becomes one per kPathWin / 2 steps or more.  Same output and result as
-// walk_path: cells goal..start (written by lane 0), their number, or kNone
-// past cap.
+// step becomes one per kPathWin / 2 steps or more.  Same result as walk_path,
+// with the cells goal..start written (by lane 0) as row-major indices
+// y * width + x, the form the host API uses; their number, or kNone past cap.
 constexpr uint32_t kPathWin = 32;
 
 __device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint32_t cap, uint64_t* win) {
     constexpr int kPer = (int)(kPathWin * kPathWin / 64);  // window words per lane
     const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t W = d.width, H = d.height;
+    const uint32_t W = d.width, H = d.height, tc = d.tile_cols;
     const uint32_t ww = W < kPathWin ? W : kPathWin, wh = H < kPathWin ? H : kPathWin;
     const gptr<const uint64_t> val = as_global<const uint64_t>(d.dense);
-    uint32_t slot = (uint32_t)d.goal;
-    uint32_t x = slot % W, y = slot / W;
+    uint32_t slot = (uint32_t)d.goal;  // tiled index
+    uint32_t x = d.goal_xy & 0xFFFFu, y = d.goal_xy >> 16;
     uint32_t n = 0;
     while (slot != kNone) {
         // the window: ww x wh cells inside the grid, the current cell near its
@@ -1899,7 +1917,7 @@ __device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint
         for (int r = 0; r < kPer; ++r) {  // all loads in flight together; positions outside read the corner
             const uint32_t k = lane + 64u * (uint32_t)r, wx = k % kPathWin, wy = k / kPathWin;
             const bool in = wx < ww && wy < wh;
-            v[r] = val[(size_t)(oy + (in ? wy : 0u)) * W + ox + (in ? wx : 0u)];
+            v[r] = val[grid_index(ox + (in ? wx : 0u), oy + (in ? wy : 0u), tc)];
         }
         __syncthreads();  // every lane is done with the previous window
 #pragma unroll
@@ -1908,34 +1926,34 @@ __device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint
         // the links inside the window; every lane follows the same chain
         while (slot != kNone && x - ox < ww && y - oy < wh) {
             if (n >= cap) return kNone;
-            if (lane == 0) out[n] = slot;
+            if (lane == 0) out[n] = (uint64_t)y * W + x;
             ++n;
             const uint32_t p = val_parent(win[(y - oy) * kPathWin + (x - ox)]);
             if (p == kNone) {
                 slot = kNone;
                 break;
             }
-            // x, y of the parent from the index difference: the eight
-            // neighbour offsets are distinct once W >= 3
-            const int64_t dd = (int64_t)p - (int64_t)slot, w = (int64_t)W;
-            int dx = 2, dy = 0;  // dx = 2: not decoded
-            if (W >= 3) {
-                if (dd == 1) dx = 1;
-                else if (dd == -1) dx = -1;
-                else if (dd == w) dx = 0, dy = 1;
-                else if (dd == -w) dx = 0, dy = -1;
-                else if (dd == w + 1) dx = 1, dy = 1;
-                else if (dd == w - 1) dx = -1, dy = 1;
-                else if (dd == 1 - w) dx = 1, dy = -1;
-                else if (dd == -1 - w) dx = -1, dy = -1;
+            // x, y of the parent: the neighbour (in the move order of
+            // GridOps::neighbour) whose tiled index it is, or, for a link to a
+            // cell that is not a neighbour, decoded from the index (a cell
+            // outside the grid then ends the walk below)
+            uint32_t px = kNone, py = 0;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const uint32_t cx = x + ((0x00221102u >> (4 * i)) & 15u) - 1u;
+                const uint32_t cy = y + ((0x02020211u >> (4 * i)) & 15u) - 1u;
+                if (px == kNone && cx < W && cy < H && grid_index(cx, cy, tc) == p) {
+                    px = cx;
+                    py = cy;
+                }
             }
-            if (dx == 2) {
-                x = p % W;
-                y = p / W;
-            } else {
-                x = (uint32_t)((int)x + dx);
-                y = (uint32_t)((int)y + dy);
+            if (px == kNone) {
+                const uint32_t t = p >> 4;
+                px = (t % tc) * 4u + (p & 3u);
+                py = (t / tc) * 4u + ((p >> 2) & 3u);
             }
+            x = px;
+            y = py;
             slot = p;
         }
         // every window holds the cell it was loaded around, so each pass takes

@@ -211,6 +211,28 @@ def test_serve_tight_budget_admits_in_turn():
         assert sol.status == ga.OK and sol.cost == oracle_cost(x)
 
 
+def test_serve_waiting_lane_does_not_hold_up_other_lanes():
+    """Admission is first come, first served per launch lane: a grid that
+    waits for its lane's only slot does not keep the puzzles behind it off the
+    other lane; within a lane the order of arrival holds."""
+    grids = [wl.grid(256, 0.2, 300 + i, num_queues=256) for i in range(2)]
+    puzzles = [wl.puzzle15(wl.instance_seed(31, i), depth=30, num_queues=256, table_slots=1 << 16) for i in range(4)]
+    insts = grids + puzzles  # all arrive at once, the grids first
+    # lanes=2 plus the 24-puzzle lane, one slot each; grids of >= 2^16 cells take lane 1
+    with ga.Engine(0, lanes=2, deep_cells=1 << 16, max_instances=3, max_iterations=2_000_000) as e:
+        sols, ss = e.serve([x.request() for x in insts], [0] * len(insts))
+    assert ss.completed == len(insts)
+    for x, sol in zip(insts, sols):
+        assert sol.status == ga.OK and sol.cost == oracle_cost(x)
+        assert replay_ok(x, sol)
+    admitted = [s.latency_ns - s.search_ns for s in sols]  # arrival 0: admission time since the start
+    # the second grid waits for the first to leave lane 1; the first puzzle does not wait for it
+    assert admitted[1] >= admitted[0] + sols[0].search_ns
+    assert admitted[2] < admitted[1]
+    # lane 0 admits its puzzles one at a time, in arrival order
+    assert admitted[2] <= admitted[3] <= admitted[4] <= admitted[5]
+
+
 def test_serve_request_over_budget_is_rejected():
     x = wl.grid(512, 0.2, 1, num_queues=256)
     with ga.Engine(0, budget_bytes=1 << 20, chunk_bytes=1 << 20) as e:
