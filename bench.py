@@ -19,6 +19,8 @@ Measured outside the timed region:
              --latency-load x the measured saturation rate (below capacity):
              p50/p99 of arrival -> result on the host at that offered load
   batched    the 1024 mixed instances of the batch configuration, one call
+             (median of --batch-reps calls), on the launch lanes and in the
+             one-launch form (ga_config.persist)
   24-puzzle  256 random-walk 5x5 boards, one call
   one at a time  --plain-sample requests of the timed stream, plain GA*
              (ga_solve) back to back on the one-at-a-time tuning
@@ -318,6 +320,7 @@ def main() -> int:
                     help="sample every n-th chunk: HIP events and device counters in its step launches")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true", help="skip the batched (1024-instance) leg")
+    ap.add_argument("--batch-reps", type=int, default=5, help="timed calls per batched form (the median is reported)")
     ap.add_argument("--skip-p24", action="store_true", help="skip the 24-puzzle leg")
     ap.add_argument("--p24-heaps", type=int, default=8192,
                     help="heaps per 24-puzzle (their own launch lane; 0 = the served 15-puzzle count)")
@@ -465,23 +468,48 @@ def main() -> int:
         log(f"[bench] latency leg at {lam:.0f} QPS offered: {latency}")
 
     # ---- batched: the 1024 mixed instances of the batch configuration in one ga_solve_batch ----
+    # twice over: on the launch lanes (one launch per iteration, chunks replayed from graphs) and in the
+    # one-launch form (every iteration of an instance inside one launch, one workgroup per instance);
+    # each the median of --batch-reps calls
     batch = None
     if not args.skip_batch:
         b_insts = wl.config4(4, 1024, threads=16)
-        b_dev = {i: e.upload(x.cells) for i, x in enumerate(b_insts) if x.kind == "grid"}
-        b_reqs = [x.request(cells_device=b_dev.get(i), num_queues=served_cfg.heaps(x.kind, x.side))
-                  for i, x in enumerate(b_insts)]
-        e.solve_batch(b_reqs[:64])  # warm-up
-        t = time.perf_counter()
-        b_sols = e.solve_batch(b_reqs)
-        b_t = time.perf_counter() - t
-        b_ok = sum(1 for s_ in b_sols if s_.status == ga.OK)
-        batch = {"solves_per_sec": round(b_ok / b_t, 3), "instances": len(b_reqs), "ok": b_ok, "seconds": round(b_t, 3),
-                 "workload": "1024 mixed: even = 15-puzzle walk depth 20..60, odd = grid side {256,512,1024,2048}, "
-                             "walls 10..25%"}
-        for a in b_dev.values():
-            e.free_device(a)
-        log(f"[bench] batched: {b_ok}/{len(b_reqs)} in {b_t:.2f}s = {b_ok / b_t:.1f} solves/s")
+        b_work = ("1024 mixed: even = 15-puzzle walk depth 20..60, odd = grid side {256,512,1024,2048}, "
+                  "walls 10..25%")
+
+        def batch_leg(eng, label):
+            dev = {i: eng.upload(x.cells) for i, x in enumerate(b_insts) if x.kind == "grid"}
+            reqs = [x.request(cells_device=dev.get(i), num_queues=served_cfg.heaps(x.kind, x.side))
+                    for i, x in enumerate(b_insts)]
+            eng.solve_batch(reqs[:64])  # warm-up
+            eng.reset_stats()
+            times, sols = [], None
+            for _ in range(max(1, args.batch_reps)):
+                t = time.perf_counter()
+                sols = eng.solve_batch(reqs)
+                times.append(time.perf_counter() - t)
+            for a in dev.values():
+                eng.free_device(a)
+            times.sort()
+            b_t = times[len(times) // 2]
+            ok = sum(1 for s_ in sols if s_.status == ga.OK)
+            log(f"[bench] batched ({label}): {ok}/{len(reqs)} in {b_t:.3f}s (median of {len(times)}) = "
+                f"{ok / b_t:.1f} solves/s; all: {[round(x, 3) for x in times]}")
+            return {"solves_per_sec": round(ok / b_t, 3), "instances": len(reqs), "ok": ok,
+                    "seconds": round(b_t, 3), "runs_s": [round(x, 3) for x in times]}, sols
+
+        batch, lane_sols = batch_leg(e, "launch lanes")
+        batch["workload"] = b_work
+        with ga.Engine(0, budget_bytes=100 << 30, chunk_bytes=8 << 30, max_iterations=MAX_ITERATIONS, persist=1,
+                       persist_block=1024, persist_cells=2048 * 2048, **served_cfg.engine) as e1:
+            one, one_sols = batch_leg(e1, "one launch")
+            one["persist_instances_per_call"] = e1.stats().persist_instances // max(1, args.batch_reps)
+        one["form"] = ("ga_config.persist: every puzzle and grid runs all of its iterations inside one launch, "
+                       "one 1024-thread workgroup (256 heaps) per instance; those that outgrow their structures "
+                       "restart on the launch lanes")
+        one["cost_mismatches_vs_lanes"] = sum(1 for a_, b_ in zip(lane_sols, one_sols)
+                                              if a_.status != b_.status or a_.cost != b_.cost)
+        batch["one_launch"] = one
 
     cores = host_cores(args.cpu_threads or 16)
     oracle_build = "portable"

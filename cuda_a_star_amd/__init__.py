@@ -96,6 +96,9 @@ class Config(ctypes.Structure):
         ("grow_in_place", ctypes.c_uint32),
         ("block", ctypes.c_uint32),
         ("step_flags", ctypes.c_uint32),
+        ("persist", ctypes.c_uint32),
+        ("persist_block", ctypes.c_uint32),
+        ("persist_cells", ctypes.c_uint64),
     ]
 
 
@@ -134,6 +137,8 @@ class EngineStats(ctypes.Structure):
         ("host_snapshot_ms", ctypes.c_double),
         ("host_launch_ms", ctypes.c_double),
         ("host_path_ms", ctypes.c_double),
+        ("persist_launches", ctypes.c_uint64),
+        ("persist_instances", ctypes.c_uint64),
     ]
 
 

@@ -120,16 +120,91 @@ __global__ __launch_bounds__(B) void step_kernel_wide(const WgEntry* __restrict_
     tally_end(sh.tally, states + inst, it, d.grow_table_at);
 }
 
+__device__ inline void seed_one(const InstDesc& d, InstState* st, int W) {
+    switch (d.kind) {
+        case KIND_PUZZLE3: seed_instance<PuzzleOps<3>>(d, st, W); break;
+        case KIND_PUZZLE4: seed_instance<PuzzleOps<4>>(d, st, W); break;
+        case KIND_GRID4: seed_instance<GridOps<4>>(d, st, W); break;
+        case KIND_GRID8: seed_instance<GridOps<8>>(d, st, W); break;
+        case KIND_PUZZLE5: seed_instance<Puzzle5Ops>(d, st, W); break;
+        default: break;
+    }
+}
+
 __global__ void seed_kernel(InstDesc d, InstDesc* descs, InstState* states, uint32_t slot, int W) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     descs[slot] = d;
-    switch (d.kind) {
-        case KIND_PUZZLE3: seed_instance<PuzzleOps<3>>(d, states + slot, W); break;
-        case KIND_PUZZLE4: seed_instance<PuzzleOps<4>>(d, states + slot, W); break;
-        case KIND_GRID4: seed_instance<GridOps<4>>(d, states + slot, W); break;
-        case KIND_GRID8: seed_instance<GridOps<8>>(d, states + slot, W); break;
-        case KIND_PUZZLE5: seed_instance<Puzzle5Ops>(d, states + slot, W); break;
-        default: break;
+    seed_one(d, states + slot, W);
+}
+
+// Seeds instances 0..n-1 of a one-launch batch (their descriptors already in
+// `descs`), one thread each.
+__global__ void seed_batch_kernel(const InstDesc* descs, InstState* states, uint32_t n, int W) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) seed_one(descs[i], states + i, W);
+}
+
+// Batched GA* with every iteration of an instance inside one launch
+// (ga_config.persist).  Workgroup b runs instance order[b] from its seed to
+// its end; the instance's heaps are the workgroup's B / W heap groups, so the
+// exchange of successor records between its heaps (the candidate buffer) and
+// the min-f termination test cross only waves of this workgroup, and an
+// iteration needs a workgroup barrier where the launch lanes need a kernel
+// boundary and a host step per chunk.  Nothing crosses workgroups, so no
+// workgroup waits for another to be resident: those beyond what the device
+// holds at once start as earlier ones finish (the host orders the instances
+// largest first).  Global stores and atomics a wave issues before the
+// barrier are seen by the other waves' loads after it: the waves of a
+// workgroup share one CU, whose vector memory pipeline keeps them in order.
+// Grow requests are off (the host cannot step in): an instance that fills
+// its heaps or closed set ends in an overflow status and is restarted on the
+// launch lanes.  `max_iters` bounds every loop (ST_ABORTED).
+template <int W, int B>
+__global__ __launch_bounds__(B) void persist_kernel(const InstDesc* __restrict__ descs, InstState* states,
+                                                    const uint32_t* __restrict__ order, uint32_t max_iters,
+                                                    StepParams prm) {
+    static_assert((B / W) * HeapRef<W>::kTop <= StepSharedT<B>::kHeapTopEntries && W >= 4, "LDS heap tops");
+    __shared__ StepSharedT<B> sh;
+    const uint32_t inst = order[blockIdx.x];
+    const InstDesc& d = descs[inst];
+    InstState* st = states + inst;
+    const uint32_t q = threadIdx.x / W;
+    const bool active = q < d.n_heaps;
+    const Tally tl{sh.tally, false};
+    for (uint32_t it = 0;; ++it) {
+        // Every thread reads the same status: the barrier that ended the
+        // previous iteration orders it after every write to it by this
+        // workgroup, and no other workgroup writes it.
+        const uint32_t status = load_relaxed(&st->status);
+        if (status != ST_RUNNING) break;
+        if (it >= max_iters) {
+            if (threadIdx.x == 0) set_status(st, ST_ABORTED);
+            break;
+        }
+        step_begin(sh);
+        switch (d.kind) {
+            case KIND_PUZZLE3:
+                run_group<W, PuzzleOps<3>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
+                                           sh.dirty);
+                break;
+            case KIND_PUZZLE4:
+                run_group<W, PuzzleOps<4>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
+                                           sh.dirty);
+                break;
+            case KIND_GRID4:
+                run_group<W, GridOps<4>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
+                                         sh.dirty);
+                break;
+            case KIND_GRID8:
+                run_group<W, GridOps<8>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
+                                         sh.dirty);
+                break;
+            default:
+                if (threadIdx.x == 0) set_status(st, ST_ABORTED);
+                break;
+        }
+        tally_end(sh.tally, st, it, 0);
+        __syncthreads();  // the last wave's flush of the tallies, and every heap and record write
     }
 }
 
@@ -685,6 +760,15 @@ struct ga_engine {
     int device = 0;
     ga_config cfg{};
     hipStream_t path_stream = nullptr;
+    // The one-launch batch form's stream, created at its first use: an engine
+    // that never uses the form keeps one stream per launch lane and the path
+    // stream only (HIP gives a process a few hardware queues; streams beyond
+    // them share queues, and lanes that share one run one after the other).
+    hipStream_t persist_stream = nullptr;
+    hipStream_t ensure_persist_stream() {
+        if (!persist_stream) hip_check(hipStreamCreateWithFlags(&persist_stream, hipStreamNonBlocking), "hipStreamCreate");
+        return persist_stream;
+    }
     Arena* arena = nullptr;
     InstDesc* d_descs = nullptr;
     InstState* d_states = nullptr;
@@ -712,6 +796,9 @@ struct ga_engine {
         if (cfg.block == 0) cfg.block = kStepBlock;
         if (cfg.block != 128 && cfg.block != 256 && cfg.block != 512)
             throw GaError(GA_E_INVALID, "block must be 128, 256 or 512");
+        if (cfg.persist_block == 0) cfg.persist_block = 1024;
+        if (cfg.persist_block != 256 && cfg.persist_block != 512 && cfg.persist_block != 1024)
+            throw GaError(GA_E_INVALID, "persist_block must be 256, 512 or 1024");
         if (cfg.lanes > GA_MAX_LANES) throw GaError(GA_E_INVALID, "lanes must be at most 4");
         if (cfg.max_instances == 0) cfg.max_instances = 1;
         if (cfg.chunk_iters == 0) cfg.chunk_iters = 1;
@@ -785,6 +872,10 @@ struct ga_engine {
         if (d_pdb) (void)hipFree(d_pdb);
         delete arena;
         if (path_stream) (void)hipStreamDestroy(path_stream);
+        if (persist_stream) {
+            (void)hipStreamSynchronize(persist_stream);
+            (void)hipStreamDestroy(persist_stream);
+        }
     }
 
     // Lane of a request: 24-puzzles take the wide-key lane; puzzles and small
@@ -812,11 +903,13 @@ struct ga_engine {
         uint64_t bytes;
     };
 
-    Plan plan(const Job& j) const {
+    // `heaps` (nonzero): that many heaps, whatever the request asks for, each
+    // holding at least `cap_min` entries (the one-launch batch form).
+    Plan plan(const Job& j, uint32_t heaps = 0, uint32_t cap_min = 0) const {
         const ga_request& r = *j.req;
         const uint32_t W = cfg.heap_arity;
         Plan p{};
-        uint64_t n_req = r.num_queues;
+        uint64_t n_req = heaps ? heaps : r.num_queues;
         uint64_t cells = (uint64_t)r.width * r.height;
         if (j.kind == KIND_PUZZLE3) {
             p.n_heaps = (uint32_t)pow2_floor(n_req ? n_req : 256);
@@ -833,6 +926,7 @@ struct ga_engine {
             p.table_slots = 0;
         }
         p.n_heaps = std::min<uint32_t>(std::max<uint32_t>(p.n_heaps, 1), 1u << 22);
+        p.heap_cap = std::max(p.heap_cap, cap_min);
         uint64_t cap = (uint64_t)p.heap_cap << j.heap_shift;
         p.heap_cap = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(cap, 16), 1u << 28);
         p.heap_stride = (uint32_t)round_up(p.heap_cap + W - 1, W);
@@ -1292,6 +1386,7 @@ struct ga_engine {
         bool ok = code != GA_E_HIP;
         for (Lane& L : lanes) ok = hipStreamSynchronize(L.stream) == hipSuccess && ok;
         ok = hipStreamSynchronize(path_stream) == hipSuccess && ok;
+        if (persist_stream) ok = hipStreamSynchronize(persist_stream) == hipSuccess && ok;
         for (Deferred& df : deferred) {  // the streams are idle: their memory is free
             for (void* f : df.frees) arena->release(f);
             if (df.fail) give_pinned(df.fail);
@@ -1627,6 +1722,296 @@ uint32_t grid_wmin(const ga_request& r) {
     return m == 256 ? 1 : m;
 }
 
+// ---------------------------------------------------------------------------
+// the one-launch batch form (ga_config.persist)
+// ---------------------------------------------------------------------------
+constexpr uint32_t kPersistMaxIters = 1u << 22;
+
+bool persist_eligible(const ga_engine* e, const Job& j) {
+    if (j.kind == KIND_PUZZLE3 || j.kind == KIND_PUZZLE4) return true;
+    if (j.kind == KIND_GRID4 || j.kind == KIND_GRID8)
+        return (uint64_t)j.req->width * j.req->height <= e->cfg.persist_cells;
+    return false;
+}
+
+// Expected work, for ordering the workgroups of a wave largest first: grids
+// by their cells, ahead of puzzles; puzzles by the Manhattan distance of their
+// start (a sliding-puzzle search grows with its depth).
+double persist_work(const Job& j) {
+    const ga_request& r = *j.req;
+    if (r.kind == GA_GRID) return 1e6 + (double)r.width * r.height;
+    const uint32_t n = r.side * r.side;
+    uint32_t pos[16] = {};
+    for (uint32_t c = 0; c < n; ++c) pos[(r.goal >> (4 * c)) & 15u] = c;
+    double h = 0;
+    for (uint32_t c = 0; c < n; ++c) {
+        const uint32_t t = (uint32_t)(r.start >> (4 * c)) & 15u;
+        if (t) h += std::abs((int)(c / r.side) - (int)(pos[t] / r.side)) + std::abs((int)(c % r.side) - (int)(pos[t] % r.side));
+    }
+    return h;
+}
+
+template <int W>
+void launch_persist_w(uint32_t block, hipStream_t s, uint32_t n, const InstDesc* descs, InstState* states,
+                      const uint32_t* order, uint32_t max_iters, const StepParams& prm) {
+    switch (block) {
+        case 256: persist_kernel<W, 256><<<n, 256, 0, s>>>(descs, states, order, max_iters, prm); break;
+        case 512: persist_kernel<W, 512><<<n, 512, 0, s>>>(descs, states, order, max_iters, prm); break;
+        default: persist_kernel<W, 1024><<<n, 1024, 0, s>>>(descs, states, order, max_iters, prm); break;
+    }
+}
+
+// The one-launch form's waves, polled by the run loop (so that the launch
+// lanes run beside them, and a lane waiting for memory a wave holds keeps
+// polling until the wave has ended and freed it).  A wave takes as many of
+// the remaining instances as the budget holds, largest first, carves and
+// clears their structures, seeds them with one launch and runs them to their
+// ends with one persist_kernel launch on the engine's persist stream; once
+// its states are back, the paths of those that reached the goal are walked
+// with one path_kernel launch.  Instances that outgrew their heaps or closed
+// set go back to the run loop to restart on the lanes with larger structures.
+class PersistRun {
+   public:
+    PersistRun(ga_engine* e, std::vector<Job*> jobs, uint64_t t0) : e(e), jobs(std::move(jobs)), t0(t0) {
+        e->ensure_persist_stream();
+        std::stable_sort(this->jobs.begin(), this->jobs.end(),
+                         [](Job* a, Job* b) { return persist_work(*a) > persist_work(*b); });
+    }
+    PersistRun(const PersistRun&) = delete;
+    PersistRun& operator=(const PersistRun&) = delete;
+    ~PersistRun() { drop(); }
+
+    bool idle() const { return stage == IDLE && next == jobs.size(); }
+
+    // One pass: moves a wave whose work has completed on to its next stage, or
+    // launches the next wave.  Finished jobs are appended to `finished`, jobs
+    // to restart on the lanes to `restart`.  Returns whether anything moved.
+    bool poll(std::vector<Job*>& finished, std::vector<Job*>& restart) {
+        if (stage != IDLE) {
+            const hipError_t q = hipEventQuery(ev);
+            if (q == hipErrorNotReady) return false;
+            hip_check(q, "one-launch batch");
+            if (stage == SEARCH) {
+                outcomes(finished, restart);
+            } else {
+                paths(finished);
+            }
+            return true;
+        }
+        if (next < jobs.size()) {
+            launch(finished);
+            return true;
+        }
+        return false;
+    }
+
+    // After a failure: waits for the stream and frees the wave.
+    void drop() {
+        if (stage != IDLE || !wave.empty()) (void)hipStreamSynchronize(e->persist_stream);
+        for (Job* j : wave) {
+            for (void* a : j->allocs) e->arena->release(a);
+            j->allocs.clear();
+        }
+        wave.clear();
+        for (void* m : meta) e->arena->release(m);
+        meta.clear();
+        if (host) e->give_pinned(host);
+        if (path_host) e->give_pinned(path_host);
+        host = path_host = nullptr;
+        if (ev) e->give_event(ev);
+        ev = nullptr;
+        stage = IDLE;
+    }
+
+   private:
+    enum Stage { IDLE, SEARCH, PATHS };
+    ga_engine* const e;
+    std::vector<Job*> jobs;
+    const uint64_t t0;
+    size_t next = 0;
+    Stage stage = IDLE;
+    std::vector<Job*> wave;
+    std::vector<void*> meta;  // the wave's own device buffers
+    Pinned* host = nullptr;   // descriptors and order up, states back
+    Pinned* path_host = nullptr;
+    hipEvent_t ev = nullptr;
+    InstDesc* d_descs = nullptr;
+    InstState* snap = nullptr;
+    std::vector<uint32_t> done;  // wave indices that reached the goal, and their path offsets
+    std::vector<uint64_t> offs;
+    uint64_t path_lens_b = 0;
+
+    void launch(std::vector<Job*>& finished) {
+        const uint32_t W = e->cfg.heap_arity, B = e->cfg.persist_block, H = B / W;
+        hipStream_t s = e->persist_stream;
+        // heap entries per heap: a workgroup's few heaps hold the whole open list
+        auto cap_min = [](const Job& j) { return (j.kind == KIND_GRID4 || j.kind == KIND_GRID8) ? 1024u : 4096u; };
+        std::vector<InstDesc> descs;
+        for (; next < jobs.size(); ++next) {
+            Job& j = *jobs[next];
+            const ga_engine::Plan p = e->plan(j, H, cap_min(j));
+            const uint64_t need = p.bytes + (j.req->kind == GA_GRID ? grid_cell_bytes(*j.req) : 0);
+            if (need > e->cfg.budget_bytes) {
+                e->finish(j, GA_E_BUDGET, t0);
+                finished.push_back(&j);
+                continue;
+            }
+            if (e->arena->in_use() + need > e->cfg.budget_bytes) break;
+            std::vector<void*> allocs;
+            bool short_of_memory = false;
+            auto take = [&](uint64_t bytes) -> void* {
+                void* ptr = short_of_memory ? nullptr : e->arena->alloc(bytes);
+                if (!ptr) short_of_memory = true;
+                else allocs.push_back(ptr);
+                return ptr;
+            };
+            InstDesc d{};
+            if (!e->carve_instance(j, p, take, d)) {
+                for (void* a : allocs) e->arena->release(a);
+                // nothing else holds memory that could be freed: waiting would not help
+                bool others = !wave.empty();
+                for (Job* o : e->resident) others = others || o != nullptr;
+                if (!others) throw GaError(GA_E_NOMEM, "device arena exhausted (one-launch batch)");
+                break;
+            }
+            d.grow_heap_at = 0;  // no grow requests: the host cannot step in during the launch
+            d.grow_table_at = 0;
+            d.start_iter = 0;
+            e->init_instance(j, p, d, s);
+            j.allocs = std::move(allocs);
+            j.desc = d;
+            j.lane = 0;
+            j.bytes = need;
+            wave.push_back(&j);
+            descs.push_back(d);
+        }
+        if (wave.empty()) return;  // the memory is held by the lanes: the next pass tries again
+        const uint32_t n = (uint32_t)wave.size();
+        e->stats.peak_bytes = std::max<uint64_t>(e->stats.peak_bytes, e->arena->in_use());
+        const uint64_t desc_b = round_up(n * sizeof(InstDesc), 256), state_b = round_up(n * sizeof(InstState), 256);
+        char* dm = (char*)e->arena->alloc(desc_b + state_b + round_up(n * sizeof(uint32_t), 256));
+        if (!dm) throw GaError(GA_E_NOMEM, "device arena exhausted (one-launch batch)");
+        meta.push_back(dm);
+        d_descs = (InstDesc*)dm;
+        InstState* d_states = (InstState*)(dm + desc_b);
+        uint32_t* d_order = (uint32_t*)(dm + desc_b + state_b);
+        host = e->take_pinned(desc_b + state_b + n * sizeof(uint32_t));
+        char* hm = (char*)host->p;
+        std::memcpy(hm, descs.data(), n * sizeof(InstDesc));
+        uint32_t* h_order = (uint32_t*)(hm + desc_b + state_b);
+        for (uint32_t i = 0; i < n; ++i) h_order[i] = i;  // the wave is already largest first
+        hip_check(hipMemcpyAsync(d_descs, hm, n * sizeof(InstDesc), hipMemcpyHostToDevice, s), "upload descs");
+        hip_check(hipMemcpyAsync(d_order, h_order, n * sizeof(uint32_t), hipMemcpyHostToDevice, s), "upload order");
+        seed_batch_kernel<<<(n + 63) / 64, 64, 0, s>>>(d_descs, d_states, n, (int)W);
+        const uint64_t t_start = now_ns();
+        for (Job* j : wave) j->t_admit = t_start;
+        // the launch's loops end by this many iterations per instance whatever the configuration
+        // (an unlimited search would otherwise only be bounded by the 32-bit counter)
+        const uint32_t max_iters =
+            e->cfg.max_iterations ? std::min(e->cfg.max_iterations, kPersistMaxIters) : kPersistMaxIters;
+        const StepParams prm{e->cfg.pops_per_iter, 1u << 16, 0u, e->cfg.rounds, e->cfg.grow_thresh,
+                             e->cfg.step_flags, {0, 0}};
+        if (W == 8) launch_persist_w<8>(B, s, n, d_descs, d_states, d_order, max_iters, prm);
+        else launch_persist_w<4>(B, s, n, d_descs, d_states, d_order, max_iters, prm);
+        hip_check(hipGetLastError(), "one-launch batch");
+        snap = (InstState*)(hm + desc_b);
+        hip_check(hipMemcpyAsync(snap, d_states, n * sizeof(InstState), hipMemcpyDeviceToHost, s), "read back states");
+        ev = e->take_event();
+        hip_check(hipEventRecord(ev, s), "record");
+        stage = SEARCH;
+        e->stats.persist_launches += 1;
+        e->stats.persist_instances += n;
+    }
+
+    // The wave's states are back: finishes what did not reach the goal,
+    // returns the overflowed to the lanes, and walks the paths of the rest.
+    void outcomes(std::vector<Job*>& finished, std::vector<Job*>& restart) {
+        const uint32_t n = (uint32_t)wave.size();
+        done.clear();
+        offs.clear();
+        uint64_t total = 0;
+        std::vector<Job*> keep;  // jobs whose memory stays until the path walk
+        for (uint32_t i = 0; i < n; ++i) {
+            Job& j = *wave[i];
+            const InstState& st = snap[i];
+            if (st.status == ST_DONE && st.best != kInf) {
+                done.push_back(i);
+                offs.push_back(total);
+                total += (uint64_t)st.best + 1;  // every move costs >= 1
+                continue;
+            }
+            e->fill_counters(j, st);
+            if (st.status == ST_OVERFLOW_HEAP || st.status == ST_OVERFLOW_TABLE) {
+                j.expanded_before += st.expanded;
+                if (j.retries >= e->cfg.max_retries) {
+                    e->finish(j, GA_E_OVERFLOW, t0);
+                    finished.push_back(&j);
+                    continue;
+                }
+                ++j.retries;  // the lanes start it again, with their heap count and 4x the structure that filled up
+                if (st.status == ST_OVERFLOW_HEAP) j.heap_shift += 2;
+                else j.table_shift += 2;
+                restart.push_back(&j);
+                continue;
+            }
+            j.out->cost = 0;
+            e->finish(j, st.status == ST_DONE ? GA_NO_PATH : GA_E_TIMEOUT, t0);
+            finished.push_back(&j);
+        }
+        e->give_event(ev);
+        ev = nullptr;
+        if (done.empty()) {
+            drop();
+            return;
+        }
+        const uint32_t m = (uint32_t)done.size();
+        const uint64_t jobs_b = round_up(m * sizeof(PathJob), 256);
+        path_lens_b = round_up(m * sizeof(uint32_t), 16);
+        char* pd = (char*)e->arena->alloc(jobs_b + path_lens_b + total * sizeof(uint64_t));
+        if (!pd) throw GaError(GA_E_NOMEM, "device arena exhausted (paths)");
+        meta.push_back(pd);
+        path_host = e->take_pinned(jobs_b + path_lens_b + total * sizeof(uint64_t));
+        PathJob* pj = (PathJob*)path_host->p;
+        for (uint32_t k = 0; k < m; ++k) pj[k] = PathJob{done[k], snap[done[k]].best + 1, offs[k]};
+        hipStream_t s = e->persist_stream;
+        hip_check(hipMemcpyAsync(pd, pj, m * sizeof(PathJob), hipMemcpyHostToDevice, s), "upload path jobs");
+        uint32_t* d_lens = (uint32_t*)(pd + jobs_b);
+        path_kernel<<<m, 64, 0, s>>>(d_descs, (const PathJob*)pd, m, (uint64_t*)(pd + jobs_b + path_lens_b), d_lens);
+        hip_check(hipGetLastError(), "path launch");
+        hip_check(hipMemcpyAsync((char*)path_host->p + jobs_b, d_lens, path_lens_b + total * sizeof(uint64_t),
+                                 hipMemcpyDeviceToHost, s),
+                  "read back paths");
+        ev = e->take_event();
+        hip_check(hipEventRecord(ev, s), "record");
+        stage = PATHS;
+    }
+
+    void paths(std::vector<Job*>& finished) {
+        const uint32_t m = (uint32_t)done.size();
+        const char* base = (const char*)path_host->p + round_up(m * sizeof(PathJob), 256);
+        const uint32_t* lens = (const uint32_t*)base;
+        const uint64_t* out = (const uint64_t*)(base + path_lens_b);
+        for (uint32_t k = 0; k < m; ++k) {
+            Job& j = *wave[done[k]];
+            const InstState& st = snap[done[k]];
+            e->fill_counters(j, st);
+            j.out->cost = st.best;
+            const uint32_t len = lens[k];
+            if (len == kNone || len == 0) {
+                e->finish(j, GA_E_OVERFLOW, t0);
+            } else {
+                j.out->path = (uint64_t*)malloc(sizeof(uint64_t) * len);
+                if (!j.out->path) throw std::bad_alloc();
+                for (uint32_t i = 0; i < len; ++i) j.out->path[i] = out[offs[k] + (len - 1 - i)];
+                j.out->path_len = len;
+                e->finish(j, GA_OK, t0);
+            }
+            finished.push_back(&j);
+        }
+        drop();
+    }
+};
+
 // The run loop shared by plain, batched and served GA* and by the online
 // server.  Jobs are added in arrival order; each step() admits what has
 // arrived and fits (retries first), retires finished path walks, and lets
@@ -1649,11 +2034,21 @@ class RunLoop {
     uint32_t inflight = 0, max_inflight = 0;
     std::vector<Job*> finished;       // jobs completed since the owner last cleared this list
 
+    PersistRun* persist = nullptr;    // the one-launch batch form's waves, run beside the lanes
+
     void add(Job* j) { pending.push_back(j); }
 
     // One pass; returns false once nothing is pending, running or walking.
     bool step() {
         e->reap_deferred(false);  // memory an in-place growth no longer needs
+        // -- the one-launch form: the next stage of its wave, or the next wave
+        // (ahead of the lanes' admissions, which then see its memory in use)
+        bool waved = false;
+        if (persist) {
+            std::vector<Job*> again;
+            waved = persist->poll(finished, again);
+            for (Job* j : again) retry.push_back(j);
+        }
         // -- admit: retries first, then arrivals in order.  Each lane admits
         // its requests first come, first served; a request that must wait
         // holds up only the requests of its own lane behind it (a big grid
@@ -1708,7 +2103,7 @@ class RunLoop {
             t_mark = t;
         };
         // -- retire path walks that have completed
-        bool progressed = false;
+        bool progressed = waved;
         while (!batches.empty() && hipEventQuery(batches.front()->ev) == hipSuccess) {
             finalize_batch(batches.front());
             batches.erase(batches.begin());
@@ -1754,6 +2149,7 @@ class RunLoop {
             lap(e->stats.host_launch_ms);
             if (L.running > 0 || L.launched > L.processed) busy = true;
         }
+        if (persist && !persist->idle()) busy = true;
         if (busy) {
             if (!progressed) std::this_thread::sleep_for(std::chrono::microseconds(20));
             return true;
@@ -1779,6 +2175,7 @@ class RunLoop {
     // After a failure: waits for the path walks in flight and frees them (the
     // lanes and resident jobs are ga_engine::abandon's).
     void discard() {
+        if (persist) persist->drop();
         (void)hipStreamSynchronize(e->path_stream);
         for (PathBatch* b : batches) {
             e->arena->release(b->dev);
@@ -1965,10 +2362,10 @@ class RunLoop {
     }
 };
 
-void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* ss) {
+void run_job_list(ga_engine* e, std::vector<Job*> order, bool timed, ga_serve_stats* ss,
+                  PersistRun* persist = nullptr) {
     RunLoop loop(e, timed);
-    std::vector<Job*> order;
-    for (auto& j : jobs) order.push_back(&j);
+    loop.persist = persist;
     std::stable_sort(order.begin(), order.end(), [](Job* a, Job* b) { return a->arrival_ns < b->arrival_ns; });
     for (Job* j : order) loop.add(j);
     try {
@@ -1981,12 +2378,12 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
         std::vector<double> lat;
         uint64_t last = 0, first = ~0ull;
         ss->completed = ss->failed = 0;
-        for (auto& j : jobs) {
-            first = std::min(first, j.arrival_ns);
-            if (j.out->status == GA_OK || j.out->status == GA_NO_PATH) {
+        for (Job* j : order) {
+            first = std::min(first, j->arrival_ns);
+            if (j->out->status == GA_OK || j->out->status == GA_NO_PATH) {
                 ++ss->completed;
-                lat.push_back(j.out->latency_ns / 1e6);
-                last = std::max(last, j.arrival_ns + j.out->latency_ns);
+                lat.push_back(j->out->latency_ns / 1e6);
+                last = std::max(last, j->arrival_ns + j->out->latency_ns);
             } else {
                 ++ss->failed;
             }
@@ -2005,6 +2402,24 @@ void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* 
     }
 }
 
+void run_jobs(ga_engine* e, std::vector<Job>& jobs, bool timed, ga_serve_stats* ss) {
+    std::vector<Job*> order;
+    for (auto& j : jobs) order.push_back(&j);
+    run_job_list(e, std::move(order), timed, ss);
+}
+
+// ga_solve_batch: with cfg.persist on, the eligible requests run in the
+// one-launch form beside the launch lanes, which take the rest.
+void run_batch(ga_engine* e, std::vector<Job>& jobs) {
+    std::vector<Job*> mine, rest;
+    for (Job& j : jobs) ((e->cfg.persist && persist_eligible(e, j)) ? mine : rest).push_back(&j);
+    if (mine.empty()) {
+        run_job_list(e, std::move(rest), false, nullptr);
+        return;
+    }
+    PersistRun pr(e, std::move(mine), now_ns());
+    run_job_list(e, std::move(rest), false, nullptr, &pr);
+}
 
 // The checks every request passes before it reaches the device: the result
 // starts as GA_PENDING; a malformed request gets GA_E_INVALID and an
@@ -2070,7 +2485,7 @@ int run_checked(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* out
 }
 
 int run_engine(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, uint32_t n, ga_result* outs,
-               ga_serve_stats* ss) {
+               ga_serve_stats* ss, bool batch = false) {
     if (!e) return GA_E_INVALID;
     if (e->serving) {  // refused before anything runs: every result says so
         e->last_error = "an online server owns the engine";
@@ -2085,7 +2500,8 @@ int run_engine(ga_engine* e, const ga_request* reqs, const uint64_t* arrivals, u
     return run_checked(e, reqs, n, outs, arrivals, [&](std::vector<Job>& live) {
         if (e->broken) throw GaError(GA_E_HIP, "the engine was stopped by a HIP error in an earlier run");
         hip_check(hipSetDevice(e->device), "hipSetDevice");
-        run_jobs(e, live, arrivals != nullptr, ss);
+        if (batch) run_batch(e, live);
+        else run_jobs(e, live, arrivals != nullptr, ss);
     });
 }
 
@@ -2433,6 +2849,9 @@ void ga_config_default(ga_config* cfg) {
     cfg->pdb24 = 1;
     cfg->grow_in_place = 1;
     cfg->block = kStepBlock;
+    cfg->persist = 0;
+    cfg->persist_block = 1024;
+    cfg->persist_cells = 0;
 }
 
 int ga_engine_create(int device, const ga_config* cfg, ga_engine** out) {
@@ -2499,7 +2918,7 @@ int ga_solve_grid(ga_engine* e, const uint8_t* cells, uint32_t width, uint32_t h
 }
 
 int ga_solve_batch(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* outs) {
-    return run_engine(e, reqs, nullptr, n, outs, nullptr);
+    return run_engine(e, reqs, nullptr, n, outs, nullptr, true);
 }
 
 int ga_serve_run(ga_engine* e, const ga_request* reqs, const uint64_t* arrival_ns, uint32_t n, ga_result* outs,

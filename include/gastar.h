@@ -130,6 +130,13 @@ typedef struct ga_config {
                                 the LDS heap-top entries a launch changed; 2 = grids read each entered
                                 cell's stored g with its weight and skip the atomicMin that could not
                                 lower it */
+    uint32_t persist;        /* ga_solve_batch only: 1 = solve the batch's 8- and 15-puzzles and its
+                                grids of at most persist_cells cells in one launch per wave, one
+                                workgroup per instance (persist_block / heap_arity heaps) running
+                                every iteration of it; the other requests, and those that outgrow
+                                their heaps or closed set, run on the launch lanes afterwards */
+    uint32_t persist_block;  /* threads per workgroup of that launch: 256, 512 or 1024 (0 = 1024) */
+    uint64_t persist_cells;  /* largest grid (cells) of the one-launch form (0 = puzzles only) */
 } ga_config;
 
 /* Engine statistics.  With cfg.profile = n, kernel time covers the timed
@@ -175,6 +182,8 @@ typedef struct ga_engine_stats {
     /* host time of the run loop: admitting requests, handling chunk read-backs (finished requests, path
        walks launched, growth), uploading launch maps and launching chunks, finishing path walks */
     double host_admit_ms, host_snapshot_ms, host_launch_ms, host_path_ms;
+    uint64_t persist_launches;  /* one-launch batch waves run (ga_config.persist) */
+    uint64_t persist_instances; /* instances those waves ran */
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {

@@ -79,3 +79,25 @@ def test_batched_config4_under_served_tuning_matches_golden():
     assert st.timed_launches > 0 and st.algo_bytes > 0  # the sampled profiling ran
     for i in range(0, 1024, 61):
         assert replay_ok(insts[i], sols[i]), i
+
+
+def test_batched_config4_one_launch_matches_golden():
+    # the same 1024 instances with the one-launch form: every puzzle and grid (up to 2048^2) runs
+    # all of its iterations inside one launch, one workgroup of 1024 threads per instance
+    gold = json.load(open(os.path.join(GOLDEN, "config4.json")))["instances"]
+    insts = wl.config4(4, 1024, threads=16)
+    cfg = tuning.load("served")
+    with ga.Engine(0, budget_bytes=BUDGET, chunk_bytes=8 << 30, max_iterations=1_000_000, persist=1,
+                   persist_block=1024, persist_cells=2048 * 2048, **cfg.engine) as e:
+        dev = {i: e.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
+        reqs = [x.request(cells_device=dev.get(i)) for i, x in enumerate(insts)]
+        sols = e.solve_batch(reqs)
+        st = e.stats()
+        for a in dev.values():
+            e.free_device(a)
+    assert st.persist_instances >= 1024
+    bad = [(i, ga.status_str(s.status), s.cost, g["cost"]) for i, (s, g) in enumerate(zip(sols, gold))
+           if s.status != ga.OK or s.cost != g["cost"]]
+    assert not bad, bad[:10]
+    for i in range(0, 1024, 61):
+        assert replay_ok(insts[i], sols[i]), i
