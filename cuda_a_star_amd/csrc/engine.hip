@@ -214,11 +214,11 @@ struct PathJob {
     uint64_t offset;  // words of the output buffer
 };
 
-// One wave (one 64-thread block) per finished instance: grids walk with the
-// whole wave through an LDS window of the closed set, the hashed sets with
-// lane 0 (their parent slots have no locality to exploit).
-__global__ __launch_bounds__(64) void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n,
-                                                  uint64_t* out, uint32_t* lens) {
+// One kPathThreads-thread workgroup per finished instance: grids walk with
+// all of its threads through an LDS window of the closed set, the hashed sets
+// with thread 0 (their parent slots have no locality to exploit).
+__global__ __launch_bounds__(kPathThreads) void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n,
+                                                            uint64_t* out, uint32_t* lens) {
     __shared__ uint64_t win[kPathWin * kPathWin];
     const uint32_t i = blockIdx.x;
     if (i >= n) return;
@@ -1976,7 +1976,7 @@ class PersistRun {
         hipStream_t s = e->persist_stream;
         hip_check(hipMemcpyAsync(pd, pj, m * sizeof(PathJob), hipMemcpyHostToDevice, s), "upload path jobs");
         uint32_t* d_lens = (uint32_t*)(pd + jobs_b);
-        path_kernel<<<m, 64, 0, s>>>(d_descs, (const PathJob*)pd, m, (uint64_t*)(pd + jobs_b + path_lens_b), d_lens);
+        path_kernel<<<m, kPathThreads, 0, s>>>(d_descs, (const PathJob*)pd, m, (uint64_t*)(pd + jobs_b + path_lens_b), d_lens);
         hip_check(hipGetLastError(), "path launch");
         hip_check(hipMemcpyAsync((char*)path_host->p + jobs_b, d_lens, path_lens_b + total * sizeof(uint64_t),
                                  hipMemcpyDeviceToHost, s),
@@ -2350,7 +2350,7 @@ class RunLoop {
         hip_check(hipEventRecord(ready, L.stream), "record");
         hip_check(hipStreamWaitEvent(e->path_stream, ready, 0), "wait");
         hip_check(hipMemcpyAsync(dj, pj, n * sizeof(PathJob), hipMemcpyHostToDevice, e->path_stream), "upload jobs");
-        path_kernel<<<(uint32_t)n, 64, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n, dout, dl);
+        path_kernel<<<(uint32_t)n, kPathThreads, 0, e->path_stream>>>(e->d_descs, (const PathJob*)dj, (uint32_t)n, dout, dl);
         hip_check(hipGetLastError(), "path launch");
         hip_check(hipMemcpyAsync(b->host->p, dl, lens_bytes + total * sizeof(uint64_t), hipMemcpyDeviceToHost,
                                  e->path_stream),
@@ -2794,7 +2794,7 @@ void solve_noalloc(ga_engine* e, const ga_request& r, ga_result* out, uint64_t* 
             throw GaError(GA_E_BUDGET, "the path does not fit the no-allocation scratch");
         const PathJob job{0u, cap, 0u};
         hip_check(hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, st), "upload path job");
-        path_kernel<<<1, 64, 0, st>>>(d_desc, d_job, 1u, d_walk, d_len);
+        path_kernel<<<1, kPathThreads, 0, st>>>(d_desc, d_job, 1u, d_walk, d_len);
         path_order_kernel<<<std::max<uint32_t>(1, std::min<uint32_t>(1024, (cap + 255) / 256)), 256, 0, st>>>(
             d_walk, d_len, d_path, is_wide(j.kind) ? 1u : 0u);
         hip_check(hipGetLastError(), "path launch");

@@ -1885,69 +1885,80 @@ __device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, 
     return n;
 }
 
-// walk_path for dense grids with the 64 lanes of one wave (all of them call
-// it, with `win` pointing at kPathWin * kPathWin words of LDS).  A node's
-// parent is one of its neighbours, so the wave loads the closed-set values of
-// the kPathWin x kPathWin cells around the current cell in one round trip and
-// follows the links inside that window; where the walk leaves it, the window
-// is reloaded around the new cell.  One dependent memory access per path
-// step becomes one per kPathWin / 2 steps or more.  Same result as walk_path,
-// with the cells goal..start written (by lane 0) as row-major indices
-// y * width + x, the form the host API uses; their number, or kNone past cap.
-constexpr uint32_t kPathWin = 32;
+// walk_path for dense grids with every thread of a kPathThreads-thread
+// workgroup (all of them call it, with `win` pointing at kPathWin * kPathWin
+// words of LDS).  A node's parent is one of its neighbours, so the threads
+// load the closed-set values of a kPathWin x kPathWin window of cells in one
+// round trip and follow the links inside it; where the walk leaves the window
+// the next one is loaded around the new cell.  Windows are aligned to the 4 x 4
+// tiles of the layout (16 threads read one tile's 128-byte line; padding cells
+// lie inside the allocation) and reach 7/8 of their width from the current
+// cell towards the start, where the walk heads.  One dependent memory access
+// per path step becomes one per window, tens of steps.  Same result as
+// walk_path, with the cells goal..start written (by thread 0) as row-major
+// indices y * width + x, the form the host API uses; their number, or kNone
+// past cap.
+constexpr uint32_t kPathWin = 64;
+constexpr uint32_t kPathThreads = 256;
+
+// Origin of a window along one axis: c the current cell, t the start's, w the
+// window's width, extent the tiled layout's cells along the axis (w and extent
+// multiples of 4, w <= extent).  The result is a multiple of 4 with
+// o <= c < o + w for every c < extent.
+__device__ inline uint32_t path_window_origin(uint32_t c, uint32_t t, uint32_t w, uint32_t extent) {
+    const uint32_t back = t < c ? w - w / 8 : w / 8;  // window cells before c
+    const uint32_t o = (c > back ? c - back : 0u) & ~3u;
+    return min(o, extent - w);
+}
 
 __device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint32_t cap, uint64_t* win) {
-    constexpr int kPer = (int)(kPathWin * kPathWin / 64);  // window words per lane
-    const uint32_t lane = threadIdx.x & 63u;
+    constexpr int kPer = (int)(kPathWin * kPathWin / kPathThreads);  // window words per thread
+    static_assert(kPathWin % 4 == 0 && kPathWin * kPathWin % kPathThreads == 0, "path window");
+    const uint32_t tid = threadIdx.x;
     const uint32_t W = d.width, H = d.height, tc = d.tile_cols;
-    const uint32_t ww = W < kPathWin ? W : kPathWin, wh = H < kPathWin ? H : kPathWin;
+    const uint32_t ex = tc * 4u, ey = ((H + 3u) / 4u) * 4u;  // the tiled layout's extent
+    const uint32_t ww = ex < kPathWin ? ex : kPathWin, wh = ey < kPathWin ? ey : kPathWin;
+    const uint32_t sx = d.start_xy & 0xFFFFu, sy = d.start_xy >> 16;
     const gptr<const uint64_t> val = as_global<const uint64_t>(d.dense);
     uint32_t slot = (uint32_t)d.goal;  // tiled index
     uint32_t x = d.goal_xy & 0xFFFFu, y = d.goal_xy >> 16;
     uint32_t n = 0;
     while (slot != kNone) {
-        // the window: ww x wh cells inside the grid, the current cell near its
-        // middle.  (Written without a select: `x < ww/2 ? 0 : min(x - ww/2, ..)`
-        // was compiled into min(x - ww/2, ..) alone, which wraps near column 0.)
-        const uint32_t ox = min(max(x, ww / 2) - ww / 2, W - ww);
-        const uint32_t oy = min(max(y, wh / 2) - wh / 2, H - wh);
+        const uint32_t ox = path_window_origin(x, sx, ww, ex), oy = path_window_origin(y, sy, wh, ey);
         const uint32_t n0 = n;
         uint64_t v[kPer];
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) {  // all loads in flight together; positions outside read the corner
-            const uint32_t k = lane + 64u * (uint32_t)r, wx = k % kPathWin, wy = k / kPathWin;
+        for (int r = 0; r < kPer; ++r) {  // all loads in flight together; positions outside read the origin
+            const uint32_t k = tid + kPathThreads * (uint32_t)r, t = k >> 4, o = k & 15u;
+            const uint32_t wx = (t % (kPathWin / 4)) * 4u + (o & 3u), wy = (t / (kPathWin / 4)) * 4u + (o >> 2);
             const bool in = wx < ww && wy < wh;
             v[r] = val[grid_index(ox + (in ? wx : 0u), oy + (in ? wy : 0u), tc)];
         }
-        __syncthreads();  // every lane is done with the previous window
+        __syncthreads();  // every thread is done with the previous window
 #pragma unroll
-        for (int r = 0; r < kPer; ++r) win[lane + 64u * (uint32_t)r] = v[r];
+        for (int r = 0; r < kPer; ++r) {
+            const uint32_t k = tid + kPathThreads * (uint32_t)r, t = k >> 4, o = k & 15u;
+            const uint32_t wx = (t % (kPathWin / 4)) * 4u + (o & 3u), wy = (t / (kPathWin / 4)) * 4u + (o >> 2);
+            win[wy * kPathWin + wx] = v[r];
+        }
         __syncthreads();
-        // the links inside the window; every lane follows the same chain
+        // the links inside the window; every thread follows the same chain
         while (slot != kNone && x - ox < ww && y - oy < wh) {
             if (n >= cap) return kNone;
-            if (lane == 0) out[n] = (uint64_t)y * W + x;
+            if (tid == 0) out[n] = (uint64_t)y * W + x;
             ++n;
             const uint32_t p = val_parent(win[(y - oy) * kPathWin + (x - ox)]);
             if (p == kNone) {
                 slot = kNone;
                 break;
             }
-            // x, y of the parent: the neighbour (in the move order of
-            // GridOps::neighbour) whose tiled index it is, or, for a link to a
-            // cell that is not a neighbour, decoded from the index (a cell
-            // outside the grid then ends the walk below)
-            uint32_t px = kNone, py = 0;
-#pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const uint32_t cx = x + ((0x00221102u >> (4 * i)) & 15u) - 1u;
-                const uint32_t cy = y + ((0x02020211u >> (4 * i)) & 15u) - 1u;
-                if (px == kNone && cx < W && cy < H && grid_index(cx, cy, tc) == p) {
-                    px = cx;
-                    py = cy;
-                }
-            }
-            if (px == kNone) {
+            // x, y of the parent: a neighbour differs from (x, y) by -1, 0 or
+            // +1 per axis, which the offset inside its tile gives (the two low
+            // bits of each coordinate); a link to any other cell is decoded
+            // from the index (one outside the grid then ends the walk below)
+            const uint32_t ddx = ((p & 3u) - (x & 3u)) & 3u, ddy = (((p >> 2) & 3u) - (y & 3u)) & 3u;
+            uint32_t px = x + (ddx == 3u ? ~0u : ddx), py = y + (ddy == 3u ? ~0u : ddy);
+            if (ddx == 2u || ddy == 2u || px >= W || py >= H || grid_index(px, py, tc) != p) {
                 const uint32_t t = p >> 4;
                 px = (t % tc) * 4u + (p & 3u);
                 py = (t / tc) * 4u + ((p >> 2) & 3u);
