@@ -230,8 +230,11 @@ int ga_solve_grid(ga_engine* e, const uint8_t* cells, uint32_t width, uint32_t h
                   uint64_t goal, uint32_t connectivity, uint32_t num_queues, ga_result* out);
 
 /* Batched GA*: all requests share each iteration's launches, with heaps and
- * closed sets partitioned per request.  No reference counterpart (the
- * reference solves one instance per a_star<>() call). */
+ * closed sets partitioned per request.  With cfg.persist the 8- and
+ * 15-puzzles and the grids of at most cfg.persist_cells cells instead run
+ * every iteration inside one launch, one workgroup per request, beside the
+ * launch lanes.  No reference counterpart (the reference solves one instance
+ * per a_star<>() call, a_star.h:223-224). */
 int ga_solve_batch(ga_engine* e, const ga_request* reqs, uint32_t n, ga_result* outs);
 
 /* Serving: replay an arrival stream (arrival_ns[i] relative to the call)
