@@ -171,6 +171,10 @@ __global__ __launch_bounds__(B) void persist_kernel(const InstDesc* __restrict__
     const uint32_t q = threadIdx.x / W;
     const bool active = q < d.n_heaps;
     const Tally tl{sh.tally, false};
+    using WL = decltype(sh.work);
+    // the heap group's heap size, kept in registers across iterations (its
+    // top levels stay in the group's LDS slice: run_group's resident form)
+    uint32_t size = active ? as_global(d.heap_size)[q] : 0u;
     for (uint32_t it = 0;; ++it) {
         // Every thread reads the same status: the barrier that ended the
         // previous iteration orders it after every write to it by this
@@ -184,20 +188,20 @@ __global__ __launch_bounds__(B) void persist_kernel(const InstDesc* __restrict__
         step_begin(sh);
         switch (d.kind) {
             case KIND_PUZZLE3:
-                run_group<W, PuzzleOps<3>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
-                                           sh.dirty);
+                run_group<W, PuzzleOps<3>, WL, true>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
+                                                     sh.heap_top, sh.dirty, &size);
                 break;
             case KIND_PUZZLE4:
-                run_group<W, PuzzleOps<4>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
-                                           sh.dirty);
+                run_group<W, PuzzleOps<4>, WL, true>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
+                                                     sh.heap_top, sh.dirty, &size);
                 break;
             case KIND_GRID4:
-                run_group<W, GridOps<4>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
-                                         sh.dirty);
+                run_group<W, GridOps<4>, WL, true>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
+                                                   sh.heap_top, sh.dirty, &size);
                 break;
             case KIND_GRID8:
-                run_group<W, GridOps<8>>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work, sh.heap_top,
-                                         sh.dirty);
+                run_group<W, GridOps<8>, WL, true>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
+                                                   sh.heap_top, sh.dirty, &size);
                 break;
             default:
                 if (threadIdx.x == 0) set_status(st, ST_ABORTED);

@@ -1436,11 +1436,22 @@ __device__ inline void push_records(const HeapRef<W>& H, uint32_t& S, const Heap
 // iteration's minimum is left in place (the heap sits the launch out) and
 // still enters the reduction.
 //
+// Res (resident heaps, the one-launch form, where a workgroup owns its
+// instance's heaps for the whole launch and heap q is the workgroup's heap
+// group q): each group keeps its own heap's top levels in its own LDS slice
+// and the heap's size in *res_size across iterations, so phase 1 reads size
+// and root without a memory access, phase 2 needs neither the copy into LDS
+// nor the write-back (the entries at or past the LDS levels live in global
+// memory as always), heaps are not packed, and the records phase 1 read stay
+// in the group's registers.  At it = 0 the slice is filled from the seeded
+// heap.
+//
 // Every thread of the workgroup calls run_group (it holds a barrier).
-template <int W, class Ops, class WL>
+template <int W, class Ops, class WL, bool Res = false>
 __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint32_t status, uint32_t it, uint32_t q,
                                           bool active, uint32_t cover, const StepParams& prm, const Tally& tl,
-                                          WL& wl, HeapEntry* heap_top, uint32_t* dirty_masks) {
+                                          WL& wl, HeapEntry* heap_top, uint32_t* dirty_masks,
+                                          uint32_t* res_size = nullptr) {
     constexpr int K = Ops::K;
     constexpr int KW = (K + W - 1) / W;
     const int lane = (int)(threadIdx.x & (W - 1));
@@ -1492,9 +1503,17 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         rec[r].aux = 0;
     }
     if (active) {
-        S = as_global(d.heap_size)[q];
-        // the root slot always lies inside the heap's storage; it is used only if S > 0
-        if (lane == 0) root_prio = (as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1))[0].prio;
+        if constexpr (Res) {
+            S = *res_size;
+            if (lane == 0 && S > 0) {
+                root_prio = it == 0 ? (as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1))[0].prio
+                                    : heap_top[(size_t)(threadIdx.x / W) * HeapRef<W>::kTop].prio;
+            }
+        } else {
+            S = as_global(d.heap_size)[q];
+            // the root slot always lies inside the heap's storage; it is used only if S > 0
+            if (lane == 0) root_prio = (as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1))[0].prio;
+        }
         if (it > 0 && q < Aw) {
             // insert_nodes_into_queues, a_star.h:101-115: heap q takes slots (q + epoch) mod N + j*N
             const uint32_t base = (q + it) & (Aw - 1);
@@ -1580,7 +1599,7 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
         }
     }
     // append the heaps with work to the list: one LDS atomic per wave
-    {
+    if (!Res) {
         const bool lead = work && lane == 0;
         const uint64_t m = __ballot(lead);
         if (m) {
@@ -1598,28 +1617,44 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     __syncthreads();
 
     // ---------------- phase 2 ----------------
-    const uint32_t n_work = wl.n;
-    if ((threadIdx.x & ~63u) / W >= n_work) return;  // no heap for this wave
-    const uint32_t g = threadIdx.x / W;
-    const bool mine = g < n_work;
-    q = mine ? wl.q[g] : 0u;
-    S = mine ? wl.S[g] : 0u;
-    const bool track = (prm.flags & STEP_DIRTY_TOPS) != 0;
+    uint32_t g;
+    bool mine;
+    if constexpr (Res) {  // every group works its own heap in its own slice
+        if (!__ballot(work)) return;  // no heap of this wave has work
+        g = threadIdx.x / W;
+        mine = work;
+    } else {
+        const uint32_t n_work = wl.n;
+        if ((threadIdx.x & ~63u) / W >= n_work) return;  // no heap for this wave
+        g = threadIdx.x / W;
+        mine = g < n_work;
+        q = mine ? wl.q[g] : 0u;
+        S = mine ? wl.S[g] : 0u;
+    }
+    const bool track = !Res && (prm.flags & STEP_DIRTY_TOPS) != 0;
     const HeapRef<W> H{as_global(d.heaps) + (size_t)q * d.heap_stride + (W - 1),
                        as_local(heap_top) + (size_t)g * HeapRef<W>::kTop, as_local(dirty_masks) + g, track};
+    if constexpr (Res) {
+        if (!mine) {  // (the records phase 1 read stay in the registers of a group with work)
 #pragma unroll
-    for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
-    if (mine && it > 0 && q < Aw) {  // the records phase 1 read for this heap (cached)
-        const uint32_t base = (q + it) & (Aw - 1);
-#pragma unroll
-        for (int r = 0; r < KW; ++r) {
-            const int j = r * W + lane;
-            if (j < K) rec[r] = cin[base + (size_t)j * Aw];
+            for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
         }
-    }
-    if (mine) {
-        if (track && lane == 0) *H.dirty = 0u;
-        H.fill(S, lane);  // the heap's top levels into LDS, with the records' loads
+        if (mine && it == 0) H.fill(S, lane);  // the seeded heap's entries into the slice, once
+    } else {
+#pragma unroll
+        for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
+        if (mine && it > 0 && q < Aw) {  // the records phase 1 read for this heap (cached)
+            const uint32_t base = (q + it) & (Aw - 1);
+#pragma unroll
+            for (int r = 0; r < KW; ++r) {
+                const int j = r * W + lane;
+                if (j < K) rec[r] = cin[base + (size_t)j * Aw];
+            }
+        }
+        if (mine) {
+            if (track && lane == 0) *H.dirty = 0u;
+            H.fill(S, lane);  // the heap's top levels into LDS, with the records' loads
+        }
     }
     group_fence();
     bool overflow_heap = false, overflow_table = false;
@@ -1755,11 +1790,15 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
             group_fence();
         }
         group_fence();
-        H.write_back(S, lane);
-        if (lane == 0) {
-            as_global(d.heap_size)[q] = S;
-            // one more iteration could bring this heap past its capacity: grow before it
-            if (d.grow_heap_at && S > d.grow_heap_at) atomicOr(&st->grow[it % 3], GROW_HEAP);
+        if constexpr (Res) {
+            *res_size = S;
+        } else {
+            H.write_back(S, lane);
+            if (lane == 0) {
+                as_global(d.heap_size)[q] = S;
+                // one more iteration could bring this heap past its capacity: grow before it
+                if (d.grow_heap_at && S > d.grow_heap_at) atomicOr(&st->grow[it % 3], GROW_HEAP);
+            }
         }
         // candidate slice of this heap: K records
         const gptr<HeapEntry> cout = as_global(d.cand) + (size_t)(it & 1) * N * K + (size_t)q * K;
