@@ -298,6 +298,52 @@ def lane_roofline(st, cfg, elapsed_s):
     return rf
 
 
+def dropin_leg(idx, insts, served_costs):
+    """The header drop-in's own loop (include/gastar/a_star.h: one instance per
+    a_star<>() call, as the reference's a_star.h:223-282): gastar_cli solves
+    the 15-puzzles idx one after another (--each-pair 1) with the reference's
+    4096 queues and reports each search's time as the reference's CLI does
+    (main.cu:26-29), in microseconds.  Rate = puzzles / summed search time;
+    costs are checked against the served ones."""
+    import subprocess
+    import tempfile
+
+    import cuda_a_star_amd as ga
+
+    cli = os.path.join(ROOT, "cuda_a_star_amd", "gastar_cli")
+
+    def tiles(state):
+        return ",".join(str(t) if t else "_" for t in ga.state_to_board(state, 4))
+
+    with tempfile.TemporaryDirectory() as tmp:
+        src, dst = os.path.join(tmp, "in.txt"), os.path.join(tmp, "out.txt")
+        with open(src, "w") as f:
+            for i in idx:
+                f.write(f"{tiles(insts[i].start)} {tiles(insts[i].goal)}\n")
+        t = time.perf_counter()
+        r = subprocess.run([cli, "--version", "sliding", "--size", "4", "--num-queues", "4096", "--each-pair", "1",
+                            "--time-unit", "us", "--input-data", src, "--output-data", dst],
+                           capture_output=True, text=True, timeout=600)
+        wall = time.perf_counter() - t
+        if r.returncode != 0:
+            raise RuntimeError(f"gastar_cli failed ({r.returncode}): {r.stdout[-500:]} {r.stderr[-500:]}")
+        times, costs = [], []
+        for line in open(dst).read().split():
+            if line.isdigit():  # a result's elapsed time starts it; its states follow, one per line
+                times.append(int(line))
+                costs.append(-1)
+            else:
+                costs[-1] += 1
+    if len(times) != len(idx):
+        raise RuntimeError(f"gastar_cli answered {len(times)} of {len(idx)} puzzles")
+    mism = sum(1 for i, c in zip(idx, costs) if served_costs.get(i) != c)
+    search_s = sum(times) / 1e6
+    return {"solves_per_sec": round(len(idx) / search_s, 3), "puzzles": len(idx), "search_s": round(search_s, 3),
+            "process_wall_s": round(wall, 3), "cost_mismatches_vs_served": mism,
+            "workload": "the first 15-puzzles of the timed stream (walk depth 30..70), 4096 heaps (the reference's "
+                        "default), one a_star<>() call each through gastar_cli; rate from the summed search times"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +357,8 @@ def main() -> int:
                     help="offered load of the latency leg, as a fraction of the measured saturation rate (0 = skip)")
     ap.add_argument("--seed", type=int, default=5)
     ap.add_argument("--plain-sample", type=int, default=200, help="requests solved one at a time")
+    ap.add_argument("--dropin-sample", type=int, default=100,
+                    help="15-puzzles of the stream solved one at a time through the header drop-in (gastar_cli)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--cpu-latency-seconds", type=float, default=5.0)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, cores available)")
@@ -546,6 +594,13 @@ def main() -> int:
         log(f"[bench] one-at-a-time: {plain_n} in {plain_t:.2f}s = {plain_sps:.2f} solves/s, "
             f"served-vs-alone cost mismatches {mismatch}")
 
+    # ---- the header drop-in: a_star<>() through gastar_cli, the same 15-puzzles one at a time ----
+    dropin = None
+    if args.dropin_sample > 0:
+        dropin = dropin_leg([i for i in timed_idx if insts[i].kind == "puzzle"][: args.dropin_sample], insts,
+                            served_costs)
+        log(f"[bench] drop-in a_star<>: {dropin}")
+
     # ---- CPU oracle on the host cores ----
     cpu = None
     if not args.skip_cpu:
@@ -593,6 +648,7 @@ def main() -> int:
         "completed": ok,
         "failed": failed,
         "one_at_a_time_solves_per_sec": round(plain_sps, 3) if plain_sps else None,
+        "dropin_one_at_a_time": dropin,
         "one_at_a_time_sample": plain_n,
         "batched": batch,
         "puzzle24": p24,

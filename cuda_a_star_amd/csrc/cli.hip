@@ -1,7 +1,8 @@
 // cli.hip -- command-line front end with the reference's flags and text formats.
 //
 //   gastar_cli --version sliding|pathfinding [--input-data F] [--output-data F]
-//              [--num-queues N] [--size S] [--device D]
+//              [--num-queues N] [--size S] [--device D] [--each-pair 0|1]
+//              [--time-unit ms|us]
 //
 // Flags, input and output formats follow main.cu:31-78 (output: elapsed
 // milliseconds, then one state per line, start..target; an empty path means
@@ -10,7 +11,10 @@
 // ("W,H", "sx,sy", "tx,ty", obstacle count and "x,y" lines, cost count and
 // "x,y,c" lines; unlisted cells cost 1).  Additions: --num-queues (the
 // reference hard-codes 4096, main.cu:18), --size (the reference hard-codes 5,
-// main.cu:72), --device (the reference hard-codes 1, main.cu:17).
+// main.cu:72), --device (the reference hard-codes 1, main.cu:17), and for the
+// sliding puzzle --each-pair 1 (solve every start/target pair of the input in
+// turn, each answered as one pair is; the reference reads the first) and
+// --time-unit us (the elapsed time in microseconds instead of milliseconds).
 //
 // The problems are plugins over the functor interface (gastar/functional_types.h)
 // solved by the header drop-in a_star<>() (gastar/a_star.h).  Differences from
@@ -140,15 +144,18 @@ void set_target(const Position<size>& target) {
 }
 
 template <unsigned char size>
-void solve(std::istream& in, std::ostream& out, uint32_t num_queues) {
+void solve(std::istream& in, std::ostream& out, uint32_t num_queues, bool each_pair, bool micros) {
     std::string a, b;
-    in >> a >> b;
-    Position<size> start(parse_tiles(a, size)), target(parse_tiles(b, size));
-    set_target(target);
-    auto res = a_star<Position<size>, 4, neighbour_generator<size>, manhattan_heuristic<size>>(start, target,
-                                                                                               num_queues);
-    out << std::chrono::duration_cast<std::chrono::milliseconds>(res.second).count() << '\n';
-    for (const auto& p : res.first) out << format(p) << '\n';
+    while (in >> a >> b) {
+        Position<size> start(parse_tiles(a, size)), target(parse_tiles(b, size));
+        set_target(target);
+        auto res = a_star<Position<size>, 4, neighbour_generator<size>, manhattan_heuristic<size>>(start, target,
+                                                                                                   num_queues);
+        if (micros) out << std::chrono::duration_cast<std::chrono::microseconds>(res.second).count() << '\n';
+        else out << std::chrono::duration_cast<std::chrono::milliseconds>(res.second).count() << '\n';
+        for (const auto& p : res.first) out << format(p) << '\n';
+        if (!each_pair) break;
+    }
 }
 
 }  // namespace SlidingPuzzle
@@ -301,6 +308,7 @@ int main(int argc, char** argv) {
     uint32_t num_queues = 4096;  // main.cu:18
     int size = 5;                // main.cu:72
     int device = 0;
+    bool each_pair = false, micros = false;
     for (int i = 1; i < argc; i += 2) {
         if (i == argc - 1) {
             std::cout << "Parameter without argument." << std::endl;
@@ -313,6 +321,8 @@ int main(int argc, char** argv) {
         else if (k == "--num-queues") num_queues = (uint32_t)std::stoul(v);
         else if (k == "--size") size = std::stoi(v);
         else if (k == "--device") device = std::stoi(v);
+        else if (k == "--each-pair") each_pair = v != "0";
+        else if (k == "--time-unit" && (v == "ms" || v == "us")) micros = v == "us";
         else {
             std::cout << "Unknown option " << k << std::endl;
             return 255;
@@ -331,9 +341,9 @@ int main(int argc, char** argv) {
         gastar::dropin::check(hipSetDevice(device), "hipSetDevice");
         if (version == "sliding") {
             switch (size) {
-                case 3: SlidingPuzzle::solve<3>(in, out, num_queues); break;
-                case 4: SlidingPuzzle::solve<4>(in, out, num_queues); break;
-                case 5: SlidingPuzzle::solve<5>(in, out, num_queues); break;
+                case 3: SlidingPuzzle::solve<3>(in, out, num_queues, each_pair, micros); break;
+                case 4: SlidingPuzzle::solve<4>(in, out, num_queues, each_pair, micros); break;
+                case 5: SlidingPuzzle::solve<5>(in, out, num_queues, each_pair, micros); break;
                 default: throw std::runtime_error("--size must be 3, 4 or 5");
             }
         } else if (version == "pathfinding") {

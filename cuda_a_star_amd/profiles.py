@@ -126,7 +126,7 @@ def bench_line(path: str) -> dict:
         return json.loads([l for l in f if l.startswith("{")][-1])
 
 
-FLAGS = "--skip-batch --skip-p24 --plain-sample 0 --skip-cpu --latency-load 0"
+FLAGS = "--skip-batch --skip-p24 --plain-sample 0 --dropin-sample 0 --skip-cpu --latency-load 0"
 LANE_KERNELS = {0: "step_kernel<4, 30u", 1: "step_kernel<4, 8u"}
 LANE_NAMES = {0: "lane 0 (15-puzzles, grids below 2 M cells): `step_kernel<4, 30u, 256>`",
               1: "lane 1 (grids of at least 2 M cells): `step_kernel<4, 8u, 256>`"}

@@ -31,6 +31,11 @@ def test_parameter_without_argument():
     assert r.returncode == 255 and "Parameter without argument." in r.stdout
 
 
+def test_time_unit_must_be_ms_or_us():
+    r = run(["--time-unit", "s"])
+    assert r.returncode == 255 and "Unknown option --time-unit" in r.stdout
+
+
 def tiles_text(state, side):
     return ",".join(str(t) if t else "_" for t in ga.state_to_board(state, side))
 
@@ -53,6 +58,34 @@ def test_sliding_matches_oracle(side, depth, seed):
     rc, ref, _ = oracle.astar_puzzle(side, start, goal, want_path=False)
     assert len(path) - 1 == ref.cost
     assert oracle.replay_puzzle(side, path, start, goal) == ref.cost
+
+
+@pytest.mark.gpu
+def test_sliding_each_pair_in_microseconds():
+    """--each-pair 1 answers every start/target pair of the input in turn, each
+    as a single pair is answered; --time-unit us gives the elapsed time in
+    microseconds (the bench times the drop-in with these)."""
+    goal = ga.puzzle_goal(4)
+    starts = [ga.gen_puzzle(4, d, s) for d, s in ((30, 11), (44, 12), (0, 13), (52, 14))]
+    text = "".join(f"{tiles_text(s, 4)} {tiles_text(goal, 4)}\n" for s in starts)
+    r = run(["--version", "sliding", "--size", "4", "--num-queues", "2048", "--each-pair", "1", "--time-unit", "us"],
+            text)
+    assert r.returncode == 0, r.stdout + r.stderr
+    results, cur = [], None
+    for line in r.stdout.split():
+        if line.isdigit():
+            cur = []
+            results.append(cur)
+        else:
+            cur.append(parse_tiles(line, 4))
+    assert len(results) == len(starts)
+    for start, path in zip(starts, results):
+        rc, ref, _ = oracle.astar_puzzle(4, start, goal, want_path=False)
+        assert len(path) - 1 == ref.cost
+        assert oracle.replay_puzzle(4, path, start, goal) == ref.cost
+    # without the flag only the first pair is answered, as by the reference
+    r1 = run(["--version", "sliding", "--size", "4", "--num-queues", "2048"], text)
+    assert r1.returncode == 0 and sum(1 for l in r1.stdout.split() if l.isdigit()) == 1
 
 
 @pytest.mark.gpu
