@@ -49,7 +49,7 @@ constexpr uint32_t kAllKinds =
 // the maximum over the kinds it can run, so a lane that holds one kind runs an
 // instantiation for that kind alone: the 15-puzzle and 4-connected grid paths
 // fit 64 VGPRs (8 waves per SIMD) where the kernel for all kinds needs 77 (6).
-template <int W, uint32_t KINDS, int B>
+template <int W, uint32_t KINDS, int B, int P = 1>
 __global__ __launch_bounds__(B) void step_kernel(const WgEntry* __restrict__ wgmap,
                                                  const InstDesc* __restrict__ descs, InstState* states,
                                                  const uint32_t* __restrict__ iter_base, uint32_t r,
@@ -80,8 +80,8 @@ __global__ __launch_bounds__(B) void step_kernel(const WgEntry* __restrict__ wgm
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID4)) != 0)
             if (d.kind == KIND_GRID4) {
-                run_group<W, GridOps<4>>(d, states + inst, status, it, q, active, cover, prm, tl, sh.work,
-                                         sh.heap_top, sh.dirty);
+                run_group<W, GridOps<4>, decltype(sh.work), false, P>(d, states + inst, status, it, q, active, cover,
+                                                                      prm, tl, sh.work, sh.heap_top, sh.dirty);
                 ran = true;
             }
         if constexpr ((KINDS & kind_bit(KIND_GRID8)) != 0)
@@ -642,6 +642,8 @@ namespace {
 // that has a single-kind instantiation, or kAllKinds), or kWideVariant for
 // the 24-puzzle kernel.
 constexpr uint32_t kWideVariant = kind_bit(KIND_PUZZLE5);
+// the 4-connected grid instantiation with paired pops (ga_config.step_flags & STEP_PAIRED_POPS)
+constexpr uint32_t kGrid4Paired = kind_bit(KIND_GRID4) | (1u << 31);
 
 template <int W, int B>
 void launch_step_w(uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map, const InstDesc* descs,
@@ -656,6 +658,9 @@ void launch_step_w(uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry
             break;
         case kind_bit(KIND_GRID4):
             step_kernel<W, kind_bit(KIND_GRID4), B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            break;
+        case kGrid4Paired:
+            step_kernel<W, kind_bit(KIND_GRID4), B, 2><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             break;
         case kWideVariant:
             step_kernel_wide<W, B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
@@ -1439,10 +1444,11 @@ struct ga_engine {
         L.map_dirty = false;
         if (!L.n_wg) return;
         // a single-kind instantiation when the lane holds one kind that has one
-        const uint32_t variant =
+        uint32_t variant =
             (kinds == kind_bit(KIND_PUZZLE3) || kinds == kind_bit(KIND_PUZZLE4) || kinds == kind_bit(KIND_GRID4))
                 ? kinds
                 : kAllKinds;
+        if (variant == kind_bit(KIND_GRID4) && (cfg.step_flags & STEP_PAIRED_POPS)) variant = kGrid4Paired;
         L.variant = variant;  // its captured chunk, if any, stays valid (graphs are kept per instantiation)
         if (L.n_wg > L.map_cap) {
             if (L.d_map) {
@@ -1484,6 +1490,7 @@ struct ga_engine {
             case kind_bit(KIND_PUZZLE3): return 1;
             case kind_bit(KIND_PUZZLE4): return 2;
             case kind_bit(KIND_GRID4): return 3;
+            case kGrid4Paired: return 4;
             default: return 0;
         }
     }

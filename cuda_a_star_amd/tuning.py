@@ -16,13 +16,18 @@ per launch):
   f slack              a heap expands its top only within this much of the last
                        iteration's minimum f (less speculative expansion)
   graph replay         chunks replayed from a captured hipGraph or launched one by one
+  kernel variants      ga_config.step_flags: paired pops on the 4-connected grid lane
+                       (4), write-back of changed heap-top entries only (1), grids'
+                       settled-cell check before the atomic (2)
 
 `python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
 candidates on a seeded sample of the serving workload on device 0 and writes
 the winners to tuning.json next to this file; `load()` reads them (or the
 built-in defaults when no file exists).  Served candidates are scored on a
 Poisson arrival stream (ga_serve_run, solves per second of the stream's wall
-time); plain candidates on the same requests solved one at a time.
+time); plain candidates on the same requests solved one at a time.  Every
+candidate is measured in a process of its own (engines created one after
+another in one process run slower).
 """
 from __future__ import annotations
 
@@ -30,6 +35,7 @@ import argparse
 import itertools
 import json
 import os
+import subprocess
 import sys
 import time
 from dataclasses import asdict, dataclass, field
@@ -102,54 +108,67 @@ def load(mode: str = "served") -> TuneConfig:
 # ---------------------------------------------------------------------------
 # sweep
 # ---------------------------------------------------------------------------
-def _measure_served(insts, dev, cfg: TuneConfig) -> float:
+def _measure_served(insts, cfg: TuneConfig) -> float:
     import cuda_a_star_amd as ga
 
     with ga.Engine(0, budget_bytes=200 << 30, chunk_bytes=8 << 30, **cfg.engine) as e:
+        dev = {i: e.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
         reqs = [x.request(cells_device=dev.get(i), num_queues=cfg.heaps(x.kind, x.side))
                 for i, x in enumerate(insts)]
         t0 = insts[0].arrival_ns
         sols, ss = e.serve(reqs, [x.arrival_ns - t0 for x in insts])
+        for a in dev.values():
+            e.free_device(a)
     if any(s.status != ga.OK for s in sols) or not ss.wall_ns:
         return 0.0
     return len(sols) / (ss.wall_ns / 1e9)
 
 
-def _measure_plain(insts, dev, cfg: TuneConfig) -> float:
+def _measure_plain(insts, cfg: TuneConfig) -> float:
     import cuda_a_star_amd as ga
 
     with ga.Engine(0, budget_bytes=200 << 30, chunk_bytes=8 << 30, **cfg.engine) as e:
+        dev = {i: e.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
         t = time.perf_counter()
         for i, x in enumerate(insts):
             s = e.solve(x.request(cells_device=dev.get(i), num_queues=cfg.heaps(x.kind, x.side)))
             if s.status != ga.OK:
                 return 0.0
             if i % 10 == 9:  # a line every ten requests: a long point is visibly alive
-                print(f"[tune] plain {i + 1}/{len(insts)} in {time.perf_counter() - t:.1f}s", flush=True)
+                print(f"[tune] plain {i + 1}/{len(insts)} in {time.perf_counter() - t:.1f}s", file=sys.stderr,
+                      flush=True)
         dt = time.perf_counter() - t
+        for a in dev.values():
+            e.free_device(a)
     return len(insts) / dt
 
 
-def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
-    import cuda_a_star_amd as ga
-    from cuda_a_star_amd import workloads as wl
+def _measure(mode: str, n: int, seed: int, qps: float, cfg: TuneConfig) -> float:
+    """One candidate, measured in a process of its own: engines created one
+    after another in one process run slower (their streams share the process's
+    few hardware queues with the earlier engines'), which would bias a sweep
+    against every candidate after the first.  The child regenerates the sample
+    from its seed and uses one engine."""
+    cmd = [sys.executable, "-m", "cuda_a_star_amd.tuning", "--measure", mode, "--n", str(n), "--seed", str(seed),
+           "--qps", str(qps), "--config", json.dumps(asdict(cfg))]
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=900,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    if r.returncode != 0:
+        print(f"[tune] measurement failed ({r.returncode}): {r.stderr[-400:]}", flush=True)
+        return 0.0
+    return float(json.loads(r.stdout.strip().splitlines()[-1])["value"])
 
-    t = time.perf_counter()
-    insts = wl.config5(seed, n, qps, threads=16)
-    up = ga.Engine(0)
-    dev = {i: up.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
-    print(f"[tune] {mode}: {n} requests generated and {len(dev)} grids uploaded in "
-          f"{time.perf_counter() - t:.1f}s", flush=True)
-    measure = _measure_served if mode == "served" else _measure_plain
+
+def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
     best = load(mode)
-    best_v = measure(insts, dev, best)
+    best_v = _measure(mode, n, seed, qps, best)
     print(f"[tune] {mode} start {best_v:.2f}/s {asdict(best)}", flush=True)
     if mode == "served":
         axes = {
             "puzzle_heaps": [256, 512, 1024, 2048, 4096, 8192, 16384],
             "grid_heaps_small": [128, 256, 512, 1024, 2048],
             "grid_heaps_mid": [256, 512, 1024, 2048, 4096],
-            "grid_heaps_big": [512, 1024, 2048, 4096, 8192],
+            "grid_heaps_big": [256, 512, 1024, 2048, 4096, 8192],
             "heap_arity": [4, 8],
             "chunk_iters": [8, 16, 32, 64, 128],
             "pops_per_iter": [1, 2, 4],
@@ -164,7 +183,7 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
             "slack_puzzle": [0, 1, 3, 5, 9],
             "slack_grid": [0, 8, 16, 24, 32, 48],
             "block": [128, 256, 512],
-            "step_flags": [0, 1, 2, 3],
+            "step_flags": [0, 1, 2, 3, 4, 6],
         }
     else:
         axes = {
@@ -192,13 +211,10 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
                 if getattr(best, name) == v:
                     continue
                 cand = TuneConfig(**{**asdict(best), name: v})
-                val = measure(insts, dev, cand)
+                val = _measure(mode, n, seed, qps, cand)
                 print(f"[tune] {mode} {name}={v}: {val:.2f}/s (best {best_v:.2f})", flush=True)
                 if val > best_v * 1.02:
                     best, best_v = cand, val
-    for a in dev.values():
-        up.free_device(a)
-    up.close()
     print(f"[tune] {mode} best {best_v:.2f}/s {asdict(best)}", flush=True)
     return best
 
@@ -210,7 +226,17 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=77)
     ap.add_argument("--qps", type=float, default=1500.0, help="offered load of the served sample")
     ap.add_argument("--out", default=TUNING_FILE, help="where to write the winners")
+    ap.add_argument("--measure", choices=["served", "plain"], help="(internal) measure --config once and print it")
+    ap.add_argument("--config", help="(internal) the TuneConfig to measure, as JSON")
     args = ap.parse_args(argv)
+    if args.measure:
+        from cuda_a_star_amd import workloads as wl
+
+        insts = wl.config5(args.seed, args.n, args.qps, threads=16)
+        cfg = TuneConfig(**json.loads(args.config))
+        fn = _measure_served if args.measure == "served" else _measure_plain
+        print(json.dumps({"value": fn(insts, cfg)}), flush=True)
+        return 0
     data = {}
     if os.path.exists(TUNING_FILE):
         with open(TUNING_FILE) as f:

@@ -263,6 +263,10 @@ VARIANTS = [
     dict(heap_arity=4, rounds=2, block=512, lanes=2, deep_cells=1 << 14),
     dict(heap_arity=8, rounds=1, block=512, use_graph=1, slack_puzzle=3),
     dict(heap_arity=8, rounds=2, block=128, adaptive_start=16, grow_thresh=4),
+    # paired pops on the 4-connected grid lane (lane 1 holds grids of >= 2^14 cells)
+    dict(heap_arity=4, rounds=1, rounds_grid=2, step_flags=4, lanes=2, deep_cells=1 << 14),
+    dict(heap_arity=4, rounds=1, rounds_grid=3, step_flags=6, lanes=2, deep_cells=1 << 14, use_graph=1,
+         slack_grid=8),
 ]
 
 
