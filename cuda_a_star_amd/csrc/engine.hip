@@ -195,9 +195,10 @@ __global__ __launch_bounds__(B) void persist_kernel(const InstDesc* __restrict__
                 run_group<W, PuzzleOps<4>, WL, true>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
                                                      sh.heap_top, sh.dirty, &size);
                 break;
-            case KIND_GRID4:
-                run_group<W, GridOps<4>, WL, true>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
-                                                   sh.heap_top, sh.dirty, &size);
+            case KIND_GRID4:  // paired pops: two nodes' atomics in flight per lane (the kernel's register
+                              // count, set by its other kinds, leaves room for them)
+                run_group<W, GridOps<4>, WL, true, 2>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
+                                                      sh.heap_top, sh.dirty, &size);
                 break;
             case KIND_GRID8:
                 run_group<W, GridOps<8>, WL, true>(d, st, status, it, q, active, d.n_heaps, prm, tl, sh.work,
