@@ -139,6 +139,7 @@ class EngineStats(ctypes.Structure):
         ("host_path_ms", ctypes.c_double),
         ("persist_launches", ctypes.c_uint64),
         ("persist_instances", ctypes.c_uint64),
+        ("persist_early_restarts", ctypes.c_uint64),
     ]
 
 

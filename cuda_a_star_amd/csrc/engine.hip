@@ -158,11 +158,17 @@ __global__ void seed_batch_kernel(const InstDesc* descs, InstState* states, uint
 // workgroup share one CU, whose vector memory pipeline keeps them in order.
 // Grow requests are off (the host cannot step in): an instance that fills
 // its heaps or closed set ends in an overflow status and is restarted on the
-// launch lanes.  `max_iters` bounds every loop (ST_ABORTED).
+// launch lanes.  `max_iters` bounds every loop (ST_ABORTED).  When an
+// instance ends, its final status | kDoneFlag goes to done[inst] in host
+// memory (coherent pinned memory, a system-scope store), where the host reads
+// it while the launch still runs: an instance that outgrew its structures is
+// restarted on the lanes at once rather than after the whole wave.
+constexpr uint32_t kDoneFlag = 0x80000000u;
+
 template <int W, int B>
 __global__ __launch_bounds__(B) void persist_kernel(const InstDesc* __restrict__ descs, InstState* states,
                                                     const uint32_t* __restrict__ order, uint32_t max_iters,
-                                                    StepParams prm) {
+                                                    StepParams prm, uint32_t* done) {
     static_assert((B / W) * HeapRef<W>::kTop <= StepSharedT<B>::kHeapTopEntries && W >= 4, "LDS heap tops");
     __shared__ StepSharedT<B> sh;
     const uint32_t inst = order[blockIdx.x];
@@ -211,6 +217,9 @@ __global__ __launch_bounds__(B) void persist_kernel(const InstDesc* __restrict__
         tally_end(sh.tally, st, it, 0);
         __syncthreads();  // the last wave's flush of the tallies, and every heap and record write
     }
+    if (threadIdx.x == 0 && done)
+        __hip_atomic_store(done + inst, kDoneFlag | load_relaxed(&st->status), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 struct PathJob {
@@ -793,6 +802,28 @@ struct ga_engine {
     std::atomic<bool> serving{false};  // an online server owns the engine (ga_serve_start .. ga_serve_stop)
     char* scratch_dev = nullptr;  // this device's scratch (ga_solve_noalloc)
     uint8_t* d_pdb = nullptr;     // 24-puzzle pattern databases (kPdbBytes + level flags), built on first use
+    // the one-launch form's completion flags: coherent pinned host memory the
+    // device writes (one word per instance of a wave), grown on demand
+    uint32_t* persist_done = nullptr;
+    uint32_t* persist_done_dev = nullptr;
+    size_t persist_done_cap = 0;
+    uint32_t* ensure_persist_done(size_t n) {
+        if (n > persist_done_cap) {
+            if (persist_done) (void)hipHostFree(persist_done);
+            persist_done = persist_done_dev = nullptr;
+            persist_done_cap = 0;
+            const size_t cap = std::max<size_t>(n, 1024);
+            void* p = nullptr;
+            hip_check(hipHostMalloc(&p, cap * sizeof(uint32_t), hipHostMallocCoherent | hipHostMallocMapped),
+                      "hipHostMalloc (completion flags)");
+            persist_done = (uint32_t*)p;
+            void* dp = nullptr;
+            hip_check(hipHostGetDevicePointer(&dp, p, 0), "hipHostGetDevicePointer");
+            persist_done_dev = (uint32_t*)dp;
+            persist_done_cap = cap;
+        }
+        return persist_done;
+    }
 
     void init(int dev, const ga_config* c) {
         device = dev;
@@ -880,6 +911,7 @@ struct ga_engine {
         if (d_descs) (void)hipFree(d_descs);
         if (d_states) (void)hipFree(d_states);
         if (d_pdb) (void)hipFree(d_pdb);
+        if (persist_done) (void)hipHostFree(persist_done);
         delete arena;
         if (path_stream) (void)hipStreamDestroy(path_stream);
         if (persist_stream) {
@@ -1765,11 +1797,11 @@ double persist_work(const Job& j) {
 
 template <int W>
 void launch_persist_w(uint32_t block, hipStream_t s, uint32_t n, const InstDesc* descs, InstState* states,
-                      const uint32_t* order, uint32_t max_iters, const StepParams& prm) {
+                      const uint32_t* order, uint32_t max_iters, const StepParams& prm, uint32_t* done) {
     switch (block) {
-        case 256: persist_kernel<W, 256><<<n, 256, 0, s>>>(descs, states, order, max_iters, prm); break;
-        case 512: persist_kernel<W, 512><<<n, 512, 0, s>>>(descs, states, order, max_iters, prm); break;
-        default: persist_kernel<W, 1024><<<n, 1024, 0, s>>>(descs, states, order, max_iters, prm); break;
+        case 256: persist_kernel<W, 256><<<n, 256, 0, s>>>(descs, states, order, max_iters, prm, done); break;
+        case 512: persist_kernel<W, 512><<<n, 512, 0, s>>>(descs, states, order, max_iters, prm, done); break;
+        default: persist_kernel<W, 1024><<<n, 1024, 0, s>>>(descs, states, order, max_iters, prm, done); break;
     }
 }
 
@@ -1799,9 +1831,11 @@ class PersistRun {
     // launches the next wave.  Finished jobs are appended to `finished`, jobs
     // to restart on the lanes to `restart`.  Returns whether anything moved.
     bool poll(std::vector<Job*>& finished, std::vector<Job*>& restart) {
+        bool moved = false;
+        if (stage == SEARCH) moved = early_restarts(restart);
         if (stage != IDLE) {
             const hipError_t q = hipEventQuery(ev);
-            if (q == hipErrorNotReady) return false;
+            if (q == hipErrorNotReady) return moved;
             hip_check(q, "one-launch batch");
             if (stage == SEARCH) {
                 outcomes(finished, restart);
@@ -1820,13 +1854,17 @@ class PersistRun {
     // After a failure: waits for the stream and frees the wave.
     void drop() {
         if (stage != IDLE || !wave.empty()) (void)hipStreamSynchronize(e->persist_stream);
-        for (Job* j : wave) {
-            for (void* a : j->allocs) e->arena->release(a);
-            j->allocs.clear();
+        for (size_t i = 0; i < wave.size(); ++i) {
+            // a job restarted early may hold the lanes' allocations by now: its
+            // wave memory was moved to `meta` when it was restarted
+            if (i < restarted.size() && restarted[i]) continue;
+            for (void* a : wave[i]->allocs) e->arena->release(a);
+            wave[i]->allocs.clear();
         }
         wave.clear();
         for (void* m : meta) e->arena->release(m);
         meta.clear();
+        restarted.clear();
         if (host) e->give_pinned(host);
         if (path_host) e->give_pinned(path_host);
         host = path_host = nullptr;
@@ -1843,7 +1881,8 @@ class PersistRun {
     size_t next = 0;
     Stage stage = IDLE;
     std::vector<Job*> wave;
-    std::vector<void*> meta;  // the wave's own device buffers
+    std::vector<void*> meta;  // the wave's own device buffers, and those of instances restarted early
+    std::vector<char> restarted;  // per wave instance: restarted on the lanes while the wave ran
     Pinned* host = nullptr;   // descriptors and order up, states back
     Pinned* path_host = nullptr;
     hipEvent_t ev = nullptr;
@@ -1923,8 +1962,11 @@ class PersistRun {
             e->cfg.max_iterations ? std::min(e->cfg.max_iterations, kPersistMaxIters) : kPersistMaxIters;
         const StepParams prm{e->cfg.pops_per_iter, 1u << 16, 0u, e->cfg.rounds, e->cfg.grow_thresh,
                              e->cfg.step_flags, {0, 0}};
-        if (W == 8) launch_persist_w<8>(B, s, n, d_descs, d_states, d_order, max_iters, prm);
-        else launch_persist_w<4>(B, s, n, d_descs, d_states, d_order, max_iters, prm);
+        uint32_t* done = e->ensure_persist_done(n);  // the previous wave's launch has ended (drop)
+        std::memset(done, 0, n * sizeof(uint32_t));
+        restarted.assign(n, 0);
+        if (W == 8) launch_persist_w<8>(B, s, n, d_descs, d_states, d_order, max_iters, prm, e->persist_done_dev);
+        else launch_persist_w<4>(B, s, n, d_descs, d_states, d_order, max_iters, prm, e->persist_done_dev);
         hip_check(hipGetLastError(), "one-launch batch");
         snap = (InstState*)(hm + desc_b);
         hip_check(hipMemcpyAsync(snap, d_states, n * sizeof(InstState), hipMemcpyDeviceToHost, s), "read back states");
@@ -1933,6 +1975,37 @@ class PersistRun {
         stage = SEARCH;
         e->stats.persist_launches += 1;
         e->stats.persist_instances += n;
+    }
+
+    // While the wave runs: an instance whose workgroup reported an overflow
+    // (its completion flag) is restarted on the lanes now, its structures 4x
+    // the one that filled up, instead of after the whole wave.  Its memory in
+    // the wave stays until the wave ends.  (The expansions of that attempt are
+    // not known yet and are not added to its result's count.)  Returns
+    // whether any was restarted.
+    bool early_restarts(std::vector<Job*>& restart) {
+        const volatile uint32_t* done = e->persist_done;
+        bool any = false;
+        for (size_t i = 0; i < wave.size(); ++i) {
+            if (restarted[i]) continue;
+            const uint32_t v = done[i];
+            if (!(v & kDoneFlag)) continue;
+            const uint32_t st = v & ~kDoneFlag;
+            if (st != ST_OVERFLOW_HEAP && st != ST_OVERFLOW_TABLE) continue;
+            Job& j = *wave[i];
+            const bool dense = j.kind == KIND_GRID4 || j.kind == KIND_GRID8;
+            if (j.retries >= e->cfg.max_retries || (dense && st == ST_OVERFLOW_TABLE)) continue;  // outcomes() ends it
+            restarted[i] = 1;
+            for (void* a : j.allocs) meta.push_back(a);
+            j.allocs.clear();
+            ++j.retries;
+            if (st == ST_OVERFLOW_HEAP) j.heap_shift += 2;
+            else j.table_shift += 2;
+            restart.push_back(&j);
+            ++e->stats.persist_early_restarts;
+            any = true;
+        }
+        return any;
     }
 
     // The wave's states are back: finishes what did not reach the goal,
@@ -1944,6 +2017,7 @@ class PersistRun {
         uint64_t total = 0;
         std::vector<Job*> keep;  // jobs whose memory stays until the path walk
         for (uint32_t i = 0; i < n; ++i) {
+            if (restarted[i]) continue;  // already back on the lanes
             Job& j = *wave[i];
             const InstState& st = snap[i];
             if (st.status == ST_DONE && st.best != kInf) {

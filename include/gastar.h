@@ -184,6 +184,8 @@ typedef struct ga_engine_stats {
     double host_admit_ms, host_snapshot_ms, host_launch_ms, host_path_ms;
     uint64_t persist_launches;  /* one-launch batch waves run (ga_config.persist) */
     uint64_t persist_instances; /* instances those waves ran */
+    uint64_t persist_early_restarts; /* instances restarted on the lanes while their wave still ran (their
+                                        workgroup's completion flag reported an overflow) */
 } ga_engine_stats;
 
 typedef struct ga_serve_stats {

@@ -71,6 +71,8 @@ def test_one_launch_batch_sends_the_rest_to_the_lanes():
         assert sol.cost == oracle_cost(x), i
         assert replay_ok(x, sol), i
     assert sols[n + 1].retries >= 1  # outgrew its closed set in the launch, restarted on the lanes
+    # ... at once: its workgroup's completion flag reached the host while the wave still ran
+    assert st.persist_early_restarts >= 1
     rc, ref, _ = oracle.astar_board(5, board, ga.board_goal(5), want_path=False)
     assert rc == oracle.OK and sols[n + 2].status == ga.OK and sols[n + 2].cost == ref.cost
     assert oracle.replay_board(5, sols[n + 2].path, board, ga.board_goal(5)) == sols[n + 2].cost
