@@ -233,14 +233,14 @@ struct PathJob {
 // with thread 0 (their parent slots have no locality to exploit).
 __global__ __launch_bounds__(kPathThreads) void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n,
                                                             uint64_t* out, uint32_t* lens) {
-    __shared__ uint64_t win[kPathWin * kPathWin];
+    __shared__ uint16_t tab[kPathTabWords];
     const uint32_t i = blockIdx.x;
     if (i >= n) return;
     const PathJob j = jobs[i];
     const InstDesc& d = descs[j.inst];
     uint32_t len = kNone;
     if (d.kind == KIND_GRID4 || d.kind == KIND_GRID8) {
-        len = walk_grid_path(d, out + j.offset, j.cap, win);
+        len = walk_grid_path(d, out + j.offset, j.cap, tab);
     } else if (threadIdx.x == 0) {
         switch (d.kind) {
             case KIND_PUZZLE3: len = walk_path<PuzzleOps<3>>(d, out + j.offset, j.cap); break;

@@ -2051,20 +2051,31 @@ __device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, 
 }
 
 // walk_path for dense grids with every thread of a kPathThreads-thread
-// workgroup (all of them call it, with `win` pointing at kPathWin * kPathWin
+// workgroup (all of them call it, with `tab` pointing at kPathTabWords 16-bit
 // words of LDS).  A node's parent is one of its neighbours, so the threads
-// load the closed-set values of a kPathWin x kPathWin window of cells in one
-// round trip and follow the links inside it; where the walk leaves the window
-// the next one is loaded around the new cell.  Windows are aligned to the 4 x 4
-// tiles of the layout (16 threads read one tile's 128-byte line; padding cells
-// lie inside the allocation) and reach 7/8 of their width from the current
-// cell towards the start, where the walk heads.  One dependent memory access
-// per path step becomes one per window, tens of steps.  Same result as
-// walk_path, with the cells goal..start written (by thread 0) as row-major
-// indices y * width + x, the form the host API uses; their number, or kNone
-// past cap.
-constexpr uint32_t kPathWin = 64;
-constexpr uint32_t kPathThreads = 256;
+// load the parent links of a kPathWin x kPathWin window of cells in one round
+// trip (windows aligned to the 4 x 4 tiles of the layout: 16 threads read one
+// tile's 128-byte line; padding cells lie inside the allocation; a window
+// reaches 7/8 of its width from the current cell towards the start, where the
+// walk heads) and turn them, in parallel, into link tables: each cell's parent
+// as an index inside the window (or a code: no parent, a parent outside the
+// window and the direction to it, or a parent that is not a neighbour), then
+// the cell 2, 4, ... steps up where every step stays inside.  The walk follows
+// the longest links, kPathHop steps per dependent LDS read, recording where
+// each hop starts, and the last few single steps lead to the window's edge
+// (or the start); then every thread writes the cells of its hops, and the
+// next window is loaded around the parent.  Same result as walk_path, with
+// the cells goal..start written as row-major indices y * width + x, the form
+// the host API uses; their number, or kNone past cap.
+constexpr uint32_t kPathWin = 128;
+constexpr uint32_t kPathThreads = 1024;
+constexpr uint32_t kPathCells = kPathWin * kPathWin;  // in-window indices: below 2^14
+constexpr int kPathLevels = 2;                        // link tables of 1, 2, ... steps
+constexpr uint32_t kPathHop = 1u << (kPathLevels - 1);
+constexpr uint32_t kPathTabWords = kPathLevels * kPathCells + kPathCells / kPathHop + 8;  // tables, hop starts, chain
+constexpr uint32_t kLinkRoot = 0xFFFFu;  // no parent (the start); in the longer tables: no link
+constexpr uint32_t kLinkFar = 0xFFFEu;   // a parent that is not a neighbour: read again from memory
+constexpr uint32_t kLinkOut = 0x8000u;   // | direction (dx + 1) + 3 (dy + 1): a neighbour outside the window
 
 // Origin of a window along one axis: c the current cell, t the start's, w the
 // window's width, extent the tiled layout's cells along the axis (w and extent
@@ -2076,67 +2087,129 @@ __device__ inline uint32_t path_window_origin(uint32_t c, uint32_t t, uint32_t w
     return min(o, extent - w);
 }
 
-__device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint32_t cap, uint64_t* win) {
-    constexpr int kPer = (int)(kPathWin * kPathWin / kPathThreads);  // window words per thread
-    static_assert(kPathWin % 4 == 0 && kPathWin * kPathWin % kPathThreads == 0, "path window");
+__device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint32_t cap, uint16_t* tab) {
+    constexpr int kPer = (int)(kPathCells / kPathThreads);  // window cells per thread
+    constexpr uint32_t kMaxHops = kPathCells / kPathHop;
+    static_assert(kPathWin % 4 == 0 && kPathCells % kPathThreads == 0 && kPathCells <= kLinkOut, "path window");
+    uint16_t* const link = tab;                                   // one step
+    uint16_t* const top = tab + (kPathLevels - 1) * kPathCells;  // kPathHop steps
+    uint16_t* const hops = tab + kPathLevels * kPathCells;       // where each hop starts
+    uint32_t* const chain = reinterpret_cast<uint32_t*>(hops + kMaxHops);  // the chain's outcome, for every wave
     const uint32_t tid = threadIdx.x;
     const uint32_t W = d.width, H = d.height, tc = d.tile_cols;
     const uint32_t ex = tc * 4u, ey = ((H + 3u) / 4u) * 4u;  // the tiled layout's extent
     const uint32_t ww = ex < kPathWin ? ex : kPathWin, wh = ey < kPathWin ? ey : kPathWin;
     const uint32_t sx = d.start_xy & 0xFFFFu, sy = d.start_xy >> 16;
-    const gptr<const uint64_t> val = as_global<const uint64_t>(d.dense);
-    uint32_t slot = (uint32_t)d.goal;  // tiled index
+    // the parent half of each closed-set value (g << 32 | parent, little-endian)
+    const gptr<const uint32_t> par = as_global<const uint32_t>(reinterpret_cast<const uint32_t*>(d.dense));
     uint32_t x = d.goal_xy & 0xFFFFu, y = d.goal_xy >> 16;
     uint32_t n = 0;
-    while (slot != kNone) {
+    for (;;) {
         const uint32_t ox = path_window_origin(x, sx, ww, ex), oy = path_window_origin(y, sy, wh, ey);
-        const uint32_t n0 = n;
-        uint64_t v[kPer];
+        auto coord = [&](uint32_t c) { return (uint64_t)(oy + c / kPathWin) * W + (ox + c % kPathWin); };
+        uint32_t v[kPer];
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {  // all loads in flight together; positions outside read the origin
             const uint32_t k = tid + kPathThreads * (uint32_t)r, t = k >> 4, o = k & 15u;
             const uint32_t wx = (t % (kPathWin / 4)) * 4u + (o & 3u), wy = (t / (kPathWin / 4)) * 4u + (o >> 2);
             const bool in = wx < ww && wy < wh;
-            v[r] = val[grid_index(ox + (in ? wx : 0u), oy + (in ? wy : 0u), tc)];
+            v[r] = par[2 * (size_t)grid_index(ox + (in ? wx : 0u), oy + (in ? wy : 0u), tc)];
         }
-        __syncthreads();  // every thread is done with the previous window
+        __syncthreads();  // every thread is done with the previous window's tables
+        // branch-free: a thread's cells share their offsets inside the tile
+        // (ix, iy) and their column; row r lies 8 r cells below the first
+        const uint32_t ix = tid & 3u, iy = (tid >> 2) & 3u;
+        const uint32_t wx = ((tid >> 4) % (kPathWin / 4)) * 4u + ix;
+        const uint32_t wy0 = (tid >> 4) / (kPathWin / 4) * 4u + iy;
+        static_assert(kPathThreads % (4 * kPathWin) == 0, "every row of a thread's cells: the same column");
+        constexpr uint32_t kRowStep = kPathThreads / kPathWin;  // cells (a multiple of 4) between a thread's rows
+        const uint32_t tile0 = ((oy + wy0) >> 2) * tc + ((ox + wx) >> 2);  // the tile of the thread's first cell
 #pragma unroll
         for (int r = 0; r < kPer; ++r) {
-            const uint32_t k = tid + kPathThreads * (uint32_t)r, t = k >> 4, o = k & 15u;
-            const uint32_t wx = (t % (kPathWin / 4)) * 4u + (o & 3u), wy = (t / (kPathWin / 4)) * 4u + (o >> 2);
-            win[wy * kPathWin + wx] = v[r];
+            const uint32_t wy = wy0 + kRowStep * (uint32_t)r, cx = ox + wx, cy = oy + wy, p = v[r];
+            // the parent's offset per axis, from the offsets inside the tile:
+            // -1, 0 or +1 for a neighbour, -2 for none
+            const int dx = __builtin_amdgcn_sbfe((int)(p - ix), 0, 2), dy = __builtin_amdgcn_sbfe((int)(p - (iy << 2)), 2, 2);
+            const uint32_t px = cx + (uint32_t)dx, py = cy + (uint32_t)dy;
+            // the parent's tile: the cell's, or the one next to it where the step crosses a tile's edge
+            const uint32_t tile = tile0 + (kRowStep / 4 * (uint32_t)r) * tc;
+            const uint32_t ptile = tile + (uint32_t)(((int)ix + dx) >> 2) + (uint32_t)__mul24(((int)iy + dy) >> 2, (int)tc);
+            const bool nb = (dx != -2) & (dy != -2) & (px < W) & (py < H) & ((p >> 4) == ptile);
+            const uint32_t rx = wx + (uint32_t)dx, ry = wy + (uint32_t)dy;
+            uint32_t code = ((rx < ww) & (ry < wh)) ? ry * kPathWin + rx : kLinkOut | (uint32_t)(dx + 3 * dy + 4);
+            code = nb ? code : kLinkFar;
+            code = p == kNone ? kLinkRoot : code;
+            code = ((wx < ww) & (wy < wh)) ? code : kLinkFar;  // positions outside the window: never reached
+            link[wy * kPathWin + wx] = (uint16_t)code;
+        }
+#pragma unroll
+        for (int l = 1; l < kPathLevels; ++l) {  // two links of the level below, where both stay inside
+            const uint16_t* const lo = tab + (l - 1) * kPathCells;
+            __syncthreads();
+#pragma unroll
+            for (int r = 0; r < kPer; ++r) {
+                const uint32_t k = tid + kPathThreads * (uint32_t)r;
+                const uint32_t a = lo[k];
+                const uint32_t b = a < kPathCells ? (uint32_t)lo[a] : kLinkRoot;
+                tab[l * kPathCells + k] = (uint16_t)(b < kPathCells ? b : kLinkRoot);
+            }
         }
         __syncthreads();
-        // the links inside the window; every thread follows the same chain
-        while (slot != kNone && x - ox < ww && y - oy < wh) {
-            if (n >= cap) return kNone;
-            if (tid == 0) out[n] = (uint64_t)y * W + x;
-            ++n;
-            const uint32_t p = val_parent(win[(y - oy) * kPathWin + (x - ox)]);
-            if (p == kNone) {
-                slot = kNone;
-                break;
+        // wave 0 follows the chain (each read is one LDS broadcast, its value
+        // uniform): it records where each hop starts, then takes the last
+        // single steps (fewer than kPathHop links, since no hop of kPathHop
+        // steps starts where they begin) to the chain's end inside the window
+        if (tid < 64) {
+            uint32_t c = (y - oy) * kPathWin + (x - ox), m = 0, nt = 0, code = kLinkFar;
+            for (uint32_t h = (uint32_t)__builtin_amdgcn_readfirstlane((int)top[c]); h < kPathCells;
+                 h = (uint32_t)__builtin_amdgcn_readfirstlane((int)top[c])) {
+                if (m == kMaxHops) break;  // links in a cycle
+                if (tid == 0) hops[m] = (uint16_t)c;
+                ++m;
+                c = h;
             }
-            // x, y of the parent: a neighbour differs from (x, y) by -1, 0 or
-            // +1 per axis, which the offset inside its tile gives (the two low
-            // bits of each coordinate); a link to any other cell is decoded
-            // from the index (one outside the grid then ends the walk below)
-            const uint32_t ddx = ((p & 3u) - (x & 3u)) & 3u, ddy = (((p >> 2) & 3u) - (y & 3u)) & 3u;
-            uint32_t px = x + (ddx == 3u ? ~0u : ddx), py = y + (ddy == 3u ? ~0u : ddy);
-            if (ddx == 2u || ddy == 2u || px >= W || py >= H || grid_index(px, py, tc) != p) {
-                const uint32_t t = p >> 4;
-                px = (t % tc) * 4u + (p & 3u);
-                py = (t / tc) * 4u + ((p >> 2) & 3u);
+            if (m < kMaxHops) {
+                for (; nt < kPathHop; ++nt) {
+                    const uint32_t i = n + kPathHop * m + nt;
+                    if (tid == 0 && i < cap) out[i] = coord(c);
+                    code = (uint32_t)__builtin_amdgcn_readfirstlane((int)link[c]);
+                    if (code >= kPathCells) break;
+                    c = code;
+                }
             }
-            x = px;
-            y = py;
-            slot = p;
+            if (tid == 0) {
+                chain[0] = m < kMaxHops && nt < kPathHop ? m : kNone;
+                chain[1] = nt + 1;
+                chain[2] = c;
+                chain[3] = code;
+            }
         }
-        // every window holds the cell it was loaded around, so each pass takes
-        // a step; one that does not ends the walk with an error, not a loop
-        if (slot != kNone && n == n0) return kNone;
+        __syncthreads();  // the hop starts and the chain's outcome are visible
+        const uint32_t m = chain[0], nt = chain[1], c = chain[2], code = chain[3];
+        if (m == kNone || n + kPathHop * m + nt > cap) return kNone;
+        for (uint32_t j = tid; j < m; j += kPathThreads) {
+            uint32_t a = hops[j];
+#pragma unroll
+            for (uint32_t i = 0; i < kPathHop; ++i) {
+                out[n + kPathHop * j + i] = coord(a);
+                if (i + 1 < kPathHop) a = link[a];
+            }
+        }
+        n += kPathHop * m + nt;
+        if (code == kLinkRoot) return n;  // the start
+        // the walk leaves the window at c: on to its parent
+        const uint32_t cx = ox + c % kPathWin, cy = oy + c / kPathWin;
+        if (code == kLinkFar) {  // decoded from the index; one outside the grid ends the walk with an error
+            const uint32_t p = par[2 * (size_t)grid_index(cx, cy, tc)], t = p >> 4;
+            x = (t % tc) * 4u + (p & 3u);
+            y = (t / tc) * 4u + ((p >> 2) & 3u);
+            if (x >= W || y >= H) return kNone;
+        } else {
+            const uint32_t dir = code & 15u;
+            x = cx + dir % 3u - 1u;
+            y = cy + dir / 3u - 1u;
+        }
     }
-    return n;
 }
 
 }  // namespace gastar

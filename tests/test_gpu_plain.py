@@ -116,10 +116,11 @@ def test_grid_unreachable(engine):
 
 
 @pytest.mark.parametrize("w,h,conn", [(1, 64, 4), (2, 50, 4), (2, 50, 8), (64, 1, 4), (3, 40, 8), (33, 2, 4),
-                                        (40, 33, 8)])
+                                        (40, 33, 8), (128, 3, 4), (129, 5, 8), (2, 300, 8), (131, 130, 4),
+                                        (256, 129, 8)])
 def test_grid_thin_and_window_edges(engine, w, h, conn):
     # widths and heights below, at and just above the path walk's window
-    # limits (rows of 1 or 2 cells, the 32-cell window): cost, then replay
+    # limits (rows of 1 or 2 cells, the 128-cell window): cost, then replay
     rng = np.random.default_rng(1000 * w + h + conn)
     cells = rng.integers(1, 10, size=w * h).astype(np.uint8)
     check_grid(engine, cells, w, h, conn=conn, num_queues=64)
@@ -128,6 +129,21 @@ def test_grid_thin_and_window_edges(engine, w, h, conn):
     ref = oracle.dial_grid(cells, w, h, w * h - 1, 0, conn)
     assert sol.status == ga.OK and sol.cost == ref
     assert oracle.replay_grid(cells, w, h, conn, sol.path, w * h - 1, 0) == sol.cost
+
+
+@pytest.mark.parametrize("conn", [4, 8])
+def test_grid_serpentine_walk(engine, conn):
+    # rows joined at alternating ends: the path runs away from the start along
+    # x on every other row, so the walk leaves its windows on every side (and
+    # diagonally when 8-connected), about 20,000 cells long
+    w, h = 300, 263
+    rng = np.random.default_rng(7 + conn)
+    cells = rng.integers(1, 10, size=(h, w)).astype(np.uint8)
+    for i, r in enumerate(range(3, h, 4)):
+        cells[r, :] = 0
+        cells[r, w - 1 if i % 2 == 0 else 0] = 1  # the gap: right end, then left end
+    sol = check_grid(engine, cells.reshape(-1), w, h, conn=conn, num_queues=1024)
+    assert len(sol.path) > 19_000
 
 
 def test_grid_1024(engine):
