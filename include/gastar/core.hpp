@@ -1710,30 +1710,22 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
     // -- 3..6. expansion rounds
     if (mine) {
         const uint32_t rounds = d.rounds ? d.rounds : prm.rounds;
-        if constexpr (P == 2) {
-            // Paired pops (grids, STEP_PAIRED_POPS): each round pops the heap's
-            // next two nodes (a stale one is dropped, not replaced) and issues
-            // the closed-set atomics of both nodes' successors before it reads
-            // any of their results, so every lane has two independent atomics
-            // in flight where a round of single pops has one.  The second
-            // node's successors go into the heap; in the last round the first
-            // node's go to the candidate buffer.  Exactness is unaffected: any
-            // order of expansions is, with re-opening and the termination test.
-            static_assert(Ops::kDense, "paired pops are for the dense grid sets");
-            uint32_t par2 = kNone;
-            uint32_t pre2[KW], pre2_g[KW];
-            auto load2 = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& gs) { Ops::load(d, t.slot, k, gs, par2); };
-            auto prefetch2 = [&](const HeapEntry& t) {
-#pragma unroll
-                for (int r = 0; r < KW; ++r) {
-                    const int i = r * W + lane;
-                    pre2[r] = i < K ? Ops::prefetch(d, t, i) : 0u;
-                    pre2_g[r] = (skip_settled && i < K) ? Ops::prefetch_g(d, t, i) : kInf;
-                }
-            };
-            HeapEntry rec2[KW];
+        if constexpr (P >= 2) {
+            // Several pops per round (grids, STEP_PAIRED_POPS): each round pops
+            // the heap's next P nodes (a stale one is dropped, not replaced)
+            // and issues the closed-set atomics of all of their successors
+            // before it reads any of their results, so every lane has P
+            // independent atomics in flight where a round of single pops has
+            // one.  The later nodes' successors go into the heap; in the last
+            // round the first node's go to the candidate buffer.  Exactness is
+            // unaffected: any order of expansions is, with re-opening and the
+            // termination test.
+            static_assert(Ops::kDense, "several pops per round are for the dense grid sets");
+            constexpr int Q = P - 1;  // the nodes after the first
+            uint32_t parq[Q], preq[Q][KW], preq_g[Q][KW];
+            HeapEntry recq[Q][KW];
             for (uint32_t round = 0; round < rounds && !gated; ++round) {
-                bool have1 = popped, have2 = false;
+                bool have1 = popped;
                 popped = false;
                 if (!have1 && S > 0) {
                     uint32_t lv = 0, hl = 0;
@@ -1746,87 +1738,140 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                         tl.prof_add(TL_POPS, have1 ? 1u : 0u);
                     }
                 }
-                HeapEntry top2;
-                top2.prio = 0;
-                top2.slot = 0;
-                top2.aux = 0;
-                typename Ops::Key key2{};
-                uint32_t g2s = 0;
-                if (!gated && S > 0) {
-                    uint32_t lv = 0, hl = 0;
-                    have2 = heap_pop<W>(H, S, lane, lv, hl, load2, prefetch2, key2, g2s, gate, top2);
-                    gated = !have2;
-                    group_fence();
-                    if (lane == 0) {
-                        tl.prof_add(TL_SIFT_DOWN, lv);
-                        tl.prof_add(TL_SIFT_DOWN_HBM, hl);
-                        tl.prof_add(TL_POPS, have2 ? 1u : 0u);
+                HeapEntry topq[Q];
+                typename Ops::Key keyq[Q];
+                uint32_t gq[Q];
+                bool haveq[Q], any = have1;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    topq[q].prio = 0;
+                    topq[q].slot = 0;
+                    topq[q].aux = 0;
+                    keyq[q] = typename Ops::Key{};
+                    gq[q] = 0;
+                    haveq[q] = false;
+                    if (!gated && S > 0) {
+                        auto loadq = [&](const HeapEntry& t, typename Ops::Key& k, uint32_t& gs) {
+                            Ops::load(d, t.slot, k, gs, parq[q]);
+                        };
+                        auto prefetchq = [&](const HeapEntry& t) {
+#pragma unroll
+                            for (int r = 0; r < KW; ++r) {
+                                const int i = r * W + lane;
+                                preq[q][r] = i < K ? Ops::prefetch(d, t, i) : 0u;
+                                preq_g[q][r] = (skip_settled && i < K) ? Ops::prefetch_g(d, t, i) : kInf;
+                            }
+                        };
+                        uint32_t lv = 0, hl = 0;
+                        haveq[q] = heap_pop<W>(H, S, lane, lv, hl, loadq, prefetchq, keyq[q], gq[q], gate, topq[q]);
+                        gated = !haveq[q];
+                        group_fence();
+                        if (lane == 0) {
+                            tl.prof_add(TL_SIFT_DOWN, lv);
+                            tl.prof_add(TL_SIFT_DOWN_HBM, hl);
+                            tl.prof_add(TL_POPS, haveq[q] ? 1u : 0u);
+                        }
                     }
+                    any = any || haveq[q];
                 }
-                if (!have1 && !have2) break;
+                if (!any) break;
                 // stale entries (g above the stored one) are dropped; the goal
                 // is tested at pop (a_star.h:80-83); a node with f >= best
                 // cannot improve it
                 const uint32_t g0a = prio_g(top.prio), fa = prio_f(top.prio);
-                const uint32_t g0b = prio_g(top2.prio), fb = prio_f(top2.prio);
+                uint32_t g0q[Q], fq[Q];
                 have1 = have1 && g0a <= top_g;
-                have2 = have2 && g0b <= g2s;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    g0q[q] = prio_g(topq[q].prio);
+                    fq[q] = prio_f(topq[q].prio);
+                    haveq[q] = haveq[q] && g0q[q] <= gq[q];
+                }
                 if (have1 && Ops::is_goal(d, top_key)) {
                     if (lane == 0) atomicMin(&st->best, g0a);
                     best = g0a < best ? g0a : best;
                     have1 = false;
                 }
-                if (have2 && Ops::is_goal(d, key2)) {
-                    if (lane == 0) atomicMin(&st->best, g0b);
-                    best = g0b < best ? g0b : best;
-                    have2 = false;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    if (haveq[q] && Ops::is_goal(d, keyq[q])) {
+                        if (lane == 0) atomicMin(&st->best, g0q[q]);
+                        best = g0q[q] < best ? g0q[q] : best;
+                        haveq[q] = false;
+                    }
                 }
                 have1 = have1 && (best == kInf || fa < best);
-                have2 = have2 && (best == kInf || fb < best);
-                n_expanded += lane == 0 ? (have1 ? 1u : 0u) + (have2 ? 1u : 0u) : 0u;
-                // both nodes' successors; their atomics issued together, their results read after
+                uint32_t nq = 0;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    haveq[q] = haveq[q] && (best == kInf || fq[q] < best);
+                    nq += haveq[q] ? 1u : 0u;
+                }
+                n_expanded += lane == 0 ? (have1 ? 1u : 0u) + nq : 0u;
+                // every node's successors; their atomics issued together, their results read after
 #pragma unroll
                 for (int r = 0; r < KW; ++r) {
                     const int i = r * W + lane;
                     rec[r].prio = kNoRecord;
-                    rec2[r].prio = kNoRecord;
-                    typename Ops::Key sa, sb;
-                    uint32_t aaux = 0, acost = 0, ath = 0, baux = 0, bcost = 0, bth = 0;
+                    typename Ops::Key sa, sq[Q];
+                    uint32_t aaux = 0, acost = 0, ath = 0, qaux[Q], qcost[Q], qth[Q];
                     const uint32_t wa = i < K ? Ops::drop_parent(d, top.aux, i, pre[r], top_par) : 0u;
-                    const uint32_t wb = i < K ? Ops::drop_parent(d, top2.aux, i, pre2[r], par2) : 0u;
                     const bool gena = have1 && i < K && Ops::succ(d, top_key, top.aux, fa - g0a, i, wa, sa, aaux, acost, ath);
-                    const bool genb = have2 && i < K && Ops::succ(d, key2, top2.aux, fb - g0b, i, wb, sb, baux, bcost, bth);
-                    n_generated += (gena ? 1u : 0u) + (genb ? 1u : 0u);
-                    const uint32_t ga = g0a + acost, gb = g0b + bcost;
+                    const uint32_t ga = g0a + acost;
                     const bool ua = gena && ga + ath < best && pre_g[r] > ga;
-                    const bool ub = genb && gb + bth < best && pre2_g[r] > gb;
-                    uint64_t olda = ~0ull, oldb = ~0ull;
+                    n_generated += gena ? 1u : 0u;
+                    uint32_t gqs[Q];
+                    bool uq[Q];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        recq[q][r].prio = kNoRecord;
+                        qaux[q] = qcost[q] = qth[q] = 0;
+                        const uint32_t wq = i < K ? Ops::drop_parent(d, topq[q].aux, i, preq[q][r], parq[q]) : 0u;
+                        const bool genq = haveq[q] && i < K &&
+                                          Ops::succ(d, keyq[q], topq[q].aux, fq[q] - g0q[q], i, wq, sq[q], qaux[q], qcost[q], qth[q]);
+                        gqs[q] = g0q[q] + qcost[q];
+                        uq[q] = genq && gqs[q] + qth[q] < best && preq_g[q][r] > gqs[q];
+                        n_generated += genq ? 1u : 0u;
+                    }
+                    uint64_t olda = ~0ull, oldq[Q];
                     if (ua)
                         olda = __hip_atomic_fetch_min(Ops::val_ptr(d, (uint32_t)sa), ((uint64_t)ga << 32) | top.slot,
                                                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    if (ub)
-                        oldb = __hip_atomic_fetch_min(Ops::val_ptr(d, (uint32_t)sb), ((uint64_t)gb << 32) | top2.slot,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    const bool ia = ua && ga < val_g(olda), ib = ub && gb < val_g(oldb);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        oldq[q] = ~0ull;
+                        if (uq[q])
+                            oldq[q] = __hip_atomic_fetch_min(Ops::val_ptr(d, (uint32_t)sq[q]),
+                                                             ((uint64_t)gqs[q] << 32) | topq[q].slot, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+                    }
+                    const bool ia = ua && ga < val_g(olda);
                     if (ia) {
                         rec[r].prio = make_prio(ga + ath, ga);
                         rec[r].slot = (uint32_t)sa;
                         rec[r].aux = aaux;
                     }
-                    if (ib) {
-                        rec2[r].prio = make_prio(gb + bth, gb);
-                        rec2[r].slot = (uint32_t)sb;
-                        rec2[r].aux = baux;
-                    }
                     tl.prof_count(TL_PROBES, ua);
-                    tl.prof_count(TL_PROBES, ub);
                     tl.prof_count(TL_LOOKUPS, ua);
-                    tl.prof_count(TL_LOOKUPS, ub);
                     tl.prof_count(TL_INSERTED, ia);
-                    tl.prof_count(TL_INSERTED, ib);
+#pragma unroll
+                    for (int q = 0; q < Q; ++q) {
+                        const bool iq = uq[q] && gqs[q] < val_g(oldq[q]);
+                        if (iq) {
+                            recq[q][r].prio = make_prio(gqs[q] + qth[q], gqs[q]);
+                            recq[q][r].slot = (uint32_t)sq[q];
+                            recq[q][r].aux = qaux[q];
+                        }
+                        tl.prof_count(TL_PROBES, uq[q]);
+                        tl.prof_count(TL_LOOKUPS, uq[q]);
+                        tl.prof_count(TL_INSERTED, iq);
+                    }
                 }
-                push_records<W, KW>(H, S, rec2, K, lane, best, d.heap_cap, overflow_heap, tl, false);
-                group_fence();
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    push_records<W, KW>(H, S, recq[q], K, lane, best, d.heap_cap, overflow_heap, tl, false);
+                    group_fence();
+                }
                 if (round + 1 == rounds) break;  // last round: the first node's records go to the candidate buffer
                 push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl, false);
 #pragma unroll
