@@ -257,7 +257,8 @@ def lane_roofline(st, cfg, elapsed_s):
         return None
     dom = max(lanes, key=lambda x: x["kernel_ms_est"])
     kernel = {0: "step_kernel<4, all kinds> (lane 0: 15-puzzles and grids below deep_cells)",
-              1: "step_kernel<4, 4-connected grids> (lane 1: grids of at least deep_cells cells)"}
+              1: "step_kernel<4, 4-connected grids" + (", two pops per round" if cfg.step_flags & 4 else "")
+                 + "> (lane 1: grids of at least deep_cells cells)"}
     rf = {"bound": "hbm", "achieved": dom["achieved_GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
           "frac": round(dom["achieved_GBps"] / HBM_PEAK_GBS, 5), "traffic": None,
           "kernel": kernel.get(dom["lane"], f"step kernel of lane {dom['lane']}"), "lane": dom["lane"],

@@ -127,9 +127,12 @@ def bench_line(path: str) -> dict:
 
 
 FLAGS = "--skip-batch --skip-p24 --plain-sample 0 --dropin-sample 0 --skip-cpu --latency-load 0"
+# each lane's step kernel: the name prefix that finds it in rocprofv3's output, and the full instantiation
+# (arity 4, the kinds it runs, 256 threads, pops per round: 2 on the grid lane, step_flags 4)
 LANE_KERNELS = {0: "step_kernel<4, 30u", 1: "step_kernel<4, 8u"}
-LANE_NAMES = {0: "lane 0 (15-puzzles, grids below 2 M cells): `step_kernel<4, 30u, 256>`",
-              1: "lane 1 (grids of at least 2 M cells): `step_kernel<4, 8u, 256>`"}
+LANE_FULL = {0: "step_kernel<4, 30u, 256, 1>", 1: "step_kernel<4, 8u, 256, 2>"}
+LANE_NAMES = {0: f"lane 0 (15-puzzles, grids below 2 M cells): `{LANE_FULL[0]}`",
+              1: f"lane 1 (grids of at least 2 M cells, two pops per round): `{LANE_FULL[1]}`"}
 WORKLOAD = ("the serving stream (seed 5, 4000 QPS offered): one warm-up segment of 512 requests, then 1024 timed "
             "requests on two launch lanes")
 
@@ -221,14 +224,14 @@ def write_summaries(out: str, profiles: str) -> dict:
                "| lane | fetched MB | algorithmic MB (fetch run) | fetch ratio | written MB | algorithmic MB (write run) "
                "| write ratio | HBM bytes per algorithmic byte |", "|---|---|---|---|---|---|---|---|"] + rows + [
                "", "The excess over the algorithmic bytes is whole lines moved for 8- and 16-byte scattered accesses: "
-               "each grid expansion touches the popped cell's line, its neighbours' weight and closed-set lines (the "
-               "rows above and below are other lines) and heap entries; the LDS copy of each working heap's top "
-               "levels is read and written back every launch; idle heaps clear their candidate slices. None of that "
-               "is counted as algorithmic.", ""]
+               "each grid expansion touches the popped cell's closed-set line, its neighbours' weight and closed-set "
+               "lines (one line per 4 x 4 tile of cells: a neighbour across the tile's edge is another line) and heap "
+               "entries; the LDS copy of each working heap's top levels is read and written back every launch; idle "
+               "heaps clear their candidate slices. None of that is counted as algorithmic.", ""]
     with open(os.path.join(profiles, "step_kernel_serving_stream_hbm_traffic.md"), "w") as f:
         f.write("\n".join(traffic))
 
-    rec = {"kernels": {str(l): LANE_KERNELS[l] + ", 256>" for l in lanes}, "workload": WORKLOAD,
+    rec = {"kernels": {str(l): LANE_FULL[l] for l in lanes}, "workload": WORKLOAD,
            "command": f"rocprofv3 --pmc FETCH_SIZE|WRITE_SIZE|TCC_HIT TCC_MISS -f csv -- python3 bench.py {FLAGS}",
            "fetch_scale": fs, "write_scale": ws, "hbm_bytes_per_algo_byte": ratio}
     if os.path.isdir(J("prof_l2")):
