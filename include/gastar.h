@@ -129,7 +129,9 @@ typedef struct ga_config {
     uint32_t step_flags;     /* step-kernel variants (each keeps results exact): 1 = write back only
                                 the LDS heap-top entries a launch changed; 2 = grids read each entered
                                 cell's stored g with its weight and skip the atomicMin that could not
-                                lower it */
+                                lower it; 4 = a launch lane that runs only 4-connected grids pops two
+                                nodes per round and issues both nodes' atomicMins before reading either
+                                result (its own kernel instantiation) */
     uint32_t persist;        /* ga_solve_batch only: 1 = solve the batch's 8- and 15-puzzles and its
                                 grids of at most persist_cells cells in one launch per wave, one
                                 workgroup per instance (persist_block / heap_arity heaps) running
