@@ -72,14 +72,28 @@ def pct(xs, p):
 
 
 def cpu_model() -> str:
+    """The host CPU as /proc/cpuinfo names it (a virtual machine may give only the vendor's line), with its
+    family / model numbers, clock, vector width and the cores this process may use."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break  # the first processor's block
+                k, _, v = line.partition(":")
+                info[k.strip()] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    name = info.get("model name", "unknown")
+    extra = []
+    if "cpu family" in info and "model" in info:
+        extra.append(f"family {info['cpu family']} model {info['model']}")
+    if "cpu MHz" in info:
+        extra.append(f"{float(info['cpu MHz']) / 1000:.2f} GHz")
+    flags = info.get("flags", "").split()
+    extra.append("AVX-512" if "avx512f" in flags else "AVX2" if "avx2" in flags else "no AVX2")
+    extra.append(f"{len(host_cores(1 << 20))} of {os.cpu_count()} logical CPUs usable")
+    return f"{name} ({', '.join(extra)})"
 
 
 def host_cores(limit: int):
