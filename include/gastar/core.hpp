@@ -1867,16 +1867,27 @@ __device__ __forceinline__ void run_group(const InstDesc& d, InstState* st, uint
                         tl.prof_count(TL_INSERTED, iq);
                     }
                 }
+                // every node's records into the heap in one push (one round
+                // trip for the parents' checks), except that in the last round
+                // the first node's go to the candidate buffer; records past K
+                // in a register block are "none" (set above)
+                const bool last = round + 1 == rounds;
+                HeapEntry all[(Q + 1) * KW];
 #pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    push_records<W, KW>(H, S, recq[q], K, lane, best, d.heap_cap, overflow_heap, tl, false);
-                    group_fence();
+                for (int q = 0; q < Q; ++q)
+#pragma unroll
+                    for (int r = 0; r < KW; ++r) all[q * KW + r] = recq[q][r];
+#pragma unroll
+                for (int r = 0; r < KW; ++r) {
+                    all[Q * KW + r] = rec[r];
+                    if (last) all[Q * KW + r].prio = kNoRecord;
                 }
-                if (round + 1 == rounds) break;  // last round: the first node's records go to the candidate buffer
-                push_records<W, KW>(H, S, rec, K, lane, best, d.heap_cap, overflow_heap, tl, false);
+                push_records<W, (Q + 1) * KW>(H, S, all, (Q + 1) * KW * W, lane, best, d.heap_cap, overflow_heap, tl,
+                                              false);
+                group_fence();
+                if (last) break;
 #pragma unroll
                 for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
-                group_fence();
             }
         } else
         for (uint32_t round = 0; round < rounds && !gated; ++round) {
