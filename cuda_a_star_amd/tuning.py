@@ -124,26 +124,43 @@ def _measure_served(insts, cfg: TuneConfig) -> float:
     return len(sols) / (ss.wall_ns / 1e9)
 
 
-def _measure_plain(insts, cfg: TuneConfig) -> float:
+def _measure_plain(insts, cfg: TuneConfig) -> Dict[str, float]:
+    """Solves per second one at a time: over the whole sample ("value", wall
+    time) and per kind ("puzzle", "grid": requests of the kind over the sum of
+    their search times), so that a parameter of one kind is judged on that
+    kind's solves, not drowned in the other's."""
     import cuda_a_star_amd as ga
 
+    zero = {"value": 0.0, "puzzle": 0.0, "grid": 0.0}
     with ga.Engine(0, budget_bytes=200 << 30, chunk_bytes=8 << 30, **cfg.engine) as e:
         dev = {i: e.upload(x.cells) for i, x in enumerate(insts) if x.kind == "grid"}
+        search = {"puzzle": [0, 0.0], "grid": [0, 0.0]}
         t = time.perf_counter()
         for i, x in enumerate(insts):
             s = e.solve(x.request(cells_device=dev.get(i), num_queues=cfg.heaps(x.kind, x.side)))
             if s.status != ga.OK:
-                return 0.0
+                return zero
+            search[x.kind][0] += 1
+            search[x.kind][1] += s.search_ns / 1e9
             if i % 10 == 9:  # a line every ten requests: a long point is visibly alive
                 print(f"[tune] plain {i + 1}/{len(insts)} in {time.perf_counter() - t:.1f}s", file=sys.stderr,
                       flush=True)
         dt = time.perf_counter() - t
         for a in dev.values():
             e.free_device(a)
-    return len(insts) / dt
+    out = {"value": len(insts) / dt}
+    for k, (n, sec) in search.items():
+        out[k] = n / sec if sec > 0 else 0.0
+    return out
 
 
-def _measure(mode: str, n: int, seed: int, qps: float, cfg: TuneConfig) -> float:
+# the kind whose solves judge a parameter in the plain sweep (others: the whole sample)
+PLAIN_AXIS_KIND = {"puzzle_heaps": "puzzle", "slack_puzzle": "puzzle", "rounds": "puzzle", "max_load_pct": "puzzle",
+                   "grid_heaps_small": "grid", "grid_heaps_mid": "grid", "grid_heaps_big": "grid",
+                   "slack_grid": "grid", "rounds_grid": "grid", "step_flags": "grid"}
+
+
+def _measure(mode: str, n: int, seed: int, qps: float, cfg: TuneConfig) -> Dict[str, float]:
     """One candidate, measured in a process of its own: engines created one
     after another in one process run slower (their streams share the process's
     few hardware queues with the earlier engines'), which would bias a sweep
@@ -155,14 +172,14 @@ def _measure(mode: str, n: int, seed: int, qps: float, cfg: TuneConfig) -> float
                        cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     if r.returncode != 0:
         print(f"[tune] measurement failed ({r.returncode}): {r.stderr[-400:]}", flush=True)
-        return 0.0
-    return float(json.loads(r.stdout.strip().splitlines()[-1])["value"])
+        return {"value": 0.0, "puzzle": 0.0, "grid": 0.0}
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
-def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
+def sweep(mode: str, n: int, seed: int, qps: float, passes: int = 2) -> TuneConfig:
     best = load(mode)
-    best_v = _measure(mode, n, seed, qps, best)
-    print(f"[tune] {mode} start {best_v:.2f}/s {asdict(best)}", flush=True)
+    best_r = _measure(mode, n, seed, qps, best)
+    print(f"[tune] {mode} start {best_r} {asdict(best)}", flush=True)
     if mode == "served":
         axes = {
             "puzzle_heaps": [256, 512, 1024, 2048, 4096, 8192, 16384],
@@ -188,34 +205,37 @@ def sweep(mode: str, n: int, seed: int, qps: float) -> TuneConfig:
     else:
         axes = {
             "puzzle_heaps": [1024, 2048, 4096, 8192, 16384],
+            "slack_puzzle": [0, 1, 3, 5],
+            "rounds": [1, 2, 4],
             "grid_heaps_small": [1024, 2048, 4096, 8192],
             "grid_heaps_mid": [1024, 2048, 4096, 8192],
             "grid_heaps_big": [2048, 4096, 8192, 16384],
+            "slack_grid": [0, 8, 16, 32, 48],
+            "rounds_grid": [0, 1, 2, 3, 4],
+            "step_flags": [0, 4],
+            "chunk_iters": [4, 8, 16, 32, 64],
+            "use_graph": [0, 1],
             "heap_arity": [4, 8],
-            "chunk_iters": [4, 8, 16, 32],
             "pops_per_iter": [1, 2, 4],
             "max_load_pct": [50, 70],
-            "rounds": [1, 2, 4],
-            "use_graph": [0, 1],
-            "adaptive_start": [0, 64, 256, 1024],
-            "grow_thresh": [8, 32, 128],
-            "rounds_grid": [0, 1, 4, 8],
-            "slack_puzzle": [0, 1, 3, 5, 9],
-            "slack_grid": [0, 1, 2, 4, 8, 16],
             "block": [128, 256, 512],
         }
-    # coordinate descent: one axis at a time, two passes
-    for _ in range(2):
+    # coordinate descent: one axis at a time; a plain parameter of one kind is
+    # judged on that kind's solves per second of search time
+    for _ in range(passes):
         for name, values in axes.items():
+            key = PLAIN_AXIS_KIND.get(name, "value") if mode == "plain" else "value"
+            if name == "rounds" and not best.rounds_grid:
+                key = "value" if mode == "plain" else key  # rounds_grid 0: the grids take `rounds` too
             for v in values:
                 if getattr(best, name) == v:
                     continue
                 cand = TuneConfig(**{**asdict(best), name: v})
-                val = _measure(mode, n, seed, qps, cand)
-                print(f"[tune] {mode} {name}={v}: {val:.2f}/s (best {best_v:.2f})", flush=True)
-                if val > best_v * 1.02:
-                    best, best_v = cand, val
-    print(f"[tune] {mode} best {best_v:.2f}/s {asdict(best)}", flush=True)
+                r = _measure(mode, n, seed, qps, cand)
+                print(f"[tune] {mode} {name}={v}: {r[key]:.2f}/s {key} (best {best_r[key]:.2f})", flush=True)
+                if r[key] > best_r[key] * 1.02:
+                    best, best_r = cand, r
+    print(f"[tune] {mode} best {best_r} {asdict(best)}", flush=True)
     return best
 
 
@@ -226,6 +246,7 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=77)
     ap.add_argument("--qps", type=float, default=1500.0, help="offered load of the served sample")
     ap.add_argument("--out", default=TUNING_FILE, help="where to write the winners")
+    ap.add_argument("--passes", type=int, default=2, help="coordinate-descent passes over the axes")
     ap.add_argument("--measure", choices=["served", "plain"], help="(internal) measure --config once and print it")
     ap.add_argument("--config", help="(internal) the TuneConfig to measure, as JSON")
     args = ap.parse_args(argv)
@@ -234,8 +255,10 @@ def main(argv=None) -> int:
 
         insts = wl.config5(args.seed, args.n, args.qps, threads=16)
         cfg = TuneConfig(**json.loads(args.config))
-        fn = _measure_served if args.measure == "served" else _measure_plain
-        print(json.dumps({"value": fn(insts, cfg)}), flush=True)
+        if args.measure == "served":
+            print(json.dumps({"value": _measure_served(insts, cfg)}), flush=True)
+        else:
+            print(json.dumps(_measure_plain(insts, cfg)), flush=True)
         return 0
     data = {}
     if os.path.exists(TUNING_FILE):
@@ -243,8 +266,8 @@ def main(argv=None) -> int:
             data = json.load(f)
     modes = ["served", "plain"] if args.mode == "both" else [args.mode]
     for m in modes:
-        n = args.n if m == "served" else max(8, args.n // 8)
-        data[m] = asdict(sweep(m, n, args.seed, args.qps))
+        n = args.n if m == "served" else max(8, args.n * 3 // 8)
+        data[m] = asdict(sweep(m, n, args.seed, args.qps, args.passes))
         with open(args.out, "w") as f:
             json.dump(data, f, indent=2, sort_keys=True)
     return 0
