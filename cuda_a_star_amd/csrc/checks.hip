@@ -187,9 +187,10 @@ int run_closed_set(const uint32_t* keys, const uint32_t* g, const uint32_t* pare
         !dslot.alloc(n) || !dout.alloc(n))
         return GA_E_NOMEM;
     const bool wide = std::is_same<Set, Wide12Set>::value;
-    if (wide && !side.alloc(slots * sizeof(State12))) return GA_E_NOMEM;
+    constexpr size_t side_bytes = sizeof(Wide12Set::Cells);  // per slot: each key word in a marked 64-bit cell
+    if (wide && !side.alloc(slots * side_bytes)) return GA_E_NOMEM;
     int rc = hip_rc(hipMemset(table.p, 0xFF, slots * slot_bytes));
-    if (rc == GA_OK && wide) rc = hip_rc(hipMemset(side.p, 0, slots * sizeof(State12)));
+    if (rc == GA_OK && wide) rc = hip_rc(hipMemset(side.p, 0xFF, slots * side_bytes));
     if (rc == GA_OK && n) rc = hip_rc(hipMemcpy(dkeys.p, keys, (size_t)n * kw * 4, hipMemcpyHostToDevice));
     if (rc == GA_OK && n) rc = hip_rc(hipMemcpy(dg.p, g, (size_t)n * 4, hipMemcpyHostToDevice));
     if (rc == GA_OK && n) rc = hip_rc(hipMemcpy(dpar.p, parent, (size_t)n * 4, hipMemcpyHostToDevice));
@@ -212,10 +213,10 @@ int run_closed_set(const uint32_t* keys, const uint32_t* g, const uint32_t* pare
     }
     if (!tag_out && !key_out && !val_out) return GA_OK;
     std::vector<uint8_t> host(slots * slot_bytes);
-    std::vector<State12> host_side(wide ? slots : 0);
+    std::vector<Wide12Set::Cells> host_side(wide ? slots : 0);
     rc = hip_rc(hipMemcpy(host.data(), table.p, host.size(), hipMemcpyDeviceToHost));
     if (rc == GA_OK && wide)
-        rc = hip_rc(hipMemcpy(host_side.data(), side.p, slots * sizeof(State12), hipMemcpyDeviceToHost));
+        rc = hip_rc(hipMemcpy(host_side.data(), side.p, slots * side_bytes, hipMemcpyDeviceToHost));
     if (rc != GA_OK) return rc;
     for (size_t s = 0; s < slots; ++s) {
         const uint8_t* b = host.data() + s * slot_bytes;
@@ -227,7 +228,8 @@ int run_closed_set(const uint32_t* keys, const uint32_t* g, const uint32_t* pare
             std::memcpy(&val, b + 16, 8);
         } else {
             std::memcpy(&val, b + 8, 8);
-            if (wide) std::memcpy(words, host_side[s].w, 12);
+            if (wide)
+                for (int w = 0; w < 3; ++w) words[w] = (uint32_t)host_side[s].c[w];
             else std::memcpy(words, b, 8);
         }
         if (tag_out) tag_out[s] = tag;

@@ -158,7 +158,11 @@ std::pair<std::vector<T>, std::chrono::nanoseconds> solve(const T& start, const 
         d.table = (Slot*)b.get(sizeof(Slot) * slots);
         d.table_mask = slots - 1;
         d.load_limit = slots / 2;
-        if (sizeof(T) > 8) d.keys = b.get(sizeof(T) * slots);
+        if constexpr (sizeof(T) > 8) {  // the side array of key words, unwritten (all ones)
+            const size_t side = sizeof(typename WideKeySet<T>::Cells) * slots;
+            d.keys = b.get(side);
+            check(hipMemset(d.keys, 0xFF, side), "hipMemset");
+        }
         const RawKey<T> ends[2] = {to_raw(start), to_raw(target)};
         void* user = b.get(sizeof(ends));
         check(hipMemcpy(user, ends, sizeof(ends), hipMemcpyHostToDevice), "hipMemcpy");

@@ -802,33 +802,57 @@ __host__ __device__ inline uint64_t hash_words(const RawKey<T>& k) {
 }
 
 // States wider than 8 bytes: Slot.key holds a tag (empty / busy / a 62-bit
-// fingerprint) and d.keys[slot] the state.  A lane that claims an empty slot
-// writes the state, fences (release: an L2 write-back on this part, whose
-// XCDs each have their own L2; a reader that matches the fingerprint fences
-// with acquire, an L2 invalidate), then publishes the fingerprint in the same loop
-// iteration; lanes that meet a busy slot look again on their next iteration,
-// so no lane waits inside an inner loop on another lane of its own wave.
-// The loop is bounded; running out counts as a closed-set overflow.
+// fingerprint) and d.keys[slot] the state, each 32-bit word in a 64-bit cell
+// that holds (1 << 32) | word once written (the side array is filled with
+// ones).  A lane that claims an empty slot stores the words, then publishes the
+// fingerprint, in the same loop iteration; a lane that meets a busy slot, or
+// its fingerprint with a word still unwritten, looks again on its next
+// iteration, so no lane waits inside an inner loop on another lane of its own
+// wave.  Every word is read and written whole with an agent-scope atomic, so
+// no fence orders the words and the tag (an agent-scope release and acquire
+// are an L2 write-back and invalidate on this part, whose XCDs each have their
+// own L2).  The loop is bounded; running out counts as a closed-set overflow.
 template <typename T>
 struct WideKeySet {
     using Key = RawKey<T>;
     static constexpr bool kDense = false;
+    static constexpr size_t kWords = sizeof(T) / 4;
+    struct Cells {
+        uint64_t c[kWords];
+    };
+    static constexpr uint64_t kUnwritten = ~0ull;  // a cell before its word is stored
     // successor i's prefetched value, or 0 (no successor) when it is the
     // popped node's stored parent; the sets without parents keep every move
     __device__ static inline uint32_t drop_parent(const InstDesc&, uint32_t, int, uint32_t w, uint32_t) { return w; }
     static constexpr uint32_t kMaxIters = 1u << 20;
 
-    __device__ static inline gptr<Key> keys(const InstDesc& d) { return as_global(reinterpret_cast<Key*>(d.keys)); }
+    __device__ static inline gptr<uint64_t> cells(const InstDesc& d, uint64_t slot) {
+        return as_global(reinterpret_cast<uint64_t*>(d.keys)) + slot * kWords;
+    }
+    // the key in a slot whose fingerprint was seen: 1 (it is `key`), 0 (another
+    // key), -1 (a word not yet visible: look again)
+    __device__ static inline int match(const InstDesc& d, uint64_t slot, const Key& key) {
+        const gptr<uint64_t> c = cells(d, slot);
+        uint64_t v[kWords];
+        for (size_t i = 0; i < kWords; ++i) v[i] = __hip_atomic_load(c + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        bool unwritten = false, same_words = true;
+        for (size_t i = 0; i < kWords; ++i) {
+            unwritten |= v[i] == kUnwritten;
+            same_words &= (uint32_t)v[i] == key.w[i];
+        }
+        return unwritten ? -1 : same_words ? 1 : 0;
+    }
+    // the key of a slot that entered a heap: its words were complete when the
+    // lane that pushed it wrote or matched them (a later launch, or that lane)
     __device__ static inline Key load_key(const InstDesc& d, uint32_t slot) {
-        // agent-scope loads: the key may have been written earlier in this launch
+        const gptr<uint64_t> c = cells(d, slot);
         Key k;
-        const gptr<const uint32_t> src = keys(d)[slot].w;
-        for (size_t i = 0; i < sizeof(T) / 4; ++i)
-            k.w[i] = __hip_atomic_load(src + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (size_t i = 0; i < kWords; ++i)
+            k.w[i] = (uint32_t)__hip_atomic_load(c + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return k;
     }
     __device__ static inline bool same(const Key& a, const Key& b) {
-        for (size_t i = 0; i < sizeof(T) / 4; ++i)
+        for (size_t i = 0; i < kWords; ++i)
             if (a.w[i] != b.w[i]) return false;
         return true;
     }
@@ -843,7 +867,7 @@ struct WideKeySet {
     // Every probe is one CAS(empty -> busy): it claims a free slot, or returns
     // the tag already there (one round trip, where a load of the tag followed
     // by the claim took two for every new key).  All side effects (claim, key
-    // write, publish) happen inside the loop body of the iteration that claims
+    // words, publish) happen inside the loop body of the iteration that claims
     // the slot; the loop exit carries none, so a lane of the same wave that
     // saw the slot busy finds it published on its next iteration (divergent
     // exits are only taken once the wave leaves).
@@ -860,18 +884,21 @@ struct WideKeySet {
             uint64_t tag = kEmptyKey;  // receives the tag found in the slot
             __hip_atomic_compare_exchange_strong(tagp, &tag, kBusyKey, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                  __HIP_MEMORY_SCOPE_AGENT);
-            if (tag == kEmptyKey) {  // claimed: write the key, then publish the tag
-                keys(d)[i] = key;
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the key before the tag
+            if (tag == kEmptyKey) {  // claimed: store the words, then publish the tag
+                const gptr<uint64_t> c = cells(d, i);
+                for (size_t w = 0; w < kWords; ++w)
+                    __hip_atomic_store(c + w, (1ull << 32) | key.w[w], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 __hip_atomic_store(tagp, fp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 outcome = 2;
             }
-            if (outcome == 0 && tag != kBusyKey) {  // a busy slot is looked at again next iteration
+            bool again = tag == kBusyKey;  // a busy slot is looked at again next iteration
+            if (outcome == 0 && !again) {
                 if (tag == fp) {
-                    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // the tag before the key
-                    if (same(load_key(d, (uint32_t)i), key)) outcome = 1;
+                    const int m = match(d, i, key);
+                    if (m > 0) outcome = 1;
+                    again = m < 0;  // published, a word not yet visible
                 }
-                if (outcome == 0) {
+                if (outcome == 0 && !again) {
                     if (++probed >= max_probes || probed > d.table_mask) outcome = 3;
                     else i = (i + 1) & d.table_mask;
                 }
@@ -889,7 +916,7 @@ struct WideKeySet {
         for (uint64_t p = 0; p <= d.table_mask; ++p) {
             const uint64_t tag = as_global(d.table)[i].key;
             if (tag == kEmptyKey) return false;
-            if (tag == fp && same(load_key(d, (uint32_t)i), key)) {
+            if (tag == fp && match(d, i, key) > 0) {
                 slot = (uint32_t)i;
                 return true;
             }
@@ -1060,9 +1087,8 @@ struct Board5 {
 // set, so a reader that finds its own lo with hi's top bit set knows the
 // board is still being written and looks again on its next iteration.  Each
 // word is read and written whole, so no fence orders the two: a claim is one
-// atomic, a lookup one atomic and one load issued together, where the
-// tag-and-side-array protocol of WideKeySet needs an agent-scope fence
-// (an L2 write-back and invalidate on this part) per new key and per match.
+// atomic, a lookup one atomic and one load issued together (WideKeySet, for
+// states of any width, keeps a tag and a side array of marked words instead).
 struct Board5KeySet {
     using Key = RawKey<Board5>;
     static constexpr bool kDense = false;
