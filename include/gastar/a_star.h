@@ -142,13 +142,13 @@ std::pair<std::vector<T>, std::chrono::nanoseconds> solve(const T& start, const 
     while (n * 2 <= std::max<uint32_t>(num_queues, 1)) n *= 2;
     uint32_t heap_cap = 1024;
     uint64_t slots = 1ull << 20;
-    const uint32_t R = 8;
+    const uint32_t R = 16;  // launches between checks of the search's status
     for (int attempt = 0; attempt < 8; ++attempt) {
         Buffers b;
         InstDesc d{};
         d.kind = KIND_USER;
         d.n_heaps = n;
-        d.a0 = n < 256 ? n : 256;  // the frontier width starts at 256 heaps and doubles on demand
+        d.a0 = n;  // every queue in use from the start, as in the reference
         d.f_slack = kInf;          // no gating of heap pops
         d.heap_cap = heap_cap;
         d.heap_stride = (heap_cap + kW - 1 + kW - 1) / kW * kW;
