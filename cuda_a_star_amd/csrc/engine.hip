@@ -233,7 +233,7 @@ struct PathJob {
 // with thread 0 (their parent slots have no locality to exploit).
 __global__ __launch_bounds__(kPathThreads) void path_kernel(const InstDesc* descs, const PathJob* jobs, uint32_t n,
                                                             uint64_t* out, uint32_t* lens) {
-    __shared__ uint16_t tab[kPathTabWords];
+    __shared__ __align__(16) uint16_t tab[kPathTabWords];  // 32-bit words sit at its end (walk_grid_path)
     const uint32_t i = blockIdx.x;
     if (i >= n) return;
     const PathJob j = jobs[i];

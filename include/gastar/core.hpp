@@ -2134,7 +2134,7 @@ __device__ inline uint32_t walk_path(const InstDesc& d, typename Ops::Key* out, 
 
 // walk_path for dense grids with every thread of a kPathThreads-thread
 // workgroup (all of them call it, with `tab` pointing at kPathTabWords 16-bit
-// words of LDS).  A node's parent is one of its neighbours, so the threads
+// words of LDS, 4-byte aligned).  A node's parent is one of its neighbours, so the threads
 // load the parent links of a kPathWin x kPathWin window of cells in one round
 // trip (windows aligned to the 4 x 4 tiles of the layout: 16 threads read one
 // tile's 128-byte line; padding cells lie inside the allocation; a window
@@ -2173,6 +2173,7 @@ __device__ inline uint32_t walk_grid_path(const InstDesc& d, uint64_t* out, uint
     constexpr int kPer = (int)(kPathCells / kPathThreads);  // window cells per thread
     constexpr uint32_t kMaxHops = kPathCells / kPathHop;
     static_assert(kPathWin % 4 == 0 && kPathCells % kPathThreads == 0 && kPathCells <= kLinkOut, "path window");
+    static_assert((kPathLevels * kPathCells + kMaxHops) % 2 == 0, "the chain's 32-bit words: 4-byte aligned in tab");
     uint16_t* const link = tab;                                   // one step
     uint16_t* const top = tab + (kPathLevels - 1) * kPathCells;  // kPathHop steps
     uint16_t* const hops = tab + kPathLevels * kPathCells;       // where each hop starts
