@@ -433,11 +433,11 @@ struct WorkListT {
 constexpr int kStepBlock = 256;
 
 // LDS of one step-kernel workgroup of B threads: the heap tops of its
-// (B / W) groups, 1 + 3W entries each (most for W = 4), the tallies and the
-// work list.
+// (B / W) groups, HeapRef<W>::kTop entries each (most for W = 4: 21 KB at
+// B = 256, so seven workgroups share a CU), the tallies and the work list.
 template <int B>
 struct StepSharedT {
-    static constexpr uint32_t kHeapTopEntries = (B / 4) * (1 + 3 * 4);
+    static constexpr uint32_t kHeapTopEntries = (B / 4) * (1 + 4 + 16);
     HeapEntry heap_top[kHeapTopEntries];
     uint32_t dirty[B / 4];  // per heap group: the heap-top entries its launch changed
     uint32_t tally[kTallies];
@@ -502,19 +502,22 @@ __device__ __forceinline__ HeapEntry load_entry(gptr<HeapEntry> p) {
     return e;
 }
 
-// A heap group's heap during one launch.  Logical entries [0, kTop) -- the
-// root, its W children and the children of nodes 1 and 2 -- are held in the
-// group's LDS slice, copied in from global memory before the launch works on
-// the heap and written back after; deeper entries stay in global memory.  The
-// heaps of a search are shallow (tens of entries), so most sift steps and
-// parent checks are LDS accesses instead of memory round trips.  A node's W
-// children lie on one side of the boundary (kTop = 1 + 3W), so the lanes that
-// read one level take the same path.  (Measured on the serving stream: 1 + 3W
-// entries beat 1 + 2W and 1 + 4W by 1-2 %, 1 + W and 1 + 5W by 3-4 %; the copy
-// in and out grows with the slice, the memory accesses it saves shrink.)
+// A heap group's heap during one launch.  Logical entries [0, kTop) -- at
+// W = 4 the root and two full levels (1 + W + W^2 = 21 entries), at W = 8 the
+// root, its children and the children of nodes 1 and 2 (1 + 3W = 25, within
+// the 32-bit dirty mask) -- are held in the group's LDS slice, copied in from
+// global memory before the launch works on the heap and written back after;
+// deeper entries stay in global memory.  The heaps of a search are shallow
+// (tens of entries), so most sift steps and parent checks are LDS accesses
+// instead of memory round trips.  A node's W children lie on one side of the
+// boundary, so the lanes that read one level take the same path.  (Measured
+// on the serving stream: with one pop per launch 1 + 3W entries beat 1 + 2W
+// and 1 + 4W by 1-2 %, 1 + W and 1 + 5W by 3-4 %; with two pops per round
+// and three rounds per grid launch, 1 + 5W = 21 beat 1 + 3W by 2.6 %: the
+// copy in and out is paid once per launch, the accesses it saves per pop.)
 template <int W>
 struct HeapRef {
-    static constexpr uint32_t kTop = 1 + 3 * W;
+    static constexpr uint32_t kTop = W == 4 ? 1 + W + W * W : 1 + 3 * W;
     static constexpr int kPer = (int)((kTop + W - 1) / W);  // LDS entries per lane in fill/write_back
     gptr<HeapEntry> g;  // logical entry 0 in global memory
     lptr<HeapEntry> t;  // logical entry 0 in LDS
@@ -561,7 +564,10 @@ struct HeapRef {
         }
     }
 };
-static_assert(HeapRef<8>::kTop <= 32, "dirty masks are 32-bit");
+static_assert(HeapRef<4>::kTop <= 32 && HeapRef<8>::kTop <= 32, "dirty masks are 32-bit");
+static_assert(StepSharedT<256>::kHeapTopEntries >= 64 * HeapRef<4>::kTop &&
+                  StepSharedT<256>::kHeapTopEntries >= 32 * HeapRef<8>::kTop,
+              "the LDS slice of every heap group of a workgroup");
 
 // Removes the minimum.  The root, the last entry and the root's W children
 // are read together; the popped state's key and stored g (via `load`) are
