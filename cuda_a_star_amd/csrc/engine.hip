@@ -669,8 +669,11 @@ void launch_step_w(uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry
         case kind_bit(KIND_GRID4):
             step_kernel<W, kind_bit(KIND_GRID4), B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             break;
-        case kGrid4Paired:
-            step_kernel<W, kind_bit(KIND_GRID4), B, 2><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+        case kGrid4Paired:  // W = 4 only (upload_map)
+            if constexpr (W == 4)
+                step_kernel<W, kind_bit(KIND_GRID4), B, 2><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            else
+                throw GaError(GA_E_INVALID, "two pops per round need heap arity 4");
             break;
         case kWideVariant:
             step_kernel_wide<W, B><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
@@ -1481,7 +1484,8 @@ struct ga_engine {
             (kinds == kind_bit(KIND_PUZZLE3) || kinds == kind_bit(KIND_PUZZLE4) || kinds == kind_bit(KIND_GRID4))
                 ? kinds
                 : kAllKinds;
-        if (variant == kind_bit(KIND_GRID4) && (cfg.step_flags & STEP_PAIRED_POPS)) variant = kGrid4Paired;
+        // two pops per round at W = 4 only (at W = 8 half of a group's lanes hold no grid successor)
+        if (variant == kind_bit(KIND_GRID4) && (cfg.step_flags & STEP_PAIRED_POPS) && W == 4) variant = kGrid4Paired;
         L.variant = variant;  // its captured chunk, if any, stays valid (graphs are kept per instantiation)
         if (L.n_wg > L.map_cap) {
             if (L.d_map) {
