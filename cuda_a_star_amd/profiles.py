@@ -128,9 +128,9 @@ def bench_line(path: str) -> dict:
 
 FLAGS = "--skip-batch --skip-p24 --plain-sample 0 --dropin-sample 0 --skip-cpu --latency-load 0"
 # each lane's step kernel: the name prefix that finds it in rocprofv3's output, and the full instantiation
-# (arity 4, the kinds it runs, 256 threads, pops per round: 2 on the grid lane, step_flags 4)
+# under the served tuning (arity 4, the kinds it runs, 128 threads, pops per round: 2 on the grid lane)
 LANE_KERNELS = {0: "step_kernel<4, 30u", 1: "step_kernel<4, 8u"}
-LANE_FULL = {0: "step_kernel<4, 30u, 256, 1>", 1: "step_kernel<4, 8u, 256, 2>"}
+LANE_FULL = {0: "step_kernel<4, 30u, 128, 1>", 1: "step_kernel<4, 8u, 128, 2>"}
 LANE_NAMES = {0: f"lane 0 (15-puzzles, grids below 2 M cells): `{LANE_FULL[0]}`",
               1: f"lane 1 (grids of at least 2 M cells, two pops per round): `{LANE_FULL[1]}`"}
 WORKLOAD = ("the serving stream (seed 5, 4000 QPS offered): one warm-up segment of 512 requests, then 1024 timed "
