@@ -652,8 +652,10 @@ namespace {
 // that has a single-kind instantiation, or kAllKinds), or kWideVariant for
 // the 24-puzzle kernel.
 constexpr uint32_t kWideVariant = kind_bit(KIND_PUZZLE5);
-// the 4-connected grid instantiation with paired pops (ga_config.step_flags & STEP_PAIRED_POPS)
+// the 4-connected grid instantiation with paired pops (ga_config.step_flags & STEP_PAIRED_POPS), and the
+// all-kinds one whose 4-connected grids pop in pairs (arity 4 only)
 constexpr uint32_t kGrid4Paired = kind_bit(KIND_GRID4) | (1u << 31);
+constexpr uint32_t kAllPaired = kAllKinds | (1u << 30);
 
 template <int W, int B>
 void launch_step_w(uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry* map, const InstDesc* descs,
@@ -672,6 +674,12 @@ void launch_step_w(uint32_t variant, hipStream_t s, uint32_t grid, const WgEntry
         case kGrid4Paired:  // W = 4 only (upload_map)
             if constexpr (W == 4)
                 step_kernel<W, kind_bit(KIND_GRID4), B, 2><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
+            else
+                throw GaError(GA_E_INVALID, "two pops per round need heap arity 4");
+            break;
+        case kAllPaired:  // W = 4 only (upload_map)
+            if constexpr (W == 4)
+                step_kernel<W, kAllKinds, B, 2><<<g, b, 0, s>>>(map, descs, states, iter, r, prm);
             else
                 throw GaError(GA_E_INVALID, "two pops per round need heap arity 4");
             break;
@@ -1484,8 +1492,12 @@ struct ga_engine {
             (kinds == kind_bit(KIND_PUZZLE3) || kinds == kind_bit(KIND_PUZZLE4) || kinds == kind_bit(KIND_GRID4))
                 ? kinds
                 : kAllKinds;
-        // two pops per round at W = 4 only (at W = 8 half of a group's lanes hold no grid successor)
-        if (variant == kind_bit(KIND_GRID4) && (cfg.step_flags & STEP_PAIRED_POPS) && W == 4) variant = kGrid4Paired;
+        // two pops per round for 4-connected grids, at W = 4 only (at W = 8 half of a group's lanes
+        // hold no grid successor)
+        if ((cfg.step_flags & STEP_PAIRED_POPS) && W == 4) {
+            if (variant == kind_bit(KIND_GRID4)) variant = kGrid4Paired;
+            else if (variant == kAllKinds && (cfg.step_flags & STEP_PAIRED_ALL)) variant = kAllPaired;
+        }
         L.variant = variant;  // its captured chunk, if any, stays valid (graphs are kept per instantiation)
         if (L.n_wg > L.map_cap) {
             if (L.d_map) {
@@ -1528,6 +1540,7 @@ struct ga_engine {
             case kind_bit(KIND_PUZZLE4): return 2;
             case kind_bit(KIND_GRID4): return 3;
             case kGrid4Paired: return 4;
+            case kAllPaired: return 5;
             default: return 0;
         }
     }

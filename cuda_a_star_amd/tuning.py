@@ -17,8 +17,9 @@ per launch):
                        iteration's minimum f (less speculative expansion)
   graph replay         chunks replayed from a captured hipGraph or launched one by one
   kernel variants      ga_config.step_flags: paired pops on the 4-connected grid lane
-                       (4), write-back of changed heap-top entries only (1), grids'
-                       settled-cell check before the atomic (2)
+                       (4; with 8, also on a lane that holds several kinds), write-back
+                       of changed heap-top entries only (1), grids' settled-cell check
+                       before the atomic (2)
 
 `python -m cuda_a_star_amd.tuning [--mode served|plain]` sweeps the
 candidates on a seeded sample of the serving workload on device 0 and writes
@@ -200,7 +201,7 @@ def sweep(mode: str, n: int, seed: int, qps: float, passes: int = 2) -> TuneConf
             "slack_puzzle": [0, 1, 3, 5, 9],
             "slack_grid": [0, 8, 16, 24, 32, 48],
             "block": [128, 256, 512],
-            "step_flags": [0, 1, 2, 3, 4, 6],
+            "step_flags": [0, 1, 2, 3, 4, 5, 6, 12],
         }
     else:
         axes = {

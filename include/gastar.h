@@ -131,7 +131,9 @@ typedef struct ga_config {
                                 cell's stored g with its weight and skip the atomicMin that could not
                                 lower it; 4 = a launch lane that runs only 4-connected grids pops two
                                 nodes per round and issues both nodes' atomicMins before reading either
-                                result (its own kernel instantiation; heap_arity 4 only, ignored at 8) */
+                                result (its own kernel instantiation; heap_arity 4 only, ignored at 8);
+                                8 (with 4) = so do the 4-connected grids of a lane that also holds
+                                other kinds */
     uint32_t persist;        /* ga_solve_batch only: 1 = solve the batch's 8- and 15-puzzles and its
                                 grids of at most persist_cells cells in one launch per wave, one
                                 workgroup per instance (persist_block / heap_arity heaps) running

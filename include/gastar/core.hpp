@@ -252,6 +252,8 @@ enum : uint32_t {
     STEP_SKIP_SETTLED = 2,   // grids: no atomicMin for a successor whose cell's stored g is already no larger
     STEP_PAIRED_POPS = 4,    // 4-connected grids: two pops per round, both nodes' atomics in flight together
                              // (selects a kernel instantiation on the host; the kernels do not read it)
+    STEP_PAIRED_ALL = 8,     // with STEP_PAIRED_POPS: so do the 4-connected grids of a lane that holds several
+                             // kinds (the all-kinds instantiation with two pops; host only as well)
 };
 
 struct StepParams {

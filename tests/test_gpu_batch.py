@@ -267,6 +267,9 @@ VARIANTS = [
     dict(heap_arity=4, rounds=1, rounds_grid=2, step_flags=4, lanes=2, deep_cells=1 << 14),
     dict(heap_arity=4, rounds=1, rounds_grid=3, step_flags=6, lanes=2, deep_cells=1 << 14, use_graph=1,
          slack_grid=8),
+    # ... and on a lane that holds puzzles and grids together (the all-kinds kernel with two pops)
+    dict(heap_arity=4, rounds=1, rounds_grid=3, step_flags=12, use_graph=1, slack_grid=8),
+    dict(heap_arity=4, rounds=2, rounds_grid=2, step_flags=13, block=128),
 ]
 
 
