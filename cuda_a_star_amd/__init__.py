@@ -99,6 +99,7 @@ class Config(ctypes.Structure):
         ("persist", ctypes.c_uint32),
         ("persist_block", ctypes.c_uint32),
         ("persist_cells", ctypes.c_uint64),
+        ("pdb15", ctypes.c_uint32),
     ]
 
 
@@ -163,6 +164,7 @@ EXPORTS = (
     "ga_serve_run", "ga_serve_start", "ga_serve_submit", "ga_serve_poll", "ga_serve_drain", "ga_serve_status",
     "ga_serve_stop", "ga_solve_noalloc", "ga_result_free", "ga_status_str", "ga_engine_last_error",
     "ga_pdb24_tables",
+    "ga_pdb15_tables",
     "ga_debug_fail_run", "ga_debug_closed_set", "ga_debug_heap",
     "ga_device_upload", "ga_device_free", "ga_atomic_rate",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
@@ -202,6 +204,7 @@ def lib() -> ctypes.CDLL:
     L.ga_serve_status.argtypes = [ctypes.c_void_p, P(ServeStats)]
     L.ga_serve_stop.argtypes = [ctypes.c_void_p]
     L.ga_pdb24_tables.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t]
+    L.ga_pdb15_tables.argtypes = [ctypes.c_void_p, _u8p, ctypes.c_size_t]
     L.ga_solve_noalloc.argtypes = [ctypes.c_void_p, P(Request), P(Result), _u64p, ctypes.c_uint32]
     L.ga_result_free.argtypes = [P(Result)]
     L.ga_result_free.restype = None
@@ -533,6 +536,14 @@ class Engine:
         rc = lib().ga_pdb24_tables(self._h, out.ctypes.data_as(_u8p), out.nbytes)
         if rc != OK:
             raise GAStarError(rc, "ga_pdb24_tables")
+        return out
+
+    def pdb15_tables(self) -> np.ndarray:
+        """The device's 15-puzzle pattern databases (ga_pdb15_tables)."""
+        out = np.empty(3 * 16 ** 5, dtype=np.uint8)
+        rc = lib().ga_pdb15_tables(self._h, out.ctypes.data_as(_u8p), out.nbytes)
+        if rc != OK:
+            raise GAStarError(rc, "ga_pdb15_tables")
         return out
 
     def stats(self) -> EngineStats:

@@ -255,34 +255,39 @@ __global__ __launch_bounds__(kPathThreads) void path_kernel(const InstDesc* desc
 }
 
 // One level of the breadth-first search that builds a pattern database of
-// NT tiles (core.hpp, kPdbTiles): every placement at distance d marks the
-// unvisited placements one tile move away with d + 1.  Several threads may
-// mark the same byte with the same value, which is harmless; a level whose
-// predecessor marked nothing returns at once.
-template <int NT>
+// NT tiles on an S x S board (core.hpp: kPdbTiles for S = 5, kPdb15Tiles for
+// S = 4; placement index: sum of cell * (S*S)^i): every placement at distance
+// d marks the unvisited placements one tile move away with d + 1.  Several
+// threads may mark the same byte with the same value, which is harmless; a
+// level whose predecessor marked nothing returns at once.
+template <int NT, int S>
 __global__ void pdb_level_kernel(uint8_t* dist, uint32_t d, uint32_t* changed) {
+    constexpr uint32_t C = S * S;
     if (d > 0 && __hip_atomic_load(&changed[d - 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) return;
-    uint32_t n = 1;
-    for (int i = 0; i < NT; ++i) n *= 25u;
+    uint32_t n = 1, pw[NT];
+    for (int i = 0; i < NT; ++i) {
+        pw[i] = n;
+        n *= C;
+    }
     bool any = false;
     for (uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x; idx < n; idx += gridDim.x * blockDim.x) {
         if (dist[idx] != d) continue;
         int pos[NT];
         uint32_t rest = idx;
         for (int i = 0; i < NT; ++i) {
-            pos[i] = (int)(rest % 25u);
-            rest /= 25u;
+            pos[i] = (int)(rest % C);
+            rest /= C;
         }
         for (int i = 0; i < NT; ++i) {
-            const int r = pos[i] / 5, c = pos[i] % 5;
+            const int r = pos[i] / S, c = pos[i] % S;
             for (int m = 0; m < 4; ++m) {  // row +1, row -1, column +1, column -1
                 const int nr = r + (m == 0) - (m == 1), nc = c + (m == 2) - (m == 3);
-                if (nr < 0 || nr > 4 || nc < 0 || nc > 4) continue;
-                const int np = nr * 5 + nc;
+                if (nr < 0 || nr >= S || nc < 0 || nc >= S) continue;
+                const int np = nr * S + nc;
                 bool taken = false;
                 for (int k = 0; k < NT; ++k) taken |= (k != i && pos[k] == np);
                 if (taken) continue;
-                const uint32_t nidx = (uint32_t)((int)idx + (np - pos[i]) * (int)pow25((uint32_t)i));
+                const uint32_t nidx = (uint32_t)((int)idx + (np - pos[i]) * (int)pw[i]);
                 if (dist[nidx] == 0xFF) {
                     dist[nidx] = (uint8_t)(d + 1);
                     any = true;
@@ -813,6 +818,7 @@ struct ga_engine {
     std::atomic<bool> serving{false};  // an online server owns the engine (ga_serve_start .. ga_serve_stop)
     char* scratch_dev = nullptr;  // this device's scratch (ga_solve_noalloc)
     uint8_t* d_pdb = nullptr;     // 24-puzzle pattern databases (kPdbBytes + level flags), built on first use
+    uint8_t* d_pdb15 = nullptr;   // 15-puzzle pattern databases (kPdb15Bytes + level flags), built on first use
     // the one-launch form's completion flags: coherent pinned host memory the
     // device writes (one word per instance of a wave), grown on demand
     uint32_t* persist_done = nullptr;
@@ -922,6 +928,7 @@ struct ga_engine {
         if (d_descs) (void)hipFree(d_descs);
         if (d_states) (void)hipFree(d_states);
         if (d_pdb) (void)hipFree(d_pdb);
+        if (d_pdb15) (void)hipFree(d_pdb15);
         if (persist_done) (void)hipHostFree(persist_done);
         delete arena;
         if (path_stream) (void)hipStreamDestroy(path_stream);
@@ -1027,9 +1034,9 @@ struct ga_engine {
             for (uint32_t d = 0; d < kPdbLevels; ++d) {
                 uint32_t* ch = changed + (uint64_t)q * kPdbLevels;
                 if (kPdbCount[q] == 5)
-                    pdb_level_kernel<5><<<4096, kBlock, 0, stream>>>(dist, d, ch);
+                    pdb_level_kernel<5, 5><<<4096, kBlock, 0, stream>>>(dist, d, ch);
                 else
-                    pdb_level_kernel<4><<<512, kBlock, 0, stream>>>(dist, d, ch);
+                    pdb_level_kernel<4, 5><<<512, kBlock, 0, stream>>>(dist, d, ch);
             }
         }
         hip_check(hipGetLastError(), "pattern database build");
@@ -1044,6 +1051,44 @@ struct ga_engine {
             }
         d_pdb = p;
         return d_pdb;
+    }
+
+    // ---- 15-puzzle pattern databases ------------------------------------------
+    static bool ordered_goal4(const ga_request& r) {
+        return r.kind == GA_PUZZLE && r.side == 4 && r.goal == kPuzzle4OrderedGoal;
+    }
+    // The same for the 15-puzzle's three 5-tile tables (kPdb15Bytes: 3 MiB).
+    const uint8_t* ensure_pdb15(hipStream_t stream) {
+        if (d_pdb15) return d_pdb15;
+        uint8_t* p = nullptr;
+        const uint64_t bytes = kPdb15Bytes + (uint64_t)kPdb15Patterns * kPdbLevels * sizeof(uint32_t);
+        if (hipMalloc(&p, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            throw GaError(GA_E_NOMEM, "pattern databases");
+        }
+        uint32_t* changed = (uint32_t*)(p + kPdb15Bytes);
+        fill_kernel<uint64_t><<<256, kBlock, 0, stream>>>((uint64_t*)p, kPdb15Bytes / 8, ~0ull);
+        fill_kernel<uint32_t><<<1, kBlock, 0, stream>>>(changed, (uint64_t)kPdb15Patterns * kPdbLevels, 0u);
+        for (int q = 0; q < kPdb15Patterns; ++q) {
+            uint32_t goal = 0;
+            for (int i = 0; i < 5; ++i) goal += (kPdb15Tiles[q][i] - 1) << (4 * i);
+            uint8_t* dist = p + (uint64_t)q * kPdb15Table;
+            put_byte_kernel<<<1, 64, 0, stream>>>(dist + goal, 0);
+            for (uint32_t d = 0; d < kPdbLevels; ++d)
+                pdb_level_kernel<5, 4><<<1024, kBlock, 0, stream>>>(dist, d, changed + (uint64_t)q * kPdbLevels);
+        }
+        hip_check(hipGetLastError(), "pattern database build");
+        std::vector<uint32_t> flags((size_t)kPdb15Patterns * kPdbLevels);
+        hip_check(hipMemcpyAsync(flags.data(), changed, flags.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
+                  "read back level flags");
+        hip_check(hipStreamSynchronize(stream), "pattern database build");
+        for (int q = 0; q < kPdb15Patterns; ++q)
+            if (flags[(size_t)q * kPdbLevels + kPdbLevels - 1]) {  // still growing at the last level
+                (void)hipFree(p);
+                throw GaError(GA_E_HIP, "pattern database deeper than the level bound");
+            }
+        d_pdb15 = p;
+        return d_pdb15;
     }
 
     // ---- admission ------------------------------------------------------------
@@ -1188,6 +1233,7 @@ struct ga_engine {
         if (!carve_instance(j, p, take, d)) return 0;
         hipStream_t stream = L.stream;
         if (cfg.pdb24 && ordered_goal5(r)) d.pdb = ensure_pdb(stream);
+        if (cfg.pdb15 && ordered_goal4(r)) d.pdb = ensure_pdb15(stream);
         set_grow_limits(j, p, d);
         init_instance(j, p, d, stream);
         int slot = L.free_slots.back();
@@ -1945,6 +1991,7 @@ class PersistRun {
             d.grow_heap_at = 0;  // no grow requests: the host cannot step in during the launch
             d.grow_table_at = 0;
             d.start_iter = 0;
+            if (e->cfg.pdb15 && ga_engine::ordered_goal4(*j.req)) d.pdb = e->ensure_pdb15(s);
             e->init_instance(j, p, d, s);
             j.allocs = std::move(allocs);
             j.desc = d;
@@ -2845,6 +2892,7 @@ void solve_noalloc(ga_engine* e, const ga_request& r, ga_result* out, uint64_t* 
                                            std::to_string(kScratchBytes >> 20) + " MiB no-allocation scratch");
         // the pattern databases when this engine has built them (building them would allocate)
         if (e->cfg.pdb24 && e->d_pdb && ga_engine::ordered_goal5(r)) d.pdb = e->d_pdb;
+        if (e->cfg.pdb15 && e->d_pdb15 && ga_engine::ordered_goal4(r)) d.pdb = e->d_pdb15;
         e->init_instance(j, p, d, st);
         fill_kernel<uint32_t><<<1, 64, 0, st>>>(d_iter, 1, 0u);
         map_kernel<<<(n_wg + 255) / 256, 256, 0, st>>>(d_map, n_wg, heaps_per_wg, p.n_heaps);
@@ -2950,6 +2998,7 @@ void ga_config_default(ga_config* cfg) {
     cfg->slack_puzzle = 0;
     cfg->slack_grid = 0;
     cfg->pdb24 = 1;
+    cfg->pdb15 = 1;
     cfg->grow_in_place = 1;
     cfg->block = kStepBlock;
     cfg->persist = 0;
@@ -3199,6 +3248,20 @@ int ga_serve_stop(ga_server* sv) {
     sv->e->serving = false;
     delete sv;
     return rc;
+}
+
+int ga_pdb15_tables(ga_engine* e, uint8_t* out, size_t bytes) {
+    if (!e || !out || bytes < kPdb15Bytes || e->serving) return GA_E_INVALID;
+    if (e->broken) return GA_E_HIP;
+    try {
+        hip_check(hipSetDevice(e->device), "hipSetDevice");
+        const uint8_t* p = e->ensure_pdb15(e->lanes[0].stream);
+        hip_check(hipMemcpy(out, p, kPdb15Bytes, hipMemcpyDeviceToHost), "copy pattern databases");
+    } catch (const GaError& err) {
+        e->last_error = err.what();
+        return err.code;
+    }
+    return GA_OK;
 }
 
 int ga_pdb24_tables(ga_engine* e, uint8_t* out, size_t bytes) {

@@ -141,6 +141,10 @@ typedef struct ga_config {
                                 their heaps or closed set, run on the launch lanes afterwards */
     uint32_t persist_block;  /* threads per workgroup of that launch: 256, 512 or 1024 (0 = 1024) */
     uint64_t persist_cells;  /* largest grid (cells) of the one-launch form (0 = puzzles only) */
+    uint32_t pdb15;          /* 1: 15-puzzles towards the ordered goal (tile t on cell t-1, blank
+                                last) use additive pattern databases (three 5-tile tables, 3 MiB,
+                                built on the device at the first such request), others Manhattan;
+                                0: Manhattan for every 15-puzzle.  Costs are the same either way */
 } ga_config;
 
 /* Engine statistics.  With cfg.profile = n, kernel time covers the timed
@@ -308,6 +312,13 @@ const char* ga_engine_last_error(ga_engine* e);
  * 48,828,125: five tables of 25^5 bytes, the fifth using its first 25^4, the
  * rest 255).  No counterpart in the reference; for tests and inspection. */
 int ga_pdb24_tables(ga_engine* e, uint8_t* out, size_t bytes);
+
+/* The 15-puzzle pattern databases of the engine's device (ga_config.pdb15),
+ * built first if need be, copied to `out` (bytes must be at least 3 * 16^5 =
+ * 3,145,728: the tables of tiles {1,2,3,5,6}, {4,7,8,11,12}, {9,10,13,14,15},
+ * indexed by the sum of cell(tile i) * 16^i, 255 where no placement is).  No
+ * counterpart in the reference; for tests and inspection. */
+int ga_pdb15_tables(ga_engine* e, uint8_t* out, size_t bytes);
 
 /* Test support, no counterpart in the reference, no device call: applies the
  * request checks of ga_solve_batch to reqs, finishes the first finish_first

@@ -177,7 +177,7 @@ struct InstDesc {
     uint8_t goal_rc[32];   // puzzle: row << 4 | col of each tile in the goal
     const void* user;      // user problems: device copy of {start, goal}
     void* keys;            // user problems with states wider than 8 bytes: key array
-    const uint8_t* pdb;    // 24-puzzle: the pattern databases (kPdbBytes), or null for Manhattan
+    const uint8_t* pdb;    // 24- and 15-puzzles: the pattern databases (kPdbBytes, kPdb15Bytes), or null for Manhattan
     uint32_t grow_heap_at; // request GROW_HEAP when a heap ends an iteration above this many entries (0 = never)
     uint32_t pad3;
     uint64_t grow_table_at;  // hashed: request GROW_TABLE when more slots than this are occupied (0 = never)
@@ -939,6 +939,32 @@ struct WideKeySet {
     __host__ __device__ static inline Key pack(const T& t) { return to_raw<T>(t); }
 };
 
+// Additive pattern databases of the 15-puzzle towards the ordered goal (tile
+// t on cell t - 1, blank on cell 15): the relaxation of the 24-puzzle's tables
+// below (a pattern's tiles step into any neighbouring cell not held by a tile
+// of the same pattern), with the 15 tiles in three disjoint patterns of five.
+// Pattern p's table (kPdb15Table bytes at p * kPdb15Table) is indexed by the
+// sum over its tiles i of cell * 16^i; the sum over the patterns is
+// admissible and consistent.
+constexpr int kPdb15Patterns = 3;
+constexpr uint32_t kPdb15Tiles[kPdb15Patterns][5] = {{1, 2, 3, 5, 6}, {4, 7, 8, 11, 12}, {9, 10, 13, 14, 15}};
+constexpr uint64_t kPdb15Table = 1ull << 20;  // 16^5
+constexpr uint64_t kPdb15Bytes = kPdb15Patterns * kPdb15Table;
+constexpr uint64_t kPuzzle4OrderedGoal = 0x0FEDCBA987654321ull;
+// pattern p: nibble t holds i + 1 for tile t at position i of the pattern, else 0
+constexpr uint64_t pdb15_pattern_word(int p) {
+    uint64_t w = 0;
+    for (int i = 0; i < 5; ++i) w |= (uint64_t)(i + 1) << (4 * kPdb15Tiles[p][i]);
+    return w;
+}
+// pattern of tile t (0..2), 3 for the blank: two bits per tile
+constexpr uint32_t pdb15_pattern_of_word() {
+    uint32_t w = 3u;  // tile 0, the blank
+    for (int p = 0; p < kPdb15Patterns; ++p)
+        for (int i = 0; i < 5; ++i) w |= (uint32_t)p << (2 * kPdb15Tiles[p][i]);
+    return w;
+}
+
 // ---------------------------------------------------------------------------
 // built-in problems
 // ---------------------------------------------------------------------------
@@ -967,8 +993,31 @@ struct PuzzleOps : HashedU64Set {
         return (uint32_t)(abs(r - gr) + abs(c - gc));
     }
 
+    // 15-puzzles towards the ordered goal with pattern databases (d.pdb): the
+    // placement index of pattern p's tiles in state s, and a table entry
+    __device__ static inline uint32_t pdb_index(uint64_t s, uint32_t p) {
+        constexpr uint64_t w0 = pdb15_pattern_word(0), w1 = pdb15_pattern_word(1), w2 = pdb15_pattern_word(2);
+        const uint64_t w = p == 0 ? w0 : p == 1 ? w1 : w2;
+        uint32_t idx = 0;
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const uint32_t at = (uint32_t)(w >> (4 * ((s >> (4 * c)) & 15u))) & 15u;  // position + 1, or 0
+            idx += at ? (uint32_t)c << (4 * (at - 1)) : 0u;
+        }
+        return idx;
+    }
+    __device__ static inline uint32_t pdb_value(const InstDesc& d, uint32_t p, uint32_t idx) {
+        return as_global(d.pdb)[(uint64_t)p * kPdb15Table + idx];
+    }
+
     __device__ static inline uint32_t heuristic(const InstDesc& d, uint64_t s, uint32_t) {
         uint32_t h = 0;
+        if constexpr (S == 4) {
+            if (d.pdb) {
+                for (uint32_t p = 0; p < kPdb15Patterns; ++p) h += pdb_value(d, p, pdb_index(s, p));
+                return h;
+            }
+        }
 #pragma unroll
         for (int c = 0; c < kCells; ++c) {
             const uint32_t t = (uint32_t)(s >> (4 * c)) & 15u;
@@ -1001,6 +1050,19 @@ struct PuzzleOps : HashedU64Set {
         t = (s & ~(0xFull << (4 * nb))) | ((uint64_t)tile << (4 * b));
         taux = (uint32_t)b + 1;
         cost = 1;
+        if constexpr (S == 4) {
+            if (d.pdb) {  // only the moved tile's pattern changes: its placement index moves by (b - nb) * 16^i
+                constexpr uint32_t pw = pdb15_pattern_of_word();
+                const uint32_t p = (pw >> (2 * tile)) & 3u;
+                const uint32_t before = pdb_index(s, p);
+                const uint32_t i = ((uint32_t)((p == 0 ? pdb15_pattern_word(0) : p == 1 ? pdb15_pattern_word(1)
+                                                                                         : pdb15_pattern_word(2)) >>
+                                               (4 * tile)) & 15u) - 1u;
+                const uint32_t after = (uint32_t)((int)before + (b - nb) * (1 << (4 * i)));
+                th = h - pdb_value(d, p, before) + pdb_value(d, p, after);
+                return true;
+            }
+        }
         th = h - md(d, tile, nb) + md(d, tile, b);
         return true;
     }

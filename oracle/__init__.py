@@ -99,6 +99,10 @@ def lib(path: str = _LIB_PATH) -> ctypes.CDLL:
         L.oracle_pdb24_build.restype = ctypes.c_int
         L.oracle_pdb24_h.argtypes = [u8p, u8p]
         L.oracle_pdb24_h.restype = ctypes.c_uint32
+        L.oracle_pdb15_build.argtypes = [u8p]
+        L.oracle_pdb15_build.restype = ctypes.c_int
+        L.oracle_pdb15_h.argtypes = [u8p, u8p]
+        L.oracle_pdb15_h.restype = ctypes.c_uint32
         L.oracle_astar_board_pdb.argtypes = [u8p, u8p, ctypes.c_uint64, ctypes.POINTER(OracleResult), u8p,
                                              ctypes.c_uint32]
         L.oracle_astar_board_pdb.restype = ctypes.c_int
@@ -254,6 +258,28 @@ def pdb24() -> np.ndarray:
 def pdb24_h(board) -> int:
     b = np.ascontiguousarray(np.asarray(board, dtype=np.uint8))
     return lib().oracle_pdb24_h(_u8(pdb24()), _u8(b))
+
+
+PDB15_BYTES = 3 * 16 ** 5
+_pdb15 = None
+
+
+def pdb15() -> np.ndarray:
+    """The 15-puzzle pattern databases (three 5-tile tables, built once per process by BFS)."""
+    global _pdb15
+    if _pdb15 is None:
+        t = np.empty(PDB15_BYTES, dtype=np.uint8)
+        rc = lib().oracle_pdb15_build(_u8(t))
+        if rc != OK:
+            raise RuntimeError(f"oracle_pdb15_build failed ({rc})")
+        _pdb15 = t
+    return _pdb15
+
+
+def pdb15_h(board) -> int:
+    """h of a 4x4 board (16 tile numbers row-major, 0 = blank) under the 15-puzzle tables."""
+    b = np.ascontiguousarray(np.asarray(board, dtype=np.uint8))
+    return lib().oracle_pdb15_h(_u8(pdb15()), _u8(b))
 
 
 def astar_board_pdb(start, max_states: int = 0, want_path: bool = True):

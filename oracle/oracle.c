@@ -726,6 +726,62 @@ static uint32_t pdb_h128(const uint8_t* pdb, u128 s) {
 
 uint32_t oracle_pdb24_h(const uint8_t* pdb, const uint8_t* board) { return pdb_h128(pdb, board_pack(5, board)); }
 
+/* ---- additive pattern databases for 4x4 boards with the ordered goal ---- */
+/* The same relaxation as the 5x5 tables above, for the 15-puzzle: its 15
+ * tiles in three disjoint patterns of five; placement index: sum over the
+ * pattern's tiles i of cell(tile i) * 16^i.  Goal: tile t on cell t - 1,
+ * blank on cell 15. */
+static const uint8_t PDB15_TILES[ORACLE_PDB15_PATTERNS][5] = {{1, 2, 3, 5, 6}, {4, 7, 8, 11, 12}, {9, 10, 13, 14, 15}};
+#define PDB15_TABLE (1u << 20) /* 16^5 */
+
+int oracle_pdb15_build(uint8_t* out) {
+    if (!out) return ORACLE_E_INVALID;
+    uint32_t* queue = (uint32_t*)malloc((size_t)PDB15_TABLE * sizeof(uint32_t));
+    if (!queue) return ORACLE_E_NOMEM;
+    for (int p = 0; p < ORACLE_PDB15_PATTERNS; ++p) {
+        uint8_t* dist = out + (size_t)p * PDB15_TABLE;
+        memset(dist, 0xFF, PDB15_TABLE);
+        uint32_t goal = 0;
+        for (int i = 0; i < 5; ++i) goal += (uint32_t)(PDB15_TILES[p][i] - 1) << (4 * i);
+        size_t head = 0, tail = 0;
+        dist[goal] = 0;
+        queue[tail++] = goal;
+        while (head < tail) { /* FIFO breadth-first search from the goal placement */
+            const uint32_t idx = queue[head++];
+            int pos[5];
+            for (int i = 0; i < 5; ++i) pos[i] = (int)(idx >> (4 * i)) & 15;
+            for (int i = 0; i < 5; ++i) {
+                const int r = pos[i] / 4, c = pos[i] % 4;
+                const int nr[4] = {r + 1, r - 1, r, r}, nc[4] = {c, c, c + 1, c - 1};
+                for (int m = 0; m < 4; ++m) {
+                    if (nr[m] < 0 || nr[m] > 3 || nc[m] < 0 || nc[m] > 3) continue;
+                    const int np = nr[m] * 4 + nc[m];
+                    int taken = 0;
+                    for (int k = 0; k < 5; ++k) taken |= (k != i && pos[k] == np);
+                    if (taken) continue;
+                    const uint32_t nidx = (idx & ~(15u << (4 * i))) | ((uint32_t)np << (4 * i));
+                    if (dist[nidx] != 0xFF) continue;
+                    dist[nidx] = (uint8_t)(dist[idx] + 1);
+                    queue[tail++] = nidx;
+                }
+            }
+        }
+    }
+    free(queue);
+    return ORACLE_OK;
+}
+
+uint32_t oracle_pdb15_h(const uint8_t* pdb, const uint8_t* board) {
+    uint32_t idx[ORACLE_PDB15_PATTERNS] = {0, 0, 0};
+    for (int c = 0; c < 16; ++c)
+        for (int p = 0; p < ORACLE_PDB15_PATTERNS; ++p)
+            for (int i = 0; i < 5; ++i)
+                if (PDB15_TILES[p][i] == board[c]) idx[p] += (uint32_t)c << (4 * i);
+    uint32_t h = 0;
+    for (int p = 0; p < ORACLE_PDB15_PATTERNS; ++p) h += pdb[(size_t)p * PDB15_TABLE + idx[p]];
+    return h;
+}
+
 /* the heuristic of one board search: Manhattan without the blank, or the sum of the pattern databases */
 typedef struct {
     int side;

@@ -47,6 +47,10 @@
  *                          across every move; a one-tile and a two-tile
  *                          closed form; equal to the engine's device-built
  *                          tables byte for byte (a GPU test).
+ *   oracle_pdb15_build   - the same checks on 4x4 boards (the 15-puzzle):
+ *   oracle_pdb15_h         h = 0 at the goal, h >= Manhattan, h <= the BFS
+ *                          depth, |h(s) - h(t)| = 1 across every move, closed
+ *                          forms; equal to the device's tables byte for byte.
  *   oracle_astar_board_pdb - equal to oracle_astar_board (Manhattan) wherever
  *                          both finish; equal to oracle_bfs_board on walks.
  *
@@ -137,6 +141,14 @@ int64_t oracle_bfs_board(int side, const uint8_t* start, const uint8_t* goal,
 int oracle_pdb24_build(uint8_t* out);
 /* Sum of the five tables for a 5x5 board (an admissible, consistent h). */
 uint32_t oracle_pdb24_h(const uint8_t* pdb, const uint8_t* board);
+/* The same for 4x4 boards (the 15-puzzle): ORACLE_PDB15_BYTES bytes, three
+ * tables of 16^5 bytes for the tile groups {1,2,3,5,6}, {4,7,8,11,12},
+ * {9,10,13,14,15}; placement index: sum of cell(tile i) * 16^i. */
+#define ORACLE_PDB15_PATTERNS 3
+#define ORACLE_PDB15_BYTES (3ull << 20)
+int oracle_pdb15_build(uint8_t* out);
+/* Sum of the three tables for a 4x4 board (16 tile numbers, 0 = blank). */
+uint32_t oracle_pdb15_h(const uint8_t* pdb, const uint8_t* board);
 /* oracle_astar_board on 5x5 boards towards the ordered goal with the
  * pattern-database heuristic instead of Manhattan. */
 int oracle_astar_board_pdb(const uint8_t* start, const uint8_t* pdb, uint64_t max_states, oracle_result* res,

@@ -238,6 +238,77 @@ def test_board_rejects_bad_inputs_and_paths():
         assert oracle.replay_board(5, jump, start, goal) == -1  # two moves at once
 
 
+# ---- 15-puzzle pattern databases (oracle_pdb15_*) ----
+GOAL4 = tuple(range(1, 16)) + (0,)
+
+
+def _neighbours4(b):
+    b = list(b)
+    z = b.index(0)
+    r, c = divmod(z, 4)
+    for nr, nc in ((r + 1, c), (r - 1, c), (r, c + 1), (r, c - 1)):
+        if 0 <= nr < 4 and 0 <= nc < 4:
+            t = b[:]
+            t[z], t[nr * 4 + nc] = t[nr * 4 + nc], 0
+            yield tuple(t)
+
+
+def test_pdb15_tables_structure():
+    t = oracle.pdb15()
+    for p in range(3):
+        seg = t[p * 16 ** 5:(p + 1) * 16 ** 5]
+        reach = seg[seg != 255]
+        # every placement of 5 distinct cells of 16 is reached, and nothing else
+        assert len(reach) == 16 * 15 * 14 * 13 * 12
+        assert reach.min() == 0 and (reach == 0).sum() == 1 and reach.max() <= 60
+    assert oracle.pdb15_h(GOAL4) == 0
+
+
+def test_pdb15_dominates_manhattan_and_is_consistent():
+    boards = [ga.state_to_board(ga.gen_puzzle(4, d, 700 + d), 4) for d in range(4, 120, 3)]
+    rng = np.random.default_rng(8)
+    for _ in range(60):  # random solvable permutations
+        b = list(rng.permutation(16))
+        if not ga.board_solvable(4, b, GOAL4):
+            i, j = [k for k in range(16) if b[k]][:2]
+            b[i], b[j] = b[j], b[i]
+        boards.append(tuple(int(x) for x in b))
+    for b in boards:
+        h = oracle.pdb15_h(b)
+        assert h >= oracle.board_manhattan(4, b, GOAL4)
+        for t in _neighbours4(b):  # one move changes h by exactly one tile step, up or down
+            assert abs(oracle.pdb15_h(t) - h) == 1
+
+
+@pytest.mark.parametrize("depth,seed", [(10, 41), (14, 42), (18, 43), (20, 44)])
+def test_pdb15_below_exact_distance(depth, seed):
+    start = ga.state_to_board(ga.gen_puzzle(4, depth, seed), 4)
+    d = oracle.bfs_board(4, start, GOAL4, depth, max_states=20_000_000)
+    assert 0 <= d <= depth and oracle.pdb15_h(start) <= d
+    rc, res, _ = oracle.astar_board(4, start, GOAL4, want_path=False)
+    assert rc == oracle.OK and res.cost == d
+
+
+def test_pdb15_closed_forms():
+    # the blank slid k cells along the bottom row: every displaced tile is one free step from home
+    for k in (1, 2, 3):
+        row = list(GOAL4)
+        for _ in range(k):
+            p = row.index(0)
+            row[p], row[p - 1] = row[p - 1], 0
+        assert oracle.pdb15_h(row) == k
+    # tiles 1 and 2 (one pattern) swapped in the top row: they cannot pass each other
+    # in the relaxation, so the table adds at least a detour of two moves to Manhattan's 2
+    b = list(GOAL4)
+    b[0], b[1] = 2, 1
+    assert oracle.pdb15_h(b) >= oracle.board_manhattan(4, b, GOAL4) + 2
+    # tiles 3 and 4 belong to different patterns and step home through each other's cell,
+    # which neither pattern holds: exactly their Manhattan distance, 2
+    b = list(GOAL4)
+    b[2], b[3] = 4, 3
+    assert oracle.pdb15_h(b) == oracle.board_manhattan(4, b, GOAL4) == 2
+
+
 # ---- 24-puzzle pattern databases (oracle_pdb24_*) ----
 GOAL5 = tuple(range(1, 25)) + (0,)
 
