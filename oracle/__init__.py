@@ -103,6 +103,9 @@ def lib(path: str = _LIB_PATH) -> ctypes.CDLL:
         L.oracle_pdb15_build.restype = ctypes.c_int
         L.oracle_pdb15_h.argtypes = [u8p, u8p]
         L.oracle_pdb15_h.restype = ctypes.c_uint32
+        L.oracle_astar_puzzle_pdb.argtypes = [ctypes.c_uint64, u8p, ctypes.c_uint64, ctypes.POINTER(OracleResult),
+                                              ctypes.POINTER(ctypes.c_uint64), ctypes.c_uint32]
+        L.oracle_astar_puzzle_pdb.restype = ctypes.c_int
         L.oracle_astar_board_pdb.argtypes = [u8p, u8p, ctypes.c_uint64, ctypes.POINTER(OracleResult), u8p,
                                              ctypes.c_uint32]
         L.oracle_astar_board_pdb.restype = ctypes.c_int
@@ -123,6 +126,17 @@ def astar_puzzle(side: int, start: int, goal: int, max_states: int = 0, want_pat
     buf = (ctypes.c_uint64 * max(cap, 1))()
     rc = lib().oracle_astar_puzzle(side, start, goal, max_states, ctypes.byref(res),
                                    buf if want_path else None, cap)
+    path = [buf[i] for i in range(res.path_len)] if (want_path and rc == OK and res.path_len <= cap) else []
+    return rc, res, path
+
+
+def astar_puzzle_pdb(start: int, max_states: int = 0, want_path: bool = True):
+    """astar_puzzle on the 15-puzzle towards the ordered goal with the pattern databases (pdb15)."""
+    res = OracleResult()
+    cap = 4096 if want_path else 0
+    buf = (ctypes.c_uint64 * max(cap, 1))()
+    rc = lib().oracle_astar_puzzle_pdb(start, _u8(pdb15()), max_states, ctypes.byref(res),
+                                       buf if want_path else None, cap)
     path = [buf[i] for i in range(res.path_len)] if (want_path and rc == OK and res.path_len <= cap) else []
     return rc, res, path
 

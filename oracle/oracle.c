@@ -202,8 +202,12 @@ static void pstore_free(pstore* st) {
     free(st->slot);
 }
 
-int oracle_astar_puzzle(int side, uint64_t start, uint64_t goal, uint64_t max_states,
-                        oracle_result* res, uint64_t* path, uint32_t path_cap) {
+static uint32_t pdb15_h_state(const uint8_t* pdb, uint64_t s);
+
+/* A* over 64-bit puzzle states; h = Manhattan without the blank, or (pdb, 4x4,
+ * the ordered goal) the sum of the 15-puzzle pattern databases */
+static int astar_puzzle_h(int side, uint64_t start, uint64_t goal, const uint8_t* pdb, uint64_t max_states,
+                          oracle_result* res, uint64_t* path, uint32_t path_cap) {
     memset(res, 0, sizeof(*res));
     res->cost = -1;
     if ((side != 3 && side != 4) || !valid_puzzle(side, start) || !valid_puzzle(side, goal))
@@ -216,7 +220,7 @@ int oracle_astar_puzzle(int side, uint64_t start, uint64_t goal, uint64_t max_st
     int64_t s0 = pstore_get(&st, start);
     if (s0 < 0) { rc = ORACLE_E_NOMEM; goto out; }
     st.g[s0] = 0; /* the start is stored with g = 0 (the reference never stores it, a_star.h:226 vs :253) */
-    uint32_t h0 = oracle_puzzle_manhattan(side, start, goal);
+    uint32_t h0 = pdb ? pdb15_h_state(pdb, start) : oracle_puzzle_manhattan(side, start, goal);
     if (pq_push(&q, (pq_item){prio_of(h0, 0), (uint32_t)s0, 0})) { rc = ORACLE_E_NOMEM; goto out; }
     int64_t goal_idx = -1;
     while (q.n) {
@@ -237,7 +241,7 @@ int oracle_astar_puzzle(int side, uint64_t start, uint64_t goal, uint64_t max_st
             if (g2 >= st.g[ti]) continue; /* rejected: fancy_hashmap.h:56-61 */
             st.g[ti] = g2;
             st.parent[ti] = it.idx;
-            uint32_t ht = oracle_puzzle_manhattan(side, t, goal);
+            uint32_t ht = pdb ? pdb15_h_state(pdb, t) : oracle_puzzle_manhattan(side, t, goal);
             if (pq_push(&q, (pq_item){prio_of(g2 + ht, g2), (uint32_t)ti, g2})) { rc = ORACLE_E_NOMEM; goto out; }
         }
     }
@@ -258,6 +262,19 @@ out:
     pstore_free(&st);
     free(q.a);
     return rc;
+}
+
+int oracle_astar_puzzle(int side, uint64_t start, uint64_t goal, uint64_t max_states,
+                        oracle_result* res, uint64_t* path, uint32_t path_cap) {
+    return astar_puzzle_h(side, start, goal, NULL, max_states, res, path, path_cap);
+}
+
+int oracle_astar_puzzle_pdb(uint64_t start, const uint8_t* pdb, uint64_t max_states, oracle_result* res,
+                            uint64_t* path, uint32_t path_cap) {
+    memset(res, 0, sizeof(*res));
+    res->cost = -1;
+    if (!pdb) return ORACLE_E_INVALID;
+    return astar_puzzle_h(4, start, ORACLE_PUZZLE4_GOAL, pdb, max_states, res, path, path_cap);
 }
 
 /* ------------------------------------------------------------------ */
@@ -769,6 +786,12 @@ int oracle_pdb15_build(uint8_t* out) {
     }
     free(queue);
     return ORACLE_OK;
+}
+
+static uint32_t pdb15_h_state(const uint8_t* pdb, uint64_t s) {
+    uint8_t board[16];
+    for (int c = 0; c < 16; ++c) board[c] = (uint8_t)((s >> (4 * c)) & 15u);
+    return oracle_pdb15_h(pdb, board);
 }
 
 uint32_t oracle_pdb15_h(const uint8_t* pdb, const uint8_t* board) {

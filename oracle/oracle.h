@@ -47,6 +47,8 @@
  *                          across every move; a one-tile and a two-tile
  *                          closed form; equal to the engine's device-built
  *                          tables byte for byte (a GPU test).
+ *   oracle_astar_puzzle_pdb - equal to oracle_astar_puzzle (Manhattan) on
+ *                          15-puzzle walks, expanding no more.
  *   oracle_pdb15_build   - the same checks on 4x4 boards (the 15-puzzle):
  *   oracle_pdb15_h         h = 0 at the goal, h >= Manhattan, h <= the BFS
  *                          depth, |h(s) - h(t)| = 1 across every move, closed
@@ -149,6 +151,11 @@ uint32_t oracle_pdb24_h(const uint8_t* pdb, const uint8_t* board);
 int oracle_pdb15_build(uint8_t* out);
 /* Sum of the three tables for a 4x4 board (16 tile numbers, 0 = blank). */
 uint32_t oracle_pdb15_h(const uint8_t* pdb, const uint8_t* board);
+/* oracle_astar_puzzle on the 15-puzzle towards the ordered goal with these
+ * tables instead of Manhattan. */
+#define ORACLE_PUZZLE4_GOAL 0x0FEDCBA987654321ull
+int oracle_astar_puzzle_pdb(uint64_t start, const uint8_t* pdb, uint64_t max_states, oracle_result* res,
+                            uint64_t* path, uint32_t path_cap);
 /* oracle_astar_board on 5x5 boards towards the ordered goal with the
  * pattern-database heuristic instead of Manhattan. */
 int oracle_astar_board_pdb(const uint8_t* start, const uint8_t* pdb, uint64_t max_states, oracle_result* res,

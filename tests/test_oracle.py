@@ -309,6 +309,16 @@ def test_pdb15_closed_forms():
     assert oracle.pdb15_h(b) == oracle.board_manhattan(4, b, GOAL4) == 2
 
 
+@pytest.mark.parametrize("depth,seed", [(30, 51), (45, 52), (60, 53)])
+def test_pdb15_astar_equals_manhattan_astar(depth, seed):
+    start, goal = ga.gen_puzzle(4, depth, seed), ga.puzzle_goal(4)
+    rc_m, m, _ = oracle.astar_puzzle(4, start, goal, want_path=False)
+    rc_p, p, path = oracle.astar_puzzle_pdb(start)
+    assert rc_m == rc_p == oracle.OK and p.cost == m.cost
+    assert p.expanded <= m.expanded
+    assert oracle.replay_puzzle(4, path, start, goal) == p.cost
+
+
 # ---- 24-puzzle pattern databases (oracle_pdb24_*) ----
 GOAL5 = tuple(range(1, 25)) + (0,)
 
