@@ -61,6 +61,26 @@ def test_served_config5_stream_matches_golden():
             assert alone.status == ga.OK and alone.cost == sols[i].cost, i
 
 
+def test_served_big_grids_expand_near_the_sequential_search():
+    # the serving stream's grids of at least deep_cells cells (the grid lane's), under the served
+    # tuning: with weights 1..9 and Manhattan x w_min, the oracle's sequential A* expands about 80 % of
+    # their cells (f < C* covers nearly the whole grid). GA* with its heaps per big grid and two pops
+    # per round must give the same cost within 1.25x the sequential expansions (1.10-1.11 measured)
+    cfg = tuning.load("served")
+    insts = [x for x in wl.config5(5, 1024, 4000.0, threads=16)
+             if x.kind == "grid" and x.side * x.side >= cfg.deep_cells][:4]
+    assert len(insts) == 4 and all(x.side >= 1448 for x in insts)
+    with bench_engine(cfg) as e:
+        sols = e.solve_batch([x.request(num_queues=cfg.heaps(x.kind, x.side)) for x in insts])
+    for i, (x, sol) in enumerate(zip(insts, sols)):
+        rc, ref, _ = oracle.astar_grid(x.cells, x.side, x.side, x.start, x.goal, 4, want_path=False)
+        assert rc == oracle.OK and ref.expanded > 0.7 * x.side * x.side, (i, x.side, ref.expanded)
+        assert sol.status == ga.OK and sol.cost == ref.cost, (i, x.side)
+        ratio = sol.expanded / ref.expanded
+        print(f"grid {x.side}^2: oracle A* {ref.expanded} expansions, GA* {sol.expanded} ({ratio:.3f}x)")
+        assert 0.9 <= ratio <= 1.25, (i, x.side, ratio)
+
+
 def test_batched_config4_under_served_tuning_matches_golden():
     # 1024 mixed instances in one ga_solve_batch on the graph-replayed served engine
     gold = json.load(open(os.path.join(GOLDEN, "config4.json")))["instances"]
