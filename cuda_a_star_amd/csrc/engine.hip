@@ -1710,7 +1710,9 @@ struct ga_engine {
         stats.push_in += s.push_in;
         algo_sampled[j.lane] += algo_bytes(j.kind, s, true);
         algo_always[j.lane] += algo_bytes(j.kind, s, false);
-        stats.lane_atomics[j.lane] += s.probes + s.lookups;
+        // one atomicMin per find-or-insert, plus one CAS or atomic load per probe of a hashed set;
+        // a dense set's probe is the cell index itself, no memory access
+        stats.lane_atomics[j.lane] += s.lookups + (dense ? 0 : s.probes);
     }
 
     // Algorithmic bytes of the step kernel: the bytes its data structures'

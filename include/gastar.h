@@ -183,8 +183,9 @@ typedef struct ga_engine_stats {
     uint64_t lane_counted_launches[GA_MAX_LANES + 1]; /* step launches of counted chunks */
     double lane_step_ms[GA_MAX_LANES + 1];           /* timed chunks' time from HIP events (one pair per chunk) */
     double lane_algo_bytes[GA_MAX_LANES + 1];        /* algorithmic bytes of all the lane's launches */
-    uint64_t lane_atomics[GA_MAX_LANES + 1];         /* closed-set atomics (probes + lookups) of the lane's
-                                                        counted launches (profile) */
+    uint64_t lane_atomics[GA_MAX_LANES + 1];         /* closed-set atomics of the lane's counted launches:
+                                                        one per find-or-insert, plus one per hashed-set
+                                                        probe (profile) */
     double lane_busy_ms[GA_MAX_LANES + 1];           /* time the lane's chunks occupied its stream, from one
                                                         HIP event pair around every chunk (profile) */
     /* host time of the run loop: admitting requests, handling chunk read-backs (finished requests, path

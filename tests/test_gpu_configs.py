@@ -72,6 +72,9 @@ def test_served_big_grids_expand_near_the_sequential_search():
     assert len(insts) == 4 and all(x.side >= 1448 for x in insts)
     with bench_engine(cfg) as e:
         sols = e.solve_batch([x.request(num_queues=cfg.heaps(x.kind, x.side)) for x in insts])
+        st = e.stats()
+    # every request ran on the grid lane, whose closed sets are dense: one atomicMin per lookup, nothing per probe
+    assert st.lookups > 0 and st.lane_atomics[cfg.lanes - 1] == st.lookups
     for i, (x, sol) in enumerate(zip(insts, sols)):
         rc, ref, _ = oracle.astar_grid(x.cells, x.side, x.side, x.start, x.goal, 4, want_path=False)
         assert rc == oracle.OK and ref.expanded > 0.7 * x.side * x.side, (i, x.side, ref.expanded)
