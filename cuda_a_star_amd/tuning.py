@@ -155,7 +155,7 @@ def _measure_plain(insts, cfg: TuneConfig) -> Dict[str, float]:
     return out
 
 
-# the kind whose solves judge a parameter in the plain sweep (others: the whole sample)
+# the kind whose solves decide a parameter in the plain sweep (others: the whole sample)
 PLAIN_AXIS_KIND = {"puzzle_heaps": "puzzle", "slack_puzzle": "puzzle", "rounds": "puzzle", "max_load_pct": "puzzle",
                    "grid_heaps_small": "grid", "grid_heaps_mid": "grid", "grid_heaps_big": "grid",
                    "slack_grid": "grid", "rounds_grid": "grid", "step_flags": "grid"}
