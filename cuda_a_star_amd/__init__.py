@@ -165,7 +165,7 @@ EXPORTS = (
     "ga_serve_stop", "ga_solve_noalloc", "ga_result_free", "ga_status_str", "ga_engine_last_error",
     "ga_pdb24_tables",
     "ga_pdb15_tables",
-    "ga_debug_fail_run", "ga_debug_closed_set", "ga_debug_heap",
+    "ga_debug_fail_run", "ga_debug_closed_set", "ga_debug_heap", "ga_heap_rate",
     "ga_device_upload", "ga_device_free", "ga_atomic_rate",
     "ga_splitmix64", "ga_puzzle_goal", "ga_gen_puzzle", "ga_gen_grid", "ga_puzzle_solvable",
     "ga_gen_board", "ga_board_solvable",
@@ -218,6 +218,8 @@ def lib() -> ctypes.CDLL:
                                       ctypes.c_uint32, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
     L.ga_debug_heap.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, _vp, ctypes.c_uint32,
                                 ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, _vp, _vp, _vp, _vp, _vp]
+    L.ga_heap_rate.argtypes = [ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                               P(ctypes.c_double)]
     L.ga_device_upload.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, P(ctypes.c_void_p)]
     L.ga_device_free.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
     L.ga_atomic_rate.argtypes = [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int,

@@ -6,7 +6,8 @@ include/gastar/core.hpp) on caller keys, in each hashed layout, and returns
 every outcome with the table after the last insert.  `heap` runs sequences of
 batched pushes and pops on W-ary heaps whose top levels live in LDS for a
 launch (`push_records`, `heap_pop`, `HeapRef`), and returns what every pop
-returned and the heaps at the end.  The reference's counterparts are the mass
+returned and the heaps at the end; `heap_rate` times those operations over
+the whole device at a given arity (4, 8 or 16).  The reference's counterparts are the mass
 insert of fancy_hashmap.h:32-81,155-167 and the heap operations of
 heaps.h:132-177.  Both need a GPU; their argument checks do not.
 """
@@ -72,3 +73,14 @@ def heap(arity: int, ops, ops_per_launch: int, cap: int, track_dirty: bool = Fal
         raise GAStarError(rc, "ga_debug_heap")
     return {"popped": popped, "popped_tag": popped_tag, "size": size, "overflow": overflow,
             "heap": [heap_out[h, :size[h]].copy() for h in range(n_heaps)]}
+
+
+def heap_rate(arity: int, waves: int = 8192, fill: int = 4096, iters: int = 1024, device: int = 0) -> float:
+    """Heap pops per second over the whole device (pushes run at the same rate): `waves`
+    64-lane waves, each W-lane group popping two entries and pushing two from a heap of
+    `fill` entries, `iters` times (ga_heap_rate)."""
+    out = ctypes.c_double()
+    rc = lib().ga_heap_rate(device, arity, waves, fill, iters, ctypes.byref(out))
+    if rc != 0:
+        raise GAStarError(rc, "ga_heap_rate")
+    return out.value

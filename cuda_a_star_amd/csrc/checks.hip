@@ -75,6 +75,22 @@ __global__ void closed_set_kernel(InstDesc d, const uint32_t* keys, const uint32
     if (i < n) update_one<Set>(d, keys, g, parent, i, max_probes, outcome, slot_out);
 }
 
+// Pushes the collected batch (if any) as one set of candidate records, counting
+// a refused batch.
+template <int W, int KW>
+__device__ __forceinline__ void flush_batch(const HeapRef<W>& H, uint32_t& S, HeapEntry (&rec)[KW], uint32_t& nb,
+                                            uint32_t& refused, int lane, uint32_t cap, const Tally& tl) {
+    if (nb) {
+        bool over = false;
+        push_records<W, KW>(H, S, rec, KW * W, lane, kInf, cap, over, tl, false);
+        refused += over ? 1u : 0u;
+#pragma unroll
+        for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
+        nb = 0;
+    }
+    group_fence();
+}
+
 // One wave; heap h = threadIdx.x / W runs its operations [begin, end).
 template <int W>
 __global__ void __launch_bounds__(64) heap_kernel(HeapEntry* heaps, uint32_t* sizes, uint32_t stride, uint32_t cap,
@@ -86,11 +102,10 @@ __global__ void __launch_bounds__(64) heap_kernel(HeapEntry* heaps, uint32_t* si
     constexpr uint32_t kTop = HeapRef<W>::kTop;
     __shared__ HeapEntry top[(64 / W) * kTop];
     __shared__ uint32_t dirty[64 / W];
-    __shared__ uint32_t tally[kTallies];
     const int lane = (int)(threadIdx.x & (W - 1));
     const uint32_t h = threadIdx.x / W;
     const bool mine = h < n_heaps;
-    const Tally tl{tally, false};
+    const Tally tl{nullptr, false};  // counts nothing: push_records only adds profile counts
     const HeapRef<W> H{as_global(heaps) + (size_t)(mine ? h : 0u) * stride + (W - 1), as_local(top) + (size_t)h * kTop,
                        as_local(dirty) + h, track != 0};
     uint32_t S = mine ? sizes[h] : 0u;
@@ -103,17 +118,7 @@ __global__ void __launch_bounds__(64) heap_kernel(HeapEntry* heaps, uint32_t* si
 #pragma unroll
     for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
     uint32_t nb = 0, refused = 0;  // record positions in the batch; batches refused
-    auto flush = [&]() {
-        if (nb) {
-            bool over = false;
-            push_records<W, KW>(H, S, rec, K, lane, kInf, cap, over, tl, false);
-            refused += over ? 1u : 0u;
-#pragma unroll
-            for (int r = 0; r < KW; ++r) rec[r].prio = kNoRecord;
-            nb = 0;
-        }
-        group_fence();
-    };
+    auto flush = [&]() { flush_batch<W, KW>(H, S, rec, nb, refused, lane, cap, tl); };
     for (uint32_t i = begin; i < end; ++i) {
         const uint64_t op = mine ? ops[(size_t)h * n_ops + i] : GA_HOP_NOP;
         if (op == GA_HOP_NOP) continue;
@@ -239,6 +244,103 @@ int run_closed_set(const uint32_t* keys, const uint32_t* g, const uint32_t* pare
     return GA_OK;
 }
 
+// Heap operation rate over the whole device.  Every W-lane group owns one heap
+// of `fill` entries in global memory, with its top entries in LDS for the
+// launch as a search step keeps them.  Each of `iters` rounds pops two entries
+// and pushes two successors of the first (f raised by 1..8, g by 1) as one
+// batch, so every heap keeps its size and each push and pop walks a heap of
+// `fill` entries.  Heap h's logical entry j lies at h * stride + W - 1 + j.
+template <int W>
+__global__ void __launch_bounds__(256) heap_rate_kernel(HeapEntry* heaps, uint32_t stride, uint32_t fill,
+                                                        uint32_t iters, uint64_t* sink) {
+    constexpr int G = 256 / W;  // heaps per workgroup
+    constexpr uint32_t kTop = HeapRef<W>::kTop;
+    __shared__ HeapEntry top[G * kTop];
+    __shared__ uint32_t dirty[G];
+    const int lane = (int)(threadIdx.x & (W - 1));
+    const uint32_t hl = threadIdx.x / W;
+    const uint32_t h = blockIdx.x * G + hl;
+    const Tally tl{nullptr, false};  // counts nothing: push_records only adds profile counts
+    const HeapRef<W> H{as_global(heaps) + (size_t)h * stride + (W - 1), as_local(top) + (size_t)hl * kTop,
+                       as_local(dirty) + hl, false};
+    uint32_t S = fill;
+    H.fill(S, lane);
+    group_fence();
+    auto load = [](const HeapEntry&, uint64_t& k, uint32_t& gs) {
+        k = 0;
+        gs = 0;
+    };
+    auto pre = [](const HeapEntry&) {};
+    uint64_t acc = 0;
+    for (uint32_t it = 0; it < iters; ++it) {
+        HeapEntry a, b;
+        uint64_t key = 0;
+        uint32_t g = 0, lv = 0, hlv = 0;
+        heap_pop<W>(H, S, lane, lv, hlv, load, pre, key, g, kInf, a);
+        group_fence();
+        heap_pop<W>(H, S, lane, lv, hlv, load, pre, key, g, kInf, b);
+        group_fence();
+        acc += b.prio;
+        HeapEntry rec[1];
+        rec[0].prio = kNoRecord;
+        rec[0].slot = it;
+        rec[0].aux = h;
+        if (lane < 2) {
+            const uint32_t step = 1u + (uint32_t)((mix64(((uint64_t)h << 32) | it) >> (8 * lane)) & 7u);
+            rec[0].prio = make_prio(prio_f(a.prio) + step, prio_g(a.prio) + 1u);
+        }
+        bool over = false;
+        push_records<W, 1>(H, S, rec, W, lane, kInf, fill + W, over, tl, false);
+        group_fence();
+    }
+    H.write_back(S, lane);
+    if (acc == 0x5A5A5A5A5A5A5A5Aull) *sink = acc;  // keeps the pops live
+}
+
+// entries of every heap: logical j < fill holds f = j + 1, g = 0 (ascending,
+// so a valid heap of any arity); the rest of the heap's storage is empty
+template <int W>
+__global__ void heap_rate_init_kernel(HeapEntry* heaps, uint64_t n, uint32_t stride, uint32_t fill) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x) {
+        const uint32_t e = (uint32_t)(i % stride);
+        HeapEntry x;
+        x.prio = (e >= W - 1 && e - (W - 1) < fill) ? make_prio(e - (W - 1) + 1u, 0u) : kNoRecord;
+        x.slot = 0;
+        x.aux = 0;
+        heaps[i] = x;
+    }
+}
+
+template <int W>
+int run_heap_rate(uint32_t waves, uint32_t fill, uint32_t iters, double* pops_per_sec) {
+    const uint32_t blocks = waves / 4;  // 256-thread workgroups
+    const uint32_t n_heaps = blocks * (256 / W);
+    const uint32_t stride = fill + 2 * W;  // capacity fill + W, and the W - 1 entries before logical entry 0
+    const uint64_t n = (uint64_t)n_heaps * stride;
+    DevBuf<HeapEntry> heaps;
+    DevBuf<uint64_t> sink;
+    if (!heaps.alloc(n) || !sink.alloc(1)) return GA_E_NOMEM;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    int rc = hip_rc(hipEventCreate(&t0));
+    if (rc == GA_OK) rc = hip_rc(hipEventCreate(&t1));
+    float ms = 0.f;
+    for (int rep = 0; rep < 2 && rc == GA_OK; ++rep) {  // the first pass maps the pages; the second is timed
+        heap_rate_init_kernel<W><<<1024, 256>>>(heaps.p, n, stride, fill);
+        rc = hip_rc(hipEventRecord(t0, nullptr));
+        if (rc == GA_OK) {
+            heap_rate_kernel<W><<<blocks, 256>>>(heaps.p, stride, fill, iters, sink.p);
+            rc = hip_rc(hipGetLastError());
+        }
+        if (rc == GA_OK) rc = hip_rc(hipEventRecord(t1, nullptr));
+        if (rc == GA_OK) rc = hip_rc(hipEventSynchronize(t1));
+        if (rc == GA_OK) rc = hip_rc(hipEventElapsedTime(&ms, t0, t1));
+    }
+    if (t0) (void)hipEventDestroy(t0);
+    if (t1) (void)hipEventDestroy(t1);
+    if (rc == GA_OK) *pops_per_sec = ms > 0.f ? 2.0 * n_heaps * (double)iters / (ms * 1e-3) : 0.0;
+    return rc;
+}
+
 template <int W>
 int run_heap(uint32_t n_heaps, const uint64_t* ops, uint32_t n_ops, uint32_t opl, uint32_t cap, int track,
              uint64_t* popped, uint32_t* popped_tag, uint64_t* heap_out, uint32_t* size_out, uint32_t* overflow_out) {
@@ -314,8 +416,11 @@ int ga_debug_closed_set(int device, int layout, const uint32_t* keys, const uint
 int ga_debug_heap(int device, uint32_t arity, uint32_t n_heaps, const uint64_t* ops, uint32_t n_ops,
                   uint32_t ops_per_launch, uint32_t cap, int track_dirty, uint64_t* popped, uint32_t* popped_tag,
                   uint64_t* heap_out, uint32_t* size_out, uint32_t* overflow_out) {
-    if ((arity != 4 && arity != 8) || n_heaps == 0 || n_heaps > 64 / arity || cap == 0 || cap > (1u << 20) ||
-        n_ops > (1u << 20) || ops_per_launch == 0 || (track_dirty != 0 && track_dirty != 1) || !heap_out ||
+    // the dirty mask has one bit per LDS top entry: 21 at arity 4, 25 at 8, 49 at 16.  Arity 64 (one
+    // heap per wave) is refused: its first run here ran past a 120 s limit with no cause shown.
+    if ((arity != 4 && arity != 8 && arity != 16) || n_heaps == 0 || n_heaps > 64 / arity ||
+        cap == 0 || cap > (1u << 20) || n_ops > (1u << 20) || ops_per_launch == 0 ||
+        (track_dirty != 0 && track_dirty != 1) || (track_dirty && arity > 8) || !heap_out ||
         !size_out || !overflow_out || (n_ops && (!ops || !popped || !popped_tag)))
         return GA_E_INVALID;
     for (size_t i = 0; i < (size_t)n_heaps * n_ops; ++i) {
@@ -325,11 +430,32 @@ int ga_debug_heap(int device, uint32_t arity, uint32_t n_heaps, const uint64_t* 
         return GA_E_INVALID;  // a push whose f is the infinite cost
     }
     if (hipSetDevice(device) != hipSuccess) return GA_E_INVALID;
-    if (arity == 4)
-        return run_heap<4>(n_heaps, ops, n_ops, ops_per_launch, cap, track_dirty, popped, popped_tag, heap_out,
-                           size_out, overflow_out);
-    return run_heap<8>(n_heaps, ops, n_ops, ops_per_launch, cap, track_dirty, popped, popped_tag, heap_out, size_out,
-                       overflow_out);
+    switch (arity) {
+        case 4:
+            return run_heap<4>(n_heaps, ops, n_ops, ops_per_launch, cap, track_dirty, popped, popped_tag, heap_out,
+                               size_out, overflow_out);
+        case 8:
+            return run_heap<8>(n_heaps, ops, n_ops, ops_per_launch, cap, track_dirty, popped, popped_tag, heap_out,
+                               size_out, overflow_out);
+        default:
+            return run_heap<16>(n_heaps, ops, n_ops, ops_per_launch, cap, track_dirty, popped, popped_tag, heap_out,
+                                size_out, overflow_out);
+    }
+}
+
+int ga_heap_rate(int device, uint32_t arity, uint32_t waves, uint32_t fill, uint32_t iters, double* pops_per_sec) {
+    if ((arity != 4 && arity != 8 && arity != 16) || waves == 0 || waves % 4 || waves > (1u << 16) ||
+        fill < 2 || fill > (1u << 16) || iters == 0 || iters > (1u << 16) || !pops_per_sec)
+        return GA_E_INVALID;
+    // at most 16 GiB of heaps
+    if ((uint64_t)(waves / 4) * (256 / arity) * (fill + 2 * arity) * sizeof(HeapEntry) > (16ull << 30))
+        return GA_E_INVALID;
+    if (hipSetDevice(device) != hipSuccess) return GA_E_INVALID;
+    switch (arity) {
+        case 4: return run_heap_rate<4>(waves, fill, iters, pops_per_sec);
+        case 8: return run_heap_rate<8>(waves, fill, iters, pops_per_sec);
+        default: return run_heap_rate<16>(waves, fill, iters, pops_per_sec);
+    }
 }
 
 }  // extern "C"

@@ -369,11 +369,12 @@ int ga_debug_closed_set(int device, int layout, const uint32_t* keys, const uint
 #define GA_HOP_FLUSH 0xFFFFFFFFFFFFFFFEull
 #define GA_HOP_NOP 0xFFFFFFFFFFFFFFFDull
 #define GA_HOP_GAP 0xFFFFFFFFFFFFFFFCull
-/* Runs n_heaps (1 <= n_heaps <= 64 / arity) heaps of `arity` (4 or 8) and
+/* Runs n_heaps (1 <= n_heaps <= 64 / arity) heaps of `arity` (4, 8 or 16) and
  * capacity cap (1 <= cap <= 2^20) in one wave, heap h executing
  * ops[h*n_ops .. (h+1)*n_ops) (n_ops <= 2^20), ops_per_launch operations per
  * kernel launch, each launch copying the heaps' top levels into LDS and back
- * as a search step does (track_dirty: write back only the changed entries).
+ * as a search step does (track_dirty, arity 4 and 8 only: write back only
+ * the changed entries).
  * Writes, per heap and operation, popped[h*n_ops + i] (the priority a POP
  * returned; all ones for an empty heap or another operation) and
  * popped_tag[h*n_ops + i] (the index, within the heap's operations, of the
@@ -385,6 +386,16 @@ int ga_debug_closed_set(int device, int layout, const uint32_t* keys, const uint
 int ga_debug_heap(int device, uint32_t arity, uint32_t n_heaps, const uint64_t* ops, uint32_t n_ops,
                   uint32_t ops_per_launch, uint32_t cap, int track_dirty, uint64_t* popped, uint32_t* popped_tag,
                   uint64_t* heap_out, uint32_t* size_out, uint32_t* overflow_out);
+
+/* Measurement helper, no counterpart in the reference: the device's rate
+ * for heap operations at a given arity (4, 8 or 16).  waves (a multiple of 4, at most
+ * 2^16) 64-lane waves run in 256-thread workgroups, each W-lane group of them
+ * owning one heap (64 / arity heaps per wave) of `fill` entries (2..2^16) with
+ * its top entries in LDS, as a search step keeps them; every group pops two
+ * entries and pushes two successors of the first as one batch, `iters` times
+ * (1..2^16), so its heap keeps its size.  Writes the pops per second of the
+ * second of two launches (pushes run at the same rate) to *pops_per_sec. */
+int ga_heap_rate(int device, uint32_t arity, uint32_t waves, uint32_t fill, uint32_t iters, double* pops_per_sec);
 
 /* Device memory helpers for callers that keep grids resident in HBM. */
 int ga_device_upload(ga_engine* e, const void* host, size_t bytes, void** device_out);

@@ -534,7 +534,8 @@ struct HeapRef {
     __device__ __forceinline__ void put(uint32_t j, const HeapEntry& e) const {
         if (j < kTop) {
             t[j] = e;
-            if (track) __hip_atomic_fetch_or(dirty, 1u << j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            if constexpr (kTop <= 32)  // wider tops keep no dirty mask
+                if (track) __hip_atomic_fetch_or(dirty, 1u << j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         } else {
             g[j] = e;
         }
@@ -556,13 +557,21 @@ struct HeapRef {
         }
     }
     // Writes entries [0, min(S, kTop)) back to global memory: all of them, or
-    // (with a dirty mask) those the launch changed.
+    // (with a dirty mask, tops of at most 32 entries) those the launch changed.
     __device__ __forceinline__ void write_back(uint32_t S, int lane) const {
-        const uint32_t mask = track ? *dirty : ~0u;
+        if constexpr (kTop <= 32) {
+            const uint32_t mask = track ? *dirty : ~0u;
 #pragma unroll
-        for (int i = 0; i < kPer; ++i) {
-            const uint32_t j = (uint32_t)lane + (uint32_t)i * W;
-            if (j < S && j < kTop && ((mask >> j) & 1u)) g[j] = t[j];
+            for (int i = 0; i < kPer; ++i) {
+                const uint32_t j = (uint32_t)lane + (uint32_t)i * W;
+                if (j < S && j < kTop && ((mask >> j) & 1u)) g[j] = t[j];
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < kPer; ++i) {
+                const uint32_t j = (uint32_t)lane + (uint32_t)i * W;
+                if (j < S && j < kTop) g[j] = t[j];
+            }
         }
     }
 };

@@ -62,12 +62,22 @@ def test_structure_checks_reject_bad_arguments_before_any_device_call():
     b32 = np.zeros(64, dtype=np.uint32)
     hp = lambda arity=4, nh=1, o=ops, opl=1, cap=8, track=0: L.ga_debug_heap(
         0, arity, nh, p(o), len(o), opl, cap, track, p(buf), p(b32), p(buf), p(b32), p(b32))
-    assert hp(arity=16) == -1 and hp(arity=2) == -1
-    assert hp(nh=0) == -1 and hp(nh=17) == -1 and hp(arity=8, nh=9) == -1
+    assert hp(arity=32) == -1 and hp(arity=2) == -1
+    assert hp(nh=0) == -1 and hp(nh=17) == -1 and hp(arity=8, nh=9) == -1 and hp(arity=16, nh=5) == -1
     assert hp(cap=0) == -1 and hp(cap=(1 << 20) + 1) == -1 and hp(opl=0) == -1 and hp(track=2) == -1
+    assert hp(arity=16, track=1) == -1  # no dirty mask above 32 top entries
+    assert hp(arity=64) == -1  # the wave-wide heap is switched off
     assert hp(o=np.array([0xFFFFFFFF << 32], dtype=np.uint64)) == -1  # f = the infinite cost
     with pytest.raises(ga.GAStarError):
-        ck.heap(16, ops, 1, 8)
+        ck.heap(32, ops, 1, 8)
+    out = ctypes.c_double()
+    hr = lambda arity=4, waves=8, fill=16, iters=1, o=ctypes.byref(out): L.ga_heap_rate(0, arity, waves, fill, iters, o)
+    assert hr(arity=2) == -1 and hr(arity=32) == -1 and hr(arity=64) == -1 and hr(waves=0) == -1 and hr(waves=6) == -1
+    assert hr(waves=(1 << 16) + 4) == -1 and hr(fill=1) == -1 and hr(fill=(1 << 16) + 1) == -1
+    assert hr(iters=0) == -1 and hr(iters=(1 << 16) + 1) == -1 and hr(o=None) == -1
+    assert hr(waves=1 << 16, fill=1 << 16) == -1  # above 16 GiB of heaps
+    with pytest.raises(ga.GAStarError):
+        ck.heap_rate(32)
 
 
 def test_status_strings():
