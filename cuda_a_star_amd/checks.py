@@ -84,3 +84,19 @@ def heap_rate(arity: int, waves: int = 8192, fill: int = 4096, iters: int = 1024
     if rc != 0:
         raise GAStarError(rc, "ga_heap_rate")
     return out.value
+
+
+if __name__ == "__main__":
+    # python -m cuda_a_star_amd.checks: the heap pop rate per arity, the device full of waves
+    import argparse
+
+    ap = argparse.ArgumentParser(description="whole-device heap operation rate per arity (ga_heap_rate)")
+    ap.add_argument("--waves", type=int, default=8192)
+    ap.add_argument("--iters", type=int, default=256)
+    ap.add_argument("--fill", type=int, nargs="+", default=[256, 4096])
+    args = ap.parse_args()
+    for fill in args.fill:
+        for arity in (4, 8, 16):
+            rate = heap_rate(arity, waves=args.waves, fill=fill, iters=args.iters)
+            print(f"heap entries {fill} arity {arity}: {args.waves * 64 // arity} heaps, "
+                  f"{rate / 1e9:.2f} G pops/s", flush=True)
