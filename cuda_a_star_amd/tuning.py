@@ -67,6 +67,7 @@ class TuneConfig:
     slack_grid: int = 0
     block: int = 256            # threads per step-kernel workgroup
     step_flags: int = 0         # step-kernel variants (ga_config.step_flags)
+    max_instances: int = 4096   # instance slots, split evenly over the launch lanes and the 24-puzzle lane
 
     def heaps(self, kind: str, side: int) -> int:
         if kind == "puzzle":
@@ -85,7 +86,7 @@ class TuneConfig:
                 "adaptive_start": self.adaptive_start, "grow_thresh": self.grow_thresh,
                 "rounds_grid": self.rounds_grid, "lanes": self.lanes, "deep_cells": self.deep_cells,
                 "slack_puzzle": self.slack_puzzle, "slack_grid": self.slack_grid, "block": self.block,
-                "step_flags": self.step_flags}
+                "step_flags": self.step_flags, "max_instances": self.max_instances}
 
     def describe(self) -> Dict[str, int]:
         return asdict(self)
