@@ -203,6 +203,7 @@ def sweep(mode: str, n: int, seed: int, qps: float, passes: int = 2) -> TuneConf
             "slack_grid": [0, 8, 16, 24, 32, 48],
             "block": [128, 256, 512],
             "step_flags": [0, 1, 2, 3, 4, 5, 6, 12],
+            "max_instances": [4096, 6144, 8192],
         }
     else:
         axes = {
